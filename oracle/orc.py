@@ -1,0 +1,400 @@
+"""ctypes front for the CPU oracle (liboracle.so) and the reference build (_ref).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py; the product package never imports this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_i64 = C.c_int64
+_f32 = C.c_float
+_vp = C.c_void_p
+_P = np.ctypeslib.ndpointer
+
+CAND_DTYPE = np.dtype([("mt", np.int32), ("mdt", np.int32), ("mwidth", np.int32), ("msnr", np.float32)])
+
+
+def build(ref=False):
+    subprocess.check_call(["make", "-s", "-C", HERE], stdout=subprocess.DEVNULL)
+    if ref:
+        subprocess.check_call(["make", "-s", "-C", HERE, "ref"], stdout=subprocess.DEVNULL)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_vp) if a is not None else None
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_plan_create.restype = _vp
+        L.orc_plan_create.argtypes = [_f32, _f32, _i64, _i64, _f32, _i64, _i64, _i64]
+        L.orc_plan_destroy.argtypes = [_vp]
+        for n in ("orc_plan_niters",):
+            getattr(L, n).restype = _i64
+            getattr(L, n).argtypes = [_vp]
+        for n in ("orc_plan_nsub", "orc_plan_nrows", "orc_plan_nmap", "orc_plan_ncopy"):
+            getattr(L, n).restype = _i64
+            getattr(L, n).argtypes = [_vp, _i64]
+        L.orc_plan_state_shape.argtypes = [_vp, _i64, _vp]
+        L.orc_plan_grid.argtypes = [_vp, _i64, _vp, _vp]
+        L.orc_plan_maps.argtypes = [_vp, _i64, _vp, _vp]
+        L.orc_plan_df.argtypes = [_vp, _i64, C.POINTER(_f32), C.POINTER(_f32)]
+        L.orc_fdmt_execute.argtypes = [_vp, _vp, _vp, C.c_int]
+        L.orc_fdmt_initialise.argtypes = [_vp, _vp, _vp, C.c_int]
+        L.orc_fdmt_bruteforce_f64.argtypes = [_vp, _vp, _vp]
+        L.orc_fdmt_track_extent.argtypes = [_vp, _vp]
+        L.orc_cfdmt_create.restype = _vp
+        L.orc_cfdmt_create.argtypes = [_f32, _f32, C.c_int, C.c_int, C.c_int]
+        L.orc_cfdmt_destroy.argtypes = [_vp]
+        L.orc_cfdmt_niter.argtypes = [_vp]
+        L.orc_cfdmt_quant.argtypes = [_vp, C.c_int]
+        L.orc_cfdmt_config.argtypes = [_vp, C.c_int, _vp, _vp]
+        L.orc_cfdmt_init_delta.argtypes = [_vp]
+        L.orc_cfdmt_process.argtypes = [_vp, _vp, _vp, C.c_int]
+        L.orc_stats.argtypes = [_vp, _vp, _i64, _i64, C.c_int, _vp, _vp, _vp, _vp]
+        L.orc_normalise.argtypes = [_vp, _vp, _i64, _i64, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp]
+        L.orc_fdmt_normalization.argtypes = [_vp, _vp, _i64, _vp]
+        L.orc_boxcar_pow2.restype = _i64
+        L.orc_boxcar_pow2.argtypes = [_vp, _i64, _i64, _f32, C.c_int, _vp, _i64]
+        L.orc_boxcar_all.restype = _i64
+        L.orc_boxcar_all.argtypes = [_vp, _i64, _i64, _f32, C.c_int, _vp, _i64]
+        L.orc_cluster.restype = _i64
+        L.orc_cluster.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp, _vp]
+        L.orc_detect_max.argtypes = [_vp, _vp, _i64, _i64, C.c_int, _vp, C.POINTER(_f32)]
+        L.orc_num_threads.restype = C.c_int
+        L.orc_set_num_threads.argtypes = [C.c_int]
+        _lib = L
+    return _lib
+
+
+def err():
+    return lib().orc_last_error().decode()
+
+
+class Plan:
+    """FDMT plan (fdmt_base.cpp:37-53,119-279) as numpy tables."""
+
+    def __init__(self, fmin, fmax, nchans, nsamps, tsamp=1.0, dt_max=0, dt_step=1, dt_min=0):
+        L = lib()
+        self.h = L.orc_plan_create(fmin, fmax, nchans, nsamps, tsamp, dt_max, dt_step, dt_min)
+        if not self.h:
+            raise RuntimeError(err())
+        self.nchans, self.nsamps = nchans, nsamps
+        self.niters = L.orc_plan_niters(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_plan_destroy(self.h)
+            self.h = None
+
+    def nrows(self, i):
+        return lib().orc_plan_nrows(self.h, i)
+
+    def state_shape(self, i):
+        out = np.zeros(5, np.int64)
+        lib().orc_plan_state_shape(self.h, i, _ptr(out))
+        return out
+
+    def grid(self, i):
+        ns = lib().orc_plan_nsub(self.h, i)
+        off = np.zeros(ns + 1, np.int64)
+        lib().orc_plan_grid(self.h, i, _ptr(off), None)
+        g = np.zeros(max(int(off[-1]), 1), np.int64)
+        lib().orc_plan_grid(self.h, i, _ptr(off), _ptr(g))
+        return off, g[: off[-1]]
+
+    def maps(self, i):
+        nm = lib().orc_plan_nmap(self.h, i)
+        nc = lib().orc_plan_ncopy(self.h, i)
+        m = np.zeros((max(nm, 1), 7), np.int64)
+        c = np.zeros((max(nc, 1), 4), np.int64)
+        lib().orc_plan_maps(self.h, i, _ptr(m), _ptr(c))
+        return m[:nm], c[:nc]
+
+    def df(self, i):
+        a, b = _f32(), _f32()
+        lib().orc_plan_df(self.h, i, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def execute(self, x, init_mode=0):
+        x = np.ascontiguousarray(x, np.float32)
+        assert x.size == self.nchans * self.nsamps
+        out = np.zeros((self.nrows(self.niters), self.nsamps), np.float32)
+        if lib().orc_fdmt_execute(self.h, _ptr(x), _ptr(out), init_mode) != 0:
+            raise RuntimeError(err())
+        return out
+
+    def initialise(self, x, init_mode=0):
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.zeros((self.nrows(0), self.nsamps), np.float32)
+        if lib().orc_fdmt_initialise(self.h, _ptr(x), _ptr(out), init_mode) != 0:
+            raise RuntimeError(err())
+        return out
+
+    def bruteforce(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.zeros((self.nrows(self.niters), self.nsamps), np.float64)
+        if lib().orc_fdmt_bruteforce_f64(self.h, _ptr(x), _ptr(out)) != 0:
+            raise RuntimeError(err())
+        return out
+
+    def track_extent(self):
+        out = np.zeros(self.nrows(self.niters), np.int64)
+        lib().orc_fdmt_track_extent(self.h, _ptr(out))
+        return out
+
+
+class CFdmt:
+    """Legacy CFdmtCpu (FdmtCpu.cpp:131-249)."""
+
+    def __init__(self, fmin, fmax, nchan, cols, imaxdt):
+        self.h = lib().orc_cfdmt_create(fmin, fmax, nchan, cols, imaxdt)
+        if not self.h:
+            raise RuntimeError(err())
+        self.nchan, self.cols, self.imaxdt = nchan, cols, imaxdt
+        self.niter = lib().orc_cfdmt_niter(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_cfdmt_destroy(self.h)
+            self.h = None
+
+    def config(self, i):
+        q = lib().orc_cfdmt_quant(self.h, i)
+        cs = np.zeros(q + 1, np.int32)
+        fr = np.zeros(q + 1, np.float32)
+        lib().orc_cfdmt_config(self.h, i, _ptr(cs), _ptr(fr))
+        return cs, fr
+
+    def init_delta(self):
+        return lib().orc_cfdmt_init_delta(self.h)
+
+    def process(self, x, b_ones=False):
+        out = np.zeros((self.imaxdt, self.cols), np.float32)
+        xa = None if x is None else np.ascontiguousarray(x, np.float32)
+        if lib().orc_cfdmt_process(self.h, _ptr(xa), _ptr(out), int(b_ones)) != 0:
+            raise RuntimeError(err())
+        return out
+
+
+def stats(x, mode=0):
+    """Row/global statistics and keep flags (Statistical_preprocessing.cu:15-88)."""
+    R, N = x.shape
+    rm = np.zeros(R, np.float64)
+    rv = np.zeros(R, np.float64)
+    keep = np.zeros(R, np.uint8)
+    st = np.zeros(5, np.float64)
+    if x.dtype == np.uint8:
+        xa = np.ascontiguousarray(x)
+        lib().orc_stats(None, _ptr(xa), R, N, mode, _ptr(rm), _ptr(rv), _ptr(keep), _ptr(st))
+    else:
+        xa = np.ascontiguousarray(x, np.float32)
+        lib().orc_stats(_ptr(xa), None, R, N, mode, _ptr(rm), _ptr(rv), _ptr(keep), _ptr(st))
+    return rm, rv, keep, st
+
+
+def normalise(x, mode=0):
+    """calc_fdmt_inputs: returns (normalised fp32 image, mask, stats tuple)."""
+    R, N = x.shape
+    rm, rv, keep, st = stats(x, mode)
+    out = np.zeros((R, N), np.float32)
+    mask = np.zeros((R, N), np.float32)
+    xa = np.ascontiguousarray(x)
+    if x.dtype == np.uint8:
+        lib().orc_normalise(None, _ptr(xa), R, N, mode, _ptr(rm), _ptr(rv), _ptr(keep), _ptr(st), _ptr(out), _ptr(mask))
+    else:
+        xa = np.ascontiguousarray(x, np.float32)
+        lib().orc_normalise(_ptr(xa), None, R, N, mode, _ptr(rm), _ptr(rv), _ptr(keep), _ptr(st), _ptr(out), _ptr(mask))
+    return out, mask, (rm, rv, keep, st)
+
+
+def fdmt_normalization(a, n):
+    a = np.ascontiguousarray(a, np.float32)
+    n = np.ascontiguousarray(n, np.float32)
+    out = np.zeros_like(a)
+    lib().orc_fdmt_normalization(_ptr(a), _ptr(n), a.size, _ptr(out))
+    return out
+
+
+def boxcar_pow2(img, thr, q, cap=1 << 20):
+    img = np.ascontiguousarray(img, np.float32)
+    out = np.zeros(cap, CAND_DTYPE)
+    n = lib().orc_boxcar_pow2(_ptr(img), img.shape[0], img.shape[1], thr, q, _ptr(out), cap)
+    if n > cap:
+        return boxcar_pow2(img, thr, q, cap=int(n))
+    return out[:n].copy()
+
+
+def boxcar_all(img, thr, w, cap=1 << 20):
+    img = np.ascontiguousarray(img, np.float32)
+    out = np.zeros(cap, CAND_DTYPE)
+    n = lib().orc_boxcar_all(_ptr(img), img.shape[0], img.shape[1], thr, w, _ptr(out), cap)
+    if n > cap:
+        return boxcar_all(img, thr, w, cap=int(n))
+    return out[:n].copy()
+
+
+def cluster(cands, bins):
+    """Returns (order, group_off, delegates (sorted by mt), group_size)."""
+    c = np.ascontiguousarray(cands, CAND_DTYPE)
+    n = c.size
+    b = np.asarray(bins, np.int32)
+    order = np.zeros(max(n, 1), np.int32)
+    goff = np.zeros(n + 1, np.int64)
+    dele = np.zeros(max(n, 1), np.int32)
+    gsz = np.zeros(max(n, 1), np.int64)
+    ng = lib().orc_cluster(_ptr(c), n, _ptr(b), _ptr(order), _ptr(goff), _ptr(dele), _ptr(gsz))
+    return order[:n], goff[: ng + 1], dele[:ng], gsz[:ng]
+
+
+def detect_max(a, norm, w):
+    a = np.ascontiguousarray(a, np.float32)
+    na = None if norm is None else np.ascontiguousarray(norm, np.float32)
+    o = np.zeros(3, np.int32)
+    s = _f32()
+    rc = lib().orc_detect_max(_ptr(a), _ptr(na), a.shape[0], a.shape[1], w, _ptr(o), C.byref(s))
+    if rc != 0:
+        return None
+    return int(o[0]), int(o[1]), int(o[2]), s.value
+
+
+# --------------------------------------------------------------------------
+# reference build (oracle/_ref), present only where /root/reference was
+# available at build time
+# --------------------------------------------------------------------------
+REF_DIR = os.path.join(HERE, "_ref")
+
+
+def ref_available():
+    return all(os.path.exists(os.path.join(REF_DIR, f"libref_{k}.so")) for k in ("pravir", "restrict", "cfdmt"))
+
+
+_ref = {}
+
+
+def ref_lib(kind):
+    if kind not in _ref:
+        L = C.CDLL(os.path.join(REF_DIR, f"libref_{kind}.so"))
+        if kind == "cfdmt":
+            L.ref_cfdmt_create.restype = _vp
+            L.ref_cfdmt_create.argtypes = [_f32, _f32, C.c_int, C.c_int, C.c_int]
+            L.ref_cfdmt_destroy.argtypes = [_vp]
+            L.ref_cfdmt_niter.argtypes = [_vp]
+            L.ref_cfdmt_quant.argtypes = [_vp, C.c_int]
+            L.ref_cfdmt_config.argtypes = [_vp, C.c_int, _vp, _vp]
+            L.ref_cfdmt_process.argtypes = [_vp, _vp, _vp, C.c_int]
+        else:
+            L.ref_fdmt_create.restype = _vp
+            L.ref_fdmt_create.argtypes = [_f32, _f32, _i64, _i64, _f32, _i64, _i64, _i64]
+            L.ref_fdmt_destroy.argtypes = [_vp]
+            L.ref_fdmt_niters.restype = _i64
+            L.ref_fdmt_niters.argtypes = [_vp]
+            for n in ("ref_fdmt_nsub", "ref_fdmt_nmap", "ref_fdmt_ncopy"):
+                getattr(L, n).restype = _i64
+                getattr(L, n).argtypes = [_vp, _i64]
+            L.ref_fdmt_state_shape.argtypes = [_vp, _i64, _vp]
+            L.ref_fdmt_grid.argtypes = [_vp, _i64, _vp, _vp]
+            L.ref_fdmt_maps.argtypes = [_vp, _i64, _vp, _vp]
+            L.ref_fdmt_df.argtypes = [_vp, _i64, C.POINTER(_f32), C.POINTER(_f32)]
+            L.ref_fdmt_execute.argtypes = [_vp, _vp, _i64, _vp, _i64]
+            L.ref_fdmt_dm_grid.argtypes = [_vp, _vp]
+        _ref[kind] = L
+    return _ref[kind]
+
+
+class RefFDMT:
+    """The reference FDMTCPU (FDMT_PRAVIR or FDMT_PRAVIR_RESTRICT) via _ref."""
+
+    def __init__(self, kind, fmin, fmax, nchans, nsamps, tsamp=1.0, dt_max=0, dt_step=1, dt_min=0):
+        self.L = ref_lib(kind)
+        self.h = self.L.ref_fdmt_create(fmin, fmax, nchans, nsamps, tsamp, dt_max, dt_step, dt_min)
+        if not self.h:
+            raise RuntimeError("reference plan builder threw")
+        self.nchans, self.nsamps = nchans, nsamps
+        self.niters = self.L.ref_fdmt_niters(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.ref_fdmt_destroy(self.h)
+            self.h = None
+
+    def state_shape(self, i):
+        out = np.zeros(5, np.int64)
+        self.L.ref_fdmt_state_shape(self.h, i, _ptr(out))
+        return out
+
+    def grid(self, i):
+        ns = self.L.ref_fdmt_nsub(self.h, i)
+        off = np.zeros(ns + 1, np.int64)
+        total = int(self.state_shape(i)[3])
+        g = np.zeros(max(total, 1), np.int64)
+        self.L.ref_fdmt_grid(self.h, i, _ptr(off), _ptr(g))
+        return off, g[: off[-1]]
+
+    def maps(self, i):
+        nm = self.L.ref_fdmt_nmap(self.h, i)
+        nc = self.L.ref_fdmt_ncopy(self.h, i)
+        m = np.zeros((max(nm, 1), 7), np.int64)
+        c = np.zeros((max(nc, 1), 4), np.int64)
+        self.L.ref_fdmt_maps(self.h, i, _ptr(m), _ptr(c))
+        return m[:nm], c[:nc]
+
+    def df(self, i):
+        a, b = _f32(), _f32()
+        self.L.ref_fdmt_df(self.h, i, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def execute(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        rows = int(self.state_shape(self.niters)[3])
+        out = np.zeros((rows, self.nsamps), np.float32)
+        if self.L.ref_fdmt_execute(self.h, _ptr(x), x.size, _ptr(out), out.size) != 0:
+            raise RuntimeError("reference execute threw")
+        return out
+
+    def dm_grid(self):
+        rows = int(self.state_shape(self.niters)[3])
+        out = np.zeros(rows, np.float32)
+        self.L.ref_fdmt_dm_grid(self.h, _ptr(out))
+        return out
+
+
+class RefCFdmt:
+    """The reference CFdmtCpu (FDMT/FdmtCpu.cpp) via _ref."""
+
+    def __init__(self, fmin, fmax, nchan, cols, imaxdt):
+        self.L = ref_lib("cfdmt")
+        self.h = self.L.ref_cfdmt_create(fmin, fmax, nchan, cols, imaxdt)
+        self.nchan, self.cols, self.imaxdt = nchan, cols, imaxdt
+        self.niter = self.L.ref_cfdmt_niter(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.ref_cfdmt_destroy(self.h)
+            self.h = None
+
+    def config(self, i):
+        q = self.L.ref_cfdmt_quant(self.h, i)
+        cs = np.zeros(q + 1, np.int32)
+        fr = np.zeros(q + 1, np.float32)
+        self.L.ref_cfdmt_config(self.h, i, _ptr(cs), _ptr(fr))
+        return cs, fr
+
+    def process(self, x, b_ones=False):
+        out = np.zeros((self.imaxdt, self.cols), np.float32)
+        xa = None if x is None else np.ascontiguousarray(x, np.float32)
+        self.L.ref_cfdmt_process(self.h, _ptr(xa), _ptr(out), int(b_ones))
+        return out

@@ -1,0 +1,973 @@
+/*
+ * sps_oracle.c -- CPU restatement of the reference single-pulse-search path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker, never the product:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load liboracle.so.  The product path (libsps.so, HIP kernels) never links,
+ * loads or calls anything from oracle/.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the reference checkout).  Floating point follows the reference CPU code op
+ * for op (fp32, IEEE division, no contraction: build with -ffp-contract=off).
+ *
+ * Parity pinning (what checks each part; see tests/test_oracle_*.py):
+ *   plan (orc_plan_*)            : tables dumped from the reference FDMT plan
+ *                                  builder compiled from source (oracle/_ref)
+ *                                  + committed golden tables + invariants.
+ *   FDMT init/merge (correct)    : bit-exact vs reference FDMT_PRAVIR FDMTCPU
+ *                                  (oracle/_ref + golden), FP64 brute force
+ *                                  shift-and-add along plan tracks, ones-input
+ *                                  closed form, linearity.
+ *   FDMT init (legacy restrict)  : bit-exact vs FDMT_PRAVIR_RESTRICT FDMTCPU.
+ *   legacy CFdmt process_image   : bit-exact vs FDMT/FdmtCpu.cpp CFdmtCpu
+ *                                  (oracle/_ref + golden), incl. b_ones.
+ *   statistics / normalisation   : closed forms (known mean/variance, an
+ *                                  outlier-variance row is zeroed), exact
+ *                                  integer sums for integer input.
+ *   fdmt_normalization           : closed form x / sqrt(n + 1e-8).
+ *   boxcar gathers               : reference golden Candidates1.csv and
+ *                                  Candidates2.csv + brute force on tiny inputs.
+ *   clustering / delegates       : reference KATs (regrouping plan {4,4},
+ *                                  delegates {6,7,10}, final sort {5,6,8}) +
+ *                                  brute-force single-linkage on tiny sets.
+ *   single-max detector          : brute force over all windows.
+ * Parity unpinned (no reference test or fixture fixes them): the reference
+ * GPU's fast-math bit patterns, CUB/thrust tie order for equal keys and
+ * atomicInc emission order; this restatement defines deterministic rules.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_EXPORT __attribute__((visibility("default")))
+
+static char g_err[256];
+static void set_err(const char* m) { snprintf(g_err, sizeof g_err, "%s", m); }
+ORC_EXPORT const char* orc_last_error(void) { return g_err; }
+
+ORC_EXPORT int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+ORC_EXPORT void orc_set_num_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+/* ------------------------------------------------------------------------ */
+/* Plan (FDMT_PRAVIR_RESTRICT/fdmt_base.cpp:87-279, fdmtUtils.cpp:7-16,63-78) */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int64_t nsub;
+    int64_t* grid_off;  /* nsub + 1 */
+    int64_t* grid;      /* concatenated dt grids */
+    int64_t* row_off;   /* nsub + 1: first state row of each sub-band */
+    int64_t nmap;       /* mappings: [out_sub, out_dt, head_sub, head_dt, tail_sub, tail_dt, offset] */
+    int64_t* map;
+    int64_t ncopy;      /* copies: [out_sub, out_dt, tail_sub, tail_dt] */
+    int64_t* copy;
+    float df_top, df_bot;
+} orc_iter;
+
+typedef struct {
+    float fmin, fmax, tsamp;
+    int64_t nchans, nsamps, dt_max, dt_step, dt_min;
+    float df, corr;
+    int64_t niters;
+    orc_iter* it;
+} orc_plan;
+
+/* fdmtUtils.cpp:7-11 -- std::pow(float, float) with kDispCoeff = -2.0f */
+static float orc_cff(float f1s, float f1e, float f2s, float f2e) {
+    return (powf(f1s, -2.0f) - powf(f1e, -2.0f)) / (powf(f2s, -2.0f) - powf(f2e, -2.0f));
+}
+
+/* fdmtUtils.cpp:13-16 */
+static int64_t orc_dt_sub(float fs, float fe, float fmin, float fmax, int64_t dt) {
+    float ratio = orc_cff(fs, fe, fmin, fmax);
+    return (int64_t)roundf((float)dt * ratio);
+}
+
+/* fdmtUtils.cpp:63-78: lower_bound, step left only when strictly closer */
+static int64_t orc_closest(const int64_t* a, int64_t n, int64_t val) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        int64_t mid = lo + (hi - lo) / 2;
+        if (a[mid] < val) lo = mid + 1; else hi = mid;
+    }
+    if (lo != n) {
+        if (lo != 0 && val - a[lo - 1] < a[lo] - val) lo--;
+    } else {
+        lo = n - 1;
+    }
+    return lo;
+}
+
+/* fdmt_base.cpp:95-105 -> writes the grid into out (cap entries), returns length */
+static int64_t orc_grid_sub(const orc_plan* p, float fs, float fe, int64_t* out, int64_t cap) {
+    int64_t hi = orc_dt_sub(fs, fe, p->fmin, p->fmax, p->dt_max);
+    int64_t lo = orc_dt_sub(fs, fe, p->fmin, p->fmax, p->dt_min);
+    int64_t n = 0;
+    for (int64_t dt = lo; dt <= hi; dt += p->dt_step) {
+        if (out && n < cap) out[n] = dt;
+        n++;
+    }
+    return n;
+}
+
+static void orc_iter_free(orc_iter* it) {
+    free(it->grid_off); free(it->grid); free(it->row_off); free(it->map); free(it->copy);
+}
+
+ORC_EXPORT void orc_plan_destroy(void* h) {
+    orc_plan* p = (orc_plan*)h;
+    if (!p) return;
+    if (p->it) for (int64_t i = 0; i <= p->niters; ++i) orc_iter_free(&p->it[i]);
+    free(p->it);
+    free(p);
+}
+
+ORC_EXPORT void* orc_plan_create(float fmin, float fmax, int64_t nchans, int64_t nsamps, float tsamp,
+                                 int64_t dt_max, int64_t dt_step, int64_t dt_min) {
+    if (nchans < 1 || nsamps < 1 || dt_step < 1 || dt_min > dt_max) {
+        set_err("invalid plan parameters");
+        return NULL;
+    }
+    orc_plan* p = (orc_plan*)calloc(1, sizeof(orc_plan));
+    p->fmin = fmin; p->fmax = fmax; p->tsamp = tsamp;
+    p->nchans = nchans; p->nsamps = nsamps; p->dt_max = dt_max; p->dt_step = dt_step; p->dt_min = dt_min;
+    /* fdmt_base.cpp:47-49,87-93 */
+    p->df = (fmax - fmin) / (float)nchans;
+    p->corr = p->df / 2;
+    p->niters = (int64_t)ceil(log2((double)nchans));
+    p->it = (orc_iter*)calloc((size_t)p->niters + 1, sizeof(orc_iter));
+
+    /* make_fdmt_plan_iter0, fdmt_base.cpp:140-173 */
+    {
+        orc_iter* it = &p->it[0];
+        it->nsub = nchans;
+        it->grid_off = (int64_t*)calloc((size_t)nchans + 1, sizeof(int64_t));
+        it->row_off = (int64_t*)calloc((size_t)nchans + 1, sizeof(int64_t));
+        int64_t total = 0;
+        for (int64_t s = 0; s < nchans; ++s) {
+            float fs = p->df * (float)s + fmin;
+            float fe = fs + p->df;
+            int64_t n = orc_grid_sub(p, fs, fe, NULL, 0);
+            it->grid_off[s + 1] = it->grid_off[s] + n;
+            total += n;
+        }
+        it->grid = (int64_t*)calloc((size_t)total + 1, sizeof(int64_t));
+        for (int64_t s = 0; s < nchans; ++s) {
+            float fs = p->df * (float)s + fmin;
+            float fe = fs + p->df;
+            orc_grid_sub(p, fs, fe, it->grid + it->grid_off[s], it->grid_off[s + 1] - it->grid_off[s]);
+            it->row_off[s + 1] = it->grid_off[s + 1];
+        }
+        it->df_top = p->df;
+        it->df_bot = p->df;
+    }
+    /* dt_grid_sub_top[0] = dt_grid[0][nchans-1] */
+    int64_t top_n = p->it[0].grid_off[nchans] - p->it[0].grid_off[nchans - 1];
+    int64_t* top = (int64_t*)malloc(sizeof(int64_t) * (size_t)(top_n + 1));
+    memcpy(top, p->it[0].grid + p->it[0].grid_off[nchans - 1], sizeof(int64_t) * (size_t)top_n);
+
+    /* make_fdmt_plan(i_iter), fdmt_base.cpp:175-279 */
+    for (int64_t ii = 1; ii <= p->niters; ++ii) {
+        orc_iter* prev = &p->it[ii - 1];
+        orc_iter* it = &p->it[ii];
+        const int64_t nprev = prev->nsub;
+        const int64_t ncur = nprev / 2 + nprev % 2;
+        const int do_copy = (nprev % 2) == 1;
+        const float df_top = do_copy ? prev->df_top : prev->df_top + prev->df_bot;
+        const float df_bot = prev->df_bot * 2;
+        it->nsub = ncur;
+        it->grid_off = (int64_t*)calloc((size_t)ncur + 1, sizeof(int64_t));
+        it->row_off = (int64_t*)calloc((size_t)ncur + 1, sizeof(int64_t));
+        /* first pass: grids */
+        int64_t cap = 16;
+        it->grid = (int64_t*)malloc(sizeof(int64_t) * (size_t)cap);
+        int64_t ngrid = 0;
+        int64_t* new_top = NULL;
+        int64_t new_top_n = 0;
+        float* fstart = (float*)malloc(sizeof(float) * (size_t)ncur);
+        float* fend = (float*)malloc(sizeof(float) * (size_t)ncur);
+        float* fmid = (float*)malloc(sizeof(float) * (size_t)ncur);
+        for (int64_t s = 0; s < ncur; ++s) {
+            float fs = df_bot * (float)s + fmin, fe, fm;
+            int64_t n;
+            int64_t* src = NULL;
+            if (s == ncur - 1) {
+                if (do_copy) {
+                    fe = fs + df_top * 2;
+                    fm = fs + df_top;
+                    n = top_n;
+                    src = top;
+                } else {
+                    fe = fs + df_top;
+                    fm = fs + df_bot / 2;
+                    n = orc_grid_sub(p, fs, fe, NULL, 0);
+                }
+            } else {
+                fe = fs + df_bot;
+                fm = fs + df_bot / 2;
+                n = orc_grid_sub(p, fs, fe, NULL, 0);
+            }
+            while (ngrid + n + 1 > cap) { cap *= 2; it->grid = (int64_t*)realloc(it->grid, sizeof(int64_t) * (size_t)cap); }
+            if (src) memcpy(it->grid + ngrid, src, sizeof(int64_t) * (size_t)n);
+            else orc_grid_sub(p, fs, fe, it->grid + ngrid, n);
+            if (s == ncur - 1 && !do_copy) {
+                new_top = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 1));
+                memcpy(new_top, it->grid + ngrid, sizeof(int64_t) * (size_t)n);
+                new_top_n = n;
+            }
+            ngrid += n;
+            it->grid_off[s + 1] = ngrid;
+            it->row_off[s + 1] = ngrid;
+            fstart[s] = fs; fend[s] = fe; fmid[s] = fm;
+        }
+        /* second pass: mappings */
+        it->map = (int64_t*)malloc(sizeof(int64_t) * 7 * (size_t)(ngrid + 1));
+        it->copy = (int64_t*)malloc(sizeof(int64_t) * 4 * (size_t)(ngrid + 1));
+        int64_t nm = 0, nc = 0;
+        for (int64_t s = 0; s < ncur; ++s) {
+            const int64_t tail_s = 2 * s, head_s = 2 * s + 1;
+            const float fs = fstart[s], fe = fend[s];
+            const float fm1 = fmid[s] - p->corr, fm2 = fmid[s] + p->corr;
+            for (int64_t k = 0; k < it->grid_off[s + 1] - it->grid_off[s]; ++k) {
+                const int64_t dt = it->grid[it->grid_off[s] + k];
+                const int64_t dt_mid1 = (int64_t)roundf((float)dt * orc_cff(fs, fm1, fs, fe));
+                const int64_t dt_mid2 = (int64_t)roundf((float)dt * orc_cff(fs, fm2, fs, fe));
+                if (dt_mid1 > dt || dt_mid2 > dt) {
+                    set_err("Invalid dt_mid values");
+                    goto fail;
+                }
+                const int64_t dt_head = dt - dt_mid2;
+                if (s == ncur - 1 && do_copy) {
+                    const int64_t* g = prev->grid + prev->grid_off[tail_s];
+                    const int64_t gn = prev->grid_off[tail_s + 1] - prev->grid_off[tail_s];
+                    int64_t* c = it->copy + 4 * nc++;
+                    c[0] = s; c[1] = k; c[2] = tail_s; c[3] = orc_closest(g, gn, dt);
+                } else {
+                    const int64_t* gh = prev->grid + prev->grid_off[head_s];
+                    const int64_t ghn = prev->grid_off[head_s + 1] - prev->grid_off[head_s];
+                    const int64_t* gt = prev->grid + prev->grid_off[tail_s];
+                    const int64_t gtn = prev->grid_off[tail_s + 1] - prev->grid_off[tail_s];
+                    if (dt_mid2 >= nsamps) {
+                        set_err("Offset is greater than input size");
+                        goto fail;
+                    }
+                    int64_t* m = it->map + 7 * nm++;
+                    m[0] = s; m[1] = k;
+                    m[2] = head_s; m[3] = orc_closest(gh, ghn, dt_head);
+                    m[4] = tail_s; m[5] = orc_closest(gt, gtn, dt_mid1);
+                    m[6] = dt_mid2;
+                }
+            }
+        }
+        it->nmap = nm;
+        it->ncopy = nc;
+        it->df_top = df_top;
+        it->df_bot = df_bot;
+        free(fstart); free(fend); free(fmid);
+        if (new_top) { free(top); top = new_top; top_n = new_top_n; }
+        continue;
+    fail:
+        free(fstart); free(fend); free(fmid); free(new_top); free(top);
+        orc_plan_destroy(p);
+        return NULL;
+    }
+    free(top);
+    return p;
+}
+
+ORC_EXPORT int64_t orc_plan_niters(void* h) { return ((orc_plan*)h)->niters; }
+ORC_EXPORT int64_t orc_plan_nsub(void* h, int64_t i) { return ((orc_plan*)h)->it[i].nsub; }
+ORC_EXPORT int64_t orc_plan_nrows(void* h, int64_t i) {
+    orc_iter* it = &((orc_plan*)h)->it[i];
+    return it->row_off[it->nsub];
+}
+/* state_shape (fdmt_base.cpp:164-170,271-277): nchans, ndt_min, ndt_max, sum ndt, nsamps */
+ORC_EXPORT void orc_plan_state_shape(void* h, int64_t i, int64_t* out5) {
+    orc_plan* p = (orc_plan*)h;
+    orc_iter* it = &p->it[i];
+    int64_t mn = INT64_MAX, mx = 0;
+    for (int64_t s = 0; s < it->nsub; ++s) {
+        int64_t n = it->grid_off[s + 1] - it->grid_off[s];
+        if (n < mn) mn = n;
+        if (n > mx) mx = n;
+    }
+    out5[0] = it->nsub; out5[1] = mn; out5[2] = mx; out5[3] = it->row_off[it->nsub]; out5[4] = p->nsamps;
+}
+ORC_EXPORT void orc_plan_grid(void* h, int64_t i, int64_t* grid_off, int64_t* grid) {
+    orc_iter* it = &((orc_plan*)h)->it[i];
+    if (grid_off) memcpy(grid_off, it->grid_off, sizeof(int64_t) * (size_t)(it->nsub + 1));
+    if (grid) memcpy(grid, it->grid, sizeof(int64_t) * (size_t)it->grid_off[it->nsub]);
+}
+ORC_EXPORT int64_t orc_plan_nmap(void* h, int64_t i) { return ((orc_plan*)h)->it[i].nmap; }
+ORC_EXPORT int64_t orc_plan_ncopy(void* h, int64_t i) { return ((orc_plan*)h)->it[i].ncopy; }
+ORC_EXPORT void orc_plan_maps(void* h, int64_t i, int64_t* map7, int64_t* copy4) {
+    orc_iter* it = &((orc_plan*)h)->it[i];
+    if (map7 && it->nmap) memcpy(map7, it->map, sizeof(int64_t) * 7 * (size_t)it->nmap);
+    if (copy4 && it->ncopy) memcpy(copy4, it->copy, sizeof(int64_t) * 4 * (size_t)it->ncopy);
+}
+ORC_EXPORT void orc_plan_df(void* h, int64_t i, float* df_top, float* df_bot) {
+    orc_iter* it = &((orc_plan*)h)->it[i];
+    *df_top = it->df_top;
+    *df_bot = it->df_bot;
+}
+
+/* ------------------------------------------------------------------------ */
+/* FDMT execute (FDMT_PRAVIR/fdmt_cpu.cpp:28-54,103-179,243-301;             */
+/*               FDMT_PRAVIR_RESTRICT/fdmt_cpu.cpp:65-121 for init_mode 1)   */
+/* ------------------------------------------------------------------------ */
+
+/* init_mode 0: FDMT_PRAVIR running mean (fdmt_cpu.cpp:115-148)
+ * init_mode 1: FDMT_PRAVIR_RESTRICT form prev*((dprev+1)+sum)/(dcur+1)
+ *              (FDMT_PRAVIR_RESTRICT/fdmt_cpu.cpp:91-114); needs dt_step 1
+ * state must hold nrows(0)*nsamps floats and is fully written (t<dt -> 0). */
+static int orc_init(const orc_plan* p, const float* x, float* state, int init_mode) {
+    const orc_iter* it = &p->it[0];
+    const int64_t N = p->nsamps;
+    if (init_mode == 1 && p->dt_step != 1) {
+        set_err("legacy restrict init requires dt_step == 1");
+        return -1;
+    }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t c = 0; c < p->nchans; ++c) {
+        const int64_t* g = it->grid + it->grid_off[c];
+        const int64_t K = it->grid_off[c + 1] - it->grid_off[c];
+        const float* xr = x + c * N;
+        float* base = state + it->row_off[c] * N;
+        memset(base, 0, sizeof(float) * (size_t)(K * N));
+        const int64_t d0 = g[0];
+        for (int64_t t = d0; t < N; ++t) {
+            float sum = 0.0f;
+            for (int64_t i = t - d0; i <= t; ++i) sum += xr[i];
+            base[t] = sum / (float)(d0 + 1);
+        }
+        for (int64_t k = 1; k < K; ++k) {
+            const int64_t dc = g[k], dp = g[k - 1];
+            float* row = base + k * N;
+            const float* prow = base + (k - 1) * N;
+            for (int64_t t = dc; t < N; ++t) {
+                float sum = 0.0f;
+                for (int64_t i = t - dc; i < t - dp; ++i) sum += xr[i];
+                if (init_mode == 1)
+                    row[t] = (prow[t] * (((float)dp + 1.0f) + sum)) / ((float)dc + 1.0f);
+                else
+                    row[t] = (prow[t] * ((float)dp + 1.0f) + sum) / ((float)dc + 1.0f);
+            }
+        }
+    }
+    return 0;
+}
+
+/* execute_iter, FDMT_PRAVIR/fdmt_cpu.cpp:243-301; add_offset_kernel fdmtUtils.cpp:18-52 */
+static void orc_merge(const orc_plan* p, int64_t ii, const float* in, float* out) {
+    const orc_iter* it = &p->it[ii];
+    const orc_iter* pv = &p->it[ii - 1];
+    const int64_t N = p->nsamps;
+#pragma omp parallel for schedule(static)
+    for (int64_t m = 0; m < it->nmap; ++m) {
+        const int64_t* mp = it->map + 7 * m;
+        const float* head = in + (pv->row_off[mp[2]] + mp[3]) * N;
+        const float* tail = in + (pv->row_off[mp[4]] + mp[5]) * N;
+        float* o = out + (it->row_off[mp[0]] + mp[1]) * N;
+        const int64_t off = mp[6];
+        memcpy(o, tail, sizeof(float) * (size_t)off);
+        for (int64_t t = off; t < N; ++t) o[t] = tail[t] + head[t - off];
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < it->ncopy; ++c) {
+        const int64_t* cp = it->copy + 4 * c;
+        const float* tail = in + (pv->row_off[cp[2]] + cp[3]) * N;
+        float* o = out + (it->row_off[cp[0]] + cp[1]) * N;
+        memcpy(o, tail, sizeof(float) * (size_t)N);
+    }
+}
+
+static int64_t orc_max_rows(const orc_plan* p) {
+    int64_t mx = 0;
+    for (int64_t i = 0; i <= p->niters; ++i) {
+        int64_t r = p->it[i].row_off[p->it[i].nsub];
+        if (r > mx) mx = r;
+    }
+    return mx;
+}
+
+/* FDMTCPU::execute (FDMT_PRAVIR/fdmt_cpu.cpp:28-54). dmt must hold
+ * nrows(niters)*nsamps floats.  niters == 0 (one channel) returns the init
+ * state (the reference copies a never-written buffer there). */
+ORC_EXPORT int orc_fdmt_execute(void* h, const float* x, float* dmt, int init_mode) {
+    orc_plan* p = (orc_plan*)h;
+    const int64_t N = p->nsamps;
+    const int64_t mr = orc_max_rows(p);
+    float* a = (float*)malloc(sizeof(float) * (size_t)(mr * N));
+    float* b = (float*)malloc(sizeof(float) * (size_t)(mr * N));
+    if (!a || !b) { free(a); free(b); set_err("out of memory"); return -1; }
+    if (orc_init(p, x, a, init_mode) != 0) { free(a); free(b); return -1; }
+    float* in = a; float* out = b;
+    if (p->niters == 0) out = a;
+    for (int64_t ii = 1; ii <= p->niters; ++ii) {
+        orc_merge(p, ii, in, out);
+        if (ii < p->niters) { float* t = in; in = out; out = t; }
+    }
+    memcpy(dmt, out, sizeof(float) * (size_t)(orc_plan_nrows(p, p->niters) * N));
+    free(a); free(b);
+    return 0;
+}
+
+/* Exposes the init state alone (FDMTCPU::initialise). */
+ORC_EXPORT int orc_fdmt_initialise(void* h, const float* x, float* state, int init_mode) {
+    return orc_init((orc_plan*)h, x, state, init_mode);
+}
+
+/* Brute force: the plan composes, for every output row, one track per
+ * channel (channel c, init row dt d_c, accumulated shift s_c); the value is
+ * sum_c [t - s_c >= d_c] mean(x[c][t-s_c-d_c .. t-s_c]) evaluated in FP64.
+ * Tracks are followed through the mappings/copies of every iteration
+ * (fdmt_cpu.cpp:259-295 read backwards). out_f64: nrows(niters)*nsamps. */
+ORC_EXPORT int orc_fdmt_bruteforce_f64(void* h, const float* x, double* out) {
+    orc_plan* p = (orc_plan*)h;
+    const int64_t N = p->nsamps;
+    const int64_t R = orc_plan_nrows(p, p->niters);
+    /* per iteration: row -> (tail_row, head_row (-1), offset) */
+    int64_t** tail = (int64_t**)calloc((size_t)p->niters + 1, sizeof(int64_t*));
+    int64_t** head = (int64_t**)calloc((size_t)p->niters + 1, sizeof(int64_t*));
+    int64_t** off = (int64_t**)calloc((size_t)p->niters + 1, sizeof(int64_t*));
+    for (int64_t ii = 1; ii <= p->niters; ++ii) {
+        orc_iter* it = &p->it[ii];
+        orc_iter* pv = &p->it[ii - 1];
+        int64_t nr = it->row_off[it->nsub];
+        tail[ii] = (int64_t*)malloc(sizeof(int64_t) * (size_t)nr);
+        head[ii] = (int64_t*)malloc(sizeof(int64_t) * (size_t)nr);
+        off[ii] = (int64_t*)malloc(sizeof(int64_t) * (size_t)nr);
+        for (int64_t m = 0; m < it->nmap; ++m) {
+            int64_t* mp = it->map + 7 * m;
+            int64_t r = it->row_off[mp[0]] + mp[1];
+            head[ii][r] = pv->row_off[mp[2]] + mp[3];
+            tail[ii][r] = pv->row_off[mp[4]] + mp[5];
+            off[ii][r] = mp[6];
+        }
+        for (int64_t c = 0; c < it->ncopy; ++c) {
+            int64_t* cp = it->copy + 4 * c;
+            int64_t r = it->row_off[cp[0]] + cp[1];
+            tail[ii][r] = pv->row_off[cp[2]] + cp[3];
+            head[ii][r] = -1;
+            off[ii][r] = 0;
+        }
+    }
+    /* init row -> (channel, dt) */
+    orc_iter* i0 = &p->it[0];
+    int64_t R0 = i0->row_off[i0->nsub];
+    int64_t* row_ch = (int64_t*)malloc(sizeof(int64_t) * (size_t)R0);
+    int64_t* row_dt = (int64_t*)malloc(sizeof(int64_t) * (size_t)R0);
+    for (int64_t c = 0; c < p->nchans; ++c)
+        for (int64_t k = i0->grid_off[c]; k < i0->grid_off[c + 1]; ++k) {
+            row_ch[k] = c;
+            row_dt[k] = i0->grid[k];
+        }
+    int ok = 1;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t r = 0; r < R; ++r) {
+        /* expand tracks with an explicit stack: (iteration, row, shift) */
+        int64_t cap = 64, top = 0;
+        int64_t* st = (int64_t*)malloc(sizeof(int64_t) * 3 * (size_t)cap);
+        double* acc = out + r * N;
+        for (int64_t t = 0; t < N; ++t) acc[t] = 0.0;
+        int64_t nleaf = 0;
+        st[0] = p->niters; st[1] = r; st[2] = 0; top = 1;
+        while (top > 0) {
+            --top;
+            int64_t ii = st[3 * top], row = st[3 * top + 1], sh = st[3 * top + 2];
+            if (ii == 0) {
+                int64_t c = row_ch[row], d = row_dt[row];
+                const float* xr = x + c * N;
+                for (int64_t t = sh + d; t < N; ++t) {
+                    double s = 0.0;
+                    for (int64_t j = t - sh - d; j <= t - sh; ++j) s += (double)xr[j];
+                    acc[t] += s / (double)(d + 1);
+                }
+                nleaf++;
+                continue;
+            }
+            if (top + 2 > cap) { cap *= 2; st = (int64_t*)realloc(st, sizeof(int64_t) * 3 * (size_t)cap); }
+            st[3 * top] = ii - 1; st[3 * top + 1] = tail[ii][row]; st[3 * top + 2] = sh; top++;
+            if (head[ii][row] >= 0) {
+                st[3 * top] = ii - 1; st[3 * top + 1] = head[ii][row]; st[3 * top + 2] = sh + off[ii][row]; top++;
+            }
+        }
+        if (nleaf != p->nchans) ok = 0;
+        free(st);
+    }
+    for (int64_t ii = 1; ii <= p->niters; ++ii) { free(tail[ii]); free(head[ii]); free(off[ii]); }
+    free(tail); free(head); free(off); free(row_ch); free(row_dt);
+    if (!ok) { set_err("track does not cover every channel once"); return -1; }
+    return 0;
+}
+
+/* Per output row: the largest accumulated delay (shift + init dt) over the
+ * row's channel tracks; for t >= that value every channel contributes. */
+ORC_EXPORT void orc_fdmt_track_extent(void* h, int64_t* out_rows) {
+    orc_plan* p = (orc_plan*)h;
+    orc_iter* i0 = &p->it[0];
+    int64_t R0 = i0->row_off[i0->nsub];
+    int64_t* cur = (int64_t*)malloc(sizeof(int64_t) * (size_t)R0);
+    for (int64_t k = 0; k < R0; ++k) cur[k] = i0->grid[k];
+    for (int64_t ii = 1; ii <= p->niters; ++ii) {
+        orc_iter* it = &p->it[ii];
+        orc_iter* pv = &p->it[ii - 1];
+        int64_t nr = it->row_off[it->nsub];
+        int64_t* nx = (int64_t*)malloc(sizeof(int64_t) * (size_t)nr);
+        for (int64_t m = 0; m < it->nmap; ++m) {
+            int64_t* mp = it->map + 7 * m;
+            int64_t a = cur[pv->row_off[mp[4]] + mp[5]];
+            int64_t b = cur[pv->row_off[mp[2]] + mp[3]] + mp[6];
+            nx[it->row_off[mp[0]] + mp[1]] = a > b ? a : b;
+        }
+        for (int64_t c = 0; c < it->ncopy; ++c) {
+            int64_t* cp = it->copy + 4 * c;
+            nx[it->row_off[cp[0]] + cp[1]] = cur[pv->row_off[cp[2]] + cp[3]];
+        }
+        free(cur);
+        cur = nx;
+    }
+    memcpy(out_rows, cur, sizeof(int64_t) * (size_t)orc_plan_nrows(p, p->niters));
+    free(cur);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Legacy CFdmt (FDMT/FdmtB.cpp:158-299, FDMT/FdmtCpu.cpp:143-372)           */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    float fmin, fmax;
+    int nchan, cols, imaxdt;
+    int niter;
+    int* quant;      /* niter+1 sub-matrix counts */
+    float** freq;    /* niter+1 arrays of quant+1 */
+    int** cumsum;    /* niter+1 arrays of quant+1 */
+} orc_cfdmt;
+
+/* FdmtB.cpp:273-276 */
+static int orc_cf_delta(const orc_cfdmt* f, float f0, float f1) {
+    return (int)(ceil((1.0 / (f0 * f0) - 1.0 / (f1 * f1)) /
+                      (1.0 / (f->fmin * f->fmin) - 1.0 / (f->fmax * f->fmax)) * (f->imaxdt - 1.0)));
+}
+
+/* FdmtCpu.cpp:369-372 */
+static float orc_cf_delay(float a, float b) { return (float)(1.0 / (a * a) - 1.0 / (b * b)); }
+
+ORC_EXPORT void orc_cfdmt_destroy(void* h) {
+    orc_cfdmt* f = (orc_cfdmt*)h;
+    if (!f) return;
+    for (int i = 0; i <= f->niter; ++i) {
+        if (f->freq) free(f->freq[i]);
+        if (f->cumsum) free(f->cumsum[i]);
+    }
+    free(f->freq); free(f->cumsum); free(f->quant); free(f);
+}
+
+/* CFdmtCpu::create_config (FdmtCpu.cpp:143-194) with
+ * calc_quant_iterations_and_lengthSubMtrxArray (FdmtB.cpp:213-270) and
+ * calcNextStateConfig (FdmtB.cpp:278-299). */
+ORC_EXPORT void* orc_cfdmt_create(float fmin, float fmax, int nchan, int cols, int imaxdt) {
+    if (nchan < 1 || cols < 1 || imaxdt < 1) { set_err("invalid CFdmt parameters"); return NULL; }
+    orc_cfdmt* f = (orc_cfdmt*)calloc(1, sizeof(orc_cfdmt));
+    f->fmin = fmin; f->fmax = fmax; f->nchan = nchan; f->cols = cols; f->imaxdt = imaxdt;
+    /* iteration count and per-iteration sub-matrix counts */
+    int q = nchan, it = 0;
+    int* quant = (int*)malloc(sizeof(int) * (size_t)(nchan + 2) + 64 * sizeof(int));
+    quant[0] = q;
+    for (it = 0; it < 100; ++it) {
+        if (q == 1) break;
+        q = (q % 2 == 1) ? q / 2 + 1 : q / 2;
+        quant[it + 1] = q;
+    }
+    f->niter = it;
+    f->quant = quant;
+    f->freq = (float**)calloc((size_t)f->niter + 1, sizeof(float*));
+    f->cumsum = (int**)calloc((size_t)f->niter + 1, sizeof(int*));
+    for (int i = 0; i <= f->niter; ++i) {
+        f->freq[i] = (float*)malloc(sizeof(float) * (size_t)(quant[i] + 1));
+        f->cumsum[i] = (int*)malloc(sizeof(int) * (size_t)(quant[i] + 1));
+    }
+    int* rows = (int*)malloc(sizeof(int) * (size_t)nchan);
+    int* rows_cur = (int*)malloc(sizeof(int) * (size_t)nchan);
+    const int d0 = orc_cf_delta(f, fmin, fmin + (fmax - fmin) / nchan);
+    for (int i = 0; i < nchan; ++i) {
+        rows[i] = d0 + 1;
+        f->freq[0][i] = fmin + i * (fmax - fmin) / nchan;
+    }
+    f->freq[0][nchan] = fmax;
+    f->cumsum[0][0] = 0;
+    for (int i = 1; i <= nchan; ++i) f->cumsum[0][i] = f->cumsum[0][i - 1] + rows[i - 1];
+    for (int k = 1; k <= f->niter; ++k) {
+        const int Q = quant[k - 1];
+        const float* F = f->freq[k - 1];
+        float* Fc = f->freq[k];
+        int i = 0, qc;
+        for (i = 0; i < Q / 2; ++i) {
+            rows_cur[i] = orc_cf_delta(f, F[2 * i], F[2 * i + 2]) + 1;
+            Fc[i] = F[2 * i];
+        }
+        i--;
+        if (Q % 2 == 1) {
+            qc = Q / 2 + 1;
+            Fc[qc - 1] = F[2 * i + 2];
+            rows_cur[qc - 1] = rows[Q - 1];
+        } else {
+            qc = Q / 2;
+        }
+        Fc[qc] = fmax;
+        memcpy(rows, rows_cur, sizeof(int) * (size_t)qc);
+        f->cumsum[k][0] = 0;
+        for (int j = 1; j <= qc; ++j) f->cumsum[k][j] = f->cumsum[k][j - 1] + rows_cur[j - 1];
+    }
+    free(rows); free(rows_cur);
+    return f;
+}
+
+ORC_EXPORT int orc_cfdmt_niter(void* h) { return ((orc_cfdmt*)h)->niter; }
+ORC_EXPORT int orc_cfdmt_quant(void* h, int i) { return ((orc_cfdmt*)h)->quant[i]; }
+ORC_EXPORT void orc_cfdmt_config(void* h, int i, int* cumsum, float* freq) {
+    orc_cfdmt* f = (orc_cfdmt*)h;
+    memcpy(cumsum, f->cumsum[i], sizeof(int) * (size_t)(f->quant[i] + 1));
+    memcpy(freq, f->freq[i], sizeof(float) * (size_t)(f->quant[i] + 1));
+}
+ORC_EXPORT int orc_cfdmt_init_delta(void* h) {
+    orc_cfdmt* f = (orc_cfdmt*)h;
+    return orc_cf_delta(f, f->fmin, f->fmin + (f->fmax - f->fmin) / f->nchan);
+}
+
+/* CFdmtCpu::process_image (FdmtCpu.cpp:197-249); x may be NULL when b_ones.
+ * out holds imaxdt*cols floats. */
+ORC_EXPORT int orc_cfdmt_process(void* h, const float* x, float* out, int b_ones) {
+    orc_cfdmt* f = (orc_cfdmt*)h;
+    const int64_t cols = f->cols;
+    const int D = orc_cfdmt_init_delta(h);
+    int64_t rows_max = 0;
+    for (int i = 0; i <= f->niter; ++i)
+        if (f->cumsum[i][f->quant[i]] > rows_max) rows_max = f->cumsum[i][f->quant[i]];
+    float* p0 = (float*)calloc((size_t)(rows_max * cols), sizeof(float));
+    float* p1 = (float*)calloc((size_t)(rows_max * cols), sizeof(float));
+    if (!p0 || !p1) { free(p0); free(p1); set_err("out of memory"); return -1; }
+    /* fnc_initC, FdmtCpu.cpp:314-367 (sequential semantics) */
+    for (int i = 0; i < f->nchan; ++i)
+        for (int64_t j = 0; j < cols; ++j)
+            p0[i * (D + 1) * cols + j] = b_ones ? 1.0f : x[i * cols + j];
+    for (int i = 0; i < f->nchan; ++i)
+        for (int dT = 1; dT < D + 1; ++dT) {
+            float* res = &p0[i * (int64_t)(D + 1) * cols + dT * cols + dT];
+            const float* arg0 = &p0[i * (int64_t)(D + 1) * cols + (dT - 1) * cols + dT];
+            for (int64_t j = 0; j < cols - dT; ++j) {
+                float t = b_ones ? 1.0f : x[i * cols + j];
+                res[j] = ((arg0[j] * ((float)dT) + t) / ((float)(dT + 1)));
+            }
+        }
+    /* fncFdmtIterationC, FdmtCpu.cpp:252-311 */
+    for (int it = 1; it <= f->niter; ++it) {
+        const int Qp = f->quant[it - 1];
+        const int* cs = f->cumsum[it - 1];
+        const float* F = f->freq[it - 1];
+        const int Qc = f->quant[it];
+        const int* csc = f->cumsum[it];
+        for (int i = 0; i < Qc; ++i) {
+            float* pout = &p1[(int64_t)csc[i] * cols];
+            const float* pin0 = &p0[(int64_t)cs[2 * i] * cols];
+            if (i * 2 + 1 >= Qp) {
+                int64_t nlast = (int64_t)(cs[Qp] - cs[Qp - 1]) * cols;
+                for (int64_t j = 0; j < nlast; ++j) pout[j] = pin0[j];
+                break;
+            }
+            const float* pin1 = &p0[(int64_t)cs[2 * i + 1] * cols];
+            const int nrows = csc[i + 1] - csc[i];
+            const float c0 = orc_cf_delay(F[2 * i], F[2 * i + 1]) / orc_cf_delay(F[2 * i], F[2 * i + 2]);
+            const float c1 = orc_cf_delay(F[2 * i + 1], F[2 * i + 2]) / orc_cf_delay(F[2 * i], F[2 * i + 2]);
+            for (int j = 0; j < nrows; ++j) {
+                const int j0 = (int)(c0 * ((float)j));
+                const int j1 = (int)(c1 * ((float)j));
+                for (int64_t k = 0; k < cols; ++k) {
+                    float v = pin0[j0 * cols + k];
+                    if (k - j0 >= 0) v += pin1[j1 * cols + k - j0];
+                    pout[j * cols + k] = v;
+                }
+            }
+        }
+        float* tmp = p0; p0 = p1; p1 = tmp;
+    }
+    memcpy(out, p0, sizeof(float) * (size_t)(cols * f->imaxdt));
+    free(p0); free(p1);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Statistics and cleaning (CDMT/Statistical_preprocessing.cu:15-88,        */
+/* 278-384,443-539).  Deterministic restatement: row sums of x and x*x in    */
+/* FP64 over 256 lane-strided partials combined by a fixed pairwise tree;    */
+/* the cross-row sums run sequentially in row order in FP64.                 */
+/* ------------------------------------------------------------------------ */
+
+#define ORC_LANES 256
+
+static void orc_row_sums(const float* xr, const uint8_t* ur, int64_t N, double* s1, double* s2) {
+    double a[ORC_LANES], b[ORC_LANES];
+    for (int l = 0; l < ORC_LANES; ++l) { a[l] = 0.0; b[l] = 0.0; }
+    for (int64_t t = 0; t < N; ++t) {
+        double v = ur ? (double)ur[t] : (double)xr[t];
+        a[t % ORC_LANES] += v;
+        b[t % ORC_LANES] += v * v;
+    }
+    for (int s = ORC_LANES / 2; s > 0; s >>= 1)
+        for (int l = 0; l < s; ++l) { a[l] += a[l + s]; b[l] += b[l + s]; }
+    *s1 = a[0];
+    *s2 = b[0];
+}
+
+/* stats_out (doubles): [mean, std, var_mean, var_std, n_keep];
+ * row_mean/row_var (doubles, R each); keep (uint8, R).  mode 0 = the
+ * reference's global mean/std (Statistical_preprocessing.cu:521-537),
+ * mode 1 = per-channel mean/std. */
+ORC_EXPORT void orc_stats(const float* x, const uint8_t* xu8, int64_t R, int64_t N, int mode,
+                          double* row_mean, double* row_var, uint8_t* keep, double* stats_out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t r = 0; r < R; ++r) {
+        double s1, s2;
+        orc_row_sums(x ? x + r * N : NULL, xu8 ? xu8 + r * N : NULL, N, &s1, &s2);
+        double m = s1 / (double)N;
+        row_mean[r] = m;
+        row_var[r] = s2 / (double)N - m * m;
+    }
+    double sm = 0.0, sq = 0.0, sv = 0.0, svv = 0.0;
+    for (int64_t r = 0; r < R; ++r) {
+        sm += row_mean[r];
+        sq += row_var[r] + row_mean[r] * row_mean[r];
+        sv += row_var[r];
+        svv += row_var[r] * row_var[r];
+    }
+    double mean = sm / (double)R;
+    double sd = sqrt(sq / (double)R - mean * mean);
+    double vm = sv / (double)R;
+    double vs = sqrt(svv / (double)R - vm * vm);
+    int64_t nk = 0;
+    for (int64_t r = 0; r < R; ++r) {
+        int drop = fabs(row_var[r] - vm) > 4.0 * vs;
+        if (mode == 1 && !(row_var[r] > 0.0)) drop = 1;
+        keep[r] = drop ? 0 : 1;
+        nk += keep[r];
+    }
+    stats_out[0] = mean; stats_out[1] = sd; stats_out[2] = vm; stats_out[3] = vs; stats_out[4] = (double)nk;
+}
+
+/* normalize_clean_and_fill_NormInput (Statistical_preprocessing.cu:510-539):
+ * kept rows x = (x - mu)/sigma in fp32 with mu, sigma rounded to fp32 (global
+ * or per row), mask 1; dropped rows x = 0, mask 0.  out may alias x (f32). */
+ORC_EXPORT void orc_normalise(const float* x, const uint8_t* xu8, int64_t R, int64_t N, int mode,
+                              const double* row_mean, const double* row_var, const uint8_t* keep,
+                              const double* stats, float* out, float* mask) {
+#pragma omp parallel for schedule(static)
+    for (int64_t r = 0; r < R; ++r) {
+        float mu, sg;
+        if (mode == 1) { mu = (float)row_mean[r]; sg = (float)sqrt(row_var[r]); }
+        else { mu = (float)stats[0]; sg = (float)stats[1]; }
+        for (int64_t t = 0; t < N; ++t) {
+            float v = xu8 ? (float)xu8[r * N + t] : x[r * N + t];
+            if (keep[r]) {
+                out[r * N + t] = (v - mu) / sg;
+                if (mask) mask[r * N + t] = 1.0f;
+            } else {
+                out[r * N + t] = 0.0f;
+                if (mask) mask[r * N + t] = 0.0f;
+            }
+        }
+    }
+}
+
+/* fdmt_normalization (CDMT/Chunk_gpu.cu:841-852): the 1e-8 literal is a
+ * double, so n + 1e-8 is formed in double and rounded to float for sqrtf. */
+ORC_EXPORT void orc_fdmt_normalization(const float* a, const float* n, int64_t len, float* out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < len; ++i) out[i] = a[i] / sqrtf((float)((double)n[i] + 1.0E-8));
+}
+
+/* ------------------------------------------------------------------------ */
+/* Boxcar candidate gathering                                                */
+/* ------------------------------------------------------------------------ */
+
+typedef struct { int32_t mt, mdt, mwidth; float msnr; } orc_cand;  /* Candidate.cuh:4-10 */
+
+/* gather_candidates_in_dynamicalArray_kernel (Clusterization.cu:419-462):
+ * widths 1,2,4,..,2^q; mwidth = log2(w); running fp32 sum left to right;
+ * snr = s / sqrtf(w); strict >.  Output in (mdt, mt, mwidth) order.
+ * Returns the true count; writes at most cap. */
+ORC_EXPORT int64_t orc_boxcar_pow2(const float* img, int64_t R, int64_t N, float thr, int q,
+                                   orc_cand* out, int64_t cap) {
+    int64_t n = 0;
+    const int64_t W = (int64_t)1 << q;
+    for (int64_t r = 0; r < R; ++r)
+        for (int64_t t = 0; t < N; ++t) {
+            float s = 0.0f;
+            int64_t next = 1;
+            int p = 0;
+            for (int64_t iw = 0; iw < W; ++iw) {
+                if (t + iw >= N) break;
+                s += img[r * N + t + iw];
+                if (iw + 1 == next) {
+                    float snr = s / sqrtf((float)(iw + 1));
+                    if (snr > thr) {
+                        if (n < cap) { out[n].mt = (int32_t)t; out[n].mdt = (int32_t)r; out[n].mwidth = p; out[n].msnr = snr; }
+                        n++;
+                    }
+                    p++;
+                    next *= 2;
+                }
+            }
+        }
+    return n;
+}
+
+/* do_plan_kernel + gather_candidates_in_heap_kernel (Clusterization.cu:
+ * 591-671): widths 1..W, each window summed from scratch left to right,
+ * mwidth = w, row-major cell order then width. */
+ORC_EXPORT int64_t orc_boxcar_all(const float* img, int64_t R, int64_t N, float thr, int W,
+                                  orc_cand* out, int64_t cap) {
+    int64_t n = 0;
+    for (int64_t r = 0; r < R; ++r)
+        for (int64_t t = 0; t < N; ++t)
+            for (int iw = 1; iw <= W; ++iw) {
+                if (t + iw - 1 >= N) continue;
+                float s = 0.0f;
+                for (int j = 0; j < iw; ++j) s += img[r * N + t + j];
+                float snr = s / sqrtf((float)iw);
+                if (snr > thr) {
+                    if (n < cap) { out[n].mt = (int32_t)t; out[n].mdt = (int32_t)r; out[n].mwidth = iw; out[n].msnr = snr; }
+                    n++;
+                }
+            }
+    return n;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Clustering (Clusterization.cu:92-166,793-998; select_delegates.cu:62-125) */
+/* Deterministic rules: stable sort within groups (ties keep the current    */
+/* order, starting from input order); split where key[i]-key[i-1] > bin;    */
+/* delegate = max msnr, ties -> lowest candidate index; delegates stably    */
+/* sorted by mt (ties keep group order); num = true group size.             */
+/* ------------------------------------------------------------------------ */
+
+static const orc_cand* g_sort_c;
+static int g_sort_key;
+static int orc_key(const orc_cand* c, int k) { return k == 0 ? c->mt : (k == 1 ? c->mdt : c->mwidth); }
+
+/* stable insertion-merge sort of idx[lo,hi) by key */
+static void orc_stable_sort(int32_t* idx, int64_t n, int32_t* tmp) {
+    if (n < 2) return;
+    int64_t h = n / 2;
+    orc_stable_sort(idx, h, tmp);
+    orc_stable_sort(idx + h, n - h, tmp);
+    int64_t i = 0, j = h, k = 0;
+    while (i < h && j < n) {
+        int a = orc_key(&g_sort_c[idx[i]], g_sort_key), b = orc_key(&g_sort_c[idx[j]], g_sort_key);
+        if (b < a) tmp[k++] = idx[j++]; else tmp[k++] = idx[i++];
+    }
+    while (i < h) tmp[k++] = idx[i++];
+    while (j < n) tmp[k++] = idx[j++];
+    memcpy(idx, tmp, sizeof(int32_t) * (size_t)n);
+}
+
+/* order (n): final member order; group_off (ngroups+1); delegates (ngroups,
+ * candidate indices sorted by mt); group_size (ngroups, aligned with
+ * delegates).  Buffers sized n (+1).  Returns ngroups (0 when n == 0). */
+ORC_EXPORT int64_t orc_cluster(const orc_cand* c, int64_t n, const int32_t* bins, int32_t* order,
+                               int64_t* group_off, int32_t* delegates, int64_t* group_size) {
+    if (n <= 0) return 0;
+    int32_t* tmp = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    int64_t* off = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 1));
+    int64_t* noff = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 1));
+    for (int64_t i = 0; i < n; ++i) order[i] = (int32_t)i;
+    int64_t ng = 1;
+    off[0] = 0; off[1] = n;
+    g_sort_c = c;
+    for (int k = 0; k < 3; ++k) {
+        g_sort_key = k;
+        int64_t nn = 0;
+        for (int64_t g = 0; g < ng; ++g) {
+            orc_stable_sort(order + off[g], off[g + 1] - off[g], tmp);
+            noff[nn++] = off[g];
+            for (int64_t i = off[g] + 1; i < off[g + 1]; ++i)
+                if (orc_key(&c[order[i]], k) - orc_key(&c[order[i - 1]], k) > bins[k]) noff[nn++] = i;
+        }
+        noff[nn] = n;
+        ng = nn;
+        memcpy(off, noff, sizeof(int64_t) * (size_t)(ng + 1));
+    }
+    memcpy(group_off, off, sizeof(int64_t) * (size_t)(ng + 1));
+    /* delegates in group order */
+    int32_t* del = (int32_t*)malloc(sizeof(int32_t) * (size_t)ng);
+    for (int64_t g = 0; g < ng; ++g) {
+        int32_t best = order[off[g]];
+        for (int64_t i = off[g] + 1; i < off[g + 1]; ++i) {
+            int32_t j = order[i];
+            if (c[j].msnr > c[best].msnr || (c[j].msnr == c[best].msnr && j < best)) best = j;
+        }
+        del[g] = best;
+    }
+    /* stable sort of groups by delegate mt */
+    int32_t* gi = (int32_t*)malloc(sizeof(int32_t) * (size_t)ng);
+    for (int64_t g = 0; g < ng; ++g) gi[g] = (int32_t)g;
+    for (int64_t a = 1; a < ng; ++a) {
+        int32_t v = gi[a];
+        int64_t b = a - 1;
+        while (b >= 0 && c[del[gi[b]]].mt > c[del[v]].mt) { gi[b + 1] = gi[b]; b--; }
+        gi[b + 1] = v;
+    }
+    for (int64_t g = 0; g < ng; ++g) {
+        delegates[g] = del[gi[g]];
+        group_size[g] = off[gi[g] + 1] - off[gi[g]];
+    }
+    free(gi); free(del); free(tmp); free(off); free(noff);
+    return ng;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Single-maximum detector (detection/kernel.cu:43-133,561-582 restated):    */
+/* global max over rows, windows 1..W fully inside, of                       */
+/* sum_{j<w} a[t+j]/sqrtf(1e-7 + n[t+j]) (fp32 left to right) / sqrtf(w).    */
+/* Ties -> smallest width, then row, then column.                            */
+/* out4 = {iwidth, irow, icol} ints and *snr.  Returns 0, or -1 if empty.    */
+/* ------------------------------------------------------------------------ */
+ORC_EXPORT int orc_detect_max(const float* a, const float* nrm, int64_t R, int64_t N, int W,
+                              int32_t* out3, float* snr) {
+    int found = 0;
+    float best = 0.0f;
+    int32_t bw = -1, br = -1, bc = -1;
+    for (int w = 1; w <= W; ++w)
+        for (int64_t r = 0; r < R; ++r)
+            for (int64_t t = 0; t + w <= N; ++t) {
+                float s = 0.0f;
+                for (int j = 0; j < w; ++j) {
+                    float v = a[r * N + t + j];
+                    if (nrm) v = v / sqrtf((float)(1.0E-7 + (double)nrm[r * N + t + j]));
+                    s += v;
+                }
+                float val = s / sqrtf((float)w);
+                if (!found || val > best) { found = 1; best = val; bw = w; br = (int32_t)r; bc = (int32_t)t; }
+            }
+    if (!found) return -1;
+    out3[0] = bw; out3[1] = br; out3[2] = bc;
+    *snr = best;
+    return 0;
+}
