@@ -1,0 +1,69 @@
+"""Builds libsps.so (HIP kernels for gfx950 + host C++ + the C ABI) in-tree.
+
+Usage: python -m cpp_projects_amd.build   (or __graft_entry__.build())
+"""
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(PKG, "csrc")
+ROOT = os.path.dirname(PKG)
+BUILD = os.path.join(PKG, "_build")
+LIB = os.path.join(PKG, "libsps.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX_HOST", "g++")
+ARCH = "gfx950"
+
+# -ffp-contract=off: fp32 results must be the IEEE results of the source's
+# operation order (bit-exact parity with the CPU oracle); packed fp32 ops are
+# disabled for all device code.
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", os.path.join(ROOT, "include"), "-Wno-unused-result",
+]
+HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-I", os.path.join(ROOT, "include")]
+
+HIP_SOURCES = ["fdmt_kernels.hip", "search_kernels.hip", "sort_kernels.hip", "fused_kernels.hip"]
+HIPCC_HOST_SOURCES = ["api.cpp", "batch.cpp"]
+GXX_SOURCES = ["plan.cpp"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build failed:\n" + " ".join(cmd) + "\n" + r.stdout)
+    return r.stdout
+
+
+def build(verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    jobs = []
+    for s in HIP_SOURCES:
+        src = os.path.join(CSRC, s)
+        if not os.path.exists(src):
+            continue
+        jobs.append([HIPCC, *HIP_FLAGS, "-c", src, "-o", os.path.join(BUILD, s + ".o")])
+    for s in HIPCC_HOST_SOURCES:
+        src = os.path.join(CSRC, s)
+        if not os.path.exists(src):
+            continue
+        jobs.append([HIPCC, *HIP_FLAGS, "-x", "hip", "-c", src, "-o", os.path.join(BUILD, s + ".o")])
+    for s in GXX_SOURCES:
+        jobs.append([CXX, *HOST_FLAGS, "-c", os.path.join(CSRC, s), "-o", os.path.join(BUILD, s + ".o")])
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        for out in ex.map(_run, jobs):
+            if verbose and out:
+                print(out)
+    objs = [j[-1] for j in jobs]
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

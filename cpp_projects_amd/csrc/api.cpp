@@ -1,0 +1,730 @@
+// api.cpp -- the extern "C" boundary of libsps (declared in include/sps.h).
+// Every entry point converts C++ exceptions into an sps_status and records
+// the message for sps_last_error().
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+#include "plan.hpp"
+
+using namespace sps;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F>
+sps_status guarded(F&& f) {
+    try {
+        f();
+        return SPS_OK;
+    } catch (const sps::Error& e) {
+        g_last_error = e.what();
+        return e.status;
+    } catch (const std::invalid_argument& e) {
+        g_last_error = e.what();
+        return SPS_ERR_INVALID_ARGUMENT;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return SPS_ERR_NO_MEMORY;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return SPS_ERR_RUNTIME;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return SPS_ERR_RUNTIME;
+    }
+}
+
+template <class T>
+void upload(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
+    b.ensure(std::max<size_t>(sizeof(T) * v.size(), 16));
+    if (!v.empty()) SPS_HIP(hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st));
+}
+
+int bitlen(uint64_t v) {
+    int b = 0;
+    while (v) {
+        ++b;
+        v >>= 1;
+    }
+    return b;
+}
+
+int64_t round_up(int64_t a, int64_t m) { return ((a + m - 1) / m) * m; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// FDMT handle
+// ---------------------------------------------------------------------------
+struct sps_fdmt {
+    std::unique_ptr<FdmtPlan> plan;    // plan-based FDMT
+    std::unique_ptr<CFdmtPlan> cplan;  // legacy CFdmt
+    EnginePlan eng;
+    int64_t pitch = 0;                 // internal state pitch (floats)
+    bool uploaded = false;
+    DevBuf d_grid, d_grid_off, d_chan_row_off;
+    std::vector<std::unique_ptr<DevBuf>> d_rows;  // per iteration
+    DevBuf state_a, state_b;
+    int64_t rows_a = 0, rows_b = 0;    // capacity of the ping-pong states in rows
+    // adjoint tables
+    bool adj_built = false;
+    std::vector<std::unique_ptr<DevBuf>> csr_off, csr_row, csr_shift;
+
+    void init_layout() {
+        pitch = round_up(eng.nsamps, 4);
+        rows_a = rows_b = 0;
+        for (int i = 0; i <= eng.niters; ++i) {
+            if (i % 2 == 0) rows_a = std::max(rows_a, eng.rows[(size_t)i]);
+            else rows_b = std::max(rows_b, eng.rows[(size_t)i]);
+        }
+    }
+
+    void ensure_device(hipStream_t st) {
+        if (uploaded) return;
+        upload(d_grid, eng.grid, st);
+        upload(d_grid_off, eng.grid_off, st);
+        upload(d_chan_row_off, eng.chan_row_off, st);
+        d_rows.clear();
+        for (int i = 0; i <= eng.niters; ++i) {
+            d_rows.emplace_back(new DevBuf());
+            std::vector<RowDescD> rd;
+            for (const auto& r : eng.iter[(size_t)i]) rd.push_back({r.out_row, r.tail_row, r.head_row, r.offset});
+            upload(*d_rows.back(), rd, st);
+        }
+        SPS_HIP(hipStreamSynchronize(st));
+        uploaded = true;
+    }
+
+    void ensure_states() {
+        state_a.ensure(sizeof(float) * (size_t)checked_mul(std::max<int64_t>(rows_a, 1), pitch, "state"));
+        if (eng.niters >= 2 || rows_b > 0)
+            state_b.ensure(sizeof(float) * (size_t)checked_mul(std::max<int64_t>(rows_b, 1), pitch, "state"));
+    }
+
+    // Runs init + merges over nsamps samples (nsamps <= eng.nsamps) writing the
+    // final rows to out (pitch out_pitch).  sa/sb: ping-pong states with pitch p.
+    void run(const void* in, int in_u8, int ones, int64_t in_pitch, const ChanNorm* norm, int64_t nsamps, float* out,
+             int64_t out_pitch, float* sa, float* sb, int64_t p, hipStream_t st) {
+        const ChanNorm* nm = norm;
+        if (eng.niters == 0) {
+            launch_fdmt_init(eng.init_mode, in, in_u8, ones, in_pitch, nm, eng.nchans, nsamps, d_grid.as<int32_t>(),
+                             d_grid_off.as<int64_t>(), d_chan_row_off.as<int64_t>(), out, out_pitch, st);
+            return;
+        }
+        launch_fdmt_init(eng.init_mode, in, in_u8, ones, in_pitch, nm, eng.nchans, nsamps, d_grid.as<int32_t>(),
+                         d_grid_off.as<int64_t>(), d_chan_row_off.as<int64_t>(), sa, p, st);
+        for (int i = 1; i <= eng.niters; ++i) {
+            const float* src = ((i - 1) % 2 == 0) ? sa : sb;
+            const bool last = (i == eng.niters);
+            float* dst = last ? out : ((i % 2 == 0) ? sa : sb);
+            const int64_t dp = last ? out_pitch : p;
+            const int64_t limit = last ? eng.out_rows : INT64_MAX;
+            launch_fdmt_merge(src, p, dst, dp, d_rows[(size_t)i]->as<RowDescD>(),
+                              (int64_t)eng.iter[(size_t)i].size(), nsamps, limit, st);
+        }
+    }
+
+    void build_adjoint(hipStream_t st) {
+        if (adj_built) return;
+        csr_off.clear();
+        csr_row.clear();
+        csr_shift.clear();
+        for (int i = 0; i <= eng.niters; ++i) {
+            csr_off.emplace_back(new DevBuf());
+            csr_row.emplace_back(new DevBuf());
+            csr_shift.emplace_back(new DevBuf());
+            if (i == 0) continue;
+            const int64_t prev_rows = eng.rows[(size_t)i - 1];
+            std::vector<std::vector<std::pair<int64_t, int64_t>>> users((size_t)prev_rows);
+            for (const auto& r : eng.iter[(size_t)i]) {
+                users[(size_t)r.tail_row].push_back({r.out_row, 0});
+                if (r.head_row >= 0) users[(size_t)r.head_row].push_back({r.out_row, r.offset});
+            }
+            std::vector<int64_t> off(1, 0), row, sh;
+            for (auto& u : users) {
+                std::sort(u.begin(), u.end());
+                for (auto& e : u) {
+                    row.push_back(e.first);
+                    sh.push_back(e.second);
+                }
+                off.push_back((int64_t)row.size());
+            }
+            upload(*csr_off.back(), off, st);
+            upload(*csr_row.back(), row, st);
+            upload(*csr_shift.back(), sh, st);
+        }
+        SPS_HIP(hipStreamSynchronize(st));
+        adj_built = true;
+    }
+};
+
+static void finish_create(sps_fdmt* h) {
+    h->init_layout();
+}
+
+extern "C" {
+
+SPS_API const char* sps_last_error(void) { return g_last_error.c_str(); }
+SPS_API int sps_version(void) { return 100; }
+
+SPS_API sps_status sps_device_count(int* h_count) {
+    return guarded([&] {
+        SPS_REQUIRE(h_count, "null output");
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        *h_count = (e == hipSuccess) ? n : 0;
+    });
+}
+
+SPS_API sps_status sps_fdmt_create(float f_min, float f_max, int64_t nchans, int64_t nsamps, float tsamp,
+                                   int64_t dt_max, int64_t dt_step, int64_t dt_min, int init_mode, sps_fdmt** out) {
+    return guarded([&] {
+        SPS_REQUIRE(out, "null output handle");
+        *out = nullptr;
+        std::unique_ptr<sps_fdmt> h(new sps_fdmt());
+        h->plan.reset(new FdmtPlan(f_min, f_max, nchans, nsamps, tsamp, dt_max, dt_step, dt_min));
+        h->eng = h->plan->lower(init_mode);
+        finish_create(h.get());
+        *out = h.release();
+    });
+}
+
+SPS_API sps_status sps_cfdmt_create(float f_min, float f_max, int nchan, int cols, int imaxdt, sps_fdmt** out) {
+    return guarded([&] {
+        SPS_REQUIRE(out, "null output handle");
+        *out = nullptr;
+        std::unique_ptr<sps_fdmt> h(new sps_fdmt());
+        h->cplan.reset(new CFdmtPlan(f_min, f_max, nchan, cols, imaxdt));
+        h->eng = h->cplan->lower();
+        finish_create(h.get());
+        *out = h.release();
+    });
+}
+
+SPS_API sps_status sps_fdmt_destroy(sps_fdmt* h) {
+    return guarded([&] { delete h; });
+}
+
+SPS_API sps_status sps_fdmt_get_niters(const sps_fdmt* h, int64_t* n) {
+    return guarded([&] {
+        SPS_REQUIRE(h && n, "null argument");
+        *n = h->eng.niters;
+    });
+}
+
+SPS_API sps_status sps_fdmt_get_state_shape(const sps_fdmt* h, int64_t iter, int64_t* s5) {
+    return guarded([&] {
+        SPS_REQUIRE(h && s5, "null argument");
+        SPS_REQUIRE(iter >= 0 && iter <= h->eng.niters, "iteration out of range");
+        if (h->plan) {
+            h->plan->state_shape(iter, s5);
+        } else {
+            const auto& c = *h->cplan;
+            const int q = c.quant[(size_t)iter];
+            int mn = INT32_MAX, mx = 0;
+            for (int i = 0; i < q; ++i) {
+                int r = c.cumsum[(size_t)iter][(size_t)i + 1] - c.cumsum[(size_t)iter][(size_t)i];
+                mn = std::min(mn, r);
+                mx = std::max(mx, r);
+            }
+            s5[0] = q;
+            s5[1] = mn;
+            s5[2] = mx;
+            s5[3] = c.cumsum[(size_t)iter][(size_t)q];
+            s5[4] = c.cols;
+        }
+    });
+}
+
+SPS_API sps_status sps_fdmt_get_output_shape(const sps_fdmt* h, int64_t* rows, int64_t* cols) {
+    return guarded([&] {
+        SPS_REQUIRE(h && rows && cols, "null argument");
+        *rows = h->eng.out_rows;
+        *cols = h->eng.nsamps;
+    });
+}
+
+SPS_API sps_status sps_fdmt_get_dt_grid_final(const sps_fdmt* h, int64_t* outv, int64_t cap, int64_t* n) {
+    return guarded([&] {
+        SPS_REQUIRE(h && n, "null argument");
+        std::vector<int64_t> g;
+        if (h->plan) g = h->plan->dt_grid_final();
+        else
+            for (int i = 0; i < h->cplan->imaxdt; ++i) g.push_back(i);
+        *n = (int64_t)g.size();
+        if (outv) std::copy(g.begin(), g.begin() + std::min<int64_t>(cap, (int64_t)g.size()), outv);
+    });
+}
+
+SPS_API sps_status sps_fdmt_get_dm_grid_final(const sps_fdmt* h, float* outv, int64_t cap, int64_t* n) {
+    return guarded([&] {
+        SPS_REQUIRE(h && n, "null argument");
+        SPS_REQUIRE(h->plan, "DM grid is defined for plan-based FDMT handles");
+        auto g = h->plan->dm_grid_final();
+        *n = (int64_t)g.size();
+        if (outv) std::copy(g.begin(), g.begin() + std::min<int64_t>(cap, (int64_t)g.size()), outv);
+    });
+}
+
+SPS_API sps_status sps_fdmt_get_df(const sps_fdmt* h, float* df, float* corr) {
+    return guarded([&] {
+        SPS_REQUIRE(h && df && corr, "null argument");
+        SPS_REQUIRE(h->plan, "df is defined for plan-based FDMT handles");
+        *df = h->plan->df;
+        *corr = h->plan->correction;
+    });
+}
+
+SPS_API sps_status sps_fdmt_get_plan_iter(const sps_fdmt* h, int64_t iter, int64_t* nsub, int64_t* grid_off,
+                                          int64_t* grid, int64_t* nmap, int64_t* map7, int64_t* ncopy,
+                                          int64_t* copy4, float* df_top, float* df_bot) {
+    return guarded([&] {
+        SPS_REQUIRE(h, "null handle");
+        SPS_REQUIRE(h->plan, "plan tables are defined for plan-based FDMT handles");
+        SPS_REQUIRE(iter >= 0 && iter <= h->plan->niters, "iteration out of range");
+        const IterPlan& it = h->plan->iters[(size_t)iter];
+        if (nsub) *nsub = it.nsub();
+        if (grid_off) {
+            grid_off[0] = 0;
+            for (int64_t s = 0; s < it.nsub(); ++s) grid_off[s + 1] = grid_off[s] + (int64_t)it.dt_grid[(size_t)s].size();
+        }
+        if (grid) {
+            int64_t k = 0;
+            for (const auto& g : it.dt_grid)
+                for (auto v : g) grid[k++] = v;
+        }
+        if (nmap) *nmap = (int64_t)it.maps.size();
+        if (map7)
+            for (size_t k = 0; k < it.maps.size(); ++k) {
+                const Mapping& m = it.maps[k];
+                int64_t* r = map7 + 7 * k;
+                r[0] = m.out.sub; r[1] = m.out.dt; r[2] = m.head.sub; r[3] = m.head.dt;
+                r[4] = m.tail.sub; r[5] = m.tail.dt; r[6] = m.offset;
+            }
+        if (ncopy) *ncopy = (int64_t)it.copies.size();
+        if (copy4)
+            for (size_t k = 0; k < it.copies.size(); ++k) {
+                const Mapping& m = it.copies[k];
+                int64_t* r = copy4 + 4 * k;
+                r[0] = m.out.sub; r[1] = m.out.dt; r[2] = m.tail.sub; r[3] = m.tail.dt;
+            }
+        if (df_top) *df_top = it.df_top;
+        if (df_bot) *df_bot = it.df_bot;
+    });
+}
+
+SPS_API sps_status sps_fdmt_get_rows(const sps_fdmt* h, int64_t iter, int64_t* n, int64_t* rows4) {
+    return guarded([&] {
+        SPS_REQUIRE(h && n, "null argument");
+        SPS_REQUIRE(iter >= 1 && iter <= h->eng.niters, "iteration out of range");
+        const auto& rd = h->eng.iter[(size_t)iter];
+        *n = (int64_t)rd.size();
+        if (rows4)
+            for (size_t k = 0; k < rd.size(); ++k) {
+                rows4[4 * k + 0] = rd[k].out_row;
+                rows4[4 * k + 1] = rd[k].tail_row;
+                rows4[4 * k + 2] = rd[k].head_row;
+                rows4[4 * k + 3] = rd[k].offset;
+            }
+    });
+}
+
+SPS_API sps_status sps_fdmt_execute(sps_fdmt* h, const void* d_wf, int dtype, size_t wf_size, float* d_dmt,
+                                    size_t dmt_size, int ones, void* stream) {
+    return guarded([&] {
+        SPS_REQUIRE(h, "null handle");
+        const EnginePlan& e = h->eng;
+        if (!ones) {
+            SPS_REQUIRE(d_wf, "null waterfall");
+            if ((int64_t)wf_size != e.nchans * e.nsamps) throw Error(SPS_ERR_INVALID_ARGUMENT, "Invalid size of waterfall");
+        }
+        SPS_REQUIRE(dtype == SPS_F32 || dtype == SPS_U8, "unknown dtype");
+        SPS_REQUIRE(d_dmt, "null dmt");
+        if ((int64_t)dmt_size != e.out_rows * e.nsamps) throw Error(SPS_ERR_INVALID_ARGUMENT, "Invalid size of dmt");
+        hipStream_t st = as_stream(stream);
+        h->ensure_device(st);
+        h->ensure_states();
+        h->run(d_wf, dtype == SPS_U8, ones, e.nsamps, nullptr, e.nsamps, d_dmt, e.nsamps, h->state_a.as<float>(),
+               h->state_b.as<float>(), h->pitch, st);
+    });
+}
+
+SPS_API sps_status sps_fdmt_initialise(sps_fdmt* h, const void* d_wf, int dtype, float* d_state, size_t state_size,
+                                       void* stream) {
+    return guarded([&] {
+        SPS_REQUIRE(h && d_wf && d_state, "null argument");
+        const EnginePlan& e = h->eng;
+        SPS_REQUIRE((int64_t)state_size == e.rows[0] * e.nsamps, "Invalid size of state");
+        hipStream_t st = as_stream(stream);
+        h->ensure_device(st);
+        launch_fdmt_init(e.init_mode, d_wf, dtype == SPS_U8, 0, e.nsamps, nullptr, e.nchans, e.nsamps,
+                         h->d_grid.as<int32_t>(), h->d_grid_off.as<int64_t>(), h->d_chan_row_off.as<int64_t>(), d_state,
+                         e.nsamps, st);
+    });
+}
+
+SPS_API sps_status sps_fdmt_adjoint(sps_fdmt* h, const float* g_dmt, size_t dmt_size, float* g_wf, size_t wf_size,
+                                    void* stream) {
+    return guarded([&] {
+        SPS_REQUIRE(h && g_dmt && g_wf, "null argument");
+        const EnginePlan& e = h->eng;
+        SPS_REQUIRE(e.init_mode != kInitRestrict, "the restrict init form is not linear; no adjoint");
+        SPS_REQUIRE((int64_t)dmt_size == e.out_rows * e.nsamps, "Invalid size of dmt");
+        SPS_REQUIRE((int64_t)wf_size == e.nchans * e.nsamps, "Invalid size of waterfall");
+        SPS_REQUIRE(e.rows.back() == e.out_rows, "adjoint needs the full final state");
+        hipStream_t st = as_stream(stream);
+        h->ensure_device(st);
+        h->ensure_states();
+        h->build_adjoint(st);
+        const int64_t N = e.nsamps, P = h->pitch;
+        float* sa = h->state_a.as<float>();
+        float* sb = h->state_b.as<float>();
+        const float* cur = g_dmt;
+        int64_t cur_pitch = N;
+        for (int i = e.niters; i >= 1; --i) {
+            float* dst = ((i - 1) % 2 == 0) ? sa : sb;
+            launch_fdmt_merge_adjoint(cur, cur_pitch, dst, P, h->csr_off[(size_t)i]->as<int64_t>(),
+                                      h->csr_row[(size_t)i]->as<int64_t>(), h->csr_shift[(size_t)i]->as<int64_t>(),
+                                      e.rows[(size_t)i - 1], N, st);
+            cur = dst;
+            cur_pitch = P;
+        }
+        launch_fdmt_init_adjoint(cur, cur_pitch, g_wf, N, e.nchans, N, h->d_grid.as<int32_t>(),
+                                 h->d_grid_off.as<int64_t>(), h->d_chan_row_off.as<int64_t>(), st);
+    });
+}
+
+// ---------------------------------------------------------------------------
+// normalisation
+// ---------------------------------------------------------------------------
+SPS_API sps_status sps_calc_fdmt_inputs(float* d_in, int64_t nrows, int64_t ncols, float* d_buff, float* d_norm_input,
+                                        int mode, void* stream) {
+    return guarded([&] {
+        SPS_REQUIRE(d_in && nrows > 0 && ncols > 0, "invalid input");
+        SPS_REQUIRE(mode == SPS_NORM_GLOBAL || mode == SPS_NORM_PER_CHANNEL, "mode must be global or per-channel");
+        hipStream_t st = as_stream(stream);
+        DevBuf rm, rv, stats, mu, sg, keep;
+        rm.ensure(sizeof(double) * (size_t)nrows);
+        rv.ensure(sizeof(double) * (size_t)nrows);
+        stats.ensure(sizeof(double) * 8);
+        mu.ensure(sizeof(float) * (size_t)nrows);
+        sg.ensure(sizeof(float) * (size_t)nrows);
+        keep.ensure((size_t)nrows);
+        launch_row_stats(d_in, 0, nrows, ncols, rm.as<double>(), rv.as<double>(), st);
+        launch_stats_finalize(rm.as<double>(), rv.as<double>(), nrows, mode, stats.as<double>(), mu.as<float>(),
+                              sg.as<float>(), keep.as<uint8_t>(), d_buff, st);
+        launch_apply_norm(d_in, d_norm_input, nrows, ncols, mu.as<float>(), sg.as<float>(), keep.as<uint8_t>(), st);
+        SPS_HIP(hipStreamSynchronize(st));
+    });
+}
+
+SPS_API sps_status sps_fdmt_normalization(const float* d_arr, const float* d_norm, int64_t len, float* d_out,
+                                          void* stream) {
+    return guarded([&] {
+        SPS_REQUIRE(d_arr && d_norm && d_out && len >= 0, "invalid argument");
+        launch_fdmt_normalization(d_arr, d_norm, len, d_out, as_stream(stream));
+    });
+}
+
+// ---------------------------------------------------------------------------
+// candidates
+// ---------------------------------------------------------------------------
+namespace {
+
+// Sorts n candidates in place by (mdt, mt, mwidth).
+void sort_cands(CandD* c, int64_t n, int64_t rows, int64_t cols, int64_t wmax, hipStream_t st) {
+    if (n <= 1) return;
+    const int bt = std::max(1, bitlen((uint64_t)cols)), bw = std::max(1, bitlen((uint64_t)wmax)),
+              br = std::max(1, bitlen((uint64_t)rows));
+    SPS_REQUIRE(bt + bw + br <= 64, "candidate key does not fit 64 bits");
+    DevBuf k, kt, v, vt, hs;
+    k.ensure(sizeof(uint64_t) * (size_t)n);
+    kt.ensure(sizeof(uint64_t) * (size_t)n);
+    v.ensure(sizeof(uint32_t) * (size_t)n);
+    vt.ensure(sizeof(uint32_t) * (size_t)n);
+    const int64_t hc = radix_hist_entries(n);
+    hs.ensure(sizeof(uint32_t) * (size_t)hc);
+    launch_cand_keys(c, n, bt, bw, k.as<uint64_t>(), v.as<uint32_t>(), st);
+    radix_sort_pairs(k.as<uint64_t>(), v.as<uint32_t>(), kt.as<uint64_t>(), vt.as<uint32_t>(), n, bt + bw + br,
+                     hs.as<uint32_t>(), hc, st);
+    launch_cand_from_keys(k.as<uint64_t>(), v.as<uint32_t>(), n, bt, bw, c, st);
+    SPS_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace
+
+SPS_API sps_status sps_boxcar_candidates(const float* d_img, int64_t rows, int64_t cols, const float* d_thr, int wnd,
+                                         int all_widths, sps_cand* d_out, int64_t cap, int64_t* h_count,
+                                         void* stream) {
+    return guarded([&] {
+        SPS_REQUIRE(d_img && d_thr && h_count, "null argument");
+        SPS_REQUIRE(rows > 0 && cols > 0 && cap >= 0, "invalid sizes");
+        SPS_REQUIRE(cap == 0 || d_out, "null output");
+        const bool pow2 = !all_widths;
+        SPS_REQUIRE(wnd >= (pow2 ? 0 : 1) && wnd <= (pow2 ? 13 : 8192), "window parameter out of range");
+        const int64_t wmax = pow2 ? (int64_t)1 << wnd : wnd;
+        hipStream_t st = as_stream(stream);
+        DevBuf cnt, tmp;
+        cnt.ensure(sizeof(unsigned long long));
+        int64_t tcap = std::max<int64_t>(cap, 1024);
+        unsigned long long n = 0;
+        for (int pass = 0; pass < 2; ++pass) {
+            tmp.ensure(sizeof(CandD) * (size_t)tcap);
+            launch_fill_u64(cnt.as<unsigned long long>(), 1, 0ull, st);
+            launch_boxcar(d_img, rows, cols, cols, d_thr, 0.f, (int)wmax, pow2, tmp.as<CandD>(), tcap,
+                          cnt.as<unsigned long long>(), st);
+            SPS_HIP(hipMemcpyAsync(&n, cnt.p, sizeof(n), hipMemcpyDeviceToHost, st));
+            SPS_HIP(hipStreamSynchronize(st));
+            if ((int64_t)n <= tcap) break;
+            tcap = (int64_t)n;
+        }
+        sort_cands(tmp.as<CandD>(), (int64_t)n, rows, cols, pow2 ? wnd : wmax, st);
+        const int64_t ncopy = std::min<int64_t>((int64_t)n, cap);
+        if (ncopy > 0)
+            SPS_HIP(hipMemcpyAsync(d_out, tmp.p, sizeof(CandD) * (size_t)ncopy, hipMemcpyDeviceToDevice, st));
+        SPS_HIP(hipStreamSynchronize(st));
+        *h_count = (int64_t)n;
+        if ((int64_t)n > cap) throw Error(SPS_ERR_CAPACITY, "more candidates than the output capacity");
+    });
+}
+
+SPS_API sps_status sps_cluster(const sps_cand* d_cands, int64_t n, const int32_t* d_bins, int32_t* d_delegates,
+                               int64_t* d_group_size, int64_t* h_ngroups, void* stream) {
+    return guarded([&] {
+        SPS_REQUIRE(h_ngroups && d_bins, "null argument");
+        SPS_REQUIRE(n >= 0 && n < 0xffffffffLL, "invalid candidate count");
+        *h_ngroups = 0;
+        if (n == 0) return;
+        SPS_REQUIRE(d_cands && d_delegates && d_group_size, "null argument");
+        hipStream_t st = as_stream(stream);
+        const CandD* c = reinterpret_cast<const CandD*>(d_cands);
+        DevBuf k, kt, v, vt, perm, grp, flags, hs, best, start, sc;
+        k.ensure(sizeof(uint64_t) * (size_t)n);
+        kt.ensure(sizeof(uint64_t) * (size_t)n);
+        v.ensure(sizeof(uint32_t) * (size_t)n);
+        vt.ensure(sizeof(uint32_t) * (size_t)n);
+        perm.ensure(sizeof(uint32_t) * (size_t)n);
+        grp.ensure(sizeof(uint32_t) * (size_t)n);
+        flags.ensure(sizeof(uint32_t) * (size_t)n);
+        const int64_t hc = radix_hist_entries(n);
+        hs.ensure(sizeof(uint32_t) * (size_t)hc);
+        sc.ensure(sizeof(uint32_t) * (size_t)scan_scratch_entries(n));
+        launch_iota_u32(perm.as<uint32_t>(), n, st);
+        for (int level = 0; level < 3; ++level) {
+            launch_cluster_keys(c, perm.as<uint32_t>(), level == 0 ? nullptr : grp.as<uint32_t>(), n, level,
+                                k.as<uint64_t>(), v.as<uint32_t>(), st);
+            radix_sort_pairs(k.as<uint64_t>(), v.as<uint32_t>(), kt.as<uint64_t>(), vt.as<uint32_t>(), n,
+                             level == 0 ? 32 : 64, hs.as<uint32_t>(), hc, st);
+            SPS_HIP(hipMemcpyAsync(perm.p, v.p, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice, st));
+            launch_cluster_flags(k.as<uint64_t>(), n, d_bins, level, flags.as<uint32_t>(), st);
+            SPS_HIP(hipMemcpyAsync(grp.p, flags.p, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice, st));
+            scan_inclusive_u32(grp.as<uint32_t>(), n, sc.as<uint32_t>(), st);
+        }
+        uint32_t ng = 0;
+        SPS_HIP(hipMemcpyAsync(&ng, grp.as<uint32_t>() + (n - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        SPS_HIP(hipStreamSynchronize(st));
+        best.ensure(sizeof(unsigned long long) * (size_t)ng);
+        start.ensure(sizeof(uint32_t) * (size_t)ng);
+        launch_fill_u64(best.as<unsigned long long>(), ng, 0ull, st);
+        launch_cluster_delegates(c, perm.as<uint32_t>(), grp.as<uint32_t>(), flags.as<uint32_t>(), n,
+                                 best.as<unsigned long long>(), start.as<uint32_t>(), st);
+        launch_delegate_keys(c, best.as<unsigned long long>(), start.as<uint32_t>(), ng, n, k.as<uint64_t>(),
+                             v.as<uint32_t>(), st);
+        radix_sort_pairs(k.as<uint64_t>(), v.as<uint32_t>(), kt.as<uint64_t>(), vt.as<uint32_t>(), ng, 64,
+                         hs.as<uint32_t>(), hc, st);
+        launch_delegate_out(best.as<unsigned long long>(), start.as<uint32_t>(), v.as<uint32_t>(), ng, n, d_delegates,
+                            d_group_size, st);
+        SPS_HIP(hipStreamSynchronize(st));
+        *h_ngroups = ng;
+    });
+}
+
+SPS_API sps_status sps_detect_max(const float* d_arr, const float* d_norm, int64_t rows, int64_t cols, int wnd,
+                                  sps_detection* d_out, void* stream) {
+    return guarded([&] {
+        SPS_REQUIRE(d_arr && d_out && rows > 0 && cols > 0 && wnd >= 1, "invalid argument");
+        hipStream_t st = as_stream(stream);
+        DevBuf blocks;
+        const int64_t nb = detect_max_blocks(rows, cols);
+        blocks.ensure(16 * (size_t)nb);
+        launch_detect_max(d_arr, d_norm, rows, cols, wnd, blocks.p, nb, d_out, st);
+        SPS_HIP(hipStreamSynchronize(st));
+    });
+}
+
+SPS_API sps_status sps_boxcar_snr(const float* d_img, int64_t rows, int64_t cols, int w, float* d_snr, void* stream) {
+    return guarded([&] {
+        SPS_REQUIRE(d_img && d_snr && rows > 0 && cols > 0 && w >= 1, "invalid argument");
+        launch_boxcar_snr(d_img, rows, cols, w, d_snr, as_stream(stream));
+    });
+}
+
+SPS_API sps_status sps_boxcar_snr_adjoint(const float* d_g, int64_t rows, int64_t cols, int w, float* d_gi,
+                                          void* stream) {
+    return guarded([&] {
+        SPS_REQUIRE(d_g && d_gi && rows > 0 && cols > 0 && w >= 1, "invalid argument");
+        launch_boxcar_snr_adjoint(d_g, rows, cols, w, d_gi, as_stream(stream));
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// fused single-beam search
+// ---------------------------------------------------------------------------
+struct sps_search {
+    sps_search_params p{};
+    std::unique_ptr<sps_fdmt> f;     // data transform (nsamps)
+    int64_t ndt = 0;
+    int64_t norm_len = 0;            // samples of the mask transform
+    DevBuf rm, rv, stats, mu, sg, keep, zeros, ones_f;
+    DevBuf norm_img, norm_a, norm_b; // mask transform output and its states
+    DevBuf dmt;                      // internal DM-time image when the caller passes none
+    bool ready = false;
+
+    void setup(hipStream_t st) {
+        if (ready) return;
+        f->ensure_device(st);
+        f->ensure_states();
+        const int64_t R = p.nchans;
+        rm.ensure(sizeof(double) * (size_t)R);
+        rv.ensure(sizeof(double) * (size_t)R);
+        stats.ensure(sizeof(double) * 8);
+        mu.ensure(sizeof(float) * (size_t)R);
+        sg.ensure(sizeof(float) * (size_t)R);
+        keep.ensure((size_t)R);
+        std::vector<float> z((size_t)R, 0.f), o((size_t)R, 1.f);
+        upload(zeros, z, st);
+        upload(ones_f, o, st);
+        if (p.norm_mode == SPS_NORM_NONE) {
+            // no cleaning: every channel kept, unit scale, mask of ones
+            std::vector<uint8_t> k((size_t)R, 1);
+            upload(keep, k, st);
+            upload(mu, z, st);
+            upload(sg, o, st);
+            std::vector<double> s = {0.0, 1.0, 0.0, 0.0, (double)R, 0.0, 0.0, 0.0};
+            upload(stats, s, st);
+        }
+        if (p.snr_norm) {
+            const int64_t L = norm_len;
+            const int64_t Pn = round_up(L, 4);
+            norm_img.ensure(sizeof(float) * (size_t)checked_mul(ndt, L, "norm"));
+            norm_a.ensure(sizeof(float) * (size_t)checked_mul(std::max<int64_t>(f->rows_a, 1), Pn, "norm"));
+            norm_b.ensure(sizeof(float) * (size_t)checked_mul(std::max<int64_t>(f->rows_b, 1), Pn, "norm"));
+        }
+        SPS_HIP(hipStreamSynchronize(st));
+        ready = true;
+    }
+};
+
+extern "C" {
+
+SPS_API sps_status sps_search_create(const sps_search_params* p, sps_search** out) {
+    return guarded([&] {
+        SPS_REQUIRE(p && out, "null argument");
+        *out = nullptr;
+        SPS_REQUIRE(p->in_dtype == SPS_F32 || p->in_dtype == SPS_U8, "unknown dtype");
+        SPS_REQUIRE(p->norm_mode >= SPS_NORM_NONE && p->norm_mode <= SPS_NORM_PER_CHANNEL, "unknown norm mode");
+        SPS_REQUIRE(p->box_q >= 0 && p->box_q <= 13, "box_q out of range");
+        SPS_REQUIRE(p->cand_capacity >= 0, "negative capacity");
+        std::unique_ptr<sps_search> s(new sps_search());
+        s->p = *p;
+        s->f.reset(new sps_fdmt());
+        s->f->plan.reset(new FdmtPlan(p->f_min, p->f_max, p->nchans, p->nsamps, p->tsamp, p->dt_max, 1, 0));
+        s->f->eng = s->f->plan->lower(kInitMean);
+        s->f->init_layout();
+        s->ndt = s->f->eng.out_rows;
+        // the mask transform is causal: samples t < L depend on inputs before
+        // L only, and beyond the longest channel track every kept channel
+        // contributes exactly 1, so the divisor there is the kept count
+        const EnginePlan& e = s->f->eng;
+        std::vector<int64_t> ext((size_t)e.rows[0]);
+        for (int64_t c = 0; c < e.nchans; ++c)
+            for (int64_t k = e.grid_off[(size_t)c]; k < e.grid_off[(size_t)c + 1]; ++k)
+                ext[(size_t)(e.chan_row_off[(size_t)c] + (k - e.grid_off[(size_t)c]))] = e.grid[(size_t)k];
+        for (int i = 1; i <= e.niters; ++i) {
+            std::vector<int64_t> nx((size_t)e.rows[(size_t)i], 0);
+            for (const auto& r : e.iter[(size_t)i]) {
+                int64_t v = ext[(size_t)r.tail_row];
+                if (r.head_row >= 0) v = std::max(v, ext[(size_t)r.head_row] + r.offset);
+                nx[(size_t)r.out_row] = v;
+            }
+            ext.swap(nx);
+        }
+        int64_t mx = 0;
+        for (auto v : ext) mx = std::max(mx, v);
+        s->norm_len = std::min<int64_t>(p->nsamps, mx + 1);
+        *out = s.release();
+    });
+}
+
+SPS_API sps_status sps_search_destroy(sps_search* s) {
+    return guarded([&] { delete s; });
+}
+
+SPS_API sps_status sps_search_get_ndt(const sps_search* s, int64_t* n) {
+    return guarded([&] {
+        SPS_REQUIRE(s && n, "null argument");
+        *n = s->ndt;
+    });
+}
+
+SPS_API sps_status sps_search_run(sps_search* s, const void* d_in, float* d_dmt, sps_cand* d_cands,
+                                  uint64_t* d_count, void* stream) {
+    return guarded([&] {
+        SPS_REQUIRE(s && d_in && d_count, "null argument");
+        SPS_REQUIRE(s->p.cand_capacity == 0 || d_cands, "null candidate buffer");
+        hipStream_t st = as_stream(stream);
+        s->setup(st);
+        const sps_search_params& p = s->p;
+        const int64_t N = p.nsamps;
+        if (p.norm_mode != SPS_NORM_NONE) {
+            launch_row_stats(d_in, p.in_dtype == SPS_U8, p.nchans, N, s->rm.as<double>(), s->rv.as<double>(), st);
+            launch_stats_finalize(s->rm.as<double>(), s->rv.as<double>(), p.nchans, p.norm_mode, s->stats.as<double>(),
+                                  s->mu.as<float>(), s->sg.as<float>(), s->keep.as<uint8_t>(), nullptr, st);
+        }
+        float* out = d_dmt;
+        if (!out) {
+            s->dmt.ensure(sizeof(float) * (size_t)checked_mul(s->ndt, N, "dmt"));
+            out = s->dmt.as<float>();
+        }
+        ChanNorm cn{s->mu.as<float>(), s->sg.as<float>(), s->keep.as<uint8_t>()};
+        const ChanNorm* np = (p.norm_mode != SPS_NORM_NONE) ? &cn : nullptr;
+        s->f->run(d_in, p.in_dtype == SPS_U8, 0, N, np, N, out, N, s->f->state_a.as<float>(), s->f->state_b.as<float>(),
+                  s->f->pitch, st);
+        if (p.snr_norm) {
+            ChanNorm mask{s->zeros.as<float>(), s->ones_f.as<float>(), s->keep.as<uint8_t>()};
+            const int64_t L = s->norm_len;
+            s->f->run(nullptr, 0, 1, N, &mask, L, s->norm_img.as<float>(), L, s->norm_a.as<float>(),
+                      s->norm_b.as<float>(), round_up(L, 4), st);
+            launch_snr_normalise(out, s->ndt, N, N, s->norm_img.as<float>(), L, s->stats.as<double>(), st);
+        }
+        launch_fill_u64(reinterpret_cast<unsigned long long*>(d_count), 1, 0ull, st);
+        launch_boxcar(out, s->ndt, N, N, nullptr, p.threshold, 1 << p.box_q, 1, reinterpret_cast<CandD*>(d_cands),
+                      p.cand_capacity, reinterpret_cast<unsigned long long*>(d_count), st);
+    });
+}
+
+SPS_API sps_status sps_search_sort_candidates(sps_search* s, sps_cand* d_cands, const uint64_t* d_count,
+                                              int64_t* h_count, void* stream) {
+    return guarded([&] {
+        SPS_REQUIRE(s && d_count && h_count, "null argument");
+        hipStream_t st = as_stream(stream);
+        unsigned long long n = 0;
+        SPS_HIP(hipMemcpyAsync(&n, d_count, sizeof(n), hipMemcpyDeviceToHost, st));
+        SPS_HIP(hipStreamSynchronize(st));
+        const int64_t m = std::min<int64_t>((int64_t)n, s->p.cand_capacity);
+        sort_cands(reinterpret_cast<CandD*>(d_cands), m, s->ndt, s->p.nsamps, s->p.box_q, st);
+        *h_count = (int64_t)n;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,113 @@
+// kernels.hpp -- host launchers for the gfx950 kernels (defined in *.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sps {
+
+// init formulas as seen by the kernels (same values as sps_init_mode)
+constexpr int kInitMeanD = 0;
+constexpr int kInitRestrictD = 1;
+constexpr int kInitCFdmtD = 2;
+
+// One merge output row: out[t] = tail[t] + (t >= offset ? head[t - offset] : 0),
+// or a verbatim copy of tail when head < 0.
+struct RowDescD {
+    int64_t out_row;
+    int64_t tail_row;
+    int64_t head_row;
+    int64_t offset;
+};
+
+// Per-channel normalisation applied while the FDMT init reads the waterfall:
+// v = keep ? (x - mu) / sigma : 0 (or keep ? 1 : 0 for the mask transform).
+struct ChanNorm {
+    const float* mu;       // per channel
+    const float* sigma;    // per channel
+    const uint8_t* keep;   // per channel
+};
+
+// FDMT init: rows of state 0 for every channel.
+//   in: nchans x nsamps (f32 or u8; ignored when ones), in_pitch elements
+//   norm: nullptr, or per-channel normalisation
+void launch_fdmt_init(int mode, const void* in, int in_u8, int ones, int64_t in_pitch, const ChanNorm* norm,
+                      int64_t nchans, int64_t nsamps, const int32_t* d_grid, const int64_t* d_grid_off,
+                      const int64_t* d_chan_row_off, float* state, int64_t pitch, hipStream_t st);
+
+// One merge iteration over nrows descriptors; rows with out_row >= out_rows_limit are skipped.
+void launch_fdmt_merge(const float* in, int64_t in_pitch, float* out, int64_t out_pitch, const RowDescD* d_rows,
+                       int64_t nrows, int64_t nsamps, int64_t out_rows_limit, hipStream_t st);
+
+// Adjoint of one merge: g_prev[p][t] = sum over (o, s) in csr[p] of g_out[o][t+s] (t+s < nsamps).
+void launch_fdmt_merge_adjoint(const float* g_out, int64_t out_pitch, float* g_prev, int64_t prev_pitch,
+                               const int64_t* d_csr_off, const int64_t* d_csr_row, const int64_t* d_csr_shift,
+                               int64_t prev_rows, int64_t nsamps, hipStream_t st);
+
+// Adjoint of the (mean) init: g_x[c][i] = sum_k sum_{t=max(i,d_k)}^{min(i+d_k,N-1)} g_s[k][t] / (d_k+1).
+void launch_fdmt_init_adjoint(const float* g_state, int64_t pitch, float* g_x, int64_t x_pitch, int64_t nchans,
+                              int64_t nsamps, const int32_t* d_grid, const int64_t* d_grid_off,
+                              const int64_t* d_chan_row_off, hipStream_t st);
+
+// ---- statistics / normalisation ----
+// row sums in FP64 (fixed lane partition) -> row mean / variance (double)
+void launch_row_stats(const void* in, int in_u8, int64_t rows, int64_t cols, double* d_row_mean, double* d_row_var,
+                      hipStream_t st);
+// global stats + keep flags + per-channel (mu, sigma) floats; d_stats (6 doubles):
+// mean, std, var_mean, var_std, n_keep, unused; d_buff (optional) gets the
+// reference float layout {means[R], vars[R], mean, std, var_mean, var_std}
+void launch_stats_finalize(const double* d_row_mean, const double* d_row_var, int64_t rows, int mode,
+                           double* d_stats, float* d_mu, float* d_sigma, uint8_t* d_keep, float* d_buff,
+                           hipStream_t st);
+void launch_apply_norm(float* x, float* mask, int64_t rows, int64_t cols, const float* d_mu, const float* d_sigma,
+                       const uint8_t* d_keep, hipStream_t st);
+void launch_fdmt_normalization(const float* a, const float* n, int64_t len, float* out, hipStream_t st);
+// S/N normalisation of the DM-time image by the mask transform: for t < norm_len
+// the divisor comes from d_norm (rows x norm_len), beyond it from *d_nkeep.
+void launch_snr_normalise(float* img, int64_t rows, int64_t cols, int64_t pitch, const float* d_norm,
+                          int64_t norm_len, const double* d_stats, hipStream_t st);
+
+// ---- boxcar ----
+struct CandD {
+    int32_t mt, mdt, mwidth;
+    float msnr;
+};
+// Emits every (row, t, width) with window sum / sqrtf(w) > *d_thr (strict), unordered.
+// pow2: widths 1,2,..,wmax (wmax a power of two), mwidth = log2 w; else widths 1..wmax, mwidth = w.
+void launch_boxcar(const float* img, int64_t rows, int64_t cols, int64_t pitch, const float* d_thr, float h_thr,
+                   int wmax, int pow2, CandD* out, int64_t cap, unsigned long long* d_count, hipStream_t st);
+void launch_boxcar_snr(const float* img, int64_t rows, int64_t cols, int w, float* snr, hipStream_t st);
+void launch_boxcar_snr_adjoint(const float* g, int64_t rows, int64_t cols, int w, float* gi, hipStream_t st);
+void launch_detect_max(const float* a, const float* n, int64_t rows, int64_t cols, int wnd, void* d_block_best,
+                       int64_t nblocks_cap, void* d_out, hipStream_t st);
+int64_t detect_max_blocks(int64_t rows, int64_t cols);
+
+// ---- sorting / scans / clustering ----
+// stable LSD radix sort of (key, val) pairs over key bits [0, nbits); the
+// result ends in (keys, vals); tmp buffers of n entries each.
+void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_tmp, uint32_t* vals_tmp, int64_t n, int nbits,
+                      uint32_t* d_hist_scratch, int64_t hist_cap, hipStream_t st);
+int64_t radix_hist_entries(int64_t n);
+// inclusive scan of n uint32 values in place; scratch of scan_scratch_entries(n)
+void scan_inclusive_u32(uint32_t* data, int64_t n, uint32_t* scratch, hipStream_t st);
+int64_t scan_scratch_entries(int64_t n);
+
+void launch_cand_keys(const CandD* c, int64_t n, int bits_t, int bits_w, uint64_t* keys, uint32_t* vals,
+                      hipStream_t st);
+void launch_cand_from_keys(const uint64_t* keys, const uint32_t* vals, int64_t n, int bits_t, int bits_w,
+                           CandD* out, hipStream_t st);
+void launch_cluster_keys(const CandD* c, const uint32_t* perm, const uint32_t* group, int64_t n, int member,
+                         uint64_t* keys, uint32_t* vals, hipStream_t st);
+void launch_cluster_flags(const uint64_t* keys, int64_t n, const int32_t* d_bins, int level, uint32_t* flags,
+                          hipStream_t st);
+void launch_cluster_delegates(const CandD* c, const uint32_t* perm, const uint32_t* gid, const uint32_t* flags,
+                              int64_t n, unsigned long long* best, uint32_t* start, hipStream_t st);
+void launch_delegate_keys(const CandD* c, const unsigned long long* best, const uint32_t* start, int64_t ng,
+                          int64_t n, uint64_t* keys, uint32_t* vals, hipStream_t st);
+void launch_delegate_out(const unsigned long long* best, const uint32_t* start, const uint32_t* order, int64_t ng,
+                         int64_t n, int32_t* delegates, int64_t* sizes, hipStream_t st);
+void launch_iota_u32(uint32_t* v, int64_t n, hipStream_t st);
+void launch_fill_u64(unsigned long long* v, int64_t n, unsigned long long val, hipStream_t st);
+
+}  // namespace sps

@@ -1,0 +1,426 @@
+// search_kernels.hip -- statistics/cleaning, S/N normalisation, boxcar
+// candidate gathering and the single-maximum detector for gfx950.
+//
+// Statistics use FP64 accumulators with a fixed lane partition and a fixed
+// reduction tree, so results do not depend on launch shape or timing:
+//   row sums   : element t goes to lane (t / 16) % 256, lanes accumulate in
+//                increasing t, then pairwise tree over the 256 lanes;
+//   cross-row  : row r goes to lane r % 256, same tree.
+// For integer (u8) input every partial sum is an exact integer in FP64.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace sps {
+
+namespace {
+
+constexpr int kLanes = 256;
+constexpr int kChunk = 16;
+
+__device__ __forceinline__ void tree_reduce2(double* a, double* b) {
+    for (int s = kLanes / 2; s > 0; s >>= 1) {
+        __syncthreads();
+        if ((int)threadIdx.x < s) {
+            a[threadIdx.x] = a[threadIdx.x] + a[threadIdx.x + s];
+            b[threadIdx.x] = b[threadIdx.x] + b[threadIdx.x + s];
+        }
+    }
+    __syncthreads();
+}
+
+template <bool U8>
+__global__ void __launch_bounds__(kLanes) k_row_stats(const void* __restrict__ in, int64_t cols,
+                                                      double* __restrict__ row_mean, double* __restrict__ row_var) {
+    __shared__ double s1[kLanes], s2[kLanes];
+    const int64_t r = blockIdx.x;
+    double a = 0.0, b = 0.0;
+    const int64_t nchunks = ceil_div(cols, kChunk);
+    for (int64_t ch = threadIdx.x; ch < nchunks; ch += kLanes) {
+        const int64_t t0 = ch * kChunk;
+        if (U8) {
+            const uint8_t* x = static_cast<const uint8_t*>(in) + r * cols;
+            if (t0 + kChunk <= cols && ((reinterpret_cast<uintptr_t>(x + t0) & 15) == 0)) {
+                const uint4 v = *reinterpret_cast<const uint4*>(x + t0);
+                const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const double d = (double)((w[q] >> (8 * e)) & 0xffu);
+                        a += d;
+                        b += d * d;
+                    }
+            } else {
+                for (int64_t t = t0; t < t0 + kChunk && t < cols; ++t) {
+                    const double d = (double)x[t];
+                    a += d;
+                    b += d * d;
+                }
+            }
+        } else {
+            const float* x = static_cast<const float*>(in) + r * cols;
+            for (int64_t t = t0; t < t0 + kChunk && t < cols; ++t) {
+                const double d = (double)x[t];
+                a += d;
+                b += d * d;
+            }
+        }
+    }
+    s1[threadIdx.x] = a;
+    s2[threadIdx.x] = b;
+    tree_reduce2(s1, s2);
+    if (threadIdx.x == 0) {
+        const double m = s1[0] / (double)cols;
+        row_mean[r] = m;
+        row_var[r] = s2[0] / (double)cols - m * m;
+    }
+}
+
+__global__ void __launch_bounds__(kLanes) k_stats_finalize(const double* __restrict__ rm, const double* __restrict__ rv,
+                                                           int64_t rows, int mode, double* __restrict__ stats,
+                                                           float* __restrict__ mu, float* __restrict__ sigma,
+                                                           uint8_t* __restrict__ keep, float* __restrict__ buff) {
+    __shared__ double a[kLanes], b[kLanes], c[kLanes], d[kLanes];
+    __shared__ double res[5];
+    double sa = 0.0, sb = 0.0, sc = 0.0, sd = 0.0;
+    for (int64_t r = threadIdx.x; r < rows; r += kLanes) {
+        sa = sa + rm[r];
+        sb = sb + (rv[r] + rm[r] * rm[r]);
+        sc = sc + rv[r];
+        sd = sd + rv[r] * rv[r];
+    }
+    a[threadIdx.x] = sa;
+    b[threadIdx.x] = sb;
+    c[threadIdx.x] = sc;
+    d[threadIdx.x] = sd;
+    tree_reduce2(a, b);
+    tree_reduce2(c, d);
+    if (threadIdx.x == 0) {
+        const double mean = a[0] / (double)rows;
+        const double sdv = sqrt(b[0] / (double)rows - mean * mean);
+        const double vm = c[0] / (double)rows;
+        const double vs = sqrt(d[0] / (double)rows - vm * vm);
+        res[0] = mean;
+        res[1] = sdv;
+        res[2] = vm;
+        res[3] = vs;
+    }
+    __syncthreads();
+    uint32_t nk = 0;
+    for (int64_t r = threadIdx.x; r < rows; r += kLanes) {
+        bool drop = fabs(rv[r] - res[2]) > 4.0 * res[3];
+        float m, s;
+        if (mode == 2) {
+            if (!(rv[r] > 0.0)) drop = true;
+            m = (float)rm[r];
+            s = (float)sqrt(rv[r]);
+        } else {
+            m = (float)res[0];
+            s = (float)res[1];
+        }
+        keep[r] = drop ? 0 : 1;
+        mu[r] = m;
+        sigma[r] = s;
+        nk += drop ? 0u : 1u;
+        if (buff) {
+            buff[r] = (float)rm[r];
+            buff[rows + r] = (float)rv[r];
+        }
+    }
+    __shared__ uint32_t cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    atomicAdd(&cnt, nk);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        stats[0] = res[0];
+        stats[1] = res[1];
+        stats[2] = res[2];
+        stats[3] = res[3];
+        stats[4] = (double)cnt;
+        stats[5] = 0.0;
+        if (buff) {
+            buff[2 * rows + 0] = (float)res[0];
+            buff[2 * rows + 1] = (float)res[1];
+            buff[2 * rows + 2] = (float)res[2];
+            buff[2 * rows + 3] = (float)res[3];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) k_apply_norm(float* __restrict__ x, float* __restrict__ mask, int64_t cols,
+                                                         const float* __restrict__ mu, const float* __restrict__ sigma,
+                                                         const uint8_t* __restrict__ keep) {
+    const int64_t r = blockIdx.y;
+    const bool k = keep[r] != 0;
+    const float m = mu[r], s = sigma[r];
+    for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < cols; t += (int64_t)gridDim.x * kThreads) {
+        const int64_t i = r * cols + t;
+        x[i] = k ? (x[i] - m) / s : 0.0f;
+        if (mask) mask[i] = k ? 1.0f : 0.0f;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) k_fdmt_normalization(const float* __restrict__ a,
+                                                                 const float* __restrict__ n, int64_t len,
+                                                                 float* __restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < len; i += (int64_t)gridDim.x * kThreads)
+        out[i] = a[i] / sqrtf((float)((double)n[i] + 1.0E-8));
+}
+
+__global__ void __launch_bounds__(kThreads) k_snr_normalise(float* __restrict__ img, int64_t cols, int64_t pitch,
+                                                            const float* __restrict__ norm, int64_t norm_len,
+                                                            const double* __restrict__ stats) {
+    const int64_t r = blockIdx.y;
+    const float nk = (float)stats[4];
+    const float dk = sqrtf((float)((double)nk + 1.0E-8));
+    for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < cols; t += (int64_t)gridDim.x * kThreads) {
+        float* p = img + r * pitch + t;
+        if (t < norm_len)
+            *p = *p / sqrtf((float)((double)norm[r * norm_len + t] + 1.0E-8));
+        else
+            *p = *p / dk;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// boxcar: exact left-to-right window sums (Clusterization.cu:419-462,591-671)
+// ---------------------------------------------------------------------------
+constexpr int kBoxCells = 1024;  // cells per block (4 per thread)
+
+template <bool POW2>
+__global__ void __launch_bounds__(kThreads) k_boxcar(const float* __restrict__ img, int64_t cols, int64_t pitch,
+                                                     const float* __restrict__ d_thr, float h_thr, int wmax,
+                                                     CandD* __restrict__ out, int64_t cap,
+                                                     unsigned long long* __restrict__ count) {
+    extern __shared__ float tile[];
+    const int64_t r = blockIdx.y;
+    const int64_t T = (int64_t)blockIdx.x * kBoxCells;
+    const int64_t span = min((int64_t)kBoxCells + wmax - 1, cols - T);
+    const float* row = img + r * pitch;
+    for (int64_t i = threadIdx.x; i < span; i += kThreads) tile[i] = row[T + i];
+    __syncthreads();
+    const float thr = d_thr ? *d_thr : h_thr;
+    for (int j = 0; j < kBoxCells / kThreads; ++j) {
+        const int64_t lt = (int64_t)j * kThreads + threadIdx.x;
+        const int64_t t = T + lt;
+        if (t >= cols) break;
+        const int64_t wlim = min((int64_t)wmax, cols - t);
+        float s = 0.0f;
+        int p = 0;
+        int64_t next = 1;
+        for (int64_t iw = 0; iw < wlim; ++iw) {
+            s += tile[lt + iw];
+            if (POW2) {
+                if (iw + 1 != next) continue;
+                next *= 2;
+            }
+            const float snr = s / sqrtf((float)(iw + 1));
+            if (snr > thr) {
+                const unsigned long long k = atomicAdd(count, 1ull);
+                if ((int64_t)k < cap) {
+                    CandD c;
+                    c.mt = (int32_t)t;
+                    c.mdt = (int32_t)r;
+                    c.mwidth = POW2 ? p : (int32_t)(iw + 1);
+                    c.msnr = snr;
+                    out[k] = c;
+                }
+            }
+            if (POW2) ++p;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) k_boxcar_snr(const float* __restrict__ img, int64_t cols, int w,
+                                                         float* __restrict__ snr) {
+    const int64_t r = blockIdx.y;
+    const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (t >= cols) return;
+    float v = 0.0f;
+    if (t + w <= cols) {
+        float s = 0.0f;
+        for (int j = 0; j < w; ++j) s += img[r * cols + t + j];
+        v = s / sqrtf((float)w);
+    }
+    snr[r * cols + t] = v;
+}
+
+__global__ void __launch_bounds__(kThreads) k_boxcar_snr_adjoint(const float* __restrict__ g, int64_t cols, int w,
+                                                                 float* __restrict__ gi) {
+    const int64_t r = blockIdx.y;
+    const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (t >= cols) return;
+    const int64_t lo = t - w + 1 > 0 ? t - w + 1 : 0;
+    const int64_t hi = t < cols - w ? t : cols - w;
+    float s = 0.0f;
+    for (int64_t k = lo; k <= hi; ++k) s += g[r * cols + k];
+    gi[r * cols + t] = s / sqrtf((float)w);
+}
+
+// ---------------------------------------------------------------------------
+// single-maximum detector
+// ---------------------------------------------------------------------------
+struct Best {
+    float snr;
+    int32_t w, r, t;
+};
+
+__device__ __forceinline__ bool better(const Best& a, const Best& b) {
+    if (a.w < 0) return false;
+    if (b.w < 0) return true;
+    if (a.snr != b.snr) return a.snr > b.snr;
+    if (a.w != b.w) return a.w < b.w;
+    if (a.r != b.r) return a.r < b.r;
+    return a.t < b.t;
+}
+
+__device__ void block_best(Best v, Best* sh) {
+    sh[threadIdx.x] = v;
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        __syncthreads();
+        if ((int)threadIdx.x < s && better(sh[threadIdx.x + s], sh[threadIdx.x])) sh[threadIdx.x] = sh[threadIdx.x + s];
+    }
+    __syncthreads();
+}
+
+constexpr int kDetCells = 256;
+
+__global__ void __launch_bounds__(kThreads) k_detect_blocks(const float* __restrict__ a, const float* __restrict__ n,
+                                                            int64_t cols, int wnd, Best* __restrict__ blocks) {
+    __shared__ Best sh[kThreads];
+    const int64_t r = blockIdx.y;
+    const int64_t t = (int64_t)blockIdx.x * kDetCells + threadIdx.x;
+    Best b{0.0f, -1, -1, -1};
+    if (t < cols) {
+        float s = 0.0f;
+        for (int w = 1; w <= wnd && t + w <= cols; ++w) {
+            float v = a[r * cols + t + w - 1];
+            if (n) v = v / sqrtf((float)(1.0E-7 + (double)n[r * cols + t + w - 1]));
+            s += v;
+            const float val = s / sqrtf((float)w);
+            Best c{val, w, (int32_t)r, (int32_t)t};
+            if (better(c, b)) b = c;
+        }
+    }
+    block_best(b, sh);
+    if (threadIdx.x == 0) blocks[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
+}
+
+__global__ void __launch_bounds__(kThreads) k_detect_final(const Best* __restrict__ blocks, int64_t nb,
+                                                           sps_detection* __restrict__ out) {
+    __shared__ Best sh[kThreads];
+    Best b{0.0f, -1, -1, -1};
+    for (int64_t i = threadIdx.x; i < nb; i += kThreads)
+        if (better(blocks[i], b)) b = blocks[i];
+    block_best(b, sh);
+    if (threadIdx.x == 0) {
+        out->iwidth = sh[0].w;
+        out->irow = sh[0].r;
+        out->icol = sh[0].t;
+        out->snr = sh[0].snr;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+void launch_row_stats(const void* in, int in_u8, int64_t rows, int64_t cols, double* d_row_mean, double* d_row_var,
+                      hipStream_t st) {
+    if (rows <= 0 || cols <= 0) return;
+    if (rows > 0x7fffffffLL) throw Error(SPS_ERR_INVALID_ARGUMENT, "too many rows");
+    if (in_u8)
+        hipLaunchKernelGGL((k_row_stats<true>), dim3((uint32_t)rows), dim3(kLanes), 0, st, in, cols, d_row_mean,
+                           d_row_var);
+    else
+        hipLaunchKernelGGL((k_row_stats<false>), dim3((uint32_t)rows), dim3(kLanes), 0, st, in, cols, d_row_mean,
+                           d_row_var);
+    SPS_HIP(hipGetLastError());
+}
+
+void launch_stats_finalize(const double* d_row_mean, const double* d_row_var, int64_t rows, int mode, double* d_stats,
+                           float* d_mu, float* d_sigma, uint8_t* d_keep, float* d_buff, hipStream_t st) {
+    hipLaunchKernelGGL(k_stats_finalize, dim3(1), dim3(kLanes), 0, st, d_row_mean, d_row_var, rows, mode, d_stats,
+                       d_mu, d_sigma, d_keep, d_buff);
+    SPS_HIP(hipGetLastError());
+}
+
+static dim3 row_grid(int64_t rows, int64_t cols) {
+    int64_t bx = ceil_div(cols, kThreads);
+    if (bx > 1024) bx = 1024;
+    return dim3((uint32_t)bx, (uint32_t)rows);
+}
+
+void launch_apply_norm(float* x, float* mask, int64_t rows, int64_t cols, const float* d_mu, const float* d_sigma,
+                       const uint8_t* d_keep, hipStream_t st) {
+    if (rows <= 0 || cols <= 0) return;
+    if (rows > 65535) throw Error(SPS_ERR_INVALID_ARGUMENT, "more than 65535 rows");
+    hipLaunchKernelGGL(k_apply_norm, row_grid(rows, cols), dim3(kThreads), 0, st, x, mask, cols, d_mu, d_sigma,
+                       d_keep);
+    SPS_HIP(hipGetLastError());
+}
+
+void launch_fdmt_normalization(const float* a, const float* n, int64_t len, float* out, hipStream_t st) {
+    if (len <= 0) return;
+    int64_t nb = ceil_div(len, kThreads);
+    if (nb > 65536) nb = 65536;
+    hipLaunchKernelGGL(k_fdmt_normalization, dim3((uint32_t)nb), dim3(kThreads), 0, st, a, n, len, out);
+    SPS_HIP(hipGetLastError());
+}
+
+void launch_snr_normalise(float* img, int64_t rows, int64_t cols, int64_t pitch, const float* d_norm,
+                          int64_t norm_len, const double* d_stats, hipStream_t st) {
+    if (rows <= 0 || cols <= 0) return;
+    if (rows > 65535) throw Error(SPS_ERR_INVALID_ARGUMENT, "more than 65535 rows");
+    hipLaunchKernelGGL(k_snr_normalise, row_grid(rows, cols), dim3(kThreads), 0, st, img, cols, pitch, d_norm,
+                       norm_len, d_stats);
+    SPS_HIP(hipGetLastError());
+}
+
+void launch_boxcar(const float* img, int64_t rows, int64_t cols, int64_t pitch, const float* d_thr, float h_thr,
+                   int wmax, int pow2, CandD* out, int64_t cap, unsigned long long* d_count, hipStream_t st) {
+    if (rows <= 0 || cols <= 0 || wmax <= 0) return;
+    if (wmax > 8192) throw Error(SPS_ERR_INVALID_ARGUMENT, "window wider than 8192 samples");
+    if (rows > 65535) throw Error(SPS_ERR_INVALID_ARGUMENT, "more than 65535 rows");
+    const dim3 grid((uint32_t)ceil_div(cols, kBoxCells), (uint32_t)rows);
+    const size_t lds = sizeof(float) * (size_t)(kBoxCells + wmax);
+    if (pow2)
+        hipLaunchKernelGGL((k_boxcar<true>), grid, dim3(kThreads), lds, st, img, cols, pitch, d_thr, h_thr, wmax, out,
+                           cap, d_count);
+    else
+        hipLaunchKernelGGL((k_boxcar<false>), grid, dim3(kThreads), lds, st, img, cols, pitch, d_thr, h_thr, wmax, out,
+                           cap, d_count);
+    SPS_HIP(hipGetLastError());
+}
+
+void launch_boxcar_snr(const float* img, int64_t rows, int64_t cols, int w, float* snr, hipStream_t st) {
+    if (rows <= 0 || cols <= 0) return;
+    if (rows > 65535) throw Error(SPS_ERR_INVALID_ARGUMENT, "more than 65535 rows");
+    hipLaunchKernelGGL(k_boxcar_snr, dim3((uint32_t)ceil_div(cols, kThreads), (uint32_t)rows), dim3(kThreads), 0, st,
+                       img, cols, w, snr);
+    SPS_HIP(hipGetLastError());
+}
+
+void launch_boxcar_snr_adjoint(const float* g, int64_t rows, int64_t cols, int w, float* gi, hipStream_t st) {
+    if (rows <= 0 || cols <= 0) return;
+    if (rows > 65535) throw Error(SPS_ERR_INVALID_ARGUMENT, "more than 65535 rows");
+    hipLaunchKernelGGL(k_boxcar_snr_adjoint, dim3((uint32_t)ceil_div(cols, kThreads), (uint32_t)rows), dim3(kThreads),
+                       0, st, g, cols, w, gi);
+    SPS_HIP(hipGetLastError());
+}
+
+int64_t detect_max_blocks(int64_t rows, int64_t cols) { return ceil_div(cols, kDetCells) * rows; }
+
+void launch_detect_max(const float* a, const float* n, int64_t rows, int64_t cols, int wnd, void* d_block_best,
+                       int64_t nblocks_cap, void* d_out, hipStream_t st) {
+    const int64_t nb = detect_max_blocks(rows, cols);
+    if (nb > nblocks_cap) throw Error(SPS_ERR_INVALID_ARGUMENT, "detector scratch too small");
+    if (rows > 65535) throw Error(SPS_ERR_INVALID_ARGUMENT, "more than 65535 rows");
+    hipLaunchKernelGGL(k_detect_blocks, dim3((uint32_t)ceil_div(cols, kDetCells), (uint32_t)rows), dim3(kThreads), 0,
+                       st, a, n, cols, wnd, static_cast<Best*>(d_block_best));
+    hipLaunchKernelGGL(k_detect_final, dim3(1), dim3(kThreads), 0, st, static_cast<const Best*>(d_block_best), nb,
+                       static_cast<sps_detection*>(d_out));
+    SPS_HIP(hipGetLastError());
+}
+
+}  // namespace sps

@@ -1,0 +1,189 @@
+"""Python mirror of the reference normalisation, detection and clustering
+calls over the C ABI (statistical_preprocessing::, fdmt_normalization,
+clusterization::, select_delegates::, doDetectSignal).  Device buffers are
+torch tensors; candidate arrays are int32 tensors of shape (n, 4) holding
+Cand{mt, mdt, mwidth, msnr} (Candidate.cuh:4-10) with msnr's bits in
+column 3.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import SPS_NORM_GLOBAL, SPS_NORM_NONE, SPS_NORM_PER_CHANNEL, SearchParams, call  # noqa: F401
+from .fdmt import _dptr, _stream
+
+CAND_DTYPE = np.dtype([("mt", np.int32), ("mdt", np.int32), ("mwidth", np.int32), ("msnr", np.float32)])
+
+
+def cands_to_numpy(t, n=None):
+    """(n, 4) int32 device/host tensor -> numpy structured array."""
+    a = t.detach().cpu().numpy() if hasattr(t, "detach") else np.asarray(t)
+    if n is not None:
+        a = a[:n]
+    return np.ascontiguousarray(a, np.int32).view(CAND_DTYPE).reshape(-1)
+
+
+def numpy_to_cands(c, device="cuda"):
+    import torch
+
+    a = np.ascontiguousarray(c, CAND_DTYPE).view(np.int32).reshape(-1, 4)
+    return torch.from_numpy(a.copy()).to(device)
+
+
+class statistical_preprocessing:
+    @staticmethod
+    def calc_fdmt_inputs(d_in, nrows, ncols, d_buff, d_norm_input, mode=SPS_NORM_GLOBAL):
+        """statistical_preprocessing::calc_fdmt_inputs (Statistical_preprocessing.cuh:9-10)."""
+        call("sps_calc_fdmt_inputs", _dptr(d_in), int(nrows), int(ncols), _dptr(d_buff), _dptr(d_norm_input),
+             int(mode), _stream())
+
+
+def fdmt_normalization(d_arr, d_norm, length, d_out):
+    """fdmt_normalization (CDMT/Chunk_gpu.cu:841-852)."""
+    call("sps_fdmt_normalization", _dptr(d_arr), _dptr(d_norm), int(length), _dptr(d_out), _stream())
+
+
+def _gather(d_fdmt, rows, cols, d_thr, wnd, all_widths, capacity=None):
+    import torch
+
+    cap = int(capacity) if capacity is not None else 1 << 16
+    while True:
+        out = torch.empty((max(cap, 1), 4), dtype=torch.int32, device=d_fdmt.device)
+        n = C.c_int64()
+        st = _lib.lib().sps_boxcar_candidates(_dptr(d_fdmt), int(rows), int(cols), _dptr(d_thr), int(wnd),
+                                              int(all_widths), _dptr(out), cap, C.byref(n), _stream())
+        if st == _lib.SPS_ERR_CAPACITY and capacity is None:
+            cap = n.value
+            continue
+        _lib.check(st)
+        return out[: n.value], n.value
+
+
+class clusterization:
+    @staticmethod
+    def gather_candidates_in_dynamicalArray(d_arr_fdmt, rows, cols, val_thresh, quant_power2_wnd):
+        """Clusterization.cuh:22-23: widths 1..2^q, mwidth = log2(w).  Returns
+        (candidates (n, 4) int32 device tensor, n); order (mdt, mt, mwidth)."""
+        return _gather(d_arr_fdmt, rows, cols, val_thresh, quant_power2_wnd, 0)
+
+    @staticmethod
+    def gather_candidates_in_heap_with_plan(d_fdmt, rows, cols, d_val_thresh, wnd_width):
+        """Clusterization.cuh:56-63: widths 1..W, mwidth = w, row-major order.
+        Returns (ok, candidates, n) with ok False when there are none."""
+        c, n = _gather(d_fdmt, rows, cols, d_val_thresh, wnd_width, 1)
+        return n > 0, c, n
+
+    @staticmethod
+    def cluster(d_cands, n, d_bins):
+        """Three-key gap clustering + delegates (Clusterization.cu:92-166).
+        Returns (delegates int32 device tensor sorted by mt, group sizes int64)."""
+        import torch
+
+        dev = d_cands.device
+        dele = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+        gsz = torch.empty(max(n, 1), dtype=torch.int64, device=dev)
+        ng = C.c_int64()
+        call("sps_cluster", _dptr(d_cands) if n else None, int(n), _dptr(d_bins), _dptr(dele), _dptr(gsz),
+             C.byref(ng), _stream())
+        return dele[: ng.value], gsz[: ng.value]
+
+    @staticmethod
+    def complete_clusterization(d_cands, n, d_bins, filename, delt_bin, t_chunk_begin, bwrite=True):
+        """Clusterization.cuh:25-32."""
+        dele, gsz = clusterization.cluster(d_cands, n, d_bins)
+        if bwrite:
+            select_delegates.write_log(cands_to_numpy(d_cands, n), dele.cpu().numpy(), gsz.cpu().numpy(), filename,
+                                       delt_bin, t_chunk_begin)
+        return dele, gsz
+
+    @staticmethod
+    def clusterization_main(d_fdmt, rows, cols, d_val_thresh, wnd_width, d_bins, filename, delt_bin, t_chunk_begin,
+                            bwrite=True):
+        """Clusterization.cuh:10-20: all-widths gather, clustering, log.
+        Returns False when no candidate passes the threshold."""
+        ok, c, n = clusterization.gather_candidates_in_heap_with_plan(d_fdmt, rows, cols, d_val_thresh, wnd_width)
+        if not ok:
+            return False
+        clusterization.complete_clusterization(c, n, d_bins, filename, delt_bin, t_chunk_begin, bwrite)
+        return True
+
+    @staticmethod
+    def writeCandHostArrayToCSV(h_cands, filename):
+        """Clusterization.cu:338-358: header mt,mdt,mwidth,msnr."""
+        with open(filename, "w") as f:
+            f.write("mt,mdt,mwidth,msnr\n")
+            for c in h_cands:
+                f.write(f"{int(c['mt'])},{int(c['mdt'])},{int(c['mwidth'])},{float(c['msnr']):g}\n")
+
+
+class select_delegates:
+    @staticmethod
+    def write_log(h_cands, delegates, group_sizes, filename, delt_bin, t_chunk_begin):
+        """select_delegates::write_log (select_delegates.cu:127-177): columns
+        N, t bin, t sec, dedisp, width, SNR, num, each 10 wide; num is the
+        delegate's own group size."""
+        with open(filename, "w") as f:
+            hdr = ["N", "t, bin", "t, sec", "dedisp", "width", "SNR", "num"]
+            f.write("".join(f"{h:>10}" for h in hdr) + "\n")
+            for i, (j, num) in enumerate(zip(delegates, group_sizes)):
+                c = h_cands[int(j)]
+                tsec = np.float32(t_chunk_begin) + np.float32(int(c["mt"])) * np.float32(delt_bin)
+                cols = [str(i + 1), str(int(c["mt"])), f"{float(tsec):g}", str(int(c["mdt"])), str(int(c["mwidth"])),
+                        f"{float(c['msnr']):g}", str(int(num))]
+                f.write("".join(f"{v:>10}" for v in cols) + "\n")
+
+
+def detect_max(d_arr, d_norm, rows, cols, wnd):
+    """doDetectSignal_v1 (detection/kernel.cu:561-582) -> (iwidth, irow, icol, snr)."""
+    import torch
+
+    out = torch.zeros(4, dtype=torch.int32, device=d_arr.device)
+    call("sps_detect_max", _dptr(d_arr), _dptr(d_norm), int(rows), int(cols), int(wnd), _dptr(out), _stream())
+    o = out.cpu()
+    return int(o[0]), int(o[1]), int(o[2]), float(o[3:4].view(torch.float32)[0])
+
+
+def boxcar_snr(d_img, rows, cols, w, d_snr):
+    call("sps_boxcar_snr", _dptr(d_img), int(rows), int(cols), int(w), _dptr(d_snr), _stream())
+
+
+def boxcar_snr_adjoint(d_g, rows, cols, w, d_gi):
+    call("sps_boxcar_snr_adjoint", _dptr(d_g), int(rows), int(cols), int(w), _dptr(d_gi), _stream())
+
+
+class SingleBeamSearch:
+    """The CChunk_gpu::process hot section (CDMT/Chunk_gpu.cu:325-425) for one
+    beam: normalise -> FDMT -> S/N normalisation -> boxcar gather."""
+
+    def __init__(self, f_min, f_max, nchans, nsamps, dt_max, tsamp=1.0, in_dtype="u8", norm_mode=SPS_NORM_GLOBAL,
+                 snr_norm=True, box_q=8, threshold=6.0, cand_capacity=1 << 20):
+        p = SearchParams()
+        p.f_min, p.f_max, p.tsamp = f_min, f_max, tsamp
+        p.nchans, p.nsamps, p.dt_max = nchans, nsamps, dt_max
+        p.in_dtype = _lib.SPS_U8 if in_dtype == "u8" else _lib.SPS_F32
+        p.norm_mode, p.snr_norm, p.box_q = norm_mode, int(snr_norm), box_q
+        p.threshold, p.cand_capacity = threshold, cand_capacity
+        self.params = p
+        self._h = C.c_void_p()
+        call("sps_search_create", C.byref(p), C.byref(self._h))
+        n = C.c_int64()
+        call("sps_search_get_ndt", self._h, C.byref(n))
+        self.ndt = n.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().sps_search_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def run(self, d_in, d_dmt, d_cands, d_count):
+        call("sps_search_run", self._h, _dptr(d_in), _dptr(d_dmt), _dptr(d_cands), _dptr(d_count), _stream())
+
+    def sort_candidates(self, d_cands, d_count):
+        n = C.c_int64()
+        call("sps_search_sort_candidates", self._h, _dptr(d_cands), _dptr(d_count), C.byref(n), _stream())
+        return n.value

@@ -1,0 +1,233 @@
+/*
+ * sps.h -- C ABI of the MI355X single-pulse-search library (libsps.so).
+ *
+ * Plain C: no C++ or torch types cross this boundary.  Pointers named d_*
+ * are HIP device pointers, h_* are host pointers.  Every call returns an
+ * sps_status; sps_last_error() returns the message of the calling thread's
+ * last failure.  Work is ordered on the stream argument (a hipStream_t passed
+ * as void*, NULL = the default stream); calls that must return a host-side
+ * count say so and synchronise that stream.  A handle is used by one host
+ * thread at a time.
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the reference checkout).
+ */
+#ifndef SPS_H
+#define SPS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPS_API __attribute__((visibility("default")))
+
+typedef enum {
+    SPS_OK = 0,
+    SPS_ERR_INVALID_ARGUMENT = 1, /* std::invalid_argument in the reference (fdmt_base.cpp:107-117) */
+    SPS_ERR_RUNTIME = 2,          /* std::runtime_error in the reference (fdmt_base.cpp:235-236,254-256) */
+    SPS_ERR_HIP = 3,              /* a HIP runtime call failed */
+    SPS_ERR_NO_MEMORY = 4,        /* device or host allocation failed */
+    SPS_ERR_CAPACITY = 5,         /* more candidates than the output capacity (count still reported) */
+    SPS_ERR_NO_DEVICE = 6         /* no HIP device is visible */
+} sps_status;
+
+typedef enum { SPS_F32 = 0, SPS_U8 = 1 } sps_dtype;
+
+/* FDMT initialisation formulas. */
+typedef enum {
+    SPS_INIT_MEAN = 0,     /* running mean, FDMT_PRAVIR/fdmt_cpu.cpp:103-179 */
+    SPS_INIT_RESTRICT = 1, /* prev*((dp+1)+sum)/(dc+1), FDMT_PRAVIR_RESTRICT/fdmt_cpu.cpp:65-121 */
+    SPS_INIT_CFDMT = 2     /* CFdmtCpu::fnc_initC, FDMT/FdmtCpu.cpp:314-367 (legacy plans only) */
+} sps_init_mode;
+
+/* Normalisation of the waterfall before the FDMT. */
+typedef enum {
+    SPS_NORM_NONE = 0,
+    SPS_NORM_GLOBAL = 1,     /* reference: global mean/std + outlier-variance rows zeroed
+                                (CDMT/Statistical_preprocessing.cu:15-88,510-539) */
+    SPS_NORM_PER_CHANNEL = 2 /* per-channel mean/std, same row cleaning */
+} sps_norm_mode;
+
+/* Candidate record, Clusterization/Candidate.cuh:4-10 (16 bytes). */
+typedef struct {
+    int32_t mt;     /* time sample of the window start */
+    int32_t mdt;    /* DM-trial row */
+    int32_t mwidth; /* log2(width) in power-of-two mode, width in all-widths mode */
+    float msnr;     /* window sum / sqrtf(width) */
+} sps_cand;
+
+/* Single-maximum detector result, detection/kernel.cu:14-20. */
+typedef struct {
+    int32_t iwidth;
+    int32_t irow;
+    int32_t icol;
+    float snr;
+} sps_detection;
+
+typedef struct sps_fdmt sps_fdmt;
+
+/* ---------------------------------------------------------------------- */
+/* library                                                                 */
+/* ---------------------------------------------------------------------- */
+SPS_API const char* sps_last_error(void);
+SPS_API int sps_version(void);
+/* number of visible HIP devices (0 on a host without a GPU) */
+SPS_API sps_status sps_device_count(int* h_count);
+
+/* ---------------------------------------------------------------------- */
+/* plan-based FDMT: FDMT / FDMTGPU (FDMT_PRAVIR_RESTRICT/fdmt_base.hpp:37-85, */
+/* fdmt_gpu.cuh:68-89).  Plan building is host-only; device state is       */
+/* allocated on first use.                                                 */
+/* ---------------------------------------------------------------------- */
+
+/* FDMT::FDMT(f_min, f_max, nchans, nsamps, tsamp, dt_max, dt_step, dt_min)
+ * (fdmt_base.cpp:37-53) + configure_fdmt_plan (fdmt_base.cpp:119-279). */
+SPS_API sps_status sps_fdmt_create(float f_min, float f_max, int64_t nchans, int64_t nsamps, float tsamp,
+                                   int64_t dt_max, int64_t dt_step, int64_t dt_min, int init_mode,
+                                   sps_fdmt** out);
+
+/* CFdmtGpu::CFdmtGpu(Fmin, Fmax, nchan, cols, imaxDt) (FDMT/FdmtGpu.cu:230-270,
+ * config from FDMT/FdmtCpu.cpp:143-194). */
+SPS_API sps_status sps_cfdmt_create(float f_min, float f_max, int nchan, int cols, int imaxdt, sps_fdmt** out);
+
+SPS_API sps_status sps_fdmt_destroy(sps_fdmt* h);
+
+/* FDMT::get_niters (fdmt_base.cpp:60); CFdmtB m_iNumIter */
+SPS_API sps_status sps_fdmt_get_niters(const sps_fdmt* h, int64_t* h_niters);
+/* plan.state_shape[i] = {nchans, ndt_min, ndt_max, sum ndt, nsamps} (fdmt_base.hpp:10-11) */
+SPS_API sps_status sps_fdmt_get_state_shape(const sps_fdmt* h, int64_t iter, int64_t* h_shape5);
+/* rows of the caller's output (dmt) and samples per row */
+SPS_API sps_status sps_fdmt_get_output_shape(const sps_fdmt* h, int64_t* h_rows, int64_t* h_cols);
+/* FDMT::get_dt_grid_final (fdmt_base.cpp:62-64); h_n receives the length */
+SPS_API sps_status sps_fdmt_get_dt_grid_final(const sps_fdmt* h, int64_t* h_out, int64_t cap, int64_t* h_n);
+/* FDMT::get_dm_grid_final (fdmt_base.cpp:65-75) */
+SPS_API sps_status sps_fdmt_get_dm_grid_final(const sps_fdmt* h, float* h_out, int64_t cap, int64_t* h_n);
+/* FDMT::get_df / get_correction (fdmt_base.cpp:56-57) */
+SPS_API sps_status sps_fdmt_get_df(const sps_fdmt* h, float* h_df, float* h_correction);
+/* plan tables of iteration iter (FDMTPlan, fdmt_base.hpp:20-35):
+ * h_grid_off (nsub+1) and h_grid (sum ndt) for dt_grid[iter];
+ * h_map7 rows {out_sub,out_dt,head_sub,head_dt,tail_sub,tail_dt,offset},
+ * h_copy4 rows {out_sub,out_dt,tail_sub,tail_dt}; any pointer may be NULL to
+ * query only the counts. */
+SPS_API sps_status sps_fdmt_get_plan_iter(const sps_fdmt* h, int64_t iter, int64_t* h_nsub, int64_t* h_grid_off,
+                                          int64_t* h_grid, int64_t* h_nmap, int64_t* h_map7, int64_t* h_ncopy,
+                                          int64_t* h_copy4, float* h_df_top, float* h_df_bot);
+/* lowered engine rows of iteration iter (>= 1): h_rows4 {out,tail,head(-1 copy),offset} */
+SPS_API sps_status sps_fdmt_get_rows(const sps_fdmt* h, int64_t iter, int64_t* h_n, int64_t* h_rows4);
+
+/* FDMT::execute(waterfall, waterfall_size, dmt, dmt_size) (fdmt_base.hpp:55-56,
+ * FDMTGPU::execute fdmt_gpu.cu:235-347) and CFdmtGpu::process_image(in, out,
+ * b_ones) (FdmtGpu.cu:273-348).  d_waterfall: nchans x nsamps (dtype f32 or
+ * u8 promoted to f32); d_dmt: output rows x nsamps.  Sizes are checked like
+ * check_inputs (fdmt_base.cpp:107-117).  ones != 0 runs the transform of an
+ * all-ones input (d_waterfall may be NULL). */
+SPS_API sps_status sps_fdmt_execute(sps_fdmt* h, const void* d_waterfall, int dtype, size_t waterfall_size,
+                                    float* d_dmt, size_t dmt_size, int ones, void* stream);
+
+/* FDMT::initialise (fdmt_base.hpp:57): the iteration-0 state, rows0 x nsamps */
+SPS_API sps_status sps_fdmt_initialise(sps_fdmt* h, const void* d_waterfall, int dtype, float* d_state,
+                                       size_t state_size, void* stream);
+
+/* Adjoint (transpose) of execute: d_grad_waterfall (nchans x nsamps) =
+ * FDMT^T d_grad_dmt (rows x nsamps).  No reference counterpart (the
+ * differentiable path of the north-star); linear, so no forward state is kept. */
+SPS_API sps_status sps_fdmt_adjoint(sps_fdmt* h, const float* d_grad_dmt, size_t dmt_size, float* d_grad_waterfall,
+                                    size_t waterfall_size, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* normalisation                                                           */
+/* ---------------------------------------------------------------------- */
+
+/* statistical_preprocessing::calc_fdmt_inputs(d_in, NRows, NCols, d_buff,
+ * d_normInput) (CDMT/Statistical_preprocessing.cuh:9-10, .cu:15-88).
+ * d_in (f32, in place) is cleaned and normalised; d_norm_input (may be NULL)
+ * receives the 0/1 mask; d_buff (>= 2*nrows+4 floats) receives
+ * {row means, row variances, mean, std, var mean, var std} like the
+ * reference's layout.  mode: sps_norm_mode (GLOBAL = reference). */
+SPS_API sps_status sps_calc_fdmt_inputs(float* d_in, int64_t nrows, int64_t ncols, float* d_buff,
+                                        float* d_norm_input, int mode, void* stream);
+
+/* fdmt_normalization(d_arr, d_norm, len, d_out) (CDMT/Chunk_gpu.cu:841-852):
+ * d_out[i] = d_arr[i] / sqrtf(d_norm[i] + 1e-8); d_out may alias d_arr. */
+SPS_API sps_status sps_fdmt_normalization(const float* d_arr, const float* d_norm, int64_t len, float* d_out,
+                                          void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* boxcar candidate gathering                                              */
+/* ---------------------------------------------------------------------- */
+
+/* clusterization::gather_candidates_in_dynamicalArray (Clusterization.cuh:22-23,
+ * .cu:56-89,419-503): widths 1,2,..,2^q, mwidth = log2(width).
+ * gather_candidates_in_heap_with_plan (Clusterization.cuh:56-63, .cu:169-202,
+ * 591-671): all_widths != 0 gives widths 1..wnd with mwidth = width.
+ * d_thr: device pointer to the threshold (the reference passes a reference
+ * to device memory).  Candidates are written in (mdt, mt, mwidth) order;
+ * h_count receives the total (synchronises the stream); more than cap
+ * returns SPS_ERR_CAPACITY with the first cap candidates in order. */
+SPS_API sps_status sps_boxcar_candidates(const float* d_img, int64_t rows, int64_t cols, const float* d_thr,
+                                         int wnd, int all_widths, sps_cand* d_out, int64_t cap, int64_t* h_count,
+                                         void* stream);
+
+/* clusterization::complete_clusterization (Clusterization.cuh:25-32, .cu:92-166)
+ * and select_delegates (select_delegates.cu:62-125): three-key gap clustering
+ * of d_cands with d_bins (device, 3 ints for mt, mdt, mwidth); d_delegates
+ * (n entries) receives delegate candidate indices sorted by mt and
+ * d_group_size their group sizes; h_ngroups the group count (synchronises). */
+SPS_API sps_status sps_cluster(const sps_cand* d_cands, int64_t n, const int32_t* d_bins, int32_t* d_delegates,
+                               int64_t* d_group_size, int64_t* h_ngroups, void* stream);
+
+/* doDetectSignal_v1 single-maximum detector (detection/kernel.cu:561-582):
+ * max over rows, windows 1..wnd fully inside, of the boxcar S/N of
+ * d_arr / sqrtf(1e-7 + d_norm) (d_norm may be NULL: no division).  Ties go to
+ * the smaller width, then row, then column. */
+SPS_API sps_status sps_detect_max(const float* d_arr, const float* d_norm, int64_t rows, int64_t cols, int wnd,
+                                  sps_detection* d_out, void* stream);
+
+/* Boxcar S/N map and its adjoint for the differentiable path (no reference
+ * counterpart): d_snr[r][t] = sum_{j<w} d_img[r][t+j] / sqrtf(w) for t+w <= cols
+ * (0 elsewhere); adjoint d_grad_img[r][t'] = sum_{t<=t'<t+w} d_grad_snr[r][t]/sqrtf(w). */
+SPS_API sps_status sps_boxcar_snr(const float* d_img, int64_t rows, int64_t cols, int w, float* d_snr, void* stream);
+SPS_API sps_status sps_boxcar_snr_adjoint(const float* d_grad_snr, int64_t rows, int64_t cols, int w,
+                                          float* d_grad_img, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* fused single-beam search: the CChunk_gpu::process hot section           */
+/* (CDMT/Chunk_gpu.cu:325-425): normalise -> FDMT -> S/N normalisation ->  */
+/* boxcar gather, without host synchronisation.                            */
+/* ---------------------------------------------------------------------- */
+typedef struct sps_search sps_search;
+
+typedef struct {
+    float f_min, f_max, tsamp;
+    int64_t nchans, nsamps, dt_max;
+    int in_dtype;      /* sps_dtype */
+    int norm_mode;     /* sps_norm_mode */
+    int snr_norm;      /* divide by sqrt(FDMT(mask) + 1e-8) like fdmt_normalization */
+    int box_q;         /* widths 1..2^box_q */
+    float threshold;   /* strict > */
+    int64_t cand_capacity;
+} sps_search_params;
+
+SPS_API sps_status sps_search_create(const sps_search_params* p, sps_search** out);
+SPS_API sps_status sps_search_destroy(sps_search* s);
+/* Runs one beam.  d_in: nchans x nsamps; d_dmt (may be NULL): the S/N
+ * normalised DM-time image (ndt x nsamps); d_cands (cand_capacity) and
+ * d_count (one uint64, total before capacity clipping) are written in
+ * stream order; candidates are in no particular order until
+ * sps_search_sort_candidates. */
+SPS_API sps_status sps_search_run(sps_search* s, const void* d_in, float* d_dmt, sps_cand* d_cands,
+                                  uint64_t* d_count, void* stream);
+/* Sorts the first min(count, capacity) candidates by (mdt, mt, mwidth) and
+ * returns the count (synchronises). */
+SPS_API sps_status sps_search_sort_candidates(sps_search* s, sps_cand* d_cands, const uint64_t* d_count,
+                                              int64_t* h_count, void* stream);
+SPS_API sps_status sps_search_get_ndt(const sps_search* s, int64_t* h_ndt);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPS_H */

@@ -190,7 +190,7 @@ class CFdmt:
         return out
 
 
-def stats(x, mode=0):
+def stats(x, mode=1):
     """Row/global statistics and keep flags (Statistical_preprocessing.cu:15-88)."""
     R, N = x.shape
     rm = np.zeros(R, np.float64)
@@ -206,7 +206,7 @@ def stats(x, mode=0):
     return rm, rv, keep, st
 
 
-def normalise(x, mode=0):
+def normalise(x, mode=1):
     """calc_fdmt_inputs: returns (normalised fp32 image, mask, stats tuple)."""
     R, N = x.shape
     rm, rv, keep, st = stats(x, mode)

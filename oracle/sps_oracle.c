@@ -722,25 +722,35 @@ ORC_EXPORT int orc_cfdmt_process(void* h, const float* x, float* out, int b_ones
 /* ------------------------------------------------------------------------ */
 
 #define ORC_LANES 256
+#define ORC_CHUNK 16
 
+static void orc_tree(double* a) {
+    for (int s = ORC_LANES / 2; s > 0; s >>= 1)
+        for (int l = 0; l < s; ++l) a[l] = a[l] + a[l + s];
+}
+
+/* row sums: element t belongs to lane (t / 16) % 256; each lane accumulates
+ * in increasing t; the 256 lane partials are combined by a pairwise tree. */
 static void orc_row_sums(const float* xr, const uint8_t* ur, int64_t N, double* s1, double* s2) {
     double a[ORC_LANES], b[ORC_LANES];
     for (int l = 0; l < ORC_LANES; ++l) { a[l] = 0.0; b[l] = 0.0; }
     for (int64_t t = 0; t < N; ++t) {
         double v = ur ? (double)ur[t] : (double)xr[t];
-        a[t % ORC_LANES] += v;
-        b[t % ORC_LANES] += v * v;
+        int l = (int)((t / ORC_CHUNK) % ORC_LANES);
+        a[l] = a[l] + v;
+        b[l] = b[l] + v * v;
     }
-    for (int s = ORC_LANES / 2; s > 0; s >>= 1)
-        for (int l = 0; l < s; ++l) { a[l] += a[l + s]; b[l] += b[l + s]; }
+    orc_tree(a);
+    orc_tree(b);
     *s1 = a[0];
     *s2 = b[0];
 }
 
 /* stats_out (doubles): [mean, std, var_mean, var_std, n_keep];
- * row_mean/row_var (doubles, R each); keep (uint8, R).  mode 0 = the
+ * row_mean/row_var (doubles, R each); keep (uint8, R).  mode 1 = the
  * reference's global mean/std (Statistical_preprocessing.cu:521-537),
- * mode 1 = per-channel mean/std. */
+ * mode 2 = per-channel mean/std (a row with zero variance is dropped).
+ * Cross-row sums: row r belongs to lane r % 256, pairwise tree. */
 ORC_EXPORT void orc_stats(const float* x, const uint8_t* xu8, int64_t R, int64_t N, int mode,
                           double* row_mean, double* row_var, uint8_t* keep, double* stats_out) {
 #pragma omp parallel for schedule(static)
@@ -751,21 +761,24 @@ ORC_EXPORT void orc_stats(const float* x, const uint8_t* xu8, int64_t R, int64_t
         row_mean[r] = m;
         row_var[r] = s2 / (double)N - m * m;
     }
-    double sm = 0.0, sq = 0.0, sv = 0.0, svv = 0.0;
+    double a[ORC_LANES], b[ORC_LANES], c[ORC_LANES], d[ORC_LANES];
+    for (int l = 0; l < ORC_LANES; ++l) { a[l] = b[l] = c[l] = d[l] = 0.0; }
     for (int64_t r = 0; r < R; ++r) {
-        sm += row_mean[r];
-        sq += row_var[r] + row_mean[r] * row_mean[r];
-        sv += row_var[r];
-        svv += row_var[r] * row_var[r];
+        int l = (int)(r % ORC_LANES);
+        a[l] = a[l] + row_mean[r];
+        b[l] = b[l] + (row_var[r] + row_mean[r] * row_mean[r]);
+        c[l] = c[l] + row_var[r];
+        d[l] = d[l] + row_var[r] * row_var[r];
     }
-    double mean = sm / (double)R;
-    double sd = sqrt(sq / (double)R - mean * mean);
-    double vm = sv / (double)R;
-    double vs = sqrt(svv / (double)R - vm * vm);
+    orc_tree(a); orc_tree(b); orc_tree(c); orc_tree(d);
+    double mean = a[0] / (double)R;
+    double sd = sqrt(b[0] / (double)R - mean * mean);
+    double vm = c[0] / (double)R;
+    double vs = sqrt(d[0] / (double)R - vm * vm);
     int64_t nk = 0;
     for (int64_t r = 0; r < R; ++r) {
         int drop = fabs(row_var[r] - vm) > 4.0 * vs;
-        if (mode == 1 && !(row_var[r] > 0.0)) drop = 1;
+        if (mode == 2 && !(row_var[r] > 0.0)) drop = 1;
         keep[r] = drop ? 0 : 1;
         nk += keep[r];
     }
@@ -774,14 +787,14 @@ ORC_EXPORT void orc_stats(const float* x, const uint8_t* xu8, int64_t R, int64_t
 
 /* normalize_clean_and_fill_NormInput (Statistical_preprocessing.cu:510-539):
  * kept rows x = (x - mu)/sigma in fp32 with mu, sigma rounded to fp32 (global
- * or per row), mask 1; dropped rows x = 0, mask 0.  out may alias x (f32). */
+ * or per row for mode 2), mask 1; dropped rows x = 0, mask 0. */
 ORC_EXPORT void orc_normalise(const float* x, const uint8_t* xu8, int64_t R, int64_t N, int mode,
                               const double* row_mean, const double* row_var, const uint8_t* keep,
                               const double* stats, float* out, float* mask) {
 #pragma omp parallel for schedule(static)
     for (int64_t r = 0; r < R; ++r) {
         float mu, sg;
-        if (mode == 1) { mu = (float)row_mean[r]; sg = (float)sqrt(row_var[r]); }
+        if (mode == 2) { mu = (float)row_mean[r]; sg = (float)sqrt(row_var[r]); }
         else { mu = (float)stats[0]; sg = (float)stats[1]; }
         for (int64_t t = 0; t < N; ++t) {
             float v = xu8 ? (float)xu8[r * N + t] : x[r * N + t];
