@@ -1,0 +1,189 @@
+"""Single-beam search benchmark on one MI355X.
+
+Workload (one step = one beam through the whole hot path, inputs resident in
+HBM): a 4096-channel x 262144-sample uint8 filterbank over 1200-1600 MHz
+(N(128,16) clipped to 0..255, 20 injected dispersed pulses, sweeps up to 4096
+samples, widths 1-256), normalised with the reference's global mean/std and
+outlier-row cleaning, FDMT with dt_max 4095 (4096 DM trials), S/N
+normalisation by the transform of the channel mask, and boxcar S/N over
+widths 1,2,..,256 with a strict 6-sigma threshold and candidate compaction.
+
+Prints ONE JSON line: DM-trials x samples / s, the merge kernel's achieved
+HBM bandwidth against 8 TB/s (HIP events on the launch stream, algorithmic
+bytes 4*nsamps*(rows_{i-1}+rows_i) per iteration), and the CPU oracle timed
+on a bounded time slice of the same beam.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "DM-trials×samples/s (FDMT+boxcar) and % of HBM peak at 1 GPU, batched vs plain"
+HBM_PEAK_GBS = 8000.0
+
+CFG = dict(f_min=1200.0, f_max=1600.0, nchans=4096, nsamps=1 << 18, dt_max=4095, box_q=8, threshold=6.0,
+           npulses=20, seed=2)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(x_u8, ndt, sample_cols, threads):
+    """Times the CPU oracle chain (normalise -> FDMT -> mask FDMT -> S/N ->
+    boxcar) on the first sample_cols samples of every channel."""
+    import numpy as np
+
+    from oracle import orc
+
+    orc.lib().orc_set_num_threads(threads)
+    xs = np.ascontiguousarray(x_u8[:, :sample_cols])
+    c = CFG
+    t0 = time.perf_counter()
+    xn, mask, _ = orc.normalise(xs, 1)
+    p = orc.Plan(c["f_min"], c["f_max"], c["nchans"], sample_cols, 1.0, c["dt_max"])
+    img = p.execute(xn)
+    norm = p.execute(mask)
+    snr = orc.fdmt_normalization(img, norm)
+    cands = orc.boxcar_pow2(snr, c["threshold"], c["box_q"])
+    dt = time.perf_counter() - t0
+    return ndt * sample_cols / dt, dt, len(cands)
+
+
+def load_traffic():
+    path = os.path.join(ROOT, "profiles", "merge_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("traffic_bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-cols", type=int, default=1 << 17)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="run one plain device copy of this size first (counter calibration under rocprofv3 --pmc)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+
+    import torch
+
+    from cpp_projects_amd import synth
+    from cpp_projects_amd.search import SingleBeamSearch
+
+    c = CFG
+    N, R = c["nsamps"], c["nchans"]
+    log(f"[bench] generating {R}x{N} u8 filterbank")
+    x, pulses = synth.u8_filterbank(R, N, c["f_min"], c["f_max"], c["seed"], npulses=c["npulses"])
+    cap = 1 << 22
+    s = SingleBeamSearch(c["f_min"], c["f_max"], R, N, c["dt_max"], in_dtype="u8", box_q=c["box_q"],
+                         threshold=c["threshold"], cand_capacity=cap)
+    ndt = s.ndt
+    dmt = torch.empty((ndt, N), dtype=torch.float32, device="cuda")
+    cands = torch.empty((cap, 4), dtype=torch.int32, device="cuda")
+    count = torch.zeros(1, dtype=torch.int64, device="cuda")
+    if args.calib_copy_mib > 0:
+        nb = args.calib_copy_mib << 20
+        a = torch.empty(nb // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy of {args.calib_copy_mib} MiB done")
+        del a, b
+    log(f"[bench] ndt={ndt}; warmup {args.warmup}")
+    for _ in range(args.warmup):
+        s.run(x, dmt, cands, count)
+    torch.cuda.synchronize()
+    log(f"[bench] timing {args.steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.run(x, dmt, cands, count)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = ndt * N * args.steps / (t1 - t0)
+    ncand = int(count.item())
+
+    # stage timing with HIP events on the same stream (separate, untimed steps)
+    s.enable_timing(True)
+    stage_sum = None
+    reps = max(3, min(args.steps, 5))
+    for _ in range(reps):
+        s.run(x, dmt, cands, count)
+        st = s.stage_times_ms()
+        stage_sum = st if stage_sum is None else [a + b for a, b in zip(stage_sum, st)]
+    s.enable_timing(False)
+    stage_ms = [v / reps for v in stage_sum]
+    sbytes = s.stage_bytes()
+    niters = len(stage_ms) - 5
+    # the plain merge kernel runs iterations 1..niters-1; the last iteration is
+    # a separately named instantiation carrying the fused S/N division
+    merge_ms = stage_ms[3:3 + niters - 1]
+    merge_bytes = sbytes[3:3 + niters - 1]
+    avg_ms = sum(merge_ms) / len(merge_ms)
+    avg_bytes = sum(merge_bytes) / len(merge_bytes)
+    achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+    names = (["stats", "mask_fdmt", "init"] + [f"merge{i}" for i in range(1, niters)] + [f"merge{niters}+snr"] +
+             ["snr_norm_separate", "boxcar"])
+    stages = {n: round(v, 4) for n, v in zip(names, stage_ms)}
+    total_alg = sum(sbytes)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "DM-trials*samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "single-beam search 4096ch x 262144 u8 (1200-1600 MHz), dt_max 4095, "
+                        "global norm + S/N norm, boxcar widths 1..256 pow2, thr 6",
+            "ndt": ndt, "nsamps": N, "nchans": R, "candidates": ncand, "path": "plain (one beam per launch sequence)",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_merge<true, false, false> (merge iterations 1..n-1)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": load_traffic(),
+            "alg_bytes_per_launch": avg_bytes,
+            "avg_launch_ms": avg_ms,
+        },
+        "stages_ms": stages,
+        "effective_GBs_whole_step": total_alg / (sum(stage_ms) * 1e-3) / 1e9,
+    }
+    if not args.no_cpu_baseline:
+        import numpy as np  # noqa: F401
+
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        xs = x[:, : args.cpu_sample_cols].cpu().numpy()
+        log(f"[bench] cpu baseline on {R}x{args.cpu_sample_cols} with {threads} threads")
+        v, dt, nc = cpu_baseline(xs, ndt, args.cpu_sample_cols, threads)
+        out["cpu_baseline"] = {"value": v, "unit": "DM-trials*samples/s", "cores": threads, "kind": "port",
+                               "sample": f"{R}ch x {args.cpu_sample_cols} samples (first slice of the beam), "
+                                         f"same chain, {dt:.2f} s"}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

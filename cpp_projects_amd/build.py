@@ -65,5 +65,18 @@ def build(verbose=False):
     return LIB
 
 
+
+HARNESS_SRC = os.path.join(ROOT, "tests", "cpp", "dropin_harness.cpp")
+HARNESS_BIN = os.path.join(ROOT, "tests", "cpp", "dropin_harness")
+
+
+def build_harness():
+    """Compiles the C++ drop-in harness (tests/cpp) against include/sps/*.hpp and libsps.so."""
+    _run([CXX, "-std=c++17", "-O1", "-Wno-unused-result", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "include"),
+          "-I", "/opt/rocm/include", HARNESS_SRC, "-o", HARNESS_BIN, "-L", PKG, "-lsps", "-L", "/opt/rocm/lib",
+          "-lamdhip64", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/../../cpp_projects_amd"])
+    return HARNESS_BIN
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))

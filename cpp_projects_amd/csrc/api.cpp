@@ -74,12 +74,14 @@ struct sps_fdmt {
     std::vector<std::unique_ptr<DevBuf>> d_rows;  // per iteration
     DevBuf state_a, state_b;
     int64_t rows_a = 0, rows_b = 0;    // capacity of the ping-pong states in rows
+    int dmax = 0;                      // largest init dt of any channel
     // adjoint tables
     bool adj_built = false;
     std::vector<std::unique_ptr<DevBuf>> csr_off, csr_row, csr_shift;
 
     void init_layout() {
         pitch = round_up(eng.nsamps, 4);
+        dmax = eng.max_init_dt();
         rows_a = rows_b = 0;
         for (int i = 0; i <= eng.niters; ++i) {
             if (i % 2 == 0) rows_a = std::max(rows_a, eng.rows[(size_t)i]);
@@ -111,16 +113,22 @@ struct sps_fdmt {
 
     // Runs init + merges over nsamps samples (nsamps <= eng.nsamps) writing the
     // final rows to out (pitch out_pitch).  sa/sb: ping-pong states with pitch p.
+    // ev (optional): one event recorded after the init and after every merge.
+    // snr (optional): S/N epilogue fused into the last merge; aux: mask transform.
     void run(const void* in, int in_u8, int ones, int64_t in_pitch, const ChanNorm* norm, int64_t nsamps, float* out,
-             int64_t out_pitch, float* sa, float* sb, int64_t p, hipStream_t st) {
+             int64_t out_pitch, float* sa, float* sb, int64_t p, hipStream_t st, hipEvent_t* ev = nullptr,
+             const MergeEpilogue* snr = nullptr, bool aux = false) {
+        SPS_REQUIRE(!snr || eng.niters > 0, "the S/N epilogue needs at least one merge");
         const ChanNorm* nm = norm;
         if (eng.niters == 0) {
             launch_fdmt_init(eng.init_mode, in, in_u8, ones, in_pitch, nm, eng.nchans, nsamps, d_grid.as<int32_t>(),
-                             d_grid_off.as<int64_t>(), d_chan_row_off.as<int64_t>(), out, out_pitch, st);
+                             d_grid_off.as<int64_t>(), d_chan_row_off.as<int64_t>(), out, out_pitch, st, dmax);
+            if (ev) SPS_HIP(hipEventRecord(ev[0], st));
             return;
         }
         launch_fdmt_init(eng.init_mode, in, in_u8, ones, in_pitch, nm, eng.nchans, nsamps, d_grid.as<int32_t>(),
-                         d_grid_off.as<int64_t>(), d_chan_row_off.as<int64_t>(), sa, p, st);
+                         d_grid_off.as<int64_t>(), d_chan_row_off.as<int64_t>(), sa, p, st, dmax);
+        if (ev) SPS_HIP(hipEventRecord(ev[0], st));
         for (int i = 1; i <= eng.niters; ++i) {
             const float* src = ((i - 1) % 2 == 0) ? sa : sb;
             const bool last = (i == eng.niters);
@@ -128,7 +136,8 @@ struct sps_fdmt {
             const int64_t dp = last ? out_pitch : p;
             const int64_t limit = last ? eng.out_rows : INT64_MAX;
             launch_fdmt_merge(src, p, dst, dp, d_rows[(size_t)i]->as<RowDescD>(),
-                              (int64_t)eng.iter[(size_t)i].size(), nsamps, limit, st);
+                              (int64_t)eng.iter[(size_t)i].size(), nsamps, limit, st, last ? snr : nullptr, aux);
+            if (ev) SPS_HIP(hipEventRecord(ev[i], st));
         }
     }
 
@@ -367,7 +376,7 @@ SPS_API sps_status sps_fdmt_initialise(sps_fdmt* h, const void* d_wf, int dtype,
         h->ensure_device(st);
         launch_fdmt_init(e.init_mode, d_wf, dtype == SPS_U8, 0, e.nsamps, nullptr, e.nchans, e.nsamps,
                          h->d_grid.as<int32_t>(), h->d_grid_off.as<int64_t>(), h->d_chan_row_off.as<int64_t>(), d_state,
-                         e.nsamps, st);
+                         e.nsamps, st, h->dmax);
     });
 }
 
@@ -589,6 +598,23 @@ struct sps_search {
     DevBuf norm_img, norm_a, norm_b; // mask transform output and its states
     DevBuf dmt;                      // internal DM-time image when the caller passes none
     bool ready = false;
+    // stage timing: ev[0] start, ev[1] stats, ev[2..] init + merges, then
+    // mask transform, S/N normalisation, boxcar
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+
+    ~sps_search() {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+    int64_t nstages() const { return (int64_t)f->eng.niters + 5; }
+    void ensure_events() {
+        const size_t need = (size_t)nstages() + 1;
+        while (ev.size() < need) {
+            hipEvent_t e;
+            SPS_HIP(hipEventCreate(&e));
+            ev.push_back(e);
+        }
+    }
 
     void setup(hipStream_t st) {
         if (ready) return;
@@ -686,30 +712,86 @@ SPS_API sps_status sps_search_run(sps_search* s, const void* d_in, float* d_dmt,
         s->setup(st);
         const sps_search_params& p = s->p;
         const int64_t N = p.nsamps;
+        const int it = s->f->eng.niters;
+        hipEvent_t* ev = nullptr;
+        if (s->timing) {
+            s->ensure_events();
+            ev = s->ev.data();
+            SPS_HIP(hipEventRecord(ev[0], st));
+        }
         if (p.norm_mode != SPS_NORM_NONE) {
             launch_row_stats(d_in, p.in_dtype == SPS_U8, p.nchans, N, s->rm.as<double>(), s->rv.as<double>(), st);
             launch_stats_finalize(s->rm.as<double>(), s->rv.as<double>(), p.nchans, p.norm_mode, s->stats.as<double>(),
                                   s->mu.as<float>(), s->sg.as<float>(), s->keep.as<uint8_t>(), nullptr, st);
         }
+        if (ev) SPS_HIP(hipEventRecord(ev[1], st));
         float* out = d_dmt;
         if (!out) {
             s->dmt.ensure(sizeof(float) * (size_t)checked_mul(s->ndt, N, "dmt"));
             out = s->dmt.as<float>();
         }
-        ChanNorm cn{s->mu.as<float>(), s->sg.as<float>(), s->keep.as<uint8_t>()};
-        const ChanNorm* np = (p.norm_mode != SPS_NORM_NONE) ? &cn : nullptr;
-        s->f->run(d_in, p.in_dtype == SPS_U8, 0, N, np, N, out, N, s->f->state_a.as<float>(), s->f->state_b.as<float>(),
-                  s->f->pitch, st);
+        // mask transform first: its output feeds the S/N epilogue of the last merge
+        MergeEpilogue epi{nullptr, 0, nullptr};
+        const bool fuse = p.snr_norm && s->f->eng.niters > 0;
         if (p.snr_norm) {
             ChanNorm mask{s->zeros.as<float>(), s->ones_f.as<float>(), s->keep.as<uint8_t>()};
             const int64_t L = s->norm_len;
             s->f->run(nullptr, 0, 1, N, &mask, L, s->norm_img.as<float>(), L, s->norm_a.as<float>(),
-                      s->norm_b.as<float>(), round_up(L, 4), st);
-            launch_snr_normalise(out, s->ndt, N, N, s->norm_img.as<float>(), L, s->stats.as<double>(), st);
+                      s->norm_b.as<float>(), round_up(L, 4), st, nullptr, nullptr, true);
+            epi = MergeEpilogue{s->norm_img.as<float>(), L, s->stats.as<double>()};
         }
+        if (ev) SPS_HIP(hipEventRecord(ev[2], st));
+        ChanNorm cn{s->mu.as<float>(), s->sg.as<float>(), s->keep.as<uint8_t>()};
+        const ChanNorm* np = (p.norm_mode != SPS_NORM_NONE) ? &cn : nullptr;
+        s->f->run(d_in, p.in_dtype == SPS_U8, 0, N, np, N, out, N, s->f->state_a.as<float>(), s->f->state_b.as<float>(),
+                  s->f->pitch, st, ev ? ev + 3 : nullptr, fuse ? &epi : nullptr);
+        if (p.snr_norm && !fuse)
+            launch_snr_normalise(out, s->ndt, N, N, s->norm_img.as<float>(), s->norm_len, s->stats.as<double>(), st);
+        if (ev) SPS_HIP(hipEventRecord(ev[it + 4], st));
         launch_fill_u64(reinterpret_cast<unsigned long long*>(d_count), 1, 0ull, st);
         launch_boxcar(out, s->ndt, N, N, nullptr, p.threshold, 1 << p.box_q, 1, reinterpret_cast<CandD*>(d_cands),
                       p.cand_capacity, reinterpret_cast<unsigned long long*>(d_count), st);
+        if (ev) SPS_HIP(hipEventRecord(ev[it + 5], st));
+    });
+}
+
+SPS_API sps_status sps_search_enable_timing(sps_search* s, int enable) {
+    return guarded([&] {
+        SPS_REQUIRE(s, "null handle");
+        s->timing = enable != 0;
+        if (s->timing) s->ensure_events();
+    });
+}
+
+SPS_API sps_status sps_search_get_timing(sps_search* s, double* h_ms, int64_t cap, int64_t* h_n) {
+    return guarded([&] {
+        SPS_REQUIRE(s && h_n, "null argument");
+        SPS_REQUIRE(s->timing, "timing is not enabled");
+        const int64_t n = s->nstages();
+        *h_n = n;
+        SPS_HIP(hipEventSynchronize(s->ev[(size_t)n]));
+        for (int64_t i = 0; i < n && i < cap; ++i) {
+            float ms = 0.f;
+            SPS_HIP(hipEventElapsedTime(&ms, s->ev[(size_t)i], s->ev[(size_t)i + 1]));
+            h_ms[i] = ms;
+        }
+    });
+}
+
+SPS_API sps_status sps_search_get_stage_bytes(const sps_search* s, double* h_b, int64_t cap, int64_t* h_n) {
+    return guarded([&] {
+        SPS_REQUIRE(s && h_n, "null argument");
+        const EnginePlan& e = s->f->eng;
+        const double N4 = 4.0 * (double)s->p.nsamps;
+        std::vector<double> b;
+        b.push_back(N4 * (double)e.nchans);                        // stats: one input read (fp32 figure)
+        b.push_back(0.0);                                          // mask transform (not counted)
+        b.push_back(N4 * (double)(e.nchans + e.rows[0]));          // init: input read + state 0 write
+        for (int i = 1; i <= e.niters; ++i) b.push_back(N4 * (double)(e.rows[(size_t)i - 1] + e.rows[(size_t)i]));
+        b.push_back(0.0);                                          // S/N normalisation (fused: not counted)
+        b.push_back(N4 * (double)e.rows.back());                   // boxcar: one read of the image
+        *h_n = (int64_t)b.size();
+        for (int64_t i = 0; i < (int64_t)b.size() && i < cap; ++i) h_b[i] = b[(size_t)i];
     });
 }
 

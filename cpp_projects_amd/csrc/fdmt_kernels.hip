@@ -31,11 +31,21 @@ typedef float float4a __attribute__((ext_vector_type(4), aligned(16)));
 // ---------------------------------------------------------------------------
 // merge
 // ---------------------------------------------------------------------------
-template <bool VEC>
+// SNR: the output is divided by sqrtf(n + 1e-8) (fdmt_normalization,
+// Chunk_gpu.cu:841-852) with n from the mask transform for t < nrm_len and the
+// kept-channel count beyond it.  AUX only renames the instantiation used for
+// the mask transform so profiles keep it apart from the data transform.
+struct SnrEpi {
+    const float* nrm;     // rows x nrm_len
+    int64_t nrm_len;
+    const double* stats;  // stats[4] = kept channels
+};
+
+template <bool VEC, bool SNR, bool AUX>
 __global__ void __launch_bounds__(kThreads) k_merge(const float* __restrict__ in, int64_t in_pitch,
                                                     float* __restrict__ out, int64_t out_pitch,
                                                     const RowDescD* __restrict__ rows, int64_t nrows,
-                                                    int64_t nsamps, int64_t out_rows_limit) {
+                                                    int64_t nsamps, int64_t out_rows_limit, SnrEpi epi) {
     const uint32_t logical = xcd_remap(blockIdx.x, gridDim.x);
     const int64_t ri = (int64_t)(logical % (uint64_t)nrows);
     const int64_t tile = (int64_t)(logical / (uint64_t)nrows);
@@ -47,6 +57,17 @@ __global__ void __launch_bounds__(kThreads) k_merge(const float* __restrict__ in
     float* __restrict__ o = out + d.out_row * out_pitch;
     const int64_t off = d.offset;
     const int64_t base = tile * kMergeTile;
+    float dk = 1.0f;
+    const float* nrow = nullptr;
+    if (SNR) {
+        dk = sqrtf((float)((double)(float)epi.stats[4] + 1.0E-8));
+        nrow = epi.nrm + d.out_row * epi.nrm_len;
+    }
+    auto div = [&](float v, int64_t t) -> float {
+        if (!SNR) return v;
+        const float dd = (t < epi.nrm_len) ? sqrtf((float)((double)nrow[t] + 1.0E-8)) : dk;
+        return v / dd;
+    };
     if (VEC) {
 #pragma unroll
         for (int j = 0; j < kMergeVecPerThread; ++j) {
@@ -67,6 +88,12 @@ __global__ void __launch_bounds__(kThreads) k_merge(const float* __restrict__ in
                         if (t + 3 >= off) a.w = a.w + head[t + 3 - off];
                     }
                 }
+                if (SNR) {
+                    a.x = div(a.x, t + 0);
+                    a.y = div(a.y, t + 1);
+                    a.z = div(a.z, t + 2);
+                    a.w = div(a.w, t + 3);
+                }
                 *reinterpret_cast<float4a*>(o + t) = a;
             } else {
                 for (int e = 0; e < 4; ++e) {
@@ -74,7 +101,7 @@ __global__ void __launch_bounds__(kThreads) k_merge(const float* __restrict__ in
                     if (tt < nsamps) {
                         float v = tail[tt];
                         if (has_head && tt >= off) v = v + head[tt - off];
-                        o[tt] = v;
+                        o[tt] = div(v, tt);
                     }
                 }
             }
@@ -85,7 +112,7 @@ __global__ void __launch_bounds__(kThreads) k_merge(const float* __restrict__ in
             if (t < nsamps) {
                 float v = tail[t];
                 if (has_head && t >= off) v = v + head[t - off];
-                o[t] = v;
+                o[t] = div(v, t);
             }
         }
     }
@@ -159,6 +186,149 @@ __global__ void __launch_bounds__(kThreads) k_init(const void* __restrict__ in, 
     }
 }
 
+// LDS-staged init: one block per (channel, 4096-sample tile).  The tile plus
+// its history of dmax samples is read once (16-byte loads when aligned),
+// normalised once into LDS, and every thread then produces 4 groups of 4
+// consecutive samples of each of the channel's rows with the same operation
+// order as k_init, storing float4 per row.
+constexpr int kInitTile = 4096;
+
+template <int IN_KIND>
+__device__ __forceinline__ void load_span(const void* __restrict__ in, int64_t row_base, int64_t lo, int n,
+                                          float* __restrict__ dst, bool aligned16) {
+    // dst[i] = raw(lo + i), i in [0, n); lo may be negative (zeros)
+    if (IN_KIND == 2) {
+        for (int i = threadIdx.x; i < n; i += kThreads) dst[i] = 1.0f;
+        return;
+    }
+    if (IN_KIND == 1 && aligned16) {
+        const uint8_t* p = static_cast<const uint8_t*>(in) + row_base;
+        const int64_t a0 = (lo >= 0 ? lo : 0) & ~(int64_t)15;
+        const int64_t hi = lo + n;
+        for (int64_t c = a0 + (int64_t)threadIdx.x * 16; c < hi; c += (int64_t)kThreads * 16) {
+            const uint4 v = *reinterpret_cast<const uint4*>(p + c);
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const int64_t idx = c + e;
+                if (idx >= lo && idx < hi) dst[idx - lo] = (float)((w[e >> 2] >> (8 * (e & 3))) & 0xffu);
+            }
+        }
+        for (int i = threadIdx.x; i < n && lo + i < 0; i += kThreads) dst[i] = 0.0f;
+        return;
+    }
+    if (IN_KIND == 0 && aligned16) {
+        const float* p = static_cast<const float*>(in) + row_base;
+        const int64_t a0 = (lo >= 0 ? lo : 0) & ~(int64_t)3;
+        const int64_t hi = lo + n;
+        for (int64_t c = a0 + (int64_t)threadIdx.x * 4; c < hi; c += (int64_t)kThreads * 4) {
+            const float4a v = *reinterpret_cast<const float4a*>(p + c);
+            const float e4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t idx = c + e;
+                if (idx >= lo && idx < hi) dst[idx - lo] = e4[e];
+            }
+        }
+        for (int i = threadIdx.x; i < n && lo + i < 0; i += kThreads) dst[i] = 0.0f;
+        return;
+    }
+    for (int i = threadIdx.x; i < n; i += kThreads) {
+        const int64_t idx = lo + i;
+        dst[i] = idx >= 0 ? load_in<IN_KIND>(in, row_base + idx) : 0.0f;
+    }
+}
+
+template <int MODE, int IN_KIND, bool NORM>
+__global__ void __launch_bounds__(kThreads) k_init_lds(const void* __restrict__ in, int64_t in_pitch, ChanNorm norm,
+                                                       int64_t nsamps, const int32_t* __restrict__ grid,
+                                                       const int64_t* __restrict__ grid_off,
+                                                       const int64_t* __restrict__ chan_row_off,
+                                                       float* __restrict__ state, int64_t pitch, int dmax_all,
+                                                       int aligned_in, int aligned_out) {
+    extern __shared__ float xs[];
+    const int64_t c = blockIdx.y;
+    const int64_t T = (int64_t)blockIdx.x * kInitTile;
+    const int64_t g0 = grid_off[c];
+    const int K = (int)(grid_off[c + 1] - g0);
+    const int32_t* g = grid + g0;
+    const int dmax = g[K - 1];
+    const int64_t n_t = min((int64_t)kInitTile, nsamps - T);
+    const int n = (int)(n_t + dmax);
+    // xs[i] holds sample T - dmax + i
+    load_span<IN_KIND>(in, c * in_pitch, T - dmax, n, xs, aligned_in != 0);
+    __syncthreads();
+    if (NORM) {
+        const bool keep = norm.keep[c] != 0;
+        const float mu = norm.mu[c], sg = norm.sigma[c];
+        for (int i = threadIdx.x; i < n; i += kThreads) xs[i] = keep ? (xs[i] - mu) / sg : 0.0f;
+        __syncthreads();
+    }
+    (void)dmax_all;
+    float* rowbase = state + chan_row_off[c] * pitch;
+    for (int grp = 0; grp < kInitTile / (kThreads * 4); ++grp) {
+    const int lt0 = grp * kThreads * 4 + threadIdx.x * 4;
+    if (lt0 >= n_t) return;
+    float v[4], prev[4];
+    const int64_t d0 = g[0];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int64_t t = T + lt0 + e;
+        const int xi = lt0 + e + dmax;  // LDS index of sample t
+        float p = 0.0f;
+        if (t < nsamps && t >= d0) {
+            if (MODE == kInitCFdmtD) {
+                p = xs[xi];
+            } else {
+                float sum = 0.0f;
+                for (int i = xi - (int)d0; i <= xi; ++i) sum += xs[i];
+                p = sum / (float)(d0 + 1);
+            }
+        }
+        prev[e] = p;
+    }
+    auto store = [&](float* row, const float* val) {
+        const int64_t t = T + lt0;
+        if (aligned_out && t + 4 <= nsamps) {
+            float4a a;
+            a.x = val[0];
+            a.y = val[1];
+            a.z = val[2];
+            a.w = val[3];
+            *reinterpret_cast<float4a*>(row + t) = a;
+        } else {
+            for (int e = 0; e < 4; ++e)
+                if (t + e < nsamps) row[t + e] = val[e];
+        }
+    };
+    store(rowbase, prev);
+    for (int k = 1; k < K; ++k) {
+        const int64_t dc = g[k], dp = g[k - 1];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int64_t t = T + lt0 + e;
+            const int xi = lt0 + e + dmax;
+            float r = 0.0f;
+            if (t < nsamps && t >= dc) {
+                if (MODE == kInitMeanD) {
+                    float sum = 0.0f;
+                    for (int i = xi - (int)dc; i < xi - (int)dp; ++i) sum += xs[i];
+                    r = (prev[e] * ((float)dp + 1.0f) + sum) / ((float)dc + 1.0f);
+                } else if (MODE == kInitRestrictD) {
+                    const float sum = 0.0f + xs[xi - (int)dc];
+                    r = (prev[e] * (((float)dp + 1.0f) + sum)) / ((float)dc + 1.0f);
+                } else {
+                    r = (prev[e] * (float)dc + xs[xi - (int)dc]) / (float)(dc + 1);
+                }
+            }
+            v[e] = r;
+            prev[e] = r;
+        }
+        store(rowbase + (int64_t)k * pitch, v);
+    }
+    }  // grp
+}
+
 // ---------------------------------------------------------------------------
 // adjoints
 // ---------------------------------------------------------------------------
@@ -207,29 +377,65 @@ __global__ void __launch_bounds__(kThreads) k_init_adjoint(const float* __restri
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+template <bool SNR, bool AUX>
+static void merge_dispatch(bool vec, uint32_t nb, const float* in, int64_t in_pitch, float* out, int64_t out_pitch,
+                           const RowDescD* d_rows, int64_t nrows, int64_t nsamps, int64_t limit, SnrEpi epi,
+                           hipStream_t st) {
+    if (vec)
+        hipLaunchKernelGGL((k_merge<true, SNR, AUX>), dim3(nb), dim3(kThreads), 0, st, in, in_pitch, out, out_pitch,
+                           d_rows, nrows, nsamps, limit, epi);
+    else
+        hipLaunchKernelGGL((k_merge<false, SNR, AUX>), dim3(nb), dim3(kThreads), 0, st, in, in_pitch, out, out_pitch,
+                           d_rows, nrows, nsamps, limit, epi);
+}
+
 void launch_fdmt_merge(const float* in, int64_t in_pitch, float* out, int64_t out_pitch, const RowDescD* d_rows,
-                       int64_t nrows, int64_t nsamps, int64_t out_rows_limit, hipStream_t st) {
+                       int64_t nrows, int64_t nsamps, int64_t out_rows_limit, hipStream_t st, const MergeEpilogue* snr,
+                       bool aux) {
     if (nrows <= 0 || nsamps <= 0) return;
     const int64_t tiles = ceil_div(nsamps, kMergeTile);
     const int64_t nb = nrows * tiles;
     if (nb > 0x7fffffffLL) throw Error(SPS_ERR_INVALID_ARGUMENT, "merge grid too large");
     const bool vec = (in_pitch % 4 == 0) && (out_pitch % 4 == 0) &&
                      (reinterpret_cast<uintptr_t>(in) % 16 == 0) && (reinterpret_cast<uintptr_t>(out) % 16 == 0);
-    if (vec)
-        hipLaunchKernelGGL((k_merge<true>), dim3((uint32_t)nb), dim3(kThreads), 0, st, in, in_pitch, out, out_pitch,
-                           d_rows, nrows, nsamps, out_rows_limit);
+    SnrEpi epi{nullptr, 0, nullptr};
+    if (snr) epi = SnrEpi{snr->nrm, snr->nrm_len, snr->stats};
+    if (snr && aux) throw Error(SPS_ERR_INVALID_ARGUMENT, "no S/N epilogue on the mask transform");
+    if (snr)
+        merge_dispatch<true, false>(vec, (uint32_t)nb, in, in_pitch, out, out_pitch, d_rows, nrows, nsamps,
+                                    out_rows_limit, epi, st);
+    else if (aux)
+        merge_dispatch<false, true>(vec, (uint32_t)nb, in, in_pitch, out, out_pitch, d_rows, nrows, nsamps,
+                                    out_rows_limit, epi, st);
     else
-        hipLaunchKernelGGL((k_merge<false>), dim3((uint32_t)nb), dim3(kThreads), 0, st, in, in_pitch, out, out_pitch,
-                           d_rows, nrows, nsamps, out_rows_limit);
+        merge_dispatch<false, false>(vec, (uint32_t)nb, in, in_pitch, out, out_pitch, d_rows, nrows, nsamps,
+                                     out_rows_limit, epi, st);
     SPS_HIP(hipGetLastError());
 }
 
 template <int MODE, int IN_KIND>
 static void init_dispatch_norm(const void* in, int64_t in_pitch, const ChanNorm* norm, int64_t nchans,
                                int64_t nsamps, const int32_t* g, const int64_t* go, const int64_t* cro, float* state,
-                               int64_t pitch, hipStream_t st) {
-    dim3 grid((uint32_t)ceil_div(nsamps, kThreads), (uint32_t)nchans);
+                               int64_t pitch, int dmax_all, hipStream_t st) {
     ChanNorm n{};
+    if (norm) n = *norm;
+    if (dmax_all >= 0 && dmax_all <= 16384) {
+        const uintptr_t ib = reinterpret_cast<uintptr_t>(in);
+        int al_in = 0;
+        if (IN_KIND == 1) al_in = (ib % 16 == 0) && (in_pitch % 16 == 0) && (nsamps % 16 == 0);
+        if (IN_KIND == 0) al_in = (ib % 16 == 0) && (in_pitch % 4 == 0) && (nsamps % 4 == 0);
+        const int al_out = (reinterpret_cast<uintptr_t>(state) % 16 == 0) && (pitch % 4 == 0);
+        const dim3 grid((uint32_t)ceil_div(nsamps, kInitTile), (uint32_t)nchans);
+        const size_t lds = sizeof(float) * (size_t)(kInitTile + dmax_all);
+        if (norm)
+            hipLaunchKernelGGL((k_init_lds<MODE, IN_KIND, true>), grid, dim3(kThreads), lds, st, in, in_pitch, n,
+                               nsamps, g, go, cro, state, pitch, dmax_all, al_in, al_out);
+        else
+            hipLaunchKernelGGL((k_init_lds<MODE, IN_KIND, false>), grid, dim3(kThreads), lds, st, in, in_pitch, n,
+                               nsamps, g, go, cro, state, pitch, dmax_all, al_in, al_out);
+        return;
+    }
+    dim3 grid((uint32_t)ceil_div(nsamps, kThreads), (uint32_t)nchans);
     if (norm) {
         n = *norm;
         hipLaunchKernelGGL((k_init<MODE, IN_KIND, true>), grid, dim3(kThreads), 0, st, in, in_pitch, n, nsamps, g, go,
@@ -243,32 +449,32 @@ static void init_dispatch_norm(const void* in, int64_t in_pitch, const ChanNorm*
 template <int MODE>
 static void init_dispatch_kind(const void* in, int in_u8, int ones, int64_t in_pitch, const ChanNorm* norm,
                                int64_t nchans, int64_t nsamps, const int32_t* g, const int64_t* go,
-                               const int64_t* cro, float* state, int64_t pitch, hipStream_t st) {
+                               const int64_t* cro, float* state, int64_t pitch, int dmax_all, hipStream_t st) {
     if (ones)
-        init_dispatch_norm<MODE, 2>(in, in_pitch, norm, nchans, nsamps, g, go, cro, state, pitch, st);
+        init_dispatch_norm<MODE, 2>(in, in_pitch, norm, nchans, nsamps, g, go, cro, state, pitch, dmax_all, st);
     else if (in_u8)
-        init_dispatch_norm<MODE, 1>(in, in_pitch, norm, nchans, nsamps, g, go, cro, state, pitch, st);
+        init_dispatch_norm<MODE, 1>(in, in_pitch, norm, nchans, nsamps, g, go, cro, state, pitch, dmax_all, st);
     else
-        init_dispatch_norm<MODE, 0>(in, in_pitch, norm, nchans, nsamps, g, go, cro, state, pitch, st);
+        init_dispatch_norm<MODE, 0>(in, in_pitch, norm, nchans, nsamps, g, go, cro, state, pitch, dmax_all, st);
 }
 
 void launch_fdmt_init(int mode, const void* in, int in_u8, int ones, int64_t in_pitch, const ChanNorm* norm,
                       int64_t nchans, int64_t nsamps, const int32_t* d_grid, const int64_t* d_grid_off,
-                      const int64_t* d_chan_row_off, float* state, int64_t pitch, hipStream_t st) {
+                      const int64_t* d_chan_row_off, float* state, int64_t pitch, hipStream_t st, int dmax_all) {
     if (nchans <= 0 || nsamps <= 0) return;
     if (nchans > 65535) throw Error(SPS_ERR_INVALID_ARGUMENT, "more than 65535 channels");
     switch (mode) {
         case kInitMeanD:
             init_dispatch_kind<kInitMeanD>(in, in_u8, ones, in_pitch, norm, nchans, nsamps, d_grid, d_grid_off,
-                                           d_chan_row_off, state, pitch, st);
+                                           d_chan_row_off, state, pitch, dmax_all, st);
             break;
         case kInitRestrictD:
             init_dispatch_kind<kInitRestrictD>(in, in_u8, ones, in_pitch, norm, nchans, nsamps, d_grid, d_grid_off,
-                                               d_chan_row_off, state, pitch, st);
+                                               d_chan_row_off, state, pitch, dmax_all, st);
             break;
         case kInitCFdmtD:
             init_dispatch_kind<kInitCFdmtD>(in, in_u8, ones, in_pitch, norm, nchans, nsamps, d_grid, d_grid_off,
-                                            d_chan_row_off, state, pitch, st);
+                                            d_chan_row_off, state, pitch, dmax_all, st);
             break;
         default:
             throw Error(SPS_ERR_INVALID_ARGUMENT, "unknown init mode");

@@ -34,11 +34,22 @@ struct ChanNorm {
 //   norm: nullptr, or per-channel normalisation
 void launch_fdmt_init(int mode, const void* in, int in_u8, int ones, int64_t in_pitch, const ChanNorm* norm,
                       int64_t nchans, int64_t nsamps, const int32_t* d_grid, const int64_t* d_grid_off,
-                      const int64_t* d_chan_row_off, float* state, int64_t pitch, hipStream_t st);
+                      const int64_t* d_chan_row_off, float* state, int64_t pitch, hipStream_t st,
+                      int dmax_all = -1);  // largest init dt of any channel; < 0 selects the unstaged kernel
+
+// Optional epilogue of the last merge: divide by sqrtf(n + 1e-8) with n from
+// nrm (rows x nrm_len) for t < nrm_len and stats[4] (kept channels) beyond.
+struct MergeEpilogue {
+    const float* nrm;
+    int64_t nrm_len;
+    const double* stats;
+};
 
 // One merge iteration over nrows descriptors; rows with out_row >= out_rows_limit are skipped.
+// aux marks launches of the mask transform (a separately named instantiation).
 void launch_fdmt_merge(const float* in, int64_t in_pitch, float* out, int64_t out_pitch, const RowDescD* d_rows,
-                       int64_t nrows, int64_t nsamps, int64_t out_rows_limit, hipStream_t st);
+                       int64_t nrows, int64_t nsamps, int64_t out_rows_limit, hipStream_t st,
+                       const MergeEpilogue* snr = nullptr, bool aux = false);
 
 // Adjoint of one merge: g_prev[p][t] = sum over (o, s) in csr[p] of g_out[o][t+s] (t+s < nsamps).
 void launch_fdmt_merge_adjoint(const float* g_out, int64_t out_pitch, float* g_prev, int64_t prev_pitch,

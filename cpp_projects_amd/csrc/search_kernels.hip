@@ -186,14 +186,49 @@ __global__ void __launch_bounds__(kThreads) k_snr_normalise(float* __restrict__ 
 // ---------------------------------------------------------------------------
 // boxcar: exact left-to-right window sums (Clusterization.cu:419-462,591-671)
 // ---------------------------------------------------------------------------
+// Candidates are staged per block in LDS and reserved in global memory with
+// one atomic per block; a full stage spills through direct atomics.
+constexpr int kStage = 512;
+
+struct CandStage {
+    CandD c[kStage];
+    uint32_t n;
+    unsigned long long base;
+};
+
+__device__ __forceinline__ void stage_emit(CandStage& st, CandD c, CandD* __restrict__ out, int64_t cap,
+                                           unsigned long long* __restrict__ count) {
+    const uint32_t k = atomicAdd(&st.n, 1u);
+    if (k < (uint32_t)kStage) {
+        st.c[k] = c;
+    } else {
+        const unsigned long long g = atomicAdd(count, 1ull);
+        if ((int64_t)g < cap) out[g] = c;
+    }
+}
+
+__device__ __forceinline__ void stage_flush(CandStage& st, CandD* __restrict__ out, int64_t cap,
+                                            unsigned long long* __restrict__ count) {
+    __syncthreads();
+    const uint32_t n = min(st.n, (uint32_t)kStage);
+    if (threadIdx.x == 0) st.base = n ? atomicAdd(count, (unsigned long long)n) : 0ull;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += kThreads) {
+        const unsigned long long g = st.base + i;
+        if ((int64_t)g < cap) out[g] = st.c[i];
+    }
+}
+
+// All widths 1..wmax: the running left-to-right sum is exact for every width.
 constexpr int kBoxCells = 1024;  // cells per block (4 per thread)
 
-template <bool POW2>
-__global__ void __launch_bounds__(kThreads) k_boxcar(const float* __restrict__ img, int64_t cols, int64_t pitch,
-                                                     const float* __restrict__ d_thr, float h_thr, int wmax,
-                                                     CandD* __restrict__ out, int64_t cap,
-                                                     unsigned long long* __restrict__ count) {
+__global__ void __launch_bounds__(kThreads) k_boxcar_all(const float* __restrict__ img, int64_t cols, int64_t pitch,
+                                                         const float* __restrict__ d_thr, float h_thr, int wmax,
+                                                         CandD* __restrict__ out, int64_t cap,
+                                                         unsigned long long* __restrict__ count) {
     extern __shared__ float tile[];
+    __shared__ CandStage st;
+    if (threadIdx.x == 0) st.n = 0;
     const int64_t r = blockIdx.y;
     const int64_t T = (int64_t)blockIdx.x * kBoxCells;
     const int64_t span = min((int64_t)kBoxCells + wmax - 1, cols - T);
@@ -207,27 +242,149 @@ __global__ void __launch_bounds__(kThreads) k_boxcar(const float* __restrict__ i
         if (t >= cols) break;
         const int64_t wlim = min((int64_t)wmax, cols - t);
         float s = 0.0f;
-        int p = 0;
-        int64_t next = 1;
         for (int64_t iw = 0; iw < wlim; ++iw) {
             s += tile[lt + iw];
-            if (POW2) {
-                if (iw + 1 != next) continue;
-                next *= 2;
-            }
             const float snr = s / sqrtf((float)(iw + 1));
-            if (snr > thr) {
-                const unsigned long long k = atomicAdd(count, 1ull);
-                if ((int64_t)k < cap) {
-                    CandD c;
-                    c.mt = (int32_t)t;
-                    c.mdt = (int32_t)r;
-                    c.mwidth = POW2 ? p : (int32_t)(iw + 1);
-                    c.msnr = snr;
-                    out[k] = c;
-                }
+            if (snr > thr) stage_emit(st, CandD{(int32_t)t, (int32_t)r, (int32_t)(iw + 1), snr}, out, cap, count);
+        }
+    }
+    stage_flush(st, out, cap, count);
+}
+
+// Power-of-two widths 1..2^q.  Exactness with a fast filter:
+//  * dyadic pairwise sums P_k(t) = P_{k-1}(t) + P_{k-1}(t + 2^{k-1}) in LDS;
+//  * the reference value is the left-to-right fp32 sum s; both are within
+//    gamma_{w-1}*sum|x| resp. gamma_k*sum|x| of the exact sum, so
+//    |P - s| <= (gamma_{w-1} + gamma_k) * w * M with M = max|x| over the
+//    block's samples (gamma_n = n u / (1 - n u), u = 2^-24);
+//  * fl(s / sqrtf(w)) > thr needs s > thr*sqrtf(w)/(1+u), so a cell whose
+//    P is below that bound minus the error term cannot pass and is skipped;
+//  * every other cell is recomputed left to right (one running sum up to its
+//    largest flagged width) and decided exactly as the reference does.
+constexpr int kPowCells = 2048;            // cells per block (8 per thread)
+constexpr int kPowPer = kPowCells / kThreads;
+
+__device__ __forceinline__ double gamma_n(double n) {
+    const double u = 0x1p-24;
+    return n * u / (1.0 - n * u);
+}
+
+__global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restrict__ img, int64_t cols, int64_t pitch,
+                                                          const float* __restrict__ d_thr, float h_thr, int q,
+                                                          CandD* __restrict__ out, int64_t cap,
+                                                          unsigned long long* __restrict__ count) {
+    extern __shared__ float lds[];
+    __shared__ float wmaxabs[kThreads / 64];
+    const int W = 1 << q;
+    const int span_max = kPowCells + W - 1;
+    float* x = lds;                  // level 0 (the samples)
+    float* la = lds + span_max;      // ping
+    float* lb = la + span_max;       // pong
+    const int64_t r = blockIdx.y;
+    const int64_t T = (int64_t)blockIdx.x * kPowCells;
+    const int span = (int)min((int64_t)span_max, cols - T);
+    const float* row = img + r * pitch;
+    float m = 0.0f;
+    for (int i = threadIdx.x; i < span; i += kThreads) {
+        const float v = row[T + i];
+        x[i] = v;
+        m = fmaxf(m, fabsf(v));
+        if (v != v) m = __int_as_float(0x7f800000);  // NaN: disable the filter
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) wmaxabs[threadIdx.x >> 6] = m;
+    __syncthreads();
+    float M = wmaxabs[0];
+    for (int i = 1; i < kThreads / 64; ++i) M = fmaxf(M, wmaxabs[i]);
+    const float thr = d_thr ? *d_thr : h_thr;
+
+    uint32_t flags[kPowPer];
+#pragma unroll
+    for (int j = 0; j < kPowPer; ++j) flags[j] = 0;
+    const float* cur = x;
+    for (int k = 0; k <= q; ++k) {
+        const int w = 1 << k;
+        if (k > 0) {
+            // level k from level k-1: valid for i + w <= span
+            float* dst = (k & 1) ? la : lb;
+            const int h = w >> 1;
+            for (int i = threadIdx.x; i + w <= span; i += kThreads) dst[i] = cur[i] + cur[i + h];
+            __syncthreads();
+            cur = dst;
+        }
+        const float c = sqrtf((float)w);
+        double lim = -INFINITY;
+        if (thr > 0.0f)
+            lim = (double)thr * (double)c * (1.0 - 0x1p-22) -
+                  (gamma_n((double)(w - 1)) + gamma_n((double)k)) * (double)w * (double)M * (1.0 + 0x1p-20);
+#pragma unroll
+        for (int j = 0; j < kPowPer; ++j) {
+            const int lt = j * kThreads + threadIdx.x;
+            if (T + lt + w <= cols && lt + w <= span) {
+                const float P = cur[lt];
+                if (!((double)P <= lim)) flags[j] |= 1u << k;
             }
-            if (POW2) ++p;
+        }
+        // the next level overwrites the buffer read two levels back only
+        __syncthreads();
+    }
+    // exact decisions: flags[j] keeps only the widths that pass
+    uint32_t mine = 0;
+#pragma unroll
+    for (int j = 0; j < kPowPer; ++j) {
+        if (!flags[j]) continue;
+        const int lt = j * kThreads + threadIdx.x;
+        const int top = 31 - __clz(flags[j]);
+        float s = 0.0f;
+        int next = 1, p = 0;
+        uint32_t pass = 0;
+        for (int iw = 0; iw < (1 << top); ++iw) {
+            s += x[lt + iw];
+            if (iw + 1 != next) continue;
+            next *= 2;
+            if ((flags[j] & (1u << p)) && s / sqrtf((float)(iw + 1)) > thr) pass |= 1u << p;
+            ++p;
+        }
+        flags[j] = pass;
+        mine += (uint32_t)__popc(pass);
+    }
+    // one global reservation per block: block-wide exclusive scan of the counts
+    __shared__ uint32_t wsum[kThreads / 64];
+    __shared__ unsigned long long gbase;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t incl = mine;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t o = __shfl_up(incl, off);
+        if (lane >= off) incl += o;
+    }
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (int w = 0; w < kThreads / 64; ++w) {
+        if (w < wv) before += wsum[w];
+        total += wsum[w];
+    }
+    if (threadIdx.x == 0) gbase = total ? atomicAdd(count, (unsigned long long)total) : 0ull;
+    __syncthreads();
+    if (!mine) return;
+    unsigned long long g = gbase + before + (incl - mine);
+#pragma unroll
+    for (int j = 0; j < kPowPer; ++j) {
+        if (!flags[j]) continue;
+        const int lt = j * kThreads + threadIdx.x;
+        const int64_t t = T + lt;
+        const int top = 31 - __clz(flags[j]);
+        float s = 0.0f;
+        int next = 1, p = 0;
+        for (int iw = 0; iw < (1 << top); ++iw) {
+            s += x[lt + iw];
+            if (iw + 1 != next) continue;
+            next *= 2;
+            if (flags[j] & (1u << p)) {
+                if ((int64_t)g < cap) out[g] = CandD{(int32_t)t, (int32_t)r, (int32_t)p, s / sqrtf((float)(iw + 1))};
+                ++g;
+            }
+            ++p;
         }
     }
 }
@@ -380,16 +537,23 @@ void launch_snr_normalise(float* img, int64_t rows, int64_t cols, int64_t pitch,
 void launch_boxcar(const float* img, int64_t rows, int64_t cols, int64_t pitch, const float* d_thr, float h_thr,
                    int wmax, int pow2, CandD* out, int64_t cap, unsigned long long* d_count, hipStream_t st) {
     if (rows <= 0 || cols <= 0 || wmax <= 0) return;
-    if (wmax > 8192) throw Error(SPS_ERR_INVALID_ARGUMENT, "window wider than 8192 samples");
     if (rows > 65535) throw Error(SPS_ERR_INVALID_ARGUMENT, "more than 65535 rows");
-    const dim3 grid((uint32_t)ceil_div(cols, kBoxCells), (uint32_t)rows);
-    const size_t lds = sizeof(float) * (size_t)(kBoxCells + wmax);
-    if (pow2)
-        hipLaunchKernelGGL((k_boxcar<true>), grid, dim3(kThreads), lds, st, img, cols, pitch, d_thr, h_thr, wmax, out,
-                           cap, d_count);
-    else
-        hipLaunchKernelGGL((k_boxcar<false>), grid, dim3(kThreads), lds, st, img, cols, pitch, d_thr, h_thr, wmax, out,
-                           cap, d_count);
+    if (pow2) {
+        int q = 0;
+        while ((1 << q) < wmax) ++q;
+        if ((1 << q) != wmax || q > 13) throw Error(SPS_ERR_INVALID_ARGUMENT, "pow2 window must be 2^q, q <= 13");
+        const dim3 grid((uint32_t)ceil_div(cols, kPowCells), (uint32_t)rows);
+        const size_t lds = sizeof(float) * 3 * (size_t)(kPowCells + wmax - 1);
+        if (lds > 150 * 1024) throw Error(SPS_ERR_INVALID_ARGUMENT, "boxcar window too wide for LDS");
+        hipLaunchKernelGGL(k_boxcar_pow2, grid, dim3(kThreads), lds, st, img, cols, pitch, d_thr, h_thr, q, out, cap,
+                           d_count);
+    } else {
+        if (wmax > 8192) throw Error(SPS_ERR_INVALID_ARGUMENT, "window wider than 8192 samples");
+        const dim3 grid((uint32_t)ceil_div(cols, kBoxCells), (uint32_t)rows);
+        const size_t lds = sizeof(float) * (size_t)(kBoxCells + wmax);
+        hipLaunchKernelGGL(k_boxcar_all, grid, dim3(kThreads), lds, st, img, cols, pitch, d_thr, h_thr, wmax, out, cap,
+                           d_count);
+    }
     SPS_HIP(hipGetLastError());
 }
 

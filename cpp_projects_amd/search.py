@@ -183,6 +183,23 @@ class SingleBeamSearch:
     def run(self, d_in, d_dmt, d_cands, d_count):
         call("sps_search_run", self._h, _dptr(d_in), _dptr(d_dmt), _dptr(d_cands), _dptr(d_count), _stream())
 
+    def enable_timing(self, on=True):
+        call("sps_search_enable_timing", self._h, int(on))
+
+    def stage_times_ms(self):
+        """Per-stage durations of the last run (HIP events on the run's stream):
+        [stats, init, merge_1..merge_n, mask transform, S/N norm, boxcar]."""
+        n = C.c_int64()
+        buf = (C.c_double * 64)()
+        call("sps_search_get_timing", self._h, buf, 64, C.byref(n))
+        return list(buf)[: n.value]
+
+    def stage_bytes(self):
+        n = C.c_int64()
+        buf = (C.c_double * 64)()
+        call("sps_search_get_stage_bytes", self._h, buf, 64, C.byref(n))
+        return list(buf)[: n.value]
+
     def sort_candidates(self, d_cands, d_count):
         n = C.c_int64()
         call("sps_search_sort_candidates", self._h, _dptr(d_cands), _dptr(d_count), C.byref(n), _stream())

@@ -132,8 +132,8 @@ SPS_API sps_status sps_fdmt_initialise(sps_fdmt* h, const void* d_waterfall, int
                                        size_t state_size, void* stream);
 
 /* Adjoint (transpose) of execute: d_grad_waterfall (nchans x nsamps) =
- * FDMT^T d_grad_dmt (rows x nsamps).  No reference counterpart (the
- * differentiable path of the north-star); linear, so no forward state is kept. */
+ * FDMT^T d_grad_dmt (rows x nsamps).  No reference counterpart (used by the
+ * differentiable search path); linear, so no forward state is kept. */
 SPS_API sps_status sps_fdmt_adjoint(sps_fdmt* h, const float* d_grad_dmt, size_t dmt_size, float* d_grad_waterfall,
                                     size_t waterfall_size, void* stream);
 
@@ -225,6 +225,17 @@ SPS_API sps_status sps_search_run(sps_search* s, const void* d_in, float* d_dmt,
 SPS_API sps_status sps_search_sort_candidates(sps_search* s, sps_cand* d_cands, const uint64_t* d_count,
                                               int64_t* h_count, void* stream);
 SPS_API sps_status sps_search_get_ndt(const sps_search* s, int64_t* h_ndt);
+/* Stage timing (no reference counterpart; a profiling aid).  When enabled,
+ * sps_search_run records HIP events on its stream between stages; after a run,
+ * sps_search_get_timing synchronises and writes the stage durations in ms:
+ * [normalise stats, mask transform, FDMT init, merge 1 .. merge niters
+ *  (the last one carries the fused S/N division), separate S/N
+ *  normalisation (zero when fused), boxcar] (h_n receives niters + 5). */
+SPS_API sps_status sps_search_enable_timing(sps_search* s, int enable);
+SPS_API sps_status sps_search_get_timing(sps_search* s, double* h_ms, int64_t cap, int64_t* h_n);
+/* Algorithmic bytes of each merge iteration, 4*nsamps*(rows_{i-1} + rows_i),
+ * and of the other stages, in the order of sps_search_get_timing. */
+SPS_API sps_status sps_search_get_stage_bytes(const sps_search* s, double* h_bytes, int64_t cap, int64_t* h_n);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,229 @@
+// sps/search.hpp -- drop-in normalisation, detection and clustering calls
+// over the libsps C ABI.
+//
+// Mirrors CDMT/Statistical_preprocessing.cuh:6-17, CDMT/Chunk_gpu.cuh:118-119
+// (fdmt_normalization), Clusterization/Candidate.cuh:4-10 (Cand),
+// Clusterization/Clusterization.cuh:8-68 (clusterization::),
+// Clusterization/select_delegates.cuh:4-26 (select_delegates::) and the
+// detector of detection/kernel.cu:14-20,561-605.  All array pointers are HIP
+// device pointers unless named h_; thresholds and bin metrics are references
+// to device memory like the reference's (Chunk_gpu.cu:373-425).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdio>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../sps.h"
+#include "fdmt.hpp"
+
+using Cand = sps_cand;  // {int mt, mdt, mwidth; float msnr}, 16 bytes
+typedef sps_detection structOutDetection;
+
+namespace statistical_preprocessing {
+// Statistical_preprocessing.cuh:9-10: d_parr_inp (rows x cols) is cleaned and
+// normalised in place; d_parrNormInput receives the 0/1 mask; d_buff holds at
+// least 2*NRows+4 floats.
+inline void calc_fdmt_inputs(float* d_parr_inp, const int NRows, const int NCols, float* d_buff,
+                             float* d_parrNormInput) {
+    sps_detail::check(sps_calc_fdmt_inputs(d_parr_inp, NRows, NCols, d_buff, d_parrNormInput, SPS_NORM_GLOBAL, nullptr));
+}
+// per-channel mean/std variant (same row cleaning)
+inline void calc_fdmt_inputs_per_channel(float* d_parr_inp, const int NRows, const int NCols, float* d_buff,
+                                         float* d_parrNormInput) {
+    sps_detail::check(
+        sps_calc_fdmt_inputs(d_parr_inp, NRows, NCols, d_buff, d_parrNormInput, SPS_NORM_PER_CHANNEL, nullptr));
+}
+inline int calcThreadsForMean_and_Disp(unsigned const int nCols) {
+    int k = 0;
+    while ((1u << k) < nCols && k < 10) ++k;
+    return 1 << k;
+}
+}  // namespace statistical_preprocessing
+
+// Chunk_gpu.cuh:118-119 (a host call here rather than a kernel launch)
+inline void fdmt_normalization(fdmt_type_* d_arr, fdmt_type_* d_norm, const int len, float* d_pOutArray) {
+    sps_detail::check(sps_fdmt_normalization(d_arr, d_norm, len, d_pOutArray, nullptr));
+}
+
+namespace select_delegates {
+// select_delegates.cuh:15-25 over plain arrays: delegates (host, sorted by mt)
+// and their group sizes; writes N, t bin, t sec, dedisp, width, SNR, num.
+inline void write_log(const Cand* d_parrCand, const int QUantCandidates, const std::vector<int>& h_delegates,
+                      const std::vector<int64_t>& h_group_sizes, const std::string& filename, const float delt_bin,
+                      const float t_chunkBegin) {
+    std::vector<Cand> h((size_t)QUantCandidates);
+    if (QUantCandidates > 0 && hipMemcpy(h.data(), d_parrCand, sizeof(Cand) * h.size(), hipMemcpyDeviceToHost) != hipSuccess)
+        throw std::runtime_error("copy of candidates failed");
+    std::ofstream f(filename);
+    f << std::setw(10) << "N" << std::setw(10) << "t, bin" << std::setw(10) << "t, sec" << std::setw(10) << "dedisp"
+      << std::setw(10) << "width" << std::setw(10) << "SNR" << std::setw(10) << "num" << std::endl;
+    for (size_t i = 0; i < h_delegates.size(); ++i) {
+        const Cand& c = h[(size_t)h_delegates[i]];
+        f << std::setw(10) << i + 1 << std::setw(10) << c.mt << std::setw(10) << t_chunkBegin + c.mt * delt_bin
+          << std::setw(10) << c.mdt << std::setw(10) << c.mwidth << std::setw(10) << c.msnr << std::setw(10)
+          << h_group_sizes[i] << std::endl;
+    }
+}
+}  // namespace select_delegates
+
+namespace clusterization {
+
+// Clusterization.cuh:22-23: widths 1,2,..,2^QUantPower2_Wnd, mwidth = log2(w).
+// *d_arrCand is hipMalloc'ed here and owned by the caller (hipFree).
+inline void gather_candidates_in_dynamicalArray(const float* d_arr_fdmt, const int Rows, const int Cols,
+                                                const float& val_thresh, const int QUantPower2_Wnd, Cand** d_arrCand,
+                                                unsigned int* h_pquantCand) {
+    int64_t n = 0;
+    sps_status s = sps_boxcar_candidates(d_arr_fdmt, Rows, Cols, &val_thresh, QUantPower2_Wnd, 0, nullptr, 0, &n, nullptr);
+    if (s != SPS_OK && s != SPS_ERR_CAPACITY) sps_detail::check(s);
+    *h_pquantCand = (unsigned int)n;
+    if (hipMalloc((void**)d_arrCand, sizeof(Cand) * (size_t)(n > 0 ? n : 1)) != hipSuccess)
+        throw std::runtime_error("hipMalloc failed");
+    if (n > 0)
+        sps_detail::check(sps_boxcar_candidates(d_arr_fdmt, Rows, Cols, &val_thresh, QUantPower2_Wnd, 0, *d_arrCand, n,
+                                                &n, nullptr));
+}
+
+// Clusterization.cuh:56-63: widths 1..WndWidth, mwidth = w; false when none.
+inline bool gather_candidates_in_heap_with_plan(const float* d_fdmt, const int Rows, const int Cols,
+                                                const float& d_VAlTresh, const int WndWidth, Cand** d_pparrCand,
+                                                unsigned int* h_piQuantCand) {
+    int64_t n = 0;
+    sps_status s = sps_boxcar_candidates(d_fdmt, Rows, Cols, &d_VAlTresh, WndWidth, 1, nullptr, 0, &n, nullptr);
+    if (s != SPS_OK && s != SPS_ERR_CAPACITY) sps_detail::check(s);
+    *h_piQuantCand = (unsigned int)n;
+    if (n == 0) return false;
+    if (hipMalloc((void**)d_pparrCand, sizeof(Cand) * (size_t)n) != hipSuccess) throw std::runtime_error("hipMalloc failed");
+    sps_detail::check(sps_boxcar_candidates(d_fdmt, Rows, Cols, &d_VAlTresh, WndWidth, 1, *d_pparrCand, n, &n, nullptr));
+    return true;
+}
+
+// Clusterization.cuh:25-32: gap clustering by (mt, mdt, mwidth) with
+// d_pbin_metrics, delegate selection, log.  Returns the delegates (host).
+inline std::vector<int> complete_clusterization(const Cand* d_parrCand, unsigned int quant_candidates,
+                                                const int* d_pbin_metrics, const std::string& filename,
+                                                const float delt_bin, const float t_chunkBegin, bool bwrite) {
+    std::vector<int> dele;
+    if (quant_candidates == 0) return dele;
+    int32_t* d_del = nullptr;
+    int64_t* d_gs = nullptr;
+    if (hipMalloc((void**)&d_del, sizeof(int32_t) * quant_candidates) != hipSuccess ||
+        hipMalloc((void**)&d_gs, sizeof(int64_t) * quant_candidates) != hipSuccess) {
+        (void)hipFree(d_del);
+        throw std::runtime_error("hipMalloc failed");
+    }
+    int64_t ng = 0;
+    sps_status s = sps_cluster(d_parrCand, quant_candidates, d_pbin_metrics, d_del, d_gs, &ng, nullptr);
+    std::vector<int64_t> gs((size_t)ng);
+    dele.resize((size_t)ng);
+    if (s == SPS_OK && ng > 0) {
+        (void)hipMemcpy(dele.data(), d_del, sizeof(int32_t) * (size_t)ng, hipMemcpyDeviceToHost);
+        (void)hipMemcpy(gs.data(), d_gs, sizeof(int64_t) * (size_t)ng, hipMemcpyDeviceToHost);
+    }
+    (void)hipFree(d_del);
+    (void)hipFree(d_gs);
+    sps_detail::check(s);
+    if (bwrite)
+        select_delegates::write_log(d_parrCand, (int)quant_candidates, dele, gs, filename, delt_bin, t_chunkBegin);
+    return dele;
+}
+
+// Clusterization.cuh:10-20 (returns true on success; the reference's success
+// path has no return statement).
+inline bool clusterization_main(const float* d_fdmt, const int Rows, const int Cols, const float& d_VAlTresh,
+                                const int WndWidth, const int* d_pbin_metrics, const std::string& filename,
+                                const float delt_bin, const float t_chunkBegin, bool bwrite = true) {
+    unsigned int n = 0;
+    Cand* d_c = nullptr;
+    if (!gather_candidates_in_heap_with_plan(d_fdmt, Rows, Cols, d_VAlTresh, WndWidth, &d_c, &n)) return false;
+    try {
+        complete_clusterization(d_c, n, d_pbin_metrics, filename, delt_bin, t_chunkBegin, bwrite);
+    } catch (...) {
+        (void)hipFree(d_c);
+        throw;
+    }
+    (void)hipFree(d_c);
+    return true;
+}
+
+// Clusterization.cu:310-358
+inline void writeCandHostArrayToCSV(const Cand* h_arrCand, int h_quantCand, const std::string& filename) {
+    std::ofstream file(filename);
+    if (!file.is_open()) throw std::runtime_error("Failed to open file: " + filename);
+    file << "mt,mdt,mwidth,msnr\n";
+    for (int i = 0; i < h_quantCand; ++i)
+        file << h_arrCand[i].mt << ',' << h_arrCand[i].mdt << ',' << h_arrCand[i].mwidth << ',' << h_arrCand[i].msnr
+             << '\n';
+}
+inline void writeCandDeviceArrayToCSV(const Cand* d_arrCand, int h_quantCand, const std::string& filename) {
+    std::vector<Cand> h((size_t)h_quantCand);
+    if (h_quantCand > 0 && hipMemcpy(h.data(), d_arrCand, sizeof(Cand) * h.size(), hipMemcpyDeviceToHost) != hipSuccess)
+        throw std::runtime_error("copy of candidates failed");
+    writeCandHostArrayToCSV(h.data(), h_quantCand, filename);
+}
+
+// Clusterization.cu:268-308: every candidate, columns N, t bin, t sec, dedisp, width, SNR
+inline void write_log_v0(const Cand* d_arrCand, const int QUantCand, const std::string& filename, const float delt_bin,
+                         const float t_chunkBegin) {
+    std::vector<Cand> h((size_t)QUantCand);
+    if (QUantCand > 0 && hipMemcpy(h.data(), d_arrCand, sizeof(Cand) * h.size(), hipMemcpyDeviceToHost) != hipSuccess)
+        throw std::runtime_error("copy of candidates failed");
+    std::ofstream f(filename);
+    f << std::setw(10) << "N" << std::setw(10) << "t, bin" << std::setw(10) << "t, sec" << std::setw(10) << "dedisp"
+      << std::setw(10) << "width" << std::setw(10) << "SNR" << std::endl;
+    for (int i = 0; i < QUantCand; ++i)
+        f << std::setw(10) << i + 1 << std::setw(10) << h[(size_t)i].mt << std::setw(10)
+          << t_chunkBegin + h[(size_t)i].mt * delt_bin << std::setw(10) << h[(size_t)i].mdt << std::setw(10)
+          << h[(size_t)i].mwidth << std::setw(10) << h[(size_t)i].msnr << std::endl;
+}
+
+// Clusterization.cu:362-415: FREDDA candidate files (snr sampno secs boxcar idt dm beamno)
+inline int countCandidatesInFreddaFile(const std::string& filename) {
+    std::ifstream file(filename);
+    if (!file.is_open()) return 0;
+    std::string line;
+    int count = 0;
+    std::getline(file, line);
+    while (std::getline(file, line)) ++count;
+    return count;
+}
+inline void readFreddaCandFile(const std::string& filename, Cand* arrCand, size_t numLines) {
+    std::ifstream file(filename);
+    if (!file.is_open()) return;
+    std::string line;
+    std::getline(file, line);
+    size_t index = 0;
+    while (std::getline(file, line) && index < numLines) {
+        std::istringstream iss(line);
+        float snr, dm, secs;
+        int sampno, boxcar, idt, beamno;
+        if (iss >> snr >> sampno >> secs >> boxcar >> idt >> dm >> beamno) {
+            arrCand[index].mt = sampno;
+            arrCand[index].mdt = idt;
+            arrCand[index].mwidth = boxcar;
+            arrCand[index].msnr = snr;
+            ++index;
+        }
+    }
+}
+
+}  // namespace clusterization
+
+// doDetectSignal_v1 (detection/kernel.cu:561-582): the best boxcar S/N of
+// d_arr / sqrtf(1e-7 + d_norm) over widths 1..WndWidth; result in *d_out
+// (device or managed).  Returns true when the best S/N reaches thr.
+inline bool doDetectSignal(fdmt_type_* d_arr, fdmt_type_* d_norm, const int Rows, const int Cols, const int WndWidth,
+                           const float thr, structOutDetection* d_out) {
+    sps_detail::check(sps_detect_max(d_arr, d_norm, Rows, Cols, WndWidth, d_out, nullptr));
+    structOutDetection h{};
+    if (hipMemcpy(&h, d_out, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess) throw std::runtime_error("copy failed");
+    return h.snr >= thr;
+}

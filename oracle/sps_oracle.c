@@ -826,29 +826,46 @@ typedef struct { int32_t mt, mdt, mwidth; float msnr; } orc_cand;  /* Candidate.
  * widths 1,2,4,..,2^q; mwidth = log2(w); running fp32 sum left to right;
  * snr = s / sqrtf(w); strict >.  Output in (mdt, mt, mwidth) order.
  * Returns the true count; writes at most cap. */
-ORC_EXPORT int64_t orc_boxcar_pow2(const float* img, int64_t R, int64_t N, float thr, int q,
-                                   orc_cand* out, int64_t cap) {
+/* one row: returns the row's count, writing at out[0..cap) */
+static int64_t orc_boxcar_pow2_row(const float* row, int64_t r, int64_t N, float thr, int q, orc_cand* out,
+                                   int64_t cap) {
     int64_t n = 0;
     const int64_t W = (int64_t)1 << q;
-    for (int64_t r = 0; r < R; ++r)
-        for (int64_t t = 0; t < N; ++t) {
-            float s = 0.0f;
-            int64_t next = 1;
-            int p = 0;
-            for (int64_t iw = 0; iw < W; ++iw) {
-                if (t + iw >= N) break;
-                s += img[r * N + t + iw];
-                if (iw + 1 == next) {
-                    float snr = s / sqrtf((float)(iw + 1));
-                    if (snr > thr) {
-                        if (n < cap) { out[n].mt = (int32_t)t; out[n].mdt = (int32_t)r; out[n].mwidth = p; out[n].msnr = snr; }
-                        n++;
-                    }
-                    p++;
-                    next *= 2;
+    for (int64_t t = 0; t < N; ++t) {
+        float s = 0.0f;
+        int64_t next = 1;
+        int p = 0;
+        for (int64_t iw = 0; iw < W; ++iw) {
+            if (t + iw >= N) break;
+            s += row[t + iw];
+            if (iw + 1 == next) {
+                float snr = s / sqrtf((float)(iw + 1));
+                if (snr > thr) {
+                    if (n < cap) { out[n].mt = (int32_t)t; out[n].mdt = (int32_t)r; out[n].mwidth = p; out[n].msnr = snr; }
+                    n++;
                 }
+                p++;
+                next *= 2;
             }
         }
+    }
+    return n;
+}
+
+ORC_EXPORT int64_t orc_boxcar_pow2(const float* img, int64_t R, int64_t N, float thr, int q,
+                                   orc_cand* out, int64_t cap) {
+    /* rows in parallel: count, prefix, then write each row at its offset */
+    int64_t* cnt = (int64_t*)calloc((size_t)R + 1, sizeof(int64_t));
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t r = 0; r < R; ++r) cnt[r + 1] = orc_boxcar_pow2_row(img + r * N, r, N, thr, q, NULL, 0);
+    for (int64_t r = 0; r < R; ++r) cnt[r + 1] += cnt[r];
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t r = 0; r < R; ++r) {
+        if (cnt[r + 1] == cnt[r] || cnt[r] >= cap) continue;
+        orc_boxcar_pow2_row(img + r * N, r, N, thr, q, out + cnt[r], cap - cnt[r]);
+    }
+    int64_t n = cnt[R];
+    free(cnt);
     return n;
 }
 

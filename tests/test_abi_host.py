@@ -101,3 +101,13 @@ def test_device_count_call_without_gpu():
     n = ctypes.c_int(-1)
     _lib.call("sps_device_count", ctypes.byref(n))
     assert n.value >= 0
+
+
+def test_dropin_headers_compile(tmp_path):
+    """include/sps/*.hpp compile with the host compiler against sps.h."""
+    import subprocess
+
+    src = os.path.join(ROOT, "tests", "cpp", "dropin_harness.cpp")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wno-unused-result", "-D__HIP_PLATFORM_AMD__", "-I",
+                        os.path.join(ROOT, "include"), "-I", "/opt/rocm/include", src], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
