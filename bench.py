@@ -70,6 +70,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-cols", type=int, default=1 << 17)
+    ap.add_argument("--schedule", type=str, default="",
+                    help="comma-separated merge fusion depths (default: autotuned during warmup)")
+    ap.add_argument("--no-tune", action="store_true", help="keep the library's default schedule")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one plain device copy of this size first (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -106,6 +109,16 @@ def main():
     for _ in range(args.warmup):
         s.run(x, dmt, cands, count)
     torch.cuda.synchronize()
+    if args.schedule:
+        s.set_schedule([int(v) for v in args.schedule.split(",")])
+    elif not args.no_tune:
+        from cpp_projects_amd.tune import tune_search
+
+        best, _ = tune_search(s, lambda: s.run(x, dmt, cands, count), reps=2, log=log)
+        log(f"[bench] tuned schedule {best}")
+    s.run(x, dmt, cands, count)
+    torch.cuda.synchronize()
+    schedule = s.get_schedule()
     log(f"[bench] timing {args.steps} steps")
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -126,19 +139,37 @@ def main():
         stage_sum = st if stage_sum is None else [a + b for a, b in zip(stage_sum, st)]
     s.enable_timing(False)
     stage_ms = [v / reps for v in stage_sum]
-    sbytes = s.stage_bytes()
-    niters = len(stage_ms) - 5
-    # the plain merge kernel runs iterations 1..niters-1; the last iteration is
-    # a separately named instantiation carrying the fused S/N division
-    merge_ms = stage_ms[3:3 + niters - 1]
-    merge_bytes = sbytes[3:3 + niters - 1]
-    avg_ms = sum(merge_ms) / len(merge_ms)
-    avg_bytes = sum(merge_bytes) / len(merge_bytes)
+    sbytes = s.stage_bytes()  # per pass: compulsory I/O, one read of its input state + one write
+    npass = len(schedule)
+    # algorithmic bytes of the unfused model: 4*N*(R_{i-1} + R_i) per iteration,
+    # credited to the pass that performs iteration i
+    from cpp_projects_amd.fdmt import FDMTGPU
+
+    plan = FDMTGPU(c["f_min"], c["f_max"], R, N, 1.0, c["dt_max"])
+    rows = [int(plan.state_shape(i)[3]) for i in range(sum(schedule) + 1)]
+    alg, io, i0 = [], [], 1
+    for L in schedule:
+        alg.append(sum(4.0 * N * (rows[i - 1] + rows[i]) for i in range(i0, i0 + L)))
+        io.append(4.0 * N * (rows[i0 - 1] + rows[i0 + L - 1]))
+        i0 += L
+    pass_ms = stage_ms[3:3 + npass]
+    # dominant kernel: the non-final fused passes of the depth with most time
+    # (the final pass is a separate instantiation carrying the S/N division)
+    by_depth = {}
+    for k in range(npass - 1):
+        by_depth.setdefault(schedule[k], []).append(k)
+    if not by_depth:
+        by_depth = {schedule[-1]: [npass - 1]}
+    dom = max(by_depth, key=lambda d: sum(pass_ms[k] for k in by_depth[d]))
+    ks = by_depth[dom]
+    avg_ms = sum(pass_ms[k] for k in ks) / len(ks)
+    avg_bytes = sum(alg[k] for k in ks) / len(ks)
+    avg_io = sum(io[k] for k in ks) / len(ks)
     achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
-    names = (["stats", "mask_fdmt", "init"] + [f"merge{i}" for i in range(1, niters)] + [f"merge{niters}+snr"] +
-             ["snr_norm_separate", "boxcar"])
+    names = (["stats", "mask_fdmt", "init"] + [f"pass{k}(L={schedule[k]})" for k in range(npass - 1)] +
+             [f"pass{npass - 1}(L={schedule[-1]})+snr"] + ["snr_norm_separate", "boxcar"])
     stages = {n: round(v, 4) for n, v in zip(names, stage_ms)}
-    total_alg = sum(sbytes)
+    total_alg = sum(sbytes[:3]) + sum(alg) + sum(sbytes[3 + npass:])
 
     out = {
         "metric": METRIC,
@@ -157,10 +188,11 @@ def main():
             "workload": "single-beam search 4096ch x 262144 u8 (1200-1600 MHz), dt_max 4095, "
                         "global norm + S/N norm, boxcar widths 1..256 pow2, thr 6",
             "ndt": ndt, "nsamps": N, "nchans": R, "candidates": ncand, "path": "plain (one beam per launch sequence)",
+            "merge_schedule": schedule,
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_merge<true, false, false> (merge iterations 1..n-1)",
+            "kernel": f"k_merge_fused<{dom}, true, false>",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -168,6 +200,9 @@ def main():
             "traffic": load_traffic(),
             "alg_bytes_per_launch": avg_bytes,
             "avg_launch_ms": avg_ms,
+            "launches_per_step": len(ks),
+            "compulsory_io_bytes_per_launch": avg_io,
+            "compulsory_io_GBs": avg_io / (avg_ms * 1e-3) / 1e9,
         },
         "stages_ms": stages,
         "effective_GBs_whole_step": total_alg / (sum(stage_ms) * 1e-3) / 1e9,

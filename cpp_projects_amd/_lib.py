@@ -79,6 +79,10 @@ SIGNATURES = {
     "sps_search_run": [_vp, _vp, _vp, _vp, _vp, _vp],
     "sps_search_sort_candidates": [_vp, _vp, _vp, _pi64, _vp],
     "sps_search_get_ndt": [_vp, _pi64],
+    "sps_fdmt_set_schedule": [_vp, C.POINTER(C.c_int32), _i64],
+    "sps_fdmt_get_schedule": [_vp, C.POINTER(C.c_int32), _i64, _pi64],
+    "sps_search_set_schedule": [_vp, C.POINTER(C.c_int32), _i64],
+    "sps_search_get_schedule": [_vp, C.POINTER(C.c_int32), _i64, _pi64],
     "sps_search_enable_timing": [_vp, _int],
     "sps_search_get_timing": [_vp, C.POINTER(C.c_double), _i64, _pi64],
     "sps_search_get_stage_bytes": [_vp, C.POINTER(C.c_double), _i64, _pi64],

@@ -79,14 +79,94 @@ struct sps_fdmt {
     bool adj_built = false;
     std::vector<std::unique_ptr<DevBuf>> csr_off, csr_row, csr_shift;
 
+    // pass schedule: fused iterations per pass (1..kMaxFuse), summing to niters
+    std::vector<int> schedule;
+    std::vector<std::unique_ptr<DevBuf>> d_trees;  // per pass
+    bool trees_ready = false;
+
     void init_layout() {
         pitch = round_up(eng.nsamps, 4);
         dmax = eng.max_init_dt();
-        rows_a = rows_b = 0;
-        for (int i = 0; i <= eng.niters; ++i) {
-            if (i % 2 == 0) rows_a = std::max(rows_a, eng.rows[(size_t)i]);
-            else rows_b = std::max(rows_b, eng.rows[(size_t)i]);
+        // state 0 in A; pass outputs alternate B, A, B, ... and rows only
+        // shrink with the iteration, so A needs rows[0] and B rows[1]
+        rows_a = eng.rows[0];
+        rows_b = eng.niters >= 1 ? eng.rows[1] : 0;
+        for (int i = 1; i <= eng.niters; ++i) {
+            SPS_REQUIRE((int64_t)eng.iter[(size_t)i].size() == eng.rows[(size_t)i], "plan rows do not cover the state");
+            for (int64_t r = 0; r < eng.rows[(size_t)i]; ++r)
+                SPS_REQUIRE(eng.iter[(size_t)i][(size_t)r].out_row == r, "plan rows out of order");
+            rows_a = std::max(rows_a, eng.rows[(size_t)i]);
         }
+        set_schedule(default_schedule(eng.niters));
+    }
+
+    static std::vector<int> default_schedule(int n) {
+        std::vector<int> s;
+        while (n > 0) {
+            const int d = (n % kMaxFuse == 1 && n > kMaxFuse) ? 2 : std::min(n, kMaxFuse);
+            s.push_back(d);
+            n -= d;
+        }
+        return s;
+    }
+
+    void set_schedule(const std::vector<int>& s) {
+        int sum = 0;
+        for (int d : s) {
+            SPS_REQUIRE(d >= 1 && d <= kMaxFuse, "fusion depth must be 1..3");
+            sum += d;
+        }
+        SPS_REQUIRE(sum == eng.niters, "schedule must cover every iteration");
+        schedule = s;
+        trees_ready = false;
+    }
+
+    template <int L>
+    void build_trees_L(int i0, std::vector<uint8_t>& bytes) const {
+        // pass covering iterations i0 .. i0+L-1 (input state i0-1)
+        const int top = i0 + L - 1;
+        std::vector<TreeDesc<L>> v((size_t)eng.rows[(size_t)top]);
+        for (int64_t r = 0; r < eng.rows[(size_t)top]; ++r) {
+            TreeDesc<L>& d = v[(size_t)r];
+            d.out_row = r;
+            for (auto& x : d.leaf) x = -1;
+            for (auto& x : d.off) x = -1;
+            // node k at depth dep corresponds to a row of iteration top - dep
+            std::vector<int64_t> node((size_t)(2 << L), -1);
+            node[1] = r;
+            for (int k = 1; k < (1 << L); ++k) {
+                if (node[(size_t)k] < 0) continue;
+                int dep = 0;
+                while ((2 << dep) <= k) ++dep;
+                const RowDesc& rd = eng.iter[(size_t)(top - dep)][(size_t)node[(size_t)k]];
+                node[(size_t)(2 * k)] = rd.tail_row;
+                if (rd.head_row >= 0) {
+                    SPS_REQUIRE(rd.offset <= INT32_MAX, "offset too large");
+                    d.off[k] = (int32_t)rd.offset;
+                    node[(size_t)(2 * k + 1)] = rd.head_row;
+                }
+            }
+            for (int k = (1 << L); k < (2 << L); ++k) d.leaf[k - (1 << L)] = node[(size_t)k];
+        }
+        bytes.resize(sizeof(TreeDesc<L>) * v.size());
+        std::memcpy(bytes.data(), v.data(), bytes.size());
+    }
+
+    void ensure_trees(hipStream_t st) {
+        if (trees_ready) return;
+        d_trees.clear();
+        int i0 = 1;
+        for (int L : schedule) {
+            std::vector<uint8_t> bytes;
+            if (L == 1) build_trees_L<1>(i0, bytes);
+            else if (L == 2) build_trees_L<2>(i0, bytes);
+            else build_trees_L<3>(i0, bytes);
+            d_trees.emplace_back(new DevBuf());
+            upload(*d_trees.back(), bytes, st);
+            i0 += L;
+        }
+        SPS_HIP(hipStreamSynchronize(st));
+        trees_ready = true;
     }
 
     void ensure_device(hipStream_t st) {
@@ -129,15 +209,31 @@ struct sps_fdmt {
         launch_fdmt_init(eng.init_mode, in, in_u8, ones, in_pitch, nm, eng.nchans, nsamps, d_grid.as<int32_t>(),
                          d_grid_off.as<int64_t>(), d_chan_row_off.as<int64_t>(), sa, p, st, dmax);
         if (ev) SPS_HIP(hipEventRecord(ev[0], st));
-        for (int i = 1; i <= eng.niters; ++i) {
-            const float* src = ((i - 1) % 2 == 0) ? sa : sb;
-            const bool last = (i == eng.niters);
-            float* dst = last ? out : ((i % 2 == 0) ? sa : sb);
-            const int64_t dp = last ? out_pitch : p;
-            const int64_t limit = last ? eng.out_rows : INT64_MAX;
-            launch_fdmt_merge(src, p, dst, dp, d_rows[(size_t)i]->as<RowDescD>(),
-                              (int64_t)eng.iter[(size_t)i].size(), nsamps, limit, st, last ? snr : nullptr, aux);
-            if (ev) SPS_HIP(hipEventRecord(ev[i], st));
+        if (aux) {
+            // the mask transform stays on single-iteration launches (a short,
+            // separately named kernel)
+            for (int i = 1; i <= eng.niters; ++i) {
+                const float* src = ((i - 1) % 2 == 0) ? sa : sb;
+                const bool last = (i == eng.niters);
+                float* dst = last ? out : ((i % 2 == 0) ? sa : sb);
+                launch_fdmt_merge(src, p, dst, last ? out_pitch : p, d_rows[(size_t)i]->as<RowDescD>(),
+                                  (int64_t)eng.iter[(size_t)i].size(), nsamps, last ? eng.out_rows : INT64_MAX, st,
+                                  nullptr, true);
+            }
+            return;
+        }
+        ensure_trees(st);
+        int i0 = 1;
+        for (size_t k = 0; k < schedule.size(); ++k) {
+            const int L = schedule[k];
+            const int top = i0 + L - 1;
+            const float* src = (k % 2 == 0) ? sa : sb;
+            const bool last = (top == eng.niters);
+            float* dst = last ? out : ((k % 2 == 0) ? sb : sa);
+            launch_fdmt_fused(L, src, p, dst, last ? out_pitch : p, d_trees[k]->p, eng.rows[(size_t)top], nsamps,
+                              last ? eng.out_rows : INT64_MAX, st, last ? snr : nullptr);
+            if (ev) SPS_HIP(hipEventRecord(ev[k + 1], st));
+            i0 += L;
         }
     }
 
@@ -606,7 +702,8 @@ struct sps_search {
     ~sps_search() {
         for (auto e : ev) (void)hipEventDestroy(e);
     }
-    int64_t nstages() const { return (int64_t)f->eng.niters + 5; }
+    // stats, mask transform, init, one per pass, separate S/N, boxcar
+    int64_t nstages() const { return (int64_t)f->schedule.size() + 5; }
     void ensure_events() {
         const size_t need = (size_t)nstages() + 1;
         while (ev.size() < need) {
@@ -712,7 +809,7 @@ SPS_API sps_status sps_search_run(sps_search* s, const void* d_in, float* d_dmt,
         s->setup(st);
         const sps_search_params& p = s->p;
         const int64_t N = p.nsamps;
-        const int it = s->f->eng.niters;
+        const int it = (int)s->f->schedule.size();  // passes
         hipEvent_t* ev = nullptr;
         if (s->timing) {
             s->ensure_events();
@@ -755,6 +852,36 @@ SPS_API sps_status sps_search_run(sps_search* s, const void* d_in, float* d_dmt,
     });
 }
 
+SPS_API sps_status sps_fdmt_set_schedule(sps_fdmt* h, const int32_t* h_depths, int64_t n) {
+    return guarded([&] {
+        SPS_REQUIRE(h && (n == 0 || h_depths), "null argument");
+        h->set_schedule(std::vector<int>(h_depths, h_depths + n));
+    });
+}
+
+SPS_API sps_status sps_fdmt_get_schedule(const sps_fdmt* h, int32_t* h_depths, int64_t cap, int64_t* h_n) {
+    return guarded([&] {
+        SPS_REQUIRE(h && h_n, "null argument");
+        *h_n = (int64_t)h->schedule.size();
+        for (int64_t i = 0; i < *h_n && i < cap && h_depths; ++i) h_depths[i] = h->schedule[(size_t)i];
+    });
+}
+
+SPS_API sps_status sps_search_set_schedule(sps_search* s, const int32_t* h_depths, int64_t n) {
+    return guarded([&] {
+        SPS_REQUIRE(s && (n == 0 || h_depths), "null argument");
+        s->f->set_schedule(std::vector<int>(h_depths, h_depths + n));
+    });
+}
+
+SPS_API sps_status sps_search_get_schedule(const sps_search* s, int32_t* h_depths, int64_t cap, int64_t* h_n) {
+    return guarded([&] {
+        SPS_REQUIRE(s && h_n, "null argument");
+        *h_n = (int64_t)s->f->schedule.size();
+        for (int64_t i = 0; i < *h_n && i < cap && h_depths; ++i) h_depths[i] = s->f->schedule[(size_t)i];
+    });
+}
+
 SPS_API sps_status sps_search_enable_timing(sps_search* s, int enable) {
     return guarded([&] {
         SPS_REQUIRE(s, "null handle");
@@ -787,7 +914,11 @@ SPS_API sps_status sps_search_get_stage_bytes(const sps_search* s, double* h_b, 
         b.push_back(N4 * (double)e.nchans);                        // stats: one input read (fp32 figure)
         b.push_back(0.0);                                          // mask transform (not counted)
         b.push_back(N4 * (double)(e.nchans + e.rows[0]));          // init: input read + state 0 write
-        for (int i = 1; i <= e.niters; ++i) b.push_back(N4 * (double)(e.rows[(size_t)i - 1] + e.rows[(size_t)i]));
+        int i0 = 1;
+        for (int L : s->f->schedule) {  // a pass reads state i0-1 once and writes state i0+L-1 once
+            b.push_back(N4 * (double)(e.rows[(size_t)i0 - 1] + e.rows[(size_t)(i0 + L - 1)]));
+            i0 += L;
+        }
         b.push_back(0.0);                                          // S/N normalisation (fused: not counted)
         b.push_back(N4 * (double)e.rows.back());                   // boxcar: one read of the image
         *h_n = (int64_t)b.size();

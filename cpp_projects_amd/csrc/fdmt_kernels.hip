@@ -119,6 +119,113 @@ __global__ void __launch_bounds__(kThreads) k_merge(const float* __restrict__ in
 }
 
 // ---------------------------------------------------------------------------
+// fused passes: L consecutive iterations in one launch
+// ---------------------------------------------------------------------------
+// Every output row of iteration i+L-1 is the root of a binary tree in heap
+// order: internal node k (1 <= k < 2^L) holds the head offset of its merge
+// (or -1 when the row is a copy), its tail child is 2k and its head child
+// 2k+1; the leaves 2^L..2^(L+1)-1 are rows of state i-1.  Evaluating
+// v(k,t) = v(2k,t) + (t >= off_k ? v(2k+1, t-off_k) : 0) performs exactly the
+// fp32 additions of the L separate iterations, in the same order, so the
+// intermediate states are never written.
+template <int L, int K>
+struct TreeEval {
+    // scalar value of node K at time t (t >= 0)
+    __device__ __forceinline__ static float one(const float* __restrict__ in, int64_t pitch, const TreeDesc<L>& d,
+                                                int64_t t) {
+        if constexpr (K >= (1 << L)) {
+            return in[d.leaf[K - (1 << L)] * pitch + t];
+        } else {
+            float v = TreeEval<L, 2 * K>::one(in, pitch, d, t);
+            const int64_t off = d.off[K];
+            if (off >= 0 && t >= off) v = v + TreeEval<L, 2 * K + 1>::one(in, pitch, d, t - off);
+            return v;
+        }
+    }
+    // four consecutive times t..t+3 (t >= 0); leaf loads are 4-byte aligned
+    // 16-byte loads, except the root tail chain which is 16-byte aligned
+    __device__ __forceinline__ static float4a four(const float* __restrict__ in, int64_t pitch, const TreeDesc<L>& d,
+                                                   int64_t t) {
+        if constexpr (K >= (1 << L)) {
+            const float4u v = *reinterpret_cast<const float4u*>(in + d.leaf[K - (1 << L)] * pitch + t);
+            float4a r;
+            r.x = v.x;
+            r.y = v.y;
+            r.z = v.z;
+            r.w = v.w;
+            return r;
+        } else {
+            float4a v = TreeEval<L, 2 * K>::four(in, pitch, d, t);
+            const int64_t off = d.off[K];
+            if (off >= 0) {
+                if (t >= off) {
+                    const float4a h = TreeEval<L, 2 * K + 1>::four(in, pitch, d, t - off);
+                    v.x = v.x + h.x;
+                    v.y = v.y + h.y;
+                    v.z = v.z + h.z;
+                    v.w = v.w + h.w;
+                } else if (t + 3 >= off) {
+                    if (t + 0 >= off) v.x = v.x + TreeEval<L, 2 * K + 1>::one(in, pitch, d, t + 0 - off);
+                    if (t + 1 >= off) v.y = v.y + TreeEval<L, 2 * K + 1>::one(in, pitch, d, t + 1 - off);
+                    if (t + 2 >= off) v.z = v.z + TreeEval<L, 2 * K + 1>::one(in, pitch, d, t + 2 - off);
+                    if (t + 3 >= off) v.w = v.w + TreeEval<L, 2 * K + 1>::one(in, pitch, d, t + 3 - off);
+                }
+            }
+            return v;
+        }
+    }
+};
+
+template <int L, bool VEC, bool SNR>
+__global__ void __launch_bounds__(kThreads) k_merge_fused(const float* __restrict__ in, int64_t in_pitch,
+                                                          float* __restrict__ out, int64_t out_pitch,
+                                                          const TreeDesc<L>* __restrict__ rows, int64_t nrows,
+                                                          int64_t nsamps, int64_t out_rows_limit, SnrEpi epi) {
+    const uint32_t logical = xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t ri = (int64_t)(logical % (uint64_t)nrows);
+    const int64_t tile = (int64_t)(logical / (uint64_t)nrows);
+    const TreeDesc<L> d = rows[ri];
+    if (d.out_row >= out_rows_limit) return;
+    float* __restrict__ o = out + d.out_row * out_pitch;
+    const int64_t base = tile * kMergeTile;
+    float dk = 1.0f;
+    const float* nrow = nullptr;
+    if (SNR) {
+        dk = sqrtf((float)((double)(float)epi.stats[4] + 1.0E-8));
+        nrow = epi.nrm + d.out_row * epi.nrm_len;
+    }
+    auto div = [&](float v, int64_t t) -> float {
+        if (!SNR) return v;
+        const float dd = (t < epi.nrm_len) ? sqrtf((float)((double)nrow[t] + 1.0E-8)) : dk;
+        return v / dd;
+    };
+    if (VEC) {
+#pragma unroll
+        for (int j = 0; j < kMergeVecPerThread; ++j) {
+            const int64_t t = base + (int64_t)j * (kThreads * 4) + (int64_t)threadIdx.x * 4;
+            if (t + 4 <= nsamps) {
+                float4a a = TreeEval<L, 1>::four(in, in_pitch, d, t);
+                if (SNR) {
+                    a.x = div(a.x, t + 0);
+                    a.y = div(a.y, t + 1);
+                    a.z = div(a.z, t + 2);
+                    a.w = div(a.w, t + 3);
+                }
+                *reinterpret_cast<float4a*>(o + t) = a;
+            } else {
+                for (int e = 0; e < 4; ++e)
+                    if (t + e < nsamps) o[t + e] = div(TreeEval<L, 1>::one(in, in_pitch, d, t + e), t + e);
+            }
+        }
+    } else {
+        for (int j = 0; j < kMergeTile / kThreads; ++j) {
+            const int64_t t = base + (int64_t)j * kThreads + threadIdx.x;
+            if (t < nsamps) o[t] = div(TreeEval<L, 1>::one(in, in_pitch, d, t), t);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // init
 // ---------------------------------------------------------------------------
 template <int IN_KIND>  // 0 f32, 1 u8, 2 ones
@@ -410,6 +517,59 @@ void launch_fdmt_merge(const float* in, int64_t in_pitch, float* out, int64_t ou
     else
         merge_dispatch<false, false>(vec, (uint32_t)nb, in, in_pitch, out, out_pitch, d_rows, nrows, nsamps,
                                      out_rows_limit, epi, st);
+    SPS_HIP(hipGetLastError());
+}
+
+template <int L, bool SNR>
+static void fused_dispatch2(bool vec, uint32_t nb, const float* in, int64_t in_pitch, float* out, int64_t out_pitch,
+                            const void* d_trees, int64_t nrows, int64_t nsamps, int64_t limit, SnrEpi epi,
+                            hipStream_t st) {
+    const TreeDesc<L>* t = static_cast<const TreeDesc<L>*>(d_trees);
+    if (vec)
+        hipLaunchKernelGGL((k_merge_fused<L, true, SNR>), dim3(nb), dim3(kThreads), 0, st, in, in_pitch, out,
+                           out_pitch, t, nrows, nsamps, limit, epi);
+    else
+        hipLaunchKernelGGL((k_merge_fused<L, false, SNR>), dim3(nb), dim3(kThreads), 0, st, in, in_pitch, out,
+                           out_pitch, t, nrows, nsamps, limit, epi);
+}
+
+template <int L>
+static void fused_dispatch(bool vec, bool snr, uint32_t nb, const float* in, int64_t in_pitch, float* out,
+                           int64_t out_pitch, const void* d_trees, int64_t nrows, int64_t nsamps, int64_t limit,
+                           SnrEpi epi, hipStream_t st) {
+    if (snr)
+        fused_dispatch2<L, true>(vec, nb, in, in_pitch, out, out_pitch, d_trees, nrows, nsamps, limit, epi, st);
+    else
+        fused_dispatch2<L, false>(vec, nb, in, in_pitch, out, out_pitch, d_trees, nrows, nsamps, limit, epi, st);
+}
+
+void launch_fdmt_fused(int L, const float* in, int64_t in_pitch, float* out, int64_t out_pitch, const void* d_trees,
+                       int64_t nrows, int64_t nsamps, int64_t out_rows_limit, hipStream_t st,
+                       const MergeEpilogue* snr) {
+    if (nrows <= 0 || nsamps <= 0) return;
+    const int64_t tiles = ceil_div(nsamps, kMergeTile);
+    const int64_t nb = nrows * tiles;
+    if (nb > 0x7fffffffLL) throw Error(SPS_ERR_INVALID_ARGUMENT, "merge grid too large");
+    const bool vec = (in_pitch % 4 == 0) && (out_pitch % 4 == 0) &&
+                     (reinterpret_cast<uintptr_t>(in) % 16 == 0) && (reinterpret_cast<uintptr_t>(out) % 16 == 0);
+    SnrEpi epi{nullptr, 0, nullptr};
+    if (snr) epi = SnrEpi{snr->nrm, snr->nrm_len, snr->stats};
+    switch (L) {
+        case 1:
+            fused_dispatch<1>(vec, snr != nullptr, (uint32_t)nb, in, in_pitch, out, out_pitch, d_trees, nrows, nsamps,
+                              out_rows_limit, epi, st);
+            break;
+        case 2:
+            fused_dispatch<2>(vec, snr != nullptr, (uint32_t)nb, in, in_pitch, out, out_pitch, d_trees, nrows, nsamps,
+                              out_rows_limit, epi, st);
+            break;
+        case 3:
+            fused_dispatch<3>(vec, snr != nullptr, (uint32_t)nb, in, in_pitch, out, out_pitch, d_trees, nrows, nsamps,
+                              out_rows_limit, epi, st);
+            break;
+        default:
+            throw Error(SPS_ERR_INVALID_ARGUMENT, "fusion depth must be 1..3");
+    }
     SPS_HIP(hipGetLastError());
 }
 

@@ -37,6 +37,18 @@ void launch_fdmt_init(int mode, const void* in, int in_u8, int ones, int64_t in_
                       const int64_t* d_chan_row_off, float* state, int64_t pitch, hipStream_t st,
                       int dmax_all = -1);  // largest init dt of any channel; < 0 selects the unstaged kernel
 
+// One output row of a pass that fuses L iterations: the merge tree in heap
+// order (node 1 = the row; node k has tail child 2k and head child 2k+1 with
+// head offset off[k], or off[k] = -1 for a copied row); leaves 2^L..2^(L+1)-1
+// are rows of the pass's input state (-1 when under a missing head).
+template <int L>
+struct TreeDesc {
+    int64_t out_row;
+    int64_t leaf[1 << L];
+    int32_t off[1 << L];
+};
+constexpr int kMaxFuse = 3;
+
 // Optional epilogue of the last merge: divide by sqrtf(n + 1e-8) with n from
 // nrm (rows x nrm_len) for t < nrm_len and stats[4] (kept channels) beyond.
 struct MergeEpilogue {
@@ -50,6 +62,11 @@ struct MergeEpilogue {
 void launch_fdmt_merge(const float* in, int64_t in_pitch, float* out, int64_t out_pitch, const RowDescD* d_rows,
                        int64_t nrows, int64_t nsamps, int64_t out_rows_limit, hipStream_t st,
                        const MergeEpilogue* snr = nullptr, bool aux = false);
+
+// A pass of L fused iterations (1 <= L <= kMaxFuse) over nrows TreeDesc<L>.
+void launch_fdmt_fused(int L, const float* in, int64_t in_pitch, float* out, int64_t out_pitch, const void* d_trees,
+                       int64_t nrows, int64_t nsamps, int64_t out_rows_limit, hipStream_t st,
+                       const MergeEpilogue* snr = nullptr);
 
 // Adjoint of one merge: g_prev[p][t] = sum over (o, s) in csr[p] of g_out[o][t+s] (t+s < nsamps).
 void launch_fdmt_merge_adjoint(const float* g_out, int64_t out_pitch, float* g_prev, int64_t prev_pitch,

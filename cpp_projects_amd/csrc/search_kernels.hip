@@ -328,64 +328,51 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
         // the next level overwrites the buffer read two levels back only
         __syncthreads();
     }
-    // exact decisions: flags[j] keeps only the widths that pass
-    uint32_t mine = 0;
+    // Exact recompute: the running sums of the thread's 8 cells advance
+    // together (independent chains), over a wave-uniform bound so that the
+    // candidates of each (cell slot, width) are emitted with one ballot and
+    // one global atomic per wave.
+    int top = -1;
 #pragma unroll
-    for (int j = 0; j < kPowPer; ++j) {
-        if (!flags[j]) continue;
-        const int lt = j * kThreads + threadIdx.x;
-        const int top = 31 - __clz(flags[j]);
-        float s = 0.0f;
-        int next = 1, p = 0;
-        uint32_t pass = 0;
-        for (int iw = 0; iw < (1 << top); ++iw) {
-            s += x[lt + iw];
-            if (iw + 1 != next) continue;
-            next *= 2;
-            if ((flags[j] & (1u << p)) && s / sqrtf((float)(iw + 1)) > thr) pass |= 1u << p;
-            ++p;
-        }
-        flags[j] = pass;
-        mine += (uint32_t)__popc(pass);
-    }
-    // one global reservation per block: block-wide exclusive scan of the counts
-    __shared__ uint32_t wsum[kThreads / 64];
-    __shared__ unsigned long long gbase;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    uint32_t incl = mine;
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t o = __shfl_up(incl, off);
-        if (lane >= off) incl += o;
-    }
-    if (lane == 63) wsum[wv] = incl;
-    __syncthreads();
-    uint32_t before = 0, total = 0;
-    for (int w = 0; w < kThreads / 64; ++w) {
-        if (w < wv) before += wsum[w];
-        total += wsum[w];
-    }
-    if (threadIdx.x == 0) gbase = total ? atomicAdd(count, (unsigned long long)total) : 0ull;
-    __syncthreads();
-    if (!mine) return;
-    unsigned long long g = gbase + before + (incl - mine);
+    for (int j = 0; j < kPowPer; ++j)
+        if (flags[j]) top = max(top, 31 - __clz(flags[j]));
+    for (int off = 32; off > 0; off >>= 1) top = max(top, __shfl_xor(top, off));
+    if (top < 0) return;
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    float s[kPowPer];
 #pragma unroll
-    for (int j = 0; j < kPowPer; ++j) {
-        if (!flags[j]) continue;
-        const int lt = j * kThreads + threadIdx.x;
-        const int64_t t = T + lt;
-        const int top = 31 - __clz(flags[j]);
-        float s = 0.0f;
-        int next = 1, p = 0;
-        for (int iw = 0; iw < (1 << top); ++iw) {
-            s += x[lt + iw];
-            if (iw + 1 != next) continue;
-            next *= 2;
-            if (flags[j] & (1u << p)) {
-                if ((int64_t)g < cap) out[g] = CandD{(int32_t)t, (int32_t)r, (int32_t)p, s / sqrtf((float)(iw + 1))};
-                ++g;
+    for (int j = 0; j < kPowPer; ++j) s[j] = 0.0f;
+    int p = 0, next = 1;
+    for (int iw = 0; iw < (1 << top); ++iw) {
+#pragma unroll
+        for (int j = 0; j < kPowPer; ++j)
+            if (flags[j] >> p) s[j] += x[j * kThreads + threadIdx.x + iw];
+        if (iw + 1 != next) continue;
+        const float c = sqrtf((float)next);
+#pragma unroll
+        for (int j = 0; j < kPowPer; ++j) {
+            float snr = 0.0f;
+            bool pred = false;
+            if ((flags[j] >> p) & 1u) {
+                snr = s[j] / c;
+                pred = snr > thr;
             }
-            ++p;
+            const uint64_t m = __ballot(pred);
+            if (m) {
+                const int leader = __ffsll((unsigned long long)m) - 1;
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(count, (unsigned long long)__popcll(m));
+                base = __shfl(base, leader);
+                if (pred) {
+                    const unsigned long long g = base + (unsigned long long)__popcll(m & lt_mask);
+                    if ((int64_t)g < cap)
+                        out[g] = CandD{(int32_t)(T + j * kThreads + threadIdx.x), (int32_t)r, (int32_t)p, snr};
+                }
+            }
         }
+        next *= 2;
+        ++p;
     }
 }
 

@@ -75,6 +75,17 @@ class _Handle:
         call("sps_fdmt_get_rows", self._h, i, C.byref(n), out.ctypes.data_as(C.POINTER(C.c_int64)))
         return out[: n.value]
 
+    def set_schedule(self, depths):
+        """Merge passes: each fuses 1..3 consecutive iterations (sum = niters)."""
+        arr = (C.c_int32 * max(len(depths), 1))(*depths)
+        call("sps_fdmt_set_schedule", self._h, arr, len(depths))
+
+    def get_schedule(self):
+        n = C.c_int64()
+        buf = (C.c_int32 * 64)()
+        call("sps_fdmt_get_schedule", self._h, buf, 64, C.byref(n))
+        return list(buf)[: n.value]
+
     # ---- device work ----
     def _execute(self, waterfall, waterfall_size, dmt, dmt_size, ones=False):
         code = SPS_F32 if waterfall is None else _dtype_code(waterfall)

@@ -183,6 +183,16 @@ class SingleBeamSearch:
     def run(self, d_in, d_dmt, d_cands, d_count):
         call("sps_search_run", self._h, _dptr(d_in), _dptr(d_dmt), _dptr(d_cands), _dptr(d_count), _stream())
 
+    def set_schedule(self, depths):
+        arr = (C.c_int32 * max(len(depths), 1))(*depths)
+        call("sps_search_set_schedule", self._h, arr, len(depths))
+
+    def get_schedule(self):
+        n = C.c_int64()
+        buf = (C.c_int32 * 64)()
+        call("sps_search_get_schedule", self._h, buf, 64, C.byref(n))
+        return list(buf)[: n.value]
+
     def enable_timing(self, on=True):
         call("sps_search_enable_timing", self._h, int(on))
 

@@ -127,6 +127,13 @@ SPS_API sps_status sps_fdmt_get_rows(const sps_fdmt* h, int64_t iter, int64_t* h
 SPS_API sps_status sps_fdmt_execute(sps_fdmt* h, const void* d_waterfall, int dtype, size_t waterfall_size,
                                     float* d_dmt, size_t dmt_size, int ones, void* stream);
 
+/* Merge schedule (no reference counterpart; a tuning knob): the merge
+ * iterations run as passes that each fuse 1..3 consecutive iterations into one
+ * launch (same fp32 operations and results; intermediate states stay out of
+ * HBM).  h_depths[k] is the depth of pass k and must sum to niters. */
+SPS_API sps_status sps_fdmt_set_schedule(sps_fdmt* h, const int32_t* h_depths, int64_t n);
+SPS_API sps_status sps_fdmt_get_schedule(const sps_fdmt* h, int32_t* h_depths, int64_t cap, int64_t* h_n);
+
 /* FDMT::initialise (fdmt_base.hpp:57): the iteration-0 state, rows0 x nsamps */
 SPS_API sps_status sps_fdmt_initialise(sps_fdmt* h, const void* d_waterfall, int dtype, float* d_state,
                                        size_t state_size, void* stream);
@@ -225,6 +232,9 @@ SPS_API sps_status sps_search_run(sps_search* s, const void* d_in, float* d_dmt,
 SPS_API sps_status sps_search_sort_candidates(sps_search* s, sps_cand* d_cands, const uint64_t* d_count,
                                               int64_t* h_count, void* stream);
 SPS_API sps_status sps_search_get_ndt(const sps_search* s, int64_t* h_ndt);
+/* merge schedule of the search's data transform (see sps_fdmt_set_schedule) */
+SPS_API sps_status sps_search_set_schedule(sps_search* s, const int32_t* h_depths, int64_t n);
+SPS_API sps_status sps_search_get_schedule(const sps_search* s, int32_t* h_depths, int64_t cap, int64_t* h_n);
 /* Stage timing (no reference counterpart; a profiling aid).  When enabled,
  * sps_search_run records HIP events on its stream between stages; after a run,
  * sps_search_get_timing synchronises and writes the stage durations in ms:
