@@ -46,6 +46,22 @@ class SearchParams(C.Structure):
     ]
 
 
+class BatchBeam(C.Structure):
+    _fields_ = [
+        ("f_min", C.c_float), ("f_max", C.c_float), ("tsamp", C.c_float),
+        ("nchans", C.c_int64), ("nsamps", C.c_int64), ("dt_max", C.c_int64),
+        ("d_in", C.c_void_p), ("d_dmt", C.c_void_p), ("d_cands", C.c_void_p),
+        ("cand_capacity", C.c_int64), ("d_count", C.c_void_p),
+    ]
+
+
+class BatchParams(C.Structure):
+    _fields_ = [
+        ("in_dtype", C.c_int), ("norm_mode", C.c_int), ("snr_norm", C.c_int), ("box_q", C.c_int),
+        ("threshold", C.c_float), ("nworkers", C.c_int),
+    ]
+
+
 _i64, _f32, _vp, _int = C.c_int64, C.c_float, C.c_void_p, C.c_int
 _pi64 = C.POINTER(C.c_int64)
 _pf32 = C.POINTER(C.c_float)
@@ -86,6 +102,14 @@ SIGNATURES = {
     "sps_search_enable_timing": [_vp, _int],
     "sps_search_get_timing": [_vp, C.POINTER(C.c_double), _i64, _pi64],
     "sps_search_get_stage_bytes": [_vp, C.POINTER(C.c_double), _i64, _pi64],
+    "sps_batch_create": [C.POINTER(BatchParams), C.POINTER(BatchBeam), _i64, C.POINTER(_vp)],
+    "sps_batch_destroy": [_vp],
+    "sps_batch_get_beam_info": [_vp, _i64, _pi64, C.POINTER(C.c_int32), _pi64],
+    "sps_batch_set_schedule": [_vp, _i64, C.POINTER(C.c_int32), _i64],
+    "sps_batch_get_schedule": [_vp, _i64, C.POINTER(C.c_int32), _i64, _pi64],
+    "sps_batch_run": [_vp, _vp],
+    "sps_batch_launch": [_vp, _vp],
+    "sps_batch_sort_candidates": [_vp, _i64, _pi64, _vp],
 }
 
 _lib = None

@@ -713,10 +713,57 @@ struct sps_search {
         }
     }
 
-    void setup(hipStream_t st) {
-        if (ready) return;
+    // device scratch of one search (floats): ping-pong states, mask transform
+    // states and image, and the DM-time image when the caller passes none.
+    // A batch hands one set, sized for the largest of its beams, to every
+    // beam that runs on the same stream.
+    struct Scratch {
+        float *sa = nullptr, *sb = nullptr, *na = nullptr, *nb = nullptr, *nimg = nullptr, *dmt = nullptr;
+    };
+    struct ScratchSize {
+        int64_t sa = 0, sb = 0, na = 0, nb = 0, nimg = 0, dmt = 0;
+    };
+    ScratchSize scratch_size() const {
+        ScratchSize z;
+        z.sa = checked_mul(std::max<int64_t>(f->rows_a, 1), f->pitch, "state");
+        z.sb = checked_mul(std::max<int64_t>(f->rows_b, 1), f->pitch, "state");
+        if (p.snr_norm) {
+            const int64_t Pn = round_up(norm_len, 4);
+            z.na = checked_mul(std::max<int64_t>(f->rows_a, 1), Pn, "norm");
+            z.nb = checked_mul(std::max<int64_t>(f->rows_b, 1), Pn, "norm");
+            z.nimg = checked_mul(ndt, norm_len, "norm");
+        }
+        z.dmt = checked_mul(ndt, p.nsamps, "dmt");
+        return z;
+    }
+
+    void setup(hipStream_t st, bool own_scratch = true) {
+        if (!ready) setup_small(st);
+        if (own_scratch) {
+            f->ensure_states();
+            if (p.snr_norm) {
+                const ScratchSize z = scratch_size();
+                norm_img.ensure(sizeof(float) * (size_t)z.nimg);
+                norm_a.ensure(sizeof(float) * (size_t)z.na);
+                norm_b.ensure(sizeof(float) * (size_t)z.nb);
+            }
+        }
+        f->ensure_trees(st);
+    }
+
+    Scratch own_scratch() {
+        Scratch s;
+        s.sa = f->state_a.as<float>();
+        s.sb = f->state_b.as<float>();
+        s.na = norm_a.as<float>();
+        s.nb = norm_b.as<float>();
+        s.nimg = norm_img.as<float>();
+        return s;
+    }
+
+    // per-search tables and statistics buffers (small)
+    void setup_small(hipStream_t st) {
         f->ensure_device(st);
-        f->ensure_states();
         const int64_t R = p.nchans;
         rm.ensure(sizeof(double) * (size_t)R);
         rv.ensure(sizeof(double) * (size_t)R);
@@ -736,17 +783,83 @@ struct sps_search {
             std::vector<double> s = {0.0, 1.0, 0.0, 0.0, (double)R, 0.0, 0.0, 0.0};
             upload(stats, s, st);
         }
-        if (p.snr_norm) {
-            const int64_t L = norm_len;
-            const int64_t Pn = round_up(L, 4);
-            norm_img.ensure(sizeof(float) * (size_t)checked_mul(ndt, L, "norm"));
-            norm_a.ensure(sizeof(float) * (size_t)checked_mul(std::max<int64_t>(f->rows_a, 1), Pn, "norm"));
-            norm_b.ensure(sizeof(float) * (size_t)checked_mul(std::max<int64_t>(f->rows_b, 1), Pn, "norm"));
-        }
         SPS_HIP(hipStreamSynchronize(st));
         ready = true;
     }
+
+    // Enqueues one beam on st (no host synchronisation, no allocation, no
+    // memset: safe under stream capture once setup() has run).
+    void enqueue(const void* d_in, float* d_dmt, sps_cand* d_cands, uint64_t* d_count, hipStream_t st,
+                 const Scratch& sc, hipEvent_t* ev) {
+        const int64_t N = p.nsamps;
+        const int it = (int)f->schedule.size();  // passes
+        if (ev) SPS_HIP(hipEventRecord(ev[0], st));
+        if (p.norm_mode != SPS_NORM_NONE) {
+            launch_row_stats(d_in, p.in_dtype == SPS_U8, p.nchans, N, rm.as<double>(), rv.as<double>(), st);
+            launch_stats_finalize(rm.as<double>(), rv.as<double>(), p.nchans, p.norm_mode, stats.as<double>(),
+                                  mu.as<float>(), sg.as<float>(), keep.as<uint8_t>(), nullptr, st);
+        }
+        if (ev) SPS_HIP(hipEventRecord(ev[1], st));
+        float* out = d_dmt ? d_dmt : sc.dmt;
+        SPS_REQUIRE(out, "no DM-time image buffer");
+        // mask transform first: its output feeds the S/N epilogue of the last merge
+        MergeEpilogue epi{nullptr, 0, nullptr};
+        const bool fuse = p.snr_norm && f->eng.niters > 0;
+        if (p.snr_norm) {
+            ChanNorm mask{zeros.as<float>(), ones_f.as<float>(), keep.as<uint8_t>()};
+            const int64_t L = norm_len;
+            f->run(nullptr, 0, 1, N, &mask, L, sc.nimg, L, sc.na, sc.nb, round_up(L, 4), st, nullptr, nullptr, true);
+            epi = MergeEpilogue{sc.nimg, L, stats.as<double>()};
+        }
+        if (ev) SPS_HIP(hipEventRecord(ev[2], st));
+        ChanNorm cn{mu.as<float>(), sg.as<float>(), keep.as<uint8_t>()};
+        const ChanNorm* np = (p.norm_mode != SPS_NORM_NONE) ? &cn : nullptr;
+        f->run(d_in, p.in_dtype == SPS_U8, 0, N, np, N, out, N, sc.sa, sc.sb, f->pitch, st, ev ? ev + 3 : nullptr,
+               fuse ? &epi : nullptr);
+        if (p.snr_norm && !fuse)
+            launch_snr_normalise(out, ndt, N, N, sc.nimg, norm_len, stats.as<double>(), st);
+        if (ev) SPS_HIP(hipEventRecord(ev[it + 4], st));
+        launch_fill_u64(reinterpret_cast<unsigned long long*>(d_count), 1, 0ull, st);
+        launch_boxcar(out, ndt, N, N, nullptr, p.threshold, 1 << p.box_q, 1, reinterpret_cast<CandD*>(d_cands),
+                      p.cand_capacity, reinterpret_cast<unsigned long long*>(d_count), st);
+        if (ev) SPS_HIP(hipEventRecord(ev[it + 5], st));
+    }
 };
+
+static std::unique_ptr<sps_search> make_search(const sps_search_params* p) {
+    SPS_REQUIRE(p->in_dtype == SPS_F32 || p->in_dtype == SPS_U8, "unknown dtype");
+    SPS_REQUIRE(p->norm_mode >= SPS_NORM_NONE && p->norm_mode <= SPS_NORM_PER_CHANNEL, "unknown norm mode");
+    SPS_REQUIRE(p->box_q >= 0 && p->box_q <= 13, "box_q out of range");
+    SPS_REQUIRE(p->cand_capacity >= 0, "negative capacity");
+    std::unique_ptr<sps_search> s(new sps_search());
+    s->p = *p;
+    s->f.reset(new sps_fdmt());
+    s->f->plan.reset(new FdmtPlan(p->f_min, p->f_max, p->nchans, p->nsamps, p->tsamp, p->dt_max, 1, 0));
+    s->f->eng = s->f->plan->lower(kInitMean);
+    s->f->init_layout();
+    s->ndt = s->f->eng.out_rows;
+    // the mask transform is causal: samples t < L depend on inputs before
+    // L only, and beyond the longest channel track every kept channel
+    // contributes exactly 1, so the divisor there is the kept count
+    const EnginePlan& e = s->f->eng;
+    std::vector<int64_t> ext((size_t)e.rows[0]);
+    for (int64_t c = 0; c < e.nchans; ++c)
+        for (int64_t k = e.grid_off[(size_t)c]; k < e.grid_off[(size_t)c + 1]; ++k)
+            ext[(size_t)(e.chan_row_off[(size_t)c] + (k - e.grid_off[(size_t)c]))] = e.grid[(size_t)k];
+    for (int i = 1; i <= e.niters; ++i) {
+        std::vector<int64_t> nx((size_t)e.rows[(size_t)i], 0);
+        for (const auto& r : e.iter[(size_t)i]) {
+            int64_t v = ext[(size_t)r.tail_row];
+            if (r.head_row >= 0) v = std::max(v, ext[(size_t)r.head_row] + r.offset);
+            nx[(size_t)r.out_row] = v;
+        }
+        ext.swap(nx);
+    }
+    int64_t mx = 0;
+    for (auto v : ext) mx = std::max(mx, v);
+    s->norm_len = std::min<int64_t>(p->nsamps, mx + 1);
+    return s;
+}
 
 extern "C" {
 
@@ -754,38 +867,7 @@ SPS_API sps_status sps_search_create(const sps_search_params* p, sps_search** ou
     return guarded([&] {
         SPS_REQUIRE(p && out, "null argument");
         *out = nullptr;
-        SPS_REQUIRE(p->in_dtype == SPS_F32 || p->in_dtype == SPS_U8, "unknown dtype");
-        SPS_REQUIRE(p->norm_mode >= SPS_NORM_NONE && p->norm_mode <= SPS_NORM_PER_CHANNEL, "unknown norm mode");
-        SPS_REQUIRE(p->box_q >= 0 && p->box_q <= 13, "box_q out of range");
-        SPS_REQUIRE(p->cand_capacity >= 0, "negative capacity");
-        std::unique_ptr<sps_search> s(new sps_search());
-        s->p = *p;
-        s->f.reset(new sps_fdmt());
-        s->f->plan.reset(new FdmtPlan(p->f_min, p->f_max, p->nchans, p->nsamps, p->tsamp, p->dt_max, 1, 0));
-        s->f->eng = s->f->plan->lower(kInitMean);
-        s->f->init_layout();
-        s->ndt = s->f->eng.out_rows;
-        // the mask transform is causal: samples t < L depend on inputs before
-        // L only, and beyond the longest channel track every kept channel
-        // contributes exactly 1, so the divisor there is the kept count
-        const EnginePlan& e = s->f->eng;
-        std::vector<int64_t> ext((size_t)e.rows[0]);
-        for (int64_t c = 0; c < e.nchans; ++c)
-            for (int64_t k = e.grid_off[(size_t)c]; k < e.grid_off[(size_t)c + 1]; ++k)
-                ext[(size_t)(e.chan_row_off[(size_t)c] + (k - e.grid_off[(size_t)c]))] = e.grid[(size_t)k];
-        for (int i = 1; i <= e.niters; ++i) {
-            std::vector<int64_t> nx((size_t)e.rows[(size_t)i], 0);
-            for (const auto& r : e.iter[(size_t)i]) {
-                int64_t v = ext[(size_t)r.tail_row];
-                if (r.head_row >= 0) v = std::max(v, ext[(size_t)r.head_row] + r.offset);
-                nx[(size_t)r.out_row] = v;
-            }
-            ext.swap(nx);
-        }
-        int64_t mx = 0;
-        for (auto v : ext) mx = std::max(mx, v);
-        s->norm_len = std::min<int64_t>(p->nsamps, mx + 1);
-        *out = s.release();
+        *out = make_search(p).release();
     });
 }
 
@@ -807,48 +889,17 @@ SPS_API sps_status sps_search_run(sps_search* s, const void* d_in, float* d_dmt,
         SPS_REQUIRE(s->p.cand_capacity == 0 || d_cands, "null candidate buffer");
         hipStream_t st = as_stream(stream);
         s->setup(st);
-        const sps_search_params& p = s->p;
-        const int64_t N = p.nsamps;
-        const int it = (int)s->f->schedule.size();  // passes
+        sps_search::Scratch sc = s->own_scratch();
+        if (!d_dmt) {
+            s->dmt.ensure(sizeof(float) * (size_t)s->scratch_size().dmt);
+            sc.dmt = s->dmt.as<float>();
+        }
         hipEvent_t* ev = nullptr;
         if (s->timing) {
             s->ensure_events();
             ev = s->ev.data();
-            SPS_HIP(hipEventRecord(ev[0], st));
         }
-        if (p.norm_mode != SPS_NORM_NONE) {
-            launch_row_stats(d_in, p.in_dtype == SPS_U8, p.nchans, N, s->rm.as<double>(), s->rv.as<double>(), st);
-            launch_stats_finalize(s->rm.as<double>(), s->rv.as<double>(), p.nchans, p.norm_mode, s->stats.as<double>(),
-                                  s->mu.as<float>(), s->sg.as<float>(), s->keep.as<uint8_t>(), nullptr, st);
-        }
-        if (ev) SPS_HIP(hipEventRecord(ev[1], st));
-        float* out = d_dmt;
-        if (!out) {
-            s->dmt.ensure(sizeof(float) * (size_t)checked_mul(s->ndt, N, "dmt"));
-            out = s->dmt.as<float>();
-        }
-        // mask transform first: its output feeds the S/N epilogue of the last merge
-        MergeEpilogue epi{nullptr, 0, nullptr};
-        const bool fuse = p.snr_norm && s->f->eng.niters > 0;
-        if (p.snr_norm) {
-            ChanNorm mask{s->zeros.as<float>(), s->ones_f.as<float>(), s->keep.as<uint8_t>()};
-            const int64_t L = s->norm_len;
-            s->f->run(nullptr, 0, 1, N, &mask, L, s->norm_img.as<float>(), L, s->norm_a.as<float>(),
-                      s->norm_b.as<float>(), round_up(L, 4), st, nullptr, nullptr, true);
-            epi = MergeEpilogue{s->norm_img.as<float>(), L, s->stats.as<double>()};
-        }
-        if (ev) SPS_HIP(hipEventRecord(ev[2], st));
-        ChanNorm cn{s->mu.as<float>(), s->sg.as<float>(), s->keep.as<uint8_t>()};
-        const ChanNorm* np = (p.norm_mode != SPS_NORM_NONE) ? &cn : nullptr;
-        s->f->run(d_in, p.in_dtype == SPS_U8, 0, N, np, N, out, N, s->f->state_a.as<float>(), s->f->state_b.as<float>(),
-                  s->f->pitch, st, ev ? ev + 3 : nullptr, fuse ? &epi : nullptr);
-        if (p.snr_norm && !fuse)
-            launch_snr_normalise(out, s->ndt, N, N, s->norm_img.as<float>(), s->norm_len, s->stats.as<double>(), st);
-        if (ev) SPS_HIP(hipEventRecord(ev[it + 4], st));
-        launch_fill_u64(reinterpret_cast<unsigned long long*>(d_count), 1, 0ull, st);
-        launch_boxcar(out, s->ndt, N, N, nullptr, p.threshold, 1 << p.box_q, 1, reinterpret_cast<CandD*>(d_cands),
-                      p.cand_capacity, reinterpret_cast<unsigned long long*>(d_count), st);
-        if (ev) SPS_HIP(hipEventRecord(ev[it + 5], st));
+        s->enqueue(d_in, d_dmt, d_cands, d_count, st, sc, ev);
     });
 }
 
@@ -936,6 +987,266 @@ SPS_API sps_status sps_search_sort_candidates(sps_search* s, sps_cand* d_cands, 
         SPS_HIP(hipStreamSynchronize(st));
         const int64_t m = std::min<int64_t>((int64_t)n, s->p.cand_capacity);
         sort_cands(reinterpret_cast<CandD*>(d_cands), m, s->ndt, s->p.nsamps, s->p.box_q, st);
+        *h_count = (int64_t)n;
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batched multi-beam search
+//
+// Every beam is a full search (its own plan, tables and statistics).  Beams
+// are spread over a few worker streams by longest-first greedy assignment on
+// their byte cost; the beams of one worker run one after the other and share
+// one set of scratch buffers sized for the largest of them, so the HBM held
+// is the inputs, the candidate buffers and nworkers x the largest beam's
+// states.  The whole fork/join launch sequence is captured once into a HIP
+// graph and replayed; it holds kernels and event waits only.
+// ---------------------------------------------------------------------------
+struct sps_batch {
+    struct Beam {
+        sps_batch_beam io{};
+        std::unique_ptr<sps_search> s;
+        int worker = 0;
+        double cost = 0.0;
+    };
+    struct Worker {
+        hipStream_t st = nullptr;
+        hipEvent_t join = nullptr;
+        DevBuf sa, sb, na, nb, nimg, dmt;
+        sps_search::Scratch sc;
+        std::vector<int64_t> beams;
+    };
+    sps_batch_params p{};
+    std::vector<Beam> beams;
+    std::vector<Worker> workers;
+    hipEvent_t fork = nullptr;
+    hipStream_t cap = nullptr;  // capture origin
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool prepared = false;
+
+    ~sps_batch() {
+        drop_graph();
+        for (auto& w : workers) {
+            if (w.join) (void)hipEventDestroy(w.join);
+            if (w.st) (void)hipStreamDestroy(w.st);
+        }
+        if (fork) (void)hipEventDestroy(fork);
+        if (cap) (void)hipStreamDestroy(cap);
+    }
+
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+
+    void assign() {
+        std::vector<int64_t> order(beams.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = (int64_t)i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int64_t a, int64_t b) { return beams[(size_t)a].cost > beams[(size_t)b].cost; });
+        std::vector<double> load(workers.size(), 0.0);
+        for (auto& w : workers) w.beams.clear();
+        for (int64_t b : order) {
+            const size_t w = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+            load[w] += beams[(size_t)b].cost;
+            beams[(size_t)b].worker = (int)w;
+        }
+        // each worker runs its beams in index order
+        for (size_t b = 0; b < beams.size(); ++b) workers[(size_t)beams[b].worker].beams.push_back((int64_t)b);
+    }
+
+    void prepare(hipStream_t st) {
+        if (prepared) return;
+        for (auto& w : workers) {
+            sps_search::ScratchSize m;
+            bool need_dmt = false;
+            for (int64_t b : w.beams) {
+                const auto z = beams[(size_t)b].s->scratch_size();
+                m.sa = std::max(m.sa, z.sa);
+                m.sb = std::max(m.sb, z.sb);
+                m.na = std::max(m.na, z.na);
+                m.nb = std::max(m.nb, z.nb);
+                m.nimg = std::max(m.nimg, z.nimg);
+                if (!beams[(size_t)b].io.d_dmt) {
+                    m.dmt = std::max(m.dmt, z.dmt);
+                    need_dmt = true;
+                }
+            }
+            if (w.beams.empty()) continue;
+            w.sa.ensure(sizeof(float) * (size_t)m.sa);
+            w.sb.ensure(sizeof(float) * (size_t)m.sb);
+            if (p.snr_norm) {
+                w.na.ensure(sizeof(float) * (size_t)m.na);
+                w.nb.ensure(sizeof(float) * (size_t)m.nb);
+                w.nimg.ensure(sizeof(float) * (size_t)m.nimg);
+            }
+            if (need_dmt) w.dmt.ensure(sizeof(float) * (size_t)m.dmt);
+            w.sc.sa = w.sa.as<float>();
+            w.sc.sb = w.sb.as<float>();
+            w.sc.na = w.na.as<float>();
+            w.sc.nb = w.nb.as<float>();
+            w.sc.nimg = w.nimg.as<float>();
+            w.sc.dmt = w.dmt.as<float>();
+        }
+        for (auto& b : beams) b.s->setup(st, false);
+        SPS_HIP(hipStreamSynchronize(st));
+        prepared = true;
+    }
+
+    // fork on `origin`, every worker runs its beams, join back on `origin`
+    void enqueue(hipStream_t origin) {
+        SPS_HIP(hipEventRecord(fork, origin));
+        for (auto& w : workers) {
+            if (w.beams.empty()) continue;
+            SPS_HIP(hipStreamWaitEvent(w.st, fork, 0));
+            for (int64_t bi : w.beams) {
+                Beam& b = beams[(size_t)bi];
+                b.s->enqueue(b.io.d_in, b.io.d_dmt, b.io.d_cands, b.io.d_count, w.st, w.sc, nullptr);
+            }
+            SPS_HIP(hipEventRecord(w.join, w.st));
+            SPS_HIP(hipStreamWaitEvent(origin, w.join, 0));
+        }
+    }
+
+    void capture() {
+        drop_graph();
+        SPS_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueue(cap);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(cap, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        SPS_HIP(hipStreamEndCapture(cap, &graph));
+        SPS_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+};
+
+extern "C" {
+
+SPS_API sps_status sps_batch_create(const sps_batch_params* p, const sps_batch_beam* beams, int64_t nbeams,
+                                    sps_batch** out) {
+    return guarded([&] {
+        SPS_REQUIRE(p && out && (nbeams == 0 || beams), "null argument");
+        *out = nullptr;
+        SPS_REQUIRE(nbeams >= 1, "a batch needs at least one beam");
+        SPS_REQUIRE(p->nworkers >= 1 && p->nworkers <= 8, "nworkers must be 1..8");
+        std::unique_ptr<sps_batch> b(new sps_batch());
+        b->p = *p;
+        b->beams.resize((size_t)nbeams);
+        for (int64_t i = 0; i < nbeams; ++i) {
+            const sps_batch_beam& io = beams[i];
+            SPS_REQUIRE(io.d_in && io.d_count, "null beam buffer");
+            SPS_REQUIRE(io.cand_capacity >= 0 && (io.cand_capacity == 0 || io.d_cands), "bad candidate buffer");
+            sps_search_params sp{};
+            sp.f_min = io.f_min;
+            sp.f_max = io.f_max;
+            sp.tsamp = io.tsamp;
+            sp.nchans = io.nchans;
+            sp.nsamps = io.nsamps;
+            sp.dt_max = io.dt_max;
+            sp.in_dtype = p->in_dtype;
+            sp.norm_mode = p->norm_mode;
+            sp.snr_norm = p->snr_norm;
+            sp.box_q = p->box_q;
+            sp.threshold = p->threshold;
+            sp.cand_capacity = io.cand_capacity;
+            sps_batch::Beam& bm = b->beams[(size_t)i];
+            bm.io = io;
+            bm.s = make_search(&sp);
+            // byte cost of the beam: input read twice, every state written
+            // once and read once, the image read by the boxcar
+            const EnginePlan& e = bm.s->f->eng;
+            double rows = (double)e.nchans * 2.0 + (double)e.rows[0];
+            for (int k = 1; k <= e.niters; ++k) rows += (double)(e.rows[(size_t)k - 1] + e.rows[(size_t)k]);
+            bm.cost = rows * (double)e.nsamps;
+        }
+        b->workers = std::vector<sps_batch::Worker>((size_t)std::min<int64_t>(p->nworkers, nbeams));
+        for (auto& w : b->workers) {
+            SPS_HIP(hipStreamCreateWithFlags(&w.st, hipStreamNonBlocking));
+            SPS_HIP(hipEventCreateWithFlags(&w.join, hipEventDisableTiming));
+        }
+        SPS_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+        SPS_HIP(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+        b->assign();
+        *out = b.release();
+    });
+}
+
+SPS_API sps_status sps_batch_destroy(sps_batch* b) {
+    return guarded([&] { delete b; });
+}
+
+SPS_API sps_status sps_batch_get_beam_info(const sps_batch* b, int64_t beam, int64_t* h_ndt, int32_t* h_worker,
+                                           int64_t* h_niters) {
+    return guarded([&] {
+        SPS_REQUIRE(b, "null handle");
+        SPS_REQUIRE(beam >= 0 && beam < (int64_t)b->beams.size(), "beam index out of range");
+        const auto& bm = b->beams[(size_t)beam];
+        if (h_ndt) *h_ndt = bm.s->ndt;
+        if (h_worker) *h_worker = bm.worker;
+        if (h_niters) *h_niters = bm.s->f->eng.niters;
+    });
+}
+
+SPS_API sps_status sps_batch_set_schedule(sps_batch* b, int64_t beam, const int32_t* h_depths, int64_t n) {
+    return guarded([&] {
+        SPS_REQUIRE(b && (n == 0 || h_depths), "null argument");
+        SPS_REQUIRE(beam >= 0 && beam < (int64_t)b->beams.size(), "beam index out of range");
+        b->beams[(size_t)beam].s->f->set_schedule(std::vector<int>(h_depths, h_depths + n));
+        b->drop_graph();
+        b->prepared = false;  // the new pass tables are built before the next capture
+    });
+}
+
+SPS_API sps_status sps_batch_get_schedule(const sps_batch* b, int64_t beam, int32_t* h_depths, int64_t cap,
+                                          int64_t* h_n) {
+    return guarded([&] {
+        SPS_REQUIRE(b && h_n, "null argument");
+        SPS_REQUIRE(beam >= 0 && beam < (int64_t)b->beams.size(), "beam index out of range");
+        const auto& s = b->beams[(size_t)beam].s->f->schedule;
+        *h_n = (int64_t)s.size();
+        for (int64_t i = 0; i < *h_n && i < cap && h_depths; ++i) h_depths[i] = s[(size_t)i];
+    });
+}
+
+SPS_API sps_status sps_batch_run(sps_batch* b, void* stream) {
+    return guarded([&] {
+        SPS_REQUIRE(b, "null handle");
+        hipStream_t st = as_stream(stream);
+        b->prepare(st);
+        b->enqueue(st);
+    });
+}
+
+SPS_API sps_status sps_batch_launch(sps_batch* b, void* stream) {
+    return guarded([&] {
+        SPS_REQUIRE(b, "null handle");
+        hipStream_t st = as_stream(stream);
+        b->prepare(st);
+        if (!b->exec) b->capture();
+        SPS_HIP(hipGraphLaunch(b->exec, st));
+    });
+}
+
+SPS_API sps_status sps_batch_sort_candidates(sps_batch* b, int64_t beam, int64_t* h_count, void* stream) {
+    return guarded([&] {
+        SPS_REQUIRE(b && h_count, "null argument");
+        SPS_REQUIRE(beam >= 0 && beam < (int64_t)b->beams.size(), "beam index out of range");
+        auto& bm = b->beams[(size_t)beam];
+        hipStream_t st = as_stream(stream);
+        unsigned long long n = 0;
+        SPS_HIP(hipMemcpyAsync(&n, bm.io.d_count, sizeof(n), hipMemcpyDeviceToHost, st));
+        SPS_HIP(hipStreamSynchronize(st));
+        const int64_t m = std::min<int64_t>((int64_t)n, bm.io.cand_capacity);
+        sort_cands(reinterpret_cast<CandD*>(bm.io.d_cands), m, bm.s->ndt, bm.s->p.nsamps, bm.s->p.box_q, st);
         *h_count = (int64_t)n;
     });
 }

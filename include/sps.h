@@ -238,14 +238,68 @@ SPS_API sps_status sps_search_get_schedule(const sps_search* s, int32_t* h_depth
 /* Stage timing (no reference counterpart; a profiling aid).  When enabled,
  * sps_search_run records HIP events on its stream between stages; after a run,
  * sps_search_get_timing synchronises and writes the stage durations in ms:
- * [normalise stats, mask transform, FDMT init, merge 1 .. merge niters
+ * [normalise stats, mask transform, FDMT init, merge pass 1 .. merge pass P
  *  (the last one carries the fused S/N division), separate S/N
- *  normalisation (zero when fused), boxcar] (h_n receives niters + 5). */
+ *  normalisation (zero when fused), boxcar] (h_n receives P + 5, P = the
+ *  number of passes of the schedule). */
 SPS_API sps_status sps_search_enable_timing(sps_search* s, int enable);
 SPS_API sps_status sps_search_get_timing(sps_search* s, double* h_ms, int64_t cap, int64_t* h_n);
-/* Algorithmic bytes of each merge iteration, 4*nsamps*(rows_{i-1} + rows_i),
- * and of the other stages, in the order of sps_search_get_timing. */
+/* Compulsory bytes of each stage in the order of sps_search_get_timing: one
+ * read of a pass's input state plus one write of its output state,
+ * 4*nsamps*(rows_in + rows_out); the input read of the statistics pass; the
+ * input read and state-0 write of the init; the image read of the boxcar. */
 SPS_API sps_status sps_search_get_stage_bytes(const sps_search* s, double* h_bytes, int64_t cap, int64_t* h_n);
+
+/* ---------------------------------------------------------------------- */
+/* batched multi-beam search (no reference counterpart: the reference runs  */
+/* one chunk per CChunk_gpu::process call, CDMT/Chunk_gpu.cu:325-425).      */
+/* Many independent beams of any channel count and length run the search    */
+/* above in one launch sequence: beams are spread over nworkers streams     */
+/* (largest first onto the least loaded), the beams of one stream share one  */
+/* set of scratch states, and sps_batch_launch captures the whole fork/join */
+/* sequence into a HIP graph once and replays it on later calls.            */
+/* ---------------------------------------------------------------------- */
+typedef struct sps_batch sps_batch;
+
+typedef struct {
+    float f_min, f_max, tsamp;
+    int64_t nchans, nsamps, dt_max;
+    const void* d_in;       /* nchans x nsamps input of the batch's dtype */
+    float* d_dmt;           /* ndt x nsamps S/N image, or NULL (kept in scratch) */
+    sps_cand* d_cands;      /* cand_capacity candidates */
+    int64_t cand_capacity;
+    uint64_t* d_count;      /* candidates found, before capacity clipping */
+} sps_batch_beam;
+
+typedef struct {
+    int in_dtype;      /* sps_dtype, every beam */
+    int norm_mode;     /* sps_norm_mode */
+    int snr_norm;
+    int box_q;
+    float threshold;
+    int nworkers;      /* concurrent streams, 1..8 */
+} sps_batch_params;
+
+/* The buffers named in beams are bound to the batch (and to its graph). */
+SPS_API sps_status sps_batch_create(const sps_batch_params* p, const sps_batch_beam* beams, int64_t nbeams,
+                                    sps_batch** out);
+SPS_API sps_status sps_batch_destroy(sps_batch* b);
+/* ndt, the stream (worker) a beam runs on and its FDMT iteration count; any
+ * output may be NULL */
+SPS_API sps_status sps_batch_get_beam_info(const sps_batch* b, int64_t beam, int64_t* h_ndt, int32_t* h_worker,
+                                           int64_t* h_niters);
+/* merge schedule of one beam (see sps_fdmt_set_schedule); drops the graph */
+SPS_API sps_status sps_batch_set_schedule(sps_batch* b, int64_t beam, const int32_t* h_depths, int64_t n);
+SPS_API sps_status sps_batch_get_schedule(const sps_batch* b, int64_t beam, int32_t* h_depths, int64_t cap,
+                                          int64_t* h_n);
+/* Enqueues every beam directly (fork from / join to stream). */
+SPS_API sps_status sps_batch_run(sps_batch* b, void* stream);
+/* Replays the captured graph on stream (captured on the first call or after
+ * a schedule change). */
+SPS_API sps_status sps_batch_launch(sps_batch* b, void* stream);
+/* Sorts one beam's candidates by (mdt, mt, mwidth) and returns its count
+ * (synchronises stream). */
+SPS_API sps_status sps_batch_sort_candidates(sps_batch* b, int64_t beam, int64_t* h_count, void* stream);
 
 #ifdef __cplusplus
 }
