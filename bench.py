@@ -12,6 +12,11 @@ Prints ONE JSON line: DM-trials x samples / s, the merge kernel's achieved
 HBM bandwidth against 8 TB/s (HIP events on the launch stream, algorithmic
 bytes 4*nsamps*(rows_{i-1}+rows_i) per iteration), and the CPU oracle timed
 on a bounded time slice of the same beam.
+
+The "batched" object measures a ragged batch of 64 float32 beams (channel
+counts 1024/2048/4096, 2^14..2^18 samples) replayed as one captured HIP graph
+per step, next to the same beams run one at a time through the single-beam
+search with a count readback after each.
 """
 import argparse
 import json
@@ -54,6 +59,192 @@ def cpu_baseline(x_u8, ndt, sample_cols, threads):
     return ndt * sample_cols / dt, dt, len(cands)
 
 
+BATCH = dict(nbeams=64, nch_choices=(1024, 2048, 4096), n_lo=1 << 14, n_hi=1 << 18, seed=3, box_q=8, threshold=6.0)
+
+
+def batch_specs(nbeams):
+    """Ragged batch: channel count from {1024, 2048, 4096}, length uniform in
+    2^14..2^18 samples, 0-5 pulses, beam b seeded 3 + b; dt_max = nch - 1."""
+    import numpy as np
+
+    rng = np.random.default_rng(BATCH["seed"])
+    specs = []
+    for b in range(nbeams):
+        nch = int(rng.choice(BATCH["nch_choices"]))
+        ns = int(rng.integers(BATCH["n_lo"], BATCH["n_hi"] + 1))
+        npulses = int(rng.integers(0, 6))
+        specs.append(dict(nch=nch, ns=ns, npulses=npulses, seed=BATCH["seed"] + b))
+    return specs
+
+
+def plan_rows(nch):
+    return plan_rows_for(nch, BATCH["n_lo"], nch - 1)
+
+
+def alg_bytes(nch, ns, rows):
+    """Unfused streaming bytes of one beam (fp32): stats read, init read +
+    state-0 write, every merge's input read + output write, boxcar read."""
+    merges = sum(rows[i - 1] + rows[i] for i in range(1, len(rows)))
+    return 4.0 * ns * (nch + (nch + rows[0]) + merges + rows[-1])
+
+
+def bench_batch(args):
+    """Batched multi-beam search (one captured graph per step) against the
+    same beams run one at a time through the single-beam search, each
+    followed by its candidate-count readback."""
+    import torch
+
+    from cpp_projects_amd import synth
+    from cpp_projects_amd.batch import BatchSearch, Beam
+    from cpp_projects_amd.search import SingleBeamSearch
+    from cpp_projects_amd.tune import tune_search
+
+    specs = batch_specs(args.batch_beams)
+    rows = {nch: plan_rows(nch) for nch in {s["nch"] for s in specs}}
+    cap = 1 << 16
+    log(f"[batch] generating {len(specs)} beams")
+    ins, cands, counts = [], [], []
+    for s in specs:
+        x, _ = synth.f32_filterbank(s["nch"], s["ns"], CFG["f_min"], CFG["f_max"], s["seed"], npulses=s["npulses"],
+                                    max_sweep=s["nch"] - 1, width_range=(1.0, 16.0))
+        ins.append(x)
+        cands.append(torch.empty((cap, 4), dtype=torch.int32, device="cuda"))
+        counts.append(torch.zeros(1, dtype=torch.int64, device="cuda"))
+    torch.cuda.synchronize()
+    mk = dict(in_dtype="f32", box_q=BATCH["box_q"], threshold=BATCH["threshold"], cand_capacity=cap)
+    # schedule per channel count, tuned on the longest beam of that count
+    sched = {}
+    for nch in sorted(rows):
+        i = max((k for k, s in enumerate(specs) if s["nch"] == nch), key=lambda k: specs[k]["ns"])
+        s = SingleBeamSearch(CFG["f_min"], CFG["f_max"], nch, specs[i]["ns"], nch - 1, **mk)
+        s.run(ins[i], None, cands[i], counts[i])
+        best, _ = tune_search(s, lambda: s.run(ins[i], None, cands[i], counts[i]), reps=1)
+        sched[nch] = best
+        log(f"[batch] nch {nch}: schedule {best}")
+        del s
+    work = sum((s["nch"]) * s["ns"] for s in specs)  # DM trials x samples (ndt = nch)
+    tot_bytes = sum(alg_bytes(s["nch"], s["ns"], rows[s["nch"]]) for s in specs)
+
+    # plain: one beam per launch sequence, synchronised per beam
+    plain_s, plain_counts = 0.0, []
+    for i, s in enumerate(specs):
+        sb = SingleBeamSearch(CFG["f_min"], CFG["f_max"], s["nch"], s["ns"], s["nch"] - 1, **mk)
+        sb.set_schedule(sched[s["nch"]])
+        sb.run(ins[i], None, cands[i], counts[i])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sb.run(ins[i], None, cands[i], counts[i])
+        plain_counts.append(int(counts[i].item()))
+        plain_s += time.perf_counter() - t0
+        del sb
+    log(f"[batch] plain: {plain_s * 1e3:.1f} ms")
+
+    beams = [Beam(CFG["f_min"], CFG["f_max"], s["nch"], s["ns"], s["nch"] - 1, ins[i], cands[i], counts[i])
+             for i, s in enumerate(specs)]
+    b = BatchSearch(beams, in_dtype="f32", box_q=BATCH["box_q"], threshold=BATCH["threshold"],
+                    nworkers=args.batch_workers)
+    for i, s in enumerate(specs):
+        b.set_schedule(i, sched[s["nch"]])
+    b.launch()  # capture + first replay
+    torch.cuda.synchronize()
+    batch_counts = [int(c.item()) for c in counts]
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        b.launch()
+    torch.cuda.synchronize()
+    batch_s = (time.perf_counter() - t0) / args.batch_steps
+    log(f"[batch] graph: {batch_s * 1e3:.1f} ms per batch")
+    t0 = time.perf_counter()
+    b.run()
+    torch.cuda.synchronize()
+    eager_s = time.perf_counter() - t0
+    same = batch_counts == plain_counts
+    del b
+    return {
+        "workload": f"{len(specs)} beams, nch in {{1024,2048,4096}}, nsamps U[2^14,2^18], f32 N(0,1) + 0-5 pulses, "
+                    "dt_max nch-1, global norm + S/N norm, boxcar widths 1..256 pow2, thr 6",
+        "value": work / batch_s,
+        "unit": "DM-trials*samples/s",
+        "ms_per_batch": batch_s * 1e3,
+        "steps": args.batch_steps,
+        "plain_value": work / plain_s,
+        "plain_ms_per_batch": plain_s * 1e3,
+        "eager_ms_per_batch": eager_s * 1e3,
+        "speedup_vs_plain": plain_s / batch_s,
+        "alg_bytes_per_batch": tot_bytes,
+        "effective_GBs": tot_bytes / batch_s / 1e9,
+        "frac_of_hbm_peak": tot_bytes / batch_s / 1e9 / HBM_PEAK_GBS,
+        "workers": args.batch_workers,
+        "candidates": sum(batch_counts),
+        "candidate_counts_equal_plain": same,
+        "schedules": {str(k): v for k, v in sched.items()},
+    }
+
+
+LARGE = dict(nchans=8192, nsamps=1 << 20, dt_max=8191, box_q=9, threshold=6.0, npulses=50, seed=5, rfi_frac=0.1,
+             rfi_sigma=5.0)
+
+
+def bench_large(args):
+    """The large single-GPU share: 8192 ch x 2^20 float32 samples (32 GiB),
+    10% of the channels replaced by N(0, 25) interference, 50 pulses (sweep up
+    to 8192 samples, widths 1-512), dt_max 8191, widths 1..512."""
+    import torch
+
+    from cpp_projects_amd import synth
+    from cpp_projects_amd.search import SingleBeamSearch
+    from cpp_projects_amd.tune import tune_search
+
+    L = LARGE
+    R, N = L["nchans"], L["nsamps"]
+    log(f"[large] generating {R}x{N} f32 filterbank")
+    x, _ = synth.f32_filterbank(R, N, CFG["f_min"], CFG["f_max"], L["seed"], npulses=L["npulses"],
+                                max_sweep=L["dt_max"], width_range=(1.0, 512.0), rfi_frac=L["rfi_frac"],
+                                rfi_sigma=L["rfi_sigma"])
+    cap = 1 << 24
+    s = SingleBeamSearch(CFG["f_min"], CFG["f_max"], R, N, L["dt_max"], in_dtype="f32", box_q=L["box_q"],
+                         threshold=L["threshold"], cand_capacity=cap)
+    cands = torch.empty((cap, 4), dtype=torch.int32, device="cuda")
+    count = torch.zeros(1, dtype=torch.int64, device="cuda")
+    s.run(x, None, cands, count)
+    torch.cuda.synchronize()
+    best, _ = tune_search(s, lambda: s.run(x, None, cands, count), reps=1)
+    log(f"[large] schedule {best}")
+    s.run(x, None, cands, count)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.large_steps):
+        s.run(x, None, cands, count)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.large_steps
+    ncand = int(count.item())
+    ndt = s.ndt
+    rows = plan_rows_for(R, N, L["dt_max"])
+    b = alg_bytes(R, N, rows)
+    del s, cands, count, x
+    return {
+        "workload": "8192ch x 2^20 f32 (1200-1600 MHz), 10% channels N(0,25), 50 pulses, dt_max 8191, "
+                    "global norm + S/N norm, boxcar widths 1..512 pow2, thr 6",
+        "value": ndt * N / dt,
+        "unit": "DM-trials*samples/s",
+        "ms_per_step": dt * 1e3,
+        "steps": args.large_steps,
+        "alg_bytes_per_step": b,
+        "effective_GBs": b / dt / 1e9,
+        "frac_of_hbm_peak": b / dt / 1e9 / HBM_PEAK_GBS,
+        "candidates": ncand,
+        "candidates_per_s": ncand / dt,
+        "merge_schedule": best,
+    }
+
+
+def plan_rows_for(nch, ns, dt_max):
+    from cpp_projects_amd.fdmt import FDMTGPU
+
+    f = FDMTGPU(CFG["f_min"], CFG["f_max"], nch, ns, 1.0, dt_max)
+    return [int(f.state_shape(i)[3]) for i in range(f.get_niters() + 1)]
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "merge_traffic.json")
     if not os.path.exists(path):
@@ -73,6 +264,12 @@ def main():
     ap.add_argument("--schedule", type=str, default="",
                     help="comma-separated merge fusion depths (default: autotuned during warmup)")
     ap.add_argument("--no-tune", action="store_true", help="keep the library's default schedule")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-beam measurement")
+    ap.add_argument("--batch-beams", type=int, default=BATCH["nbeams"])
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--batch-workers", type=int, default=4)
+    ap.add_argument("--no-large", action="store_true", help="skip the 8192 x 2^20 large-share measurement")
+    ap.add_argument("--large-steps", type=int, default=3)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one plain device copy of this size first (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -207,11 +404,22 @@ def main():
         "stages_ms": stages,
         "effective_GBs_whole_step": total_alg / (sum(stage_ms) * 1e-3) / 1e9,
     }
+    xs = x[:, : args.cpu_sample_cols].cpu().numpy() if not args.no_cpu_baseline else None
+    if not args.no_batch:
+        del s, dmt, cands, count, x
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        out["batched"] = bench_batch(args)
+        torch.cuda.empty_cache()
+    if not args.no_large:
+        if args.no_batch:
+            del s, dmt, cands, count, x
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        out["large"] = bench_large(args)
+        torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
-        import numpy as np  # noqa: F401
-
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        xs = x[:, : args.cpu_sample_cols].cpu().numpy()
         log(f"[bench] cpu baseline on {R}x{args.cpu_sample_cols} with {threads} threads")
         v, dt, nc = cpu_baseline(xs, ndt, args.cpu_sample_cols, threads)
         out["cpu_baseline"] = {"value": v, "unit": "DM-trials*samples/s", "cores": threads, "kind": "port",

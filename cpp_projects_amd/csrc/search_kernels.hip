@@ -328,34 +328,56 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
         // the next level overwrites the buffer read two levels back only
         __syncthreads();
     }
-    // Exact recompute: the running sums of the thread's 8 cells advance
-    // together (independent chains), over a wave-uniform bound so that the
-    // candidates of each (cell slot, width) are emitted with one ballot and
-    // one global atomic per wave.
-    int top = -1;
+    // Compact the flagged cells into an LDS queue (cell << 14 | width flags;
+    // the level buffers are free now), so the exact recompute runs one chain
+    // per lane over flagged cells only.
+    uint32_t* queue = reinterpret_cast<uint32_t*>(la);
+    __shared__ uint32_t wave_tot[kThreads / 64];
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    uint32_t mine = 0;
+#pragma unroll
+    for (int j = 0; j < kPowPer; ++j) mine += flags[j] ? 1u : 0u;
+    uint32_t incl = mine;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t v = __shfl_up(incl, d);
+        if (lane >= d) incl += v;
+    }
+    if (lane == 63) wave_tot[wid] = incl;
+    __syncthreads();
+    uint32_t pos = incl - mine, total = 0;
+    for (int k = 0; k < kThreads / 64; ++k) {
+        if (k < wid) pos += wave_tot[k];
+        total += wave_tot[k];
+    }
+    if (total == 0) return;  // block-uniform
 #pragma unroll
     for (int j = 0; j < kPowPer; ++j)
-        if (flags[j]) top = max(top, 31 - __clz(flags[j]));
-    for (int off = 32; off > 0; off >>= 1) top = max(top, __shfl_xor(top, off));
-    if (top < 0) return;
-    const int lane = threadIdx.x & 63;
+        if (flags[j]) queue[pos++] = ((uint32_t)(j * kThreads + threadIdx.x) << 14) | flags[j];
+    __syncthreads();
+
+    // Exact recompute: lane i of a round takes queue entry i and runs the
+    // left-to-right running sum of its cell up to its largest flagged width;
+    // the loop bound is wave-uniform, so the candidates of each width are
+    // emitted with one ballot and one global atomic per wave.
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    float s[kPowPer];
-#pragma unroll
-    for (int j = 0; j < kPowPer; ++j) s[j] = 0.0f;
-    int p = 0, next = 1;
-    for (int iw = 0; iw < (1 << top); ++iw) {
-#pragma unroll
-        for (int j = 0; j < kPowPer; ++j)
-            if (flags[j] >> p) s[j] += x[j * kThreads + threadIdx.x + iw];
-        if (iw + 1 != next) continue;
-        const float c = sqrtf((float)next);
-#pragma unroll
-        for (int j = 0; j < kPowPer; ++j) {
+    for (uint32_t q0 = 0; q0 < total; q0 += kThreads) {
+        const uint32_t qi = q0 + threadIdx.x;
+        const uint32_t e = qi < total ? queue[qi] : 0u;
+        const int lt = (int)(e >> 14);
+        const uint32_t fl = e & 0x3fffu;
+        int top = fl ? 31 - __clz(fl) : -1;
+        for (int off = 32; off > 0; off >>= 1) top = max(top, __shfl_xor(top, off));
+        if (top < 0) continue;  // wave-uniform
+        float s = 0.0f;
+        int p = 0, next = 1;
+        for (int iw = 0; iw < (1 << top); ++iw) {
+            if (fl >> p) s += x[lt + iw];
+            if (iw + 1 != next) continue;
             float snr = 0.0f;
             bool pred = false;
-            if ((flags[j] >> p) & 1u) {
-                snr = s[j] / c;
+            if ((fl >> p) & 1u) {
+                snr = s / sqrtf((float)next);
                 pred = snr > thr;
             }
             const uint64_t m = __ballot(pred);
@@ -366,13 +388,12 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
                 base = __shfl(base, leader);
                 if (pred) {
                     const unsigned long long g = base + (unsigned long long)__popcll(m & lt_mask);
-                    if ((int64_t)g < cap)
-                        out[g] = CandD{(int32_t)(T + j * kThreads + threadIdx.x), (int32_t)r, (int32_t)p, snr};
+                    if ((int64_t)g < cap) out[g] = CandD{(int32_t)(T + lt), (int32_t)r, (int32_t)p, snr};
                 }
             }
+            next *= 2;
+            ++p;
         }
-        next *= 2;
-        ++p;
     }
 }
 
