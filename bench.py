@@ -129,7 +129,8 @@ def bench_batch(args):
     plain_s, plain_counts = 0.0, []
     for i, s in enumerate(specs):
         sb = SingleBeamSearch(CFG["f_min"], CFG["f_max"], s["nch"], s["ns"], s["nch"] - 1, **mk)
-        sb.set_schedule(sched[s["nch"]])
+        sb.set_staged(sched[s["nch"]]["staged"])
+        sb.set_schedule(sched[s["nch"]]["schedule"])
         sb.run(ins[i], None, cands[i], counts[i])
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -144,7 +145,8 @@ def bench_batch(args):
     b = BatchSearch(beams, in_dtype="f32", box_q=BATCH["box_q"], threshold=BATCH["threshold"],
                     nworkers=args.batch_workers)
     for i, s in enumerate(specs):
-        b.set_schedule(i, sched[s["nch"]])
+        b.set_staged(i, sched[s["nch"]]["staged"])
+        b.set_schedule(i, sched[s["nch"]]["schedule"])
     b.launch()  # capture + first replay
     torch.cuda.synchronize()
     batch_counts = [int(c.item()) for c in counts]
@@ -264,6 +266,7 @@ def main():
     ap.add_argument("--schedule", type=str, default="",
                     help="comma-separated merge fusion depths (default: autotuned during warmup)")
     ap.add_argument("--no-tune", action="store_true", help="keep the library's default schedule")
+    ap.add_argument("--staged", type=int, default=0, help="with --schedule/--no-tune: 1 = LDS-staged passes")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-beam measurement")
     ap.add_argument("--batch-beams", type=int, default=BATCH["nbeams"])
     ap.add_argument("--batch-steps", type=int, default=3)
@@ -307,6 +310,7 @@ def main():
         s.run(x, dmt, cands, count)
     torch.cuda.synchronize()
     if args.schedule:
+        s.set_staged(args.staged)
         s.set_schedule([int(v) for v in args.schedule.split(",")])
     elif not args.no_tune:
         from cpp_projects_amd.tune import tune_search
@@ -316,6 +320,7 @@ def main():
     s.run(x, dmt, cands, count)
     torch.cuda.synchronize()
     schedule = s.get_schedule()
+    staged = bool(args.staged) if (args.schedule or args.no_tune) else best["staged"]
     log(f"[bench] timing {args.steps} steps")
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -386,10 +391,11 @@ def main():
                         "global norm + S/N norm, boxcar widths 1..256 pow2, thr 6",
             "ndt": ndt, "nsamps": N, "nchans": R, "candidates": ncand, "path": "plain (one beam per launch sequence)",
             "merge_schedule": schedule,
+            "staged_passes": staged,
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"k_merge_fused<{dom}, true, false>",
+            "kernel": f"k_merge_group<{dom}, false>" if staged else f"k_merge_fused<{dom}, true, false>",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -68,6 +68,9 @@ class BatchSearch:
         arr = (C.c_int32 * max(len(depths), 1))(*depths)
         call("sps_batch_set_schedule", self._h, int(i), arr, len(depths))
 
+    def set_staged(self, i, on=True):
+        call("sps_batch_set_staged", self._h, int(i), int(on))
+
     def get_schedule(self, i):
         n = C.c_int64()
         buf = (C.c_int32 * 64)()

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -79,10 +80,21 @@ struct sps_fdmt {
     bool adj_built = false;
     std::vector<std::unique_ptr<DevBuf>> csr_off, csr_row, csr_shift;
 
-    // pass schedule: fused iterations per pass (1..kMaxFuse), summing to niters
+    // pass schedule: fused iterations per pass (1..kMaxFuse, or 1..kMaxGroupFuse
+    // for staged passes), summing to niters
     std::vector<int> schedule;
     std::vector<std::unique_ptr<DevBuf>> d_trees;  // per pass
     bool trees_ready = false;
+    // staged (LDS-window) passes
+    bool staged = false;
+    int group_floats = 16384;  // LDS window budget per block (floats)
+    int group_rows = 16;       // output rows per group at most
+    struct StagedPass {
+        DevBuf groups, leaves, rows;
+        int64_t ngroups = 0;
+        int lds_floats = 0;
+    };
+    std::vector<std::unique_ptr<StagedPass>> d_staged;  // per pass
 
     void init_layout() {
         pitch = round_up(eng.nsamps, 4);
@@ -112,13 +124,131 @@ struct sps_fdmt {
 
     void set_schedule(const std::vector<int>& s) {
         int sum = 0;
+        const int maxd = staged ? kMaxGroupFuse : kMaxFuse;
         for (int d : s) {
-            SPS_REQUIRE(d >= 1 && d <= kMaxFuse, "fusion depth must be 1..3");
+            SPS_REQUIRE(d >= 1 && d <= maxd, staged ? "staged fusion depth must be 1..4" : "fusion depth must be 1..3");
             sum += d;
         }
         SPS_REQUIRE(sum == eng.niters, "schedule must cover every iteration");
         schedule = s;
         trees_ready = false;
+    }
+
+    void set_staged(bool on) {
+        if (on == staged) return;
+        staged = on;
+        bool ok = true;
+        for (int d : schedule) ok = ok && d <= (staged ? kMaxGroupFuse : kMaxFuse);
+        if (!ok) schedule = default_schedule(eng.niters);
+        trees_ready = false;
+    }
+
+    // Staged pass covering iterations i0 .. i0+L-1: group consecutive output
+    // rows while their distinct leaves fit the LDS budget.
+    template <int L>
+    void build_groups_L(int i0, StagedPass& sp, hipStream_t st) const {
+        const int top = i0 + L - 1;
+        const int64_t R = eng.rows[(size_t)top];
+        const int NL = 1 << L;
+        std::vector<int64_t> leaf((size_t)R * NL, -1), shift((size_t)R * NL, 0);
+        std::vector<int32_t> offs((size_t)R * NL, -1);
+        for (int64_t r = 0; r < R; ++r) {
+            std::vector<int64_t> node((size_t)(2 << L), -1), sh((size_t)(2 << L), 0);
+            node[1] = r;
+            for (int k = 1; k < NL; ++k) {
+                if (node[(size_t)k] < 0) continue;
+                int dep = 0;
+                while ((2 << dep) <= k) ++dep;
+                const RowDesc& rd = eng.iter[(size_t)(top - dep)][(size_t)node[(size_t)k]];
+                node[(size_t)(2 * k)] = rd.tail_row;
+                sh[(size_t)(2 * k)] = sh[(size_t)k];
+                if (rd.head_row >= 0) {
+                    SPS_REQUIRE(rd.offset <= INT32_MAX, "offset too large");
+                    offs[(size_t)r * NL + k] = (int32_t)rd.offset;
+                    node[(size_t)(2 * k + 1)] = rd.head_row;
+                    sh[(size_t)(2 * k + 1)] = sh[(size_t)k] + rd.offset;
+                }
+            }
+            for (int j = 0; j < NL; ++j) {
+                leaf[(size_t)r * NL + j] = node[(size_t)(NL + j)];
+                shift[(size_t)r * NL + j] = sh[(size_t)(NL + j)];
+            }
+        }
+        auto units = [](int64_t smin, int64_t smax) -> int64_t {
+            return (kGroupTile + round_up(smax, 4) - smin + 3) / 4;
+        };
+        constexpr int64_t kMaxUnits = 192;  // per leaf window: 64 lanes x 3 float4
+        constexpr size_t kMaxLeaves = 64;
+        std::vector<GroupDescD> groups;
+        std::vector<LeafDescD> leaves;
+        std::vector<GRowDesc<L>> rows((size_t)R);
+        int maxf = 0;
+        int64_t r = 0;
+        while (r < R) {
+            std::map<int64_t, std::pair<int64_t, int64_t>> S;  // leaf row -> (smin, smax)
+            const int64_t r0 = r;
+            while (r < R && r - r0 < group_rows) {
+                auto S2 = S;
+                for (int j = 0; j < NL; ++j) {
+                    const int64_t lr = leaf[(size_t)r * NL + j];
+                    if (lr < 0) continue;
+                    const int64_t s = shift[(size_t)r * NL + j];
+                    auto it = S2.find(lr);
+                    if (it == S2.end())
+                        S2.emplace(lr, std::make_pair(s, s));
+                    else
+                        it->second = {std::min(it->second.first, s), std::max(it->second.second, s)};
+                }
+                bool ok = S2.size() <= kMaxLeaves;
+                int64_t floats = 0;
+                for (const auto& kv : S2) {
+                    const int64_t u = units(kv.second.first, kv.second.second);
+                    ok = ok && u <= kMaxUnits;
+                    floats += 4 * u;
+                }
+                ok = ok && floats <= group_floats;
+                if (!ok) {
+                    SPS_REQUIRE(r > r0, "one merge tree exceeds the staged window");
+                    break;
+                }
+                S.swap(S2);
+                ++r;
+            }
+            GroupDescD g{};
+            g.out_row0 = r0;
+            g.nrows = (int32_t)(r - r0);
+            g.leaf0 = (int64_t)leaves.size();
+            g.nleaves = (int32_t)S.size();
+            std::map<int64_t, int32_t> base;
+            int32_t off = 0;
+            for (const auto& kv : S) {
+                LeafDescD lf{};
+                lf.row = kv.first;
+                lf.smax4 = (int32_t)round_up(kv.second.second, 4);
+                lf.lds_off = off;
+                lf.nunits = (int32_t)units(kv.second.first, kv.second.second);
+                base[kv.first] = off + lf.smax4;
+                off += 4 * lf.nunits;
+                leaves.push_back(lf);
+            }
+            g.lds_floats = off;
+            maxf = std::max(maxf, (int)off);
+            groups.push_back(g);
+            for (int64_t rr = r0; rr < r; ++rr) {
+                GRowDesc<L>& d = rows[(size_t)rr];
+                d.out_row = rr;
+                for (int j = 0; j < NL; ++j) {
+                    const int64_t lr = leaf[(size_t)rr * NL + j];
+                    d.lbase[j] = lr >= 0 ? base[lr] : -1;
+                }
+                for (int k = 0; k < NL; ++k) d.off[k] = offs[(size_t)rr * NL + k];
+            }
+        }
+        upload(sp.groups, groups, st);
+        upload(sp.leaves, leaves, st);
+        upload(sp.rows, rows, st);
+        sp.ngroups = (int64_t)groups.size();
+        sp.lds_floats = maxf;
     }
 
     template <int L>
@@ -155,7 +285,22 @@ struct sps_fdmt {
     void ensure_trees(hipStream_t st) {
         if (trees_ready) return;
         d_trees.clear();
+        d_staged.clear();
         int i0 = 1;
+        if (staged) {
+            for (int L : schedule) {
+                d_staged.emplace_back(new StagedPass());
+                StagedPass& sp = *d_staged.back();
+                if (L == 1) build_groups_L<1>(i0, sp, st);
+                else if (L == 2) build_groups_L<2>(i0, sp, st);
+                else if (L == 3) build_groups_L<3>(i0, sp, st);
+                else build_groups_L<4>(i0, sp, st);
+                i0 += L;
+            }
+            SPS_HIP(hipStreamSynchronize(st));
+            trees_ready = true;
+            return;
+        }
         for (int L : schedule) {
             std::vector<uint8_t> bytes;
             if (L == 1) build_trees_L<1>(i0, bytes);
@@ -230,8 +375,15 @@ struct sps_fdmt {
             const float* src = (k % 2 == 0) ? sa : sb;
             const bool last = (top == eng.niters);
             float* dst = last ? out : ((k % 2 == 0) ? sb : sa);
-            launch_fdmt_fused(L, src, p, dst, last ? out_pitch : p, d_trees[k]->p, eng.rows[(size_t)top], nsamps,
-                              last ? eng.out_rows : INT64_MAX, st, last ? snr : nullptr);
+            if (staged) {
+                const StagedPass& sp = *d_staged[k];
+                launch_fdmt_group(L, src, p, dst, last ? out_pitch : p, sp.groups.p, sp.ngroups, sp.leaves.p,
+                                  sp.rows.p, sp.lds_floats, nsamps, last ? eng.out_rows : INT64_MAX, st,
+                                  last ? snr : nullptr);
+            } else {
+                launch_fdmt_fused(L, src, p, dst, last ? out_pitch : p, d_trees[k]->p, eng.rows[(size_t)top], nsamps,
+                                  last ? eng.out_rows : INT64_MAX, st, last ? snr : nullptr);
+            }
             if (ev) SPS_HIP(hipEventRecord(ev[k + 1], st));
             i0 += L;
         }
@@ -918,6 +1070,20 @@ SPS_API sps_status sps_fdmt_get_schedule(const sps_fdmt* h, int32_t* h_depths, i
     });
 }
 
+SPS_API sps_status sps_fdmt_set_staged(sps_fdmt* h, int staged) {
+    return guarded([&] {
+        SPS_REQUIRE(h, "null handle");
+        h->set_staged(staged != 0);
+    });
+}
+
+SPS_API sps_status sps_search_set_staged(sps_search* s, int staged) {
+    return guarded([&] {
+        SPS_REQUIRE(s, "null handle");
+        s->f->set_staged(staged != 0);
+    });
+}
+
 SPS_API sps_status sps_search_set_schedule(sps_search* s, const int32_t* h_depths, int64_t n) {
     return guarded([&] {
         SPS_REQUIRE(s && (n == 0 || h_depths), "null argument");
@@ -1203,6 +1369,16 @@ SPS_API sps_status sps_batch_set_schedule(sps_batch* b, int64_t beam, const int3
         b->beams[(size_t)beam].s->f->set_schedule(std::vector<int>(h_depths, h_depths + n));
         b->drop_graph();
         b->prepared = false;  // the new pass tables are built before the next capture
+    });
+}
+
+SPS_API sps_status sps_batch_set_staged(sps_batch* b, int64_t beam, int staged) {
+    return guarded([&] {
+        SPS_REQUIRE(b, "null handle");
+        SPS_REQUIRE(beam >= 0 && beam < (int64_t)b->beams.size(), "beam index out of range");
+        b->beams[(size_t)beam].s->f->set_staged(staged != 0);
+        b->drop_graph();
+        b->prepared = false;
     });
 }
 

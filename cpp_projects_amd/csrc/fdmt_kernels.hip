@@ -226,6 +226,95 @@ __global__ void __launch_bounds__(kThreads) k_merge_fused(const float* __restric
 }
 
 // ---------------------------------------------------------------------------
+// LDS-staged passes: one block = (group of consecutive output rows, 512-sample
+// tile).  Every distinct leaf row of the group is read from HBM once per tile
+// (aligned float4 units, each wave staging its own leaves with up to 12 loads
+// in flight per lane); the trees of all rows are then evaluated from LDS with
+// the same fp32 additions, in the same order, as the separate iterations.
+// ---------------------------------------------------------------------------
+template <int L, int K>
+struct LdsTree {
+    __device__ __forceinline__ static float eval(const float* __restrict__ lds, const GRowDesc<L>& d, int64_t t,
+                                                 int64_t T) {
+        if constexpr (K >= (1 << L)) {
+            return lds[d.lbase[K - (1 << L)] + (int)(t - T)];
+        } else {
+            float v = LdsTree<L, 2 * K>::eval(lds, d, t, T);
+            const int32_t off = d.off[K];
+            if (off >= 0 && t >= off) v = v + LdsTree<L, 2 * K + 1>::eval(lds, d, t - off, T);
+            return v;
+        }
+    }
+};
+
+constexpr int kStageLeavesPerRound = 4;  // leaves a wave stages per round
+constexpr int kStageUnitsPerLeaf = 3;    // float4 units per lane per leaf (<= 192 units = 768 floats)
+
+template <int L, bool SNR>
+__global__ void __launch_bounds__(kThreads) k_merge_group(const float* __restrict__ in, int64_t in_pitch,
+                                                          float* __restrict__ out, int64_t out_pitch,
+                                                          const GroupDescD* __restrict__ groups, int64_t ngroups,
+                                                          const LeafDescD* __restrict__ leaves,
+                                                          const GRowDesc<L>* __restrict__ rows, int64_t nsamps,
+                                                          int64_t out_rows_limit, SnrEpi epi) {
+    extern __shared__ float lds[];
+    const uint32_t logical = xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t gi = (int64_t)(logical % (uint64_t)ngroups);
+    const int64_t T = (int64_t)(logical / (uint64_t)ngroups) * kGroupTile;
+    const GroupDescD g = groups[gi];
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    constexpr int kWaves = kThreads / 64;
+    for (int l0 = wid * kStageLeavesPerRound; l0 < g.nleaves; l0 += kWaves * kStageLeavesPerRound) {
+        float4a buf[kStageLeavesPerRound][kStageUnitsPerLeaf];
+#pragma unroll
+        for (int a = 0; a < kStageLeavesPerRound; ++a) {
+            const int l = l0 + a;
+            LeafDescD lf{};
+            if (l < g.nleaves) lf = leaves[g.leaf0 + l];
+            const float* src = in + lf.row * in_pitch;
+#pragma unroll
+            for (int b = 0; b < kStageUnitsPerLeaf; ++b) {
+                const int k = lane + 64 * b;
+                const int64_t t = T - lf.smax4 + 4 * (int64_t)k;
+                float4a v = {0.0f, 0.0f, 0.0f, 0.0f};
+                if (l < g.nleaves && k < lf.nunits && t >= 0 && t + 4 <= in_pitch)
+                    v = *reinterpret_cast<const float4a*>(src + t);
+                buf[a][b] = v;
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < kStageLeavesPerRound; ++a) {
+            const int l = l0 + a;
+            if (l >= g.nleaves) continue;
+            const LeafDescD lf = leaves[g.leaf0 + l];
+#pragma unroll
+            for (int b = 0; b < kStageUnitsPerLeaf; ++b) {
+                const int k = lane + 64 * b;
+                if (k < lf.nunits) *reinterpret_cast<float4a*>(lds + lf.lds_off + 4 * k) = buf[a][b];
+            }
+        }
+    }
+    __syncthreads();
+    float dk = 1.0f;
+    if (SNR) dk = sqrtf((float)((double)(float)epi.stats[4] + 1.0E-8));
+    for (int i = 0; i < g.nrows; ++i) {
+        const GRowDesc<L> d = rows[g.out_row0 + i];
+        if (d.out_row >= out_rows_limit) continue;
+        float* __restrict__ o = out + d.out_row * out_pitch;
+        const float* nrow = SNR ? epi.nrm + d.out_row * epi.nrm_len : nullptr;
+#pragma unroll
+        for (int j = 0; j < kGroupTile / kThreads; ++j) {
+            const int64_t t = T + j * kThreads + threadIdx.x;
+            if (t < nsamps) {
+                float v = LdsTree<L, 1>::eval(lds, d, t, T);
+                if (SNR) v = v / ((t < epi.nrm_len) ? sqrtf((float)((double)nrow[t] + 1.0E-8)) : dk);
+                o[t] = v;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // init
 // ---------------------------------------------------------------------------
 template <int IN_KIND>  // 0 f32, 1 u8, 2 ones
@@ -569,6 +658,57 @@ void launch_fdmt_fused(int L, const float* in, int64_t in_pitch, float* out, int
             break;
         default:
             throw Error(SPS_ERR_INVALID_ARGUMENT, "fusion depth must be 1..3");
+    }
+    SPS_HIP(hipGetLastError());
+}
+
+template <int L>
+static void group_dispatch(bool snr, uint32_t nb, size_t lds, const float* in, int64_t in_pitch, float* out,
+                           int64_t out_pitch, const void* g, int64_t ng, const void* lv, const void* rw,
+                           int64_t nsamps, int64_t limit, SnrEpi epi, hipStream_t st) {
+    const GroupDescD* gp = static_cast<const GroupDescD*>(g);
+    const LeafDescD* lp = static_cast<const LeafDescD*>(lv);
+    const GRowDesc<L>* rp = static_cast<const GRowDesc<L>*>(rw);
+    if (snr)
+        hipLaunchKernelGGL((k_merge_group<L, true>), dim3(nb), dim3(kThreads), lds, st, in, in_pitch, out, out_pitch,
+                           gp, ng, lp, rp, nsamps, limit, epi);
+    else
+        hipLaunchKernelGGL((k_merge_group<L, false>), dim3(nb), dim3(kThreads), lds, st, in, in_pitch, out, out_pitch,
+                           gp, ng, lp, rp, nsamps, limit, epi);
+}
+
+void launch_fdmt_group(int L, const float* in, int64_t in_pitch, float* out, int64_t out_pitch, const void* d_groups,
+                       int64_t ngroups, const void* d_leaves, const void* d_rows, int lds_floats_max, int64_t nsamps,
+                       int64_t out_rows_limit, hipStream_t st, const MergeEpilogue* snr) {
+    if (ngroups <= 0 || nsamps <= 0) return;
+    if (in_pitch % 4 != 0 || reinterpret_cast<uintptr_t>(in) % 16 != 0)
+        throw Error(SPS_ERR_INVALID_ARGUMENT, "staged pass needs a 16-byte aligned input state");
+    if (lds_floats_max <= 0 || lds_floats_max > 40 * 1024)
+        throw Error(SPS_ERR_INVALID_ARGUMENT, "staged pass LDS window out of range");
+    const int64_t nb = ngroups * ceil_div(nsamps, kGroupTile);
+    if (nb > 0x7fffffffLL) throw Error(SPS_ERR_INVALID_ARGUMENT, "merge grid too large");
+    SnrEpi epi{nullptr, 0, nullptr};
+    if (snr) epi = SnrEpi{snr->nrm, snr->nrm_len, snr->stats};
+    const size_t lds = sizeof(float) * (size_t)lds_floats_max;
+    switch (L) {
+        case 1:
+            group_dispatch<1>(snr, (uint32_t)nb, lds, in, in_pitch, out, out_pitch, d_groups, ngroups, d_leaves,
+                              d_rows, nsamps, out_rows_limit, epi, st);
+            break;
+        case 2:
+            group_dispatch<2>(snr, (uint32_t)nb, lds, in, in_pitch, out, out_pitch, d_groups, ngroups, d_leaves,
+                              d_rows, nsamps, out_rows_limit, epi, st);
+            break;
+        case 3:
+            group_dispatch<3>(snr, (uint32_t)nb, lds, in, in_pitch, out, out_pitch, d_groups, ngroups, d_leaves,
+                              d_rows, nsamps, out_rows_limit, epi, st);
+            break;
+        case 4:
+            group_dispatch<4>(snr, (uint32_t)nb, lds, in, in_pitch, out, out_pitch, d_groups, ngroups, d_leaves,
+                              d_rows, nsamps, out_rows_limit, epi, st);
+            break;
+        default:
+            throw Error(SPS_ERR_INVALID_ARGUMENT, "staged fusion depth must be 1..4");
     }
     SPS_HIP(hipGetLastError());
 }

@@ -49,6 +49,37 @@ struct TreeDesc {
 };
 constexpr int kMaxFuse = 3;
 
+// LDS-staged pass of L fused iterations.  Consecutive output rows are grouped
+// so that the distinct input rows their trees read (the leaves) fit one LDS
+// window per block: a block (group, 512-sample tile) loads every leaf row of
+// the group once, over times [T - smax4, T - smax4 + nunits*4), and evaluates
+// the trees of all the group's rows from LDS.
+constexpr int kGroupTile = 512;
+constexpr int kMaxGroupFuse = 4;
+struct GroupDescD {
+    int64_t out_row0;  // first output row (rows are consecutive)
+    int64_t leaf0;     // first LeafDescD of the group
+    int32_t nrows;
+    int32_t nleaves;
+    int32_t lds_floats;
+    int32_t pad;
+};
+struct LeafDescD {
+    int64_t row;       // input state row
+    int32_t smax4;     // largest shift it is read with, rounded up to 4
+    int32_t lds_off;   // float offset of its window (multiple of 4)
+    int32_t nunits;    // window length in float4 units
+    int32_t pad;
+};
+// tree of one output row with leaves as LDS bases: the leaf read at time u
+// is lds[lbase + u - T] (lbase = lds_off + smax4), -1 for a missing leaf
+template <int L>
+struct GRowDesc {
+    int64_t out_row;
+    int32_t lbase[1 << L];
+    int32_t off[1 << L];
+};
+
 // Optional epilogue of the last merge: divide by sqrtf(n + 1e-8) with n from
 // nrm (rows x nrm_len) for t < nrm_len and stats[4] (kept channels) beyond.
 struct MergeEpilogue {
@@ -67,6 +98,12 @@ void launch_fdmt_merge(const float* in, int64_t in_pitch, float* out, int64_t ou
 void launch_fdmt_fused(int L, const float* in, int64_t in_pitch, float* out, int64_t out_pitch, const void* d_trees,
                        int64_t nrows, int64_t nsamps, int64_t out_rows_limit, hipStream_t st,
                        const MergeEpilogue* snr = nullptr);
+
+// A staged pass (1 <= L <= kMaxGroupFuse) over ngroups GroupDescD; in must be
+// 16-byte aligned with in_pitch % 4 == 0 (the internal states are).
+void launch_fdmt_group(int L, const float* in, int64_t in_pitch, float* out, int64_t out_pitch, const void* d_groups,
+                       int64_t ngroups, const void* d_leaves, const void* d_rows, int lds_floats_max, int64_t nsamps,
+                       int64_t out_rows_limit, hipStream_t st, const MergeEpilogue* snr = nullptr);
 
 // Adjoint of one merge: g_prev[p][t] = sum over (o, s) in csr[p] of g_out[o][t+s] (t+s < nsamps).
 void launch_fdmt_merge_adjoint(const float* g_out, int64_t out_pitch, float* g_prev, int64_t prev_pitch,

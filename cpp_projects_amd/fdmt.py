@@ -76,9 +76,15 @@ class _Handle:
         return out[: n.value]
 
     def set_schedule(self, depths):
-        """Merge passes: each fuses 1..3 consecutive iterations (sum = niters)."""
+        """Merge passes: each fuses 1..3 consecutive iterations, 1..4 when
+        staged (sum = niters)."""
         arr = (C.c_int32 * max(len(depths), 1))(*depths)
         call("sps_fdmt_set_schedule", self._h, arr, len(depths))
+
+    def set_staged(self, on=True):
+        """LDS-staged passes (grouped output rows, each input row read once
+        per time tile)."""
+        call("sps_fdmt_set_staged", self._h, int(on))
 
     def get_schedule(self):
         n = C.c_int64()

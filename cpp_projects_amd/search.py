@@ -187,6 +187,9 @@ class SingleBeamSearch:
         arr = (C.c_int32 * max(len(depths), 1))(*depths)
         call("sps_search_set_schedule", self._h, arr, len(depths))
 
+    def set_staged(self, on=True):
+        call("sps_search_set_staged", self._h, int(on))
+
     def get_schedule(self):
         n = C.c_int64()
         buf = (C.c_int32 * 64)()

@@ -133,6 +133,13 @@ SPS_API sps_status sps_fdmt_execute(sps_fdmt* h, const void* d_waterfall, int dt
  * HBM).  h_depths[k] is the depth of pass k and must sum to niters. */
 SPS_API sps_status sps_fdmt_set_schedule(sps_fdmt* h, const int32_t* h_depths, int64_t n);
 SPS_API sps_status sps_fdmt_get_schedule(const sps_fdmt* h, int32_t* h_depths, int64_t cap, int64_t* h_n);
+/* Staged passes (no reference counterpart; a tuning knob): each pass groups
+ * consecutive output rows so that the input rows their merge trees read fit
+ * one LDS window per block, reads each of those rows once per time tile and
+ * evaluates the trees from LDS (same fp32 operations and results).  Staged
+ * passes fuse 1..4 iterations; turning staging off resets a schedule with
+ * depth-4 passes to the default. */
+SPS_API sps_status sps_fdmt_set_staged(sps_fdmt* h, int staged);
 
 /* FDMT::initialise (fdmt_base.hpp:57): the iteration-0 state, rows0 x nsamps */
 SPS_API sps_status sps_fdmt_initialise(sps_fdmt* h, const void* d_waterfall, int dtype, float* d_state,
@@ -232,6 +239,8 @@ SPS_API sps_status sps_search_run(sps_search* s, const void* d_in, float* d_dmt,
 SPS_API sps_status sps_search_sort_candidates(sps_search* s, sps_cand* d_cands, const uint64_t* d_count,
                                               int64_t* h_count, void* stream);
 SPS_API sps_status sps_search_get_ndt(const sps_search* s, int64_t* h_ndt);
+/* staged passes of the search's data transform (see sps_fdmt_set_staged) */
+SPS_API sps_status sps_search_set_staged(sps_search* s, int staged);
 /* merge schedule of the search's data transform (see sps_fdmt_set_schedule) */
 SPS_API sps_status sps_search_set_schedule(sps_search* s, const int32_t* h_depths, int64_t n);
 SPS_API sps_status sps_search_get_schedule(const sps_search* s, int32_t* h_depths, int64_t cap, int64_t* h_n);
@@ -288,6 +297,8 @@ SPS_API sps_status sps_batch_destroy(sps_batch* b);
  * output may be NULL */
 SPS_API sps_status sps_batch_get_beam_info(const sps_batch* b, int64_t beam, int64_t* h_ndt, int32_t* h_worker,
                                            int64_t* h_niters);
+/* staged passes of one beam (see sps_fdmt_set_staged); drops the graph */
+SPS_API sps_status sps_batch_set_staged(sps_batch* b, int64_t beam, int staged);
 /* merge schedule of one beam (see sps_fdmt_set_schedule); drops the graph */
 SPS_API sps_status sps_batch_set_schedule(sps_batch* b, int64_t beam, const int32_t* h_depths, int64_t n);
 SPS_API sps_status sps_batch_get_schedule(const sps_batch* b, int64_t beam, int32_t* h_depths, int64_t cap,

@@ -157,6 +157,15 @@ def test_batch_schedule_change_and_scratch_images(torch_cuda):
     bufs.poison()
     b.launch()
     check(torch, b, bufs, ref, "after schedule change")
+    for i in (0, 3, 5):
+        n = b.beam_info(i)[2]
+        b.set_staged(i, True)
+        s = [4] * (n // 4) + [n % 4] * (n % 4 > 0)
+        b.set_schedule(i, s)
+        assert b.get_schedule(i) == s
+    bufs.poison()
+    b.launch()
+    check(torch, b, bufs, ref, "staged passes")
 
 
 def test_batch_beam_vs_oracle(torch_cuda):

@@ -126,3 +126,111 @@ def test_search_schedules_same_candidates(torch_cuda):
         n = s.sort_candidates(c, cnt)
         outs.append((sha(dmt.cpu().numpy()), cands_to_numpy(c, n).tobytes()))
     assert all(o == outs[0] for o in outs)
+
+
+def test_staged_all_schedules_small_odd_plan(torch_cuda):
+    from cpp_projects_amd.fdmt import FDMTGPU
+
+    nch, ns, dtm = 37, 300, 36
+    p = orc.Plan(400.0, 800.0, nch, ns, 1.0, dtm)
+    x = np.random.default_rng(1).standard_normal((nch, ns)).astype(np.float32)
+    ref = p.execute(x)
+    f = FDMTGPU(400.0, 800.0, nch, ns, 1.0, dtm)
+    f.set_staged(True)
+    scheds = list(compositions(p.niters, (1, 2, 3, 4)))
+    assert any(4 in s for s in scheds)
+    for s in scheds:
+        f.set_schedule(s)
+        assert f.get_schedule() == s
+        np.testing.assert_array_equal(run(torch_cuda, f, x).view(np.uint32), ref.view(np.uint32), err_msg=str(s))
+
+
+def test_staged_on_golden(torch_cuda, golden):
+    from cpp_projects_amd.fdmt import FDMTGPU
+
+    man, arr = golden
+    for name in ("c1000odd", "c1024", "c3000", "c256step2", "c512file"):
+        e = man["fdmt"][name]
+        x = make_input(e["input"], e["nchans"], e["nsamps"], e["seed"], arr)
+        f = FDMTGPU(e["fmin"], e["fmax"], e["nchans"], e["nsamps"], 1.0, e["dt_max"], e["dt_step"], e["dt_min"])
+        f.set_staged(True)
+        n = e["niters"]
+        cands = [[4] * (n // 4) + [n % 4] * (n % 4 > 0), [1] * n, [3] * (n // 3) + [n % 3] * (n % 3 > 0),
+                 [2, 4] + [3] * ((n - 6) // 3) + [(n - 6) % 3] * ((n - 6) % 3 > 0)]
+        for s in cands:
+            s = [d for d in s if d > 0]
+            assert sum(s) == n
+            f.set_schedule(s)
+            print("case staged", name, s, flush=True)
+            assert sha(run(torch_cuda, f, x)) == e["out_sha256"], (name, s)
+
+
+def test_staged_long_rows_vs_oracle(torch_cuda):
+    """Many time tiles, a length that is not a multiple of 4 or of the tile,
+    and a wide delay range (groups whose leaves span large shifts)."""
+    from cpp_projects_amd.fdmt import FDMTGPU
+
+    for nch, ns, dtm in ((512, 5001, 511), (256, 3000, 1000), (1024, 2050, 1023)):
+        p = orc.Plan(1200.0, 1600.0, nch, ns, 1.0, dtm)
+        x = np.random.default_rng(nch).standard_normal((nch, ns)).astype(np.float32)
+        ref = p.execute(x)
+        f = FDMTGPU(1200.0, 1600.0, nch, ns, 1.0, dtm)
+        f.set_staged(True)
+        n = p.niters
+        for s in ([4] * (n // 4) + [n % 4] * (n % 4 > 0), [2] * (n // 2) + [1] * (n % 2)):
+            s = [d for d in s if d > 0]
+            f.set_schedule(s)
+            print("case staged long", nch, ns, dtm, s, flush=True)
+            np.testing.assert_array_equal(run(torch_cuda, f, x).view(np.uint32), ref.view(np.uint32),
+                                          err_msg=f"{nch} {ns} {s}")
+
+
+def test_staged_cfdmt_and_toggle(torch_cuda, golden):
+    from cpp_projects_amd._lib import InvalidArgument
+    from cpp_projects_amd.fdmt import CFdmtGpu
+
+    torch = torch_cuda
+    man, arr = golden
+    e = man["cfdmt"]["lc1000"]
+    x = make_input(e["input"], e["nchan"], e["cols"], e["seed"], arr)
+    f = CFdmtGpu(e["fmin"], e["fmax"], e["nchan"], e["cols"], e["imaxdt"])
+    n = f.get_niters()
+    d_in = torch.from_numpy(x).cuda()
+    d_out = torch.empty((e["imaxdt"], e["cols"]), dtype=torch.float32, device="cuda")
+    f.set_staged(True)
+    f.set_schedule([4] * (n // 4) + [1] * (n % 4))
+    f.process_image(d_in, d_out, False)
+    torch.cuda.synchronize()
+    assert sha(d_out.cpu().numpy()) == e["out_sha256"]
+    # leaving staged mode drops a schedule with depth-4 passes
+    f.set_staged(False)
+    assert max(f.get_schedule()) <= 3 and sum(f.get_schedule()) == n
+    with pytest.raises(InvalidArgument):
+        f.set_schedule([4] * (n // 4) + [1] * (n % 4))
+    f.process_image(d_in, d_out, False)
+    torch.cuda.synchronize()
+    assert sha(d_out.cpu().numpy()) == e["out_sha256"]
+
+
+def test_search_staged_same_candidates(torch_cuda):
+    from cpp_projects_amd.search import SingleBeamSearch, cands_to_numpy
+
+    torch = torch_cuda
+    rng = np.random.default_rng(19)
+    nch, ns = 256, 8190
+    x = np.clip(np.round(rng.normal(128, 16, (nch, ns))), 0, 255).astype(np.uint8)
+    x[:, 3000:3004] = 200
+    s = SingleBeamSearch(1200.0, 1600.0, nch, ns, nch - 1, in_dtype="u8", box_q=6, threshold=5.0,
+                         cand_capacity=1 << 18)
+    d_in = torch.from_numpy(x).cuda()
+    dmt = torch.empty((s.ndt, ns), device="cuda")
+    c = torch.empty((1 << 18, 4), dtype=torch.int32, device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    outs = []
+    for staged, sched in ((False, [3, 3, 2]), (True, [4, 4]), (True, [1, 3, 4]), (True, [2, 2, 2, 2])):
+        s.set_staged(staged)
+        s.set_schedule(sched)
+        s.run(d_in, dmt, c, cnt)
+        n = s.sort_candidates(c, cnt)
+        outs.append((sha(dmt.cpu().numpy()), cands_to_numpy(c, n).tobytes()))
+    assert all(o == outs[0] for o in outs)
