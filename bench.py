@@ -77,6 +77,21 @@ def batch_specs(nbeams):
     return specs
 
 
+def live_rows(plan, out_rows):
+    """Rows of each FDMT state that some delivered output row depends on (the
+    library computes only these)."""
+    n = plan.get_niters()
+    live = [set() for _ in range(n + 1)]
+    live[n] = set(range(out_rows))
+    for i in range(n, 0, -1):
+        for o, t, h, _ in plan.rows(i):
+            if int(o) in live[i]:
+                live[i - 1].add(int(t))
+                if h >= 0:
+                    live[i - 1].add(int(h))
+    return [len(v) for v in live]
+
+
 def plan_rows(nch):
     return plan_rows_for(nch, BATCH["n_lo"], nch - 1)
 
@@ -130,6 +145,8 @@ def bench_batch(args):
     for i, s in enumerate(specs):
         sb = SingleBeamSearch(CFG["f_min"], CFG["f_max"], s["nch"], s["ns"], s["nch"] - 1, **mk)
         sb.set_staged(sched[s["nch"]]["staged"])
+        if sched[s["nch"]]["shape"]:
+            sb.set_group_shape(*sched[s["nch"]]["shape"])
         sb.set_schedule(sched[s["nch"]]["schedule"])
         sb.run(ins[i], None, cands[i], counts[i])
         torch.cuda.synchronize()
@@ -146,6 +163,8 @@ def bench_batch(args):
                     nworkers=args.batch_workers)
     for i, s in enumerate(specs):
         b.set_staged(i, sched[s["nch"]]["staged"])
+        if sched[s["nch"]]["shape"]:
+            b.set_group_shape(i, *sched[s["nch"]]["shape"])
         b.set_schedule(i, sched[s["nch"]]["schedule"])
     b.launch()  # capture + first replay
     torch.cuda.synchronize()
@@ -266,7 +285,11 @@ def main():
     ap.add_argument("--schedule", type=str, default="",
                     help="comma-separated merge fusion depths (default: autotuned during warmup)")
     ap.add_argument("--no-tune", action="store_true", help="keep the library's default schedule")
-    ap.add_argument("--staged", type=int, default=0, help="with --schedule/--no-tune: 1 = LDS-staged passes")
+    ap.add_argument("--staged", type=int, default=0,
+                    help="with --schedule/--no-tune: staging mode 0..4 (0 tree passes, 1 staged passes with a "
+                         "tree final pass, 3 all staged; 2, 4 add the fused init)")
+    ap.add_argument("--group-shape", type=str, default="6144,16,16",
+                    help="with --schedule and --staged > 0: LDS floats, rows per group, tiles per block")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-beam measurement")
     ap.add_argument("--batch-beams", type=int, default=BATCH["nbeams"])
     ap.add_argument("--batch-steps", type=int, default=3)
@@ -309,18 +332,23 @@ def main():
     for _ in range(args.warmup):
         s.run(x, dmt, cands, count)
     torch.cuda.synchronize()
+    shape = None
     if args.schedule:
         s.set_staged(args.staged)
+        if args.staged:
+            shape = [int(v) for v in args.group_shape.split(",")]
+            s.set_group_shape(*shape)
         s.set_schedule([int(v) for v in args.schedule.split(",")])
     elif not args.no_tune:
         from cpp_projects_amd.tune import tune_search
 
         best, _ = tune_search(s, lambda: s.run(x, dmt, cands, count), reps=2, log=log)
+        shape = best["shape"]
         log(f"[bench] tuned schedule {best}")
     s.run(x, dmt, cands, count)
     torch.cuda.synchronize()
     schedule = s.get_schedule()
-    staged = bool(args.staged) if (args.schedule or args.no_tune) else best["staged"]
+    staged = int(args.staged) if (args.schedule or args.no_tune) else int(best["staged"])
     log(f"[bench] timing {args.steps} steps")
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -349,21 +377,37 @@ def main():
 
     plan = FDMTGPU(c["f_min"], c["f_max"], R, N, 1.0, c["dt_max"])
     rows = [int(plan.state_shape(i)[3]) for i in range(sum(schedule) + 1)]
+    live = live_rows(plan, ndt)
     alg, io, i0 = [], [], 1
     for L in schedule:
         alg.append(sum(4.0 * N * (rows[i - 1] + rows[i]) for i in range(i0, i0 + L)))
-        io.append(4.0 * N * (rows[i0 - 1] + rows[i0 + L - 1]))
+        # compulsory traffic: the live rows of the pass's input and output states
+        io.append(4.0 * N * (live[i0 - 1] + live[i0 + L - 1]))
         i0 += L
     pass_ms = stage_ms[3:3 + npass]
-    # dominant kernel: the non-final fused passes of the depth with most time
-    # (the final pass is a separate instantiation carrying the S/N division)
-    by_depth = {}
+    fused_init = staged in (2, 4) and npass >= 2 and stage_ms[2] < 0.05
+    if fused_init:
+        # the first pass also performs the init: u8 input read, no state 0
+        alg[0] += 4.0 * N * (R + rows[0])
+        io[0] = 1.0 * N * R + 4.0 * N * live[schedule[0]]
+
+    def kname(k):
+        L, last = schedule[k], k == npass - 1
+        if k == 0 and fused_init:
+            return f"k_merge_group_init<{L}, 1, true>"
+        if staged and not (last and staged in (1, 2) and L <= 3):
+            return f"k_merge_group<{L}, {'true' if last else 'false'}>"
+        return f"k_merge_fused<{L}, true, {'true' if last else 'false'}>"
+
+    # dominant kernel: the non-final passes of the kernel instantiation with
+    # the most time (the final pass carries the S/N division)
+    by_kernel = {}
     for k in range(npass - 1):
-        by_depth.setdefault(schedule[k], []).append(k)
-    if not by_depth:
-        by_depth = {schedule[-1]: [npass - 1]}
-    dom = max(by_depth, key=lambda d: sum(pass_ms[k] for k in by_depth[d]))
-    ks = by_depth[dom]
+        by_kernel.setdefault(kname(k), []).append(k)
+    if not by_kernel:
+        by_kernel = {kname(npass - 1): [npass - 1]}
+    dom = max(by_kernel, key=lambda d: sum(pass_ms[k] for k in by_kernel[d]))
+    ks = by_kernel[dom]
     avg_ms = sum(pass_ms[k] for k in ks) / len(ks)
     avg_bytes = sum(alg[k] for k in ks) / len(ks)
     avg_io = sum(io[k] for k in ks) / len(ks)
@@ -371,7 +415,7 @@ def main():
     names = (["stats", "mask_fdmt", "init"] + [f"pass{k}(L={schedule[k]})" for k in range(npass - 1)] +
              [f"pass{npass - 1}(L={schedule[-1]})+snr"] + ["snr_norm_separate", "boxcar"])
     stages = {n: round(v, 4) for n, v in zip(names, stage_ms)}
-    total_alg = sum(sbytes[:3]) + sum(alg) + sum(sbytes[3 + npass:])
+    total_alg = sum(sbytes[:2 if fused_init else 3]) + sum(alg) + sum(sbytes[3 + npass:])
 
     out = {
         "metric": METRIC,
@@ -392,10 +436,12 @@ def main():
             "ndt": ndt, "nsamps": N, "nchans": R, "candidates": ncand, "path": "plain (one beam per launch sequence)",
             "merge_schedule": schedule,
             "staged_passes": staged,
+            "group_shape": shape if staged else None,
+            "live_rows": live,
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"k_merge_group<{dom}, false>" if staged else f"k_merge_fused<{dom}, true, false>",
+            "kernel": dom,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -101,6 +101,9 @@ SIGNATURES = {
     "sps_search_set_staged": [_vp, _int],
     "sps_fdmt_set_staged": [_vp, _int],
     "sps_batch_set_staged": [_vp, _i64, _int],
+    "sps_fdmt_set_group_shape": [_vp, _int, _int, _int],
+    "sps_search_set_group_shape": [_vp, _int, _int, _int],
+    "sps_batch_set_group_shape": [_vp, _i64, _int, _int, _int],
     "sps_search_get_schedule": [_vp, C.POINTER(C.c_int32), _i64, _pi64],
     "sps_search_enable_timing": [_vp, _int],
     "sps_search_get_timing": [_vp, C.POINTER(C.c_double), _i64, _pi64],

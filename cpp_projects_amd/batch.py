@@ -68,8 +68,11 @@ class BatchSearch:
         arr = (C.c_int32 * max(len(depths), 1))(*depths)
         call("sps_batch_set_schedule", self._h, int(i), arr, len(depths))
 
-    def set_staged(self, i, on=True):
-        call("sps_batch_set_staged", self._h, int(i), int(on))
+    def set_staged(self, i, mode=1):
+        call("sps_batch_set_staged", self._h, int(i), int(mode))
+
+    def set_group_shape(self, i, lds_floats=8192, max_rows=16, tiles_per_block=8):
+        call("sps_batch_set_group_shape", self._h, int(i), int(lds_floats), int(max_rows), int(tiles_per_block))
 
     def get_schedule(self, i):
         n = C.c_int64()

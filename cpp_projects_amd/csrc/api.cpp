@@ -72,7 +72,8 @@ struct sps_fdmt {
     int64_t pitch = 0;                 // internal state pitch (floats)
     bool uploaded = false;
     DevBuf d_grid, d_grid_off, d_chan_row_off;
-    std::vector<std::unique_ptr<DevBuf>> d_rows;  // per iteration
+    std::vector<std::unique_ptr<DevBuf>> d_rows;  // per iteration: RowDescD of its live output rows
+    std::vector<int64_t> d_rows_n;                // per iteration: number of those rows
     DevBuf state_a, state_b;
     int64_t rows_a = 0, rows_b = 0;    // capacity of the ping-pong states in rows
     int dmax = 0;                      // largest init dt of any channel
@@ -83,18 +84,25 @@ struct sps_fdmt {
     // pass schedule: fused iterations per pass (1..kMaxFuse, or 1..kMaxGroupFuse
     // for staged passes), summing to niters
     std::vector<int> schedule;
-    std::vector<std::unique_ptr<DevBuf>> d_trees;  // per pass
+    std::vector<std::unique_ptr<DevBuf>> d_trees;  // per pass: TreeDesc of its live output rows
+    std::vector<int64_t> tree_rows;                // per pass: number of those rows
     bool trees_ready = false;
     // staged (LDS-window) passes
     bool staged = false;
-    int group_floats = 16384;  // LDS window budget per block (floats)
+    bool fuse_init = false;    // first staged pass builds state 0 itself (mean init)
+    bool staged_last = false;  // the final pass is staged too
+    int group_floats = 8192;   // LDS window per tile buffer (floats; two buffers per block)
     int group_rows = 16;       // output rows per group at most
+    int group_tiles = 8;       // consecutive 512-sample tiles per block
+    static constexpr int kInitGroupFloats = 16384;  // first pass with the init fused
     struct StagedPass {
-        DevBuf groups, leaves, rows;
+        DevBuf groups, leaves, rows, unit_leaf;
         int64_t ngroups = 0;
         int lds_floats = 0;
+        bool init = false;     // leaves are ChanWinD (init fused)
     };
     std::vector<std::unique_ptr<StagedPass>> d_staged;  // per pass
+    std::unique_ptr<StagedPass> d_stage0_init;          // first pass with the init fused
 
     void init_layout() {
         pitch = round_up(eng.nsamps, 4);
@@ -109,7 +117,31 @@ struct sps_fdmt {
                 SPS_REQUIRE(eng.iter[(size_t)i][(size_t)r].out_row == r, "plan rows out of order");
             rows_a = std::max(rows_a, eng.rows[(size_t)i]);
         }
+        compute_live();
         set_schedule(default_schedule(eng.niters));
+    }
+
+    // Rows of an intermediate state that no delivered output row depends on
+    // (the plan's dt grid per sub-band is wider than what the last merges
+    // read) are never computed: skipping them changes no output value.
+    std::vector<std::vector<uint8_t>> live;  // per state
+    DevBuf d_live0;                          // state-0 flags for the init kernels
+    void compute_live() {
+        const int n = eng.niters;
+        live.assign((size_t)n + 1, {});
+        for (int i = 0; i <= n; ++i) live[(size_t)i].assign((size_t)eng.rows[(size_t)i], 0);
+        for (int64_t r = 0; r < eng.rows[(size_t)n]; ++r) live[(size_t)n][(size_t)r] = r < eng.out_rows ? 1 : 0;
+        for (int i = n; i >= 1; --i)
+            for (const auto& rd : eng.iter[(size_t)i]) {
+                if (!live[(size_t)i][(size_t)rd.out_row]) continue;
+                live[(size_t)i - 1][(size_t)rd.tail_row] = 1;
+                if (rd.head_row >= 0) live[(size_t)i - 1][(size_t)rd.head_row] = 1;
+            }
+    }
+    int64_t live_rows(int i) const {
+        int64_t c = 0;
+        for (auto v : live[(size_t)i]) c += v;
+        return c;
     }
 
     static std::vector<int> default_schedule(int n) {
@@ -134,7 +166,29 @@ struct sps_fdmt {
         trees_ready = false;
     }
 
-    void set_staged(bool on) {
+    // LDS window per tile buffer (floats), output rows per group and tiles
+    // per block of staged passes
+    void set_group_shape(int floats, int rows, int tiles) {
+        SPS_REQUIRE(floats >= 2048 && floats <= 4 * kGroupMaxUnits, "group window must be 2048..9216 floats");
+        SPS_REQUIRE(rows >= 1 && rows <= 64, "group rows must be 1..64");
+        SPS_REQUIRE(tiles >= 1 && tiles <= 512, "tiles per block must be 1..512");
+        group_floats = floats;
+        group_rows = rows;
+        group_tiles = tiles;
+        trees_ready = false;
+    }
+
+    // 0: tree passes; 1: staged passes, the final pass on the tree kernel when
+    // its depth allows; 2: as 1 with the init fused into the first pass (mean
+    // init and at least two passes; otherwise as 1); 3, 4: as 1, 2 with the
+    // final pass staged too
+    void set_staged(int mode) {
+        SPS_REQUIRE(mode >= 0 && mode <= 4, "staging mode must be 0..4");
+        const bool on = mode > 0;
+        const bool fi = mode == 2 || mode == 4, sl = mode >= 3;
+        if (fuse_init != fi || staged_last != sl) trees_ready = false;
+        fuse_init = fi;
+        staged_last = sl;
         if (on == staged) return;
         staged = on;
         bool ok = true;
@@ -142,12 +196,28 @@ struct sps_fdmt {
         if (!ok) schedule = default_schedule(eng.niters);
         trees_ready = false;
     }
+    // the final pass of a staged schedule runs on the tree kernel
+    bool tree_last() const { return staged && !staged_last && !schedule.empty() && schedule.back() <= kMaxFuse; }
 
     // Staged pass covering iterations i0 .. i0+L-1: group consecutive output
     // rows while their distinct leaves fit the LDS budget.
+    // init_leaves: the pass reads the waterfall and builds its state-0 leaves
+    // itself (first pass only); the LDS then holds every init row of each
+    // channel the group touches.
     template <int L>
-    void build_groups_L(int i0, StagedPass& sp, hipStream_t st) const {
+    bool build_groups_L(int i0, StagedPass& sp, hipStream_t st, bool init_leaves = false) const {
         const int top = i0 + L - 1;
+        std::vector<int64_t> row_chan;
+        if (init_leaves) {
+            SPS_REQUIRE(i0 == 1, "only the first pass can build state 0");
+            row_chan.resize((size_t)eng.rows[0]);
+            for (int64_t c = 0; c < eng.nchans; ++c)
+                for (int64_t q = eng.chan_row_off[(size_t)c]; q < eng.chan_row_off[(size_t)c + 1]; ++q)
+                    row_chan[(size_t)q] = c;
+        }
+        auto chan_rows = [&](int64_t c) -> int64_t {
+            return eng.chan_row_off[(size_t)c + 1] - eng.chan_row_off[(size_t)c];
+        };
         const int64_t R = eng.rows[(size_t)top];
         const int NL = 1 << L;
         std::vector<int64_t> leaf((size_t)R * NL, -1), shift((size_t)R * NL, 0);
@@ -177,17 +247,23 @@ struct sps_fdmt {
         auto units = [](int64_t smin, int64_t smax) -> int64_t {
             return (kGroupTile + round_up(smax, 4) - smin + 3) / 4;
         };
-        constexpr int64_t kMaxUnits = 192;  // per leaf window: 64 lanes x 3 float4
-        constexpr size_t kMaxLeaves = 64;
+        constexpr size_t kMaxLeaves = 64;  // leaf indices of the unit table are bytes
+        std::vector<uint8_t> unit_leaf;
         std::vector<GroupDescD> groups;
         std::vector<LeafDescD> leaves;
-        std::vector<GRowDesc<L>> rows((size_t)R);
+        std::vector<ChanWinD> chans;
+        std::vector<GRowDesc<L>> rows;
+        std::vector<int64_t> order;  // live output rows, in row order
+        for (int64_t r = 0; r < R; ++r)
+            if (live[(size_t)top][(size_t)r]) order.push_back(r);
+        const int64_t nr = (int64_t)order.size();
         int maxf = 0;
-        int64_t r = 0;
-        while (r < R) {
+        int64_t q = 0;
+        while (q < nr) {
             std::map<int64_t, std::pair<int64_t, int64_t>> S;  // leaf row -> (smin, smax)
-            const int64_t r0 = r;
-            while (r < R && r - r0 < group_rows) {
+            const int64_t q0 = q;
+            while (q < nr && q - q0 < group_rows) {
+                const int64_t r = order[(size_t)q];
                 auto S2 = S;
                 for (int j = 0; j < NL; ++j) {
                     const int64_t lr = leaf[(size_t)r * NL + j];
@@ -201,63 +277,127 @@ struct sps_fdmt {
                 }
                 bool ok = S2.size() <= kMaxLeaves;
                 int64_t floats = 0;
-                for (const auto& kv : S2) {
-                    const int64_t u = units(kv.second.first, kv.second.second);
-                    ok = ok && u <= kMaxUnits;
-                    floats += 4 * u;
+                if (init_leaves) {
+                    // one window per channel, covering every use of any of its rows
+                    std::map<int64_t, std::pair<int64_t, int64_t>> C;
+                    for (const auto& kv : S2) {
+                        const int64_t c = row_chan[(size_t)kv.first];
+                        auto it = C.find(c);
+                        if (it == C.end())
+                            C.emplace(c, kv.second);
+                        else
+                            it->second = {std::min(it->second.first, kv.second.first),
+                                          std::max(it->second.second, kv.second.second)};
+                    }
+                    ok = C.size() <= kMaxLeaves;
+                    for (const auto& kv : C)
+                        floats += 4 * units(kv.second.first, kv.second.second) * chan_rows(kv.first);
+                } else {
+                    for (const auto& kv : S2) floats += 4 * units(kv.second.first, kv.second.second);
                 }
-                ok = ok && floats <= group_floats;
+                ok = ok && floats <= (init_leaves ? (int64_t)kInitGroupFloats : (int64_t)group_floats);
                 if (!ok) {
-                    SPS_REQUIRE(r > r0, "one merge tree exceeds the staged window");
-                    break;
+                    if (q > q0) break;
+                    // a lone tree over the budget: with init leaves the pass
+                    // keeps the separate init; with state leaves it is taken
+                    // anyway (2^L windows of at most 130 units each)
+                    if (init_leaves) return false;
                 }
                 S.swap(S2);
-                ++r;
+                ++q;
             }
             GroupDescD g{};
-            g.out_row0 = r0;
-            g.nrows = (int32_t)(r - r0);
-            g.leaf0 = (int64_t)leaves.size();
-            g.nleaves = (int32_t)S.size();
+            g.row0 = (int64_t)rows.size();
+            g.nrows = (int32_t)(q - q0);
             std::map<int64_t, int32_t> base;
             int32_t off = 0;
-            for (const auto& kv : S) {
-                LeafDescD lf{};
-                lf.row = kv.first;
-                lf.smax4 = (int32_t)round_up(kv.second.second, 4);
-                lf.lds_off = off;
-                lf.nunits = (int32_t)units(kv.second.first, kv.second.second);
-                base[kv.first] = off + lf.smax4;
-                off += 4 * lf.nunits;
-                leaves.push_back(lf);
+            if (init_leaves) {
+                std::map<int64_t, std::pair<int64_t, int64_t>> C;
+                for (const auto& kv : S) {
+                    const int64_t c = row_chan[(size_t)kv.first];
+                    auto it = C.find(c);
+                    if (it == C.end())
+                        C.emplace(c, kv.second);
+                    else
+                        it->second = {std::min(it->second.first, kv.second.first),
+                                      std::max(it->second.second, kv.second.second)};
+                }
+                g.leaf0 = (int64_t)chans.size();
+                g.nleaves = (int32_t)C.size();
+                for (const auto& kv : C) {
+                    const int64_t c = kv.first;
+                    ChanWinD cw{};
+                    cw.chan = c;
+                    cw.grid0 = eng.grid_off[(size_t)c];
+                    cw.nrows = (int32_t)chan_rows(c);
+                    cw.s4 = (int32_t)round_up(kv.second.second, 4);
+                    cw.units = (int32_t)units(kv.second.first, kv.second.second);
+                    cw.row_off = off;
+                    for (int64_t k = 0; k < cw.nrows; ++k)
+                        base[eng.chan_row_off[(size_t)c] + k] = off + (int32_t)(4 * cw.units * k) + cw.s4;
+                    off += 4 * cw.units * cw.nrows;
+                    chans.push_back(cw);
+                }
+            } else {
+                g.leaf0 = (int64_t)leaves.size();
+                g.nleaves = (int32_t)S.size();
+                g.unit0 = (int64_t)unit_leaf.size();
+                int li = 0;
+                for (const auto& kv : S) {
+                    LeafDescD lf{};
+                    lf.row = kv.first;
+                    lf.smax4 = (int32_t)round_up(kv.second.second, 4);
+                    lf.lds_off = off;
+                    lf.nunits = (int32_t)units(kv.second.first, kv.second.second);
+                    base[kv.first] = off + lf.smax4;
+                    off += 4 * lf.nunits;
+                    leaves.push_back(lf);
+                    unit_leaf.insert(unit_leaf.end(), (size_t)lf.nunits, (uint8_t)li++);
+                }
+                g.nunits = off / 4;
+                SPS_REQUIRE(g.nunits <= kGroupMaxUnits, "staged group window too large");
             }
             g.lds_floats = off;
             maxf = std::max(maxf, (int)off);
             groups.push_back(g);
-            for (int64_t rr = r0; rr < r; ++rr) {
-                GRowDesc<L>& d = rows[(size_t)rr];
+            for (int64_t qq = q0; qq < q; ++qq) {
+                const int64_t rr = order[(size_t)qq];
+                GRowDesc<L> d{};
                 d.out_row = rr;
+                int64_t ms = 0;
                 for (int j = 0; j < NL; ++j) {
                     const int64_t lr = leaf[(size_t)rr * NL + j];
                     d.lbase[j] = lr >= 0 ? base[lr] : -1;
+                    if (lr >= 0) ms = std::max(ms, shift[(size_t)rr * NL + j]);
                 }
+                SPS_REQUIRE(ms <= INT32_MAX, "shift too large");
+                d.maxshift = (int32_t)ms;
                 for (int k = 0; k < NL; ++k) d.off[k] = offs[(size_t)rr * NL + k];
+                rows.push_back(d);
             }
         }
         upload(sp.groups, groups, st);
-        upload(sp.leaves, leaves, st);
+        if (init_leaves)
+            upload(sp.leaves, chans, st);
+        else
+            upload(sp.leaves, leaves, st);
         upload(sp.rows, rows, st);
+        upload(sp.unit_leaf, unit_leaf, st);
         sp.ngroups = (int64_t)groups.size();
         sp.lds_floats = maxf;
+        sp.init = init_leaves;
+        return true;
     }
 
     template <int L>
     void build_trees_L(int i0, std::vector<uint8_t>& bytes) const {
         // pass covering iterations i0 .. i0+L-1 (input state i0-1)
         const int top = i0 + L - 1;
-        std::vector<TreeDesc<L>> v((size_t)eng.rows[(size_t)top]);
+        std::vector<TreeDesc<L>> v;
         for (int64_t r = 0; r < eng.rows[(size_t)top]; ++r) {
-            TreeDesc<L>& d = v[(size_t)r];
+            if (!live[(size_t)top][(size_t)r]) continue;  // dead rows are not computed
+            v.emplace_back();
+            TreeDesc<L>& d = v.back();
             d.out_row = r;
             for (auto& x : d.leaf) x = -1;
             for (auto& x : d.off) x = -1;
@@ -280,39 +420,58 @@ struct sps_fdmt {
         }
         bytes.resize(sizeof(TreeDesc<L>) * v.size());
         std::memcpy(bytes.data(), v.data(), bytes.size());
+        SPS_REQUIRE(!v.empty(), "a pass without live rows");
     }
 
     void ensure_trees(hipStream_t st) {
         if (trees_ready) return;
         d_trees.clear();
+        tree_rows.clear();
         d_staged.clear();
+        d_stage0_init.reset();
         int i0 = 1;
-        if (staged) {
-            for (int L : schedule) {
-                d_staged.emplace_back(new StagedPass());
-                StagedPass& sp = *d_staged.back();
+        const size_t np = schedule.size();
+        d_trees.resize(np);
+        tree_rows.assign(np, 0);
+        d_staged.resize(np);
+        for (size_t k = 0; k < np; ++k) {
+            const int L = schedule[k];
+            if (pass_staged(k)) {
+                d_staged[k].reset(new StagedPass());
+                StagedPass& sp = *d_staged[k];
                 if (L == 1) build_groups_L<1>(i0, sp, st);
                 else if (L == 2) build_groups_L<2>(i0, sp, st);
                 else if (L == 3) build_groups_L<3>(i0, sp, st);
                 else build_groups_L<4>(i0, sp, st);
-                i0 += L;
+            } else {
+                std::vector<uint8_t> bytes;
+                if (L == 1) build_trees_L<1>(i0, bytes);
+                else if (L == 2) build_trees_L<2>(i0, bytes);
+                else build_trees_L<3>(i0, bytes);
+                d_trees[k].reset(new DevBuf());
+                upload(*d_trees[k], bytes, st);
+                tree_rows[k] = live_rows(i0 + L - 1);
             }
-            SPS_HIP(hipStreamSynchronize(st));
-            trees_ready = true;
-            return;
-        }
-        for (int L : schedule) {
-            std::vector<uint8_t> bytes;
-            if (L == 1) build_trees_L<1>(i0, bytes);
-            else if (L == 2) build_trees_L<2>(i0, bytes);
-            else build_trees_L<3>(i0, bytes);
-            d_trees.emplace_back(new DevBuf());
-            upload(*d_trees.back(), bytes, st);
             i0 += L;
+        }
+        if (staged) {
+            if (fuse_init && eng.init_mode == kInitMean && np >= 2 && pass_staged(0)) {
+                // a first pass whose trees need more init rows than one LDS
+                // window holds keeps the separate init
+                std::unique_ptr<StagedPass> sp(new StagedPass());
+                const int L = schedule[0];
+                bool ok;
+                if (L == 1) ok = build_groups_L<1>(1, *sp, st, true);
+                else if (L == 2) ok = build_groups_L<2>(1, *sp, st, true);
+                else if (L == 3) ok = build_groups_L<3>(1, *sp, st, true);
+                else ok = build_groups_L<4>(1, *sp, st, true);
+                if (ok) d_stage0_init = std::move(sp);
+            }
         }
         SPS_HIP(hipStreamSynchronize(st));
         trees_ready = true;
     }
+    bool pass_staged(size_t k) const { return staged && !(k + 1 == schedule.size() && tree_last()); }
 
     void ensure_device(hipStream_t st) {
         if (uploaded) return;
@@ -320,12 +479,16 @@ struct sps_fdmt {
         upload(d_grid_off, eng.grid_off, st);
         upload(d_chan_row_off, eng.chan_row_off, st);
         d_rows.clear();
+        d_rows_n.clear();
         for (int i = 0; i <= eng.niters; ++i) {
             d_rows.emplace_back(new DevBuf());
             std::vector<RowDescD> rd;
-            for (const auto& r : eng.iter[(size_t)i]) rd.push_back({r.out_row, r.tail_row, r.head_row, r.offset});
+            for (const auto& r : eng.iter[(size_t)i])
+                if (live[(size_t)i][(size_t)r.out_row]) rd.push_back({r.out_row, r.tail_row, r.head_row, r.offset});
             upload(*d_rows.back(), rd, st);
+            d_rows_n.push_back((int64_t)rd.size());
         }
+        upload(d_live0, live[0], st);
         SPS_HIP(hipStreamSynchronize(st));
         uploaded = true;
     }
@@ -351,8 +514,13 @@ struct sps_fdmt {
             if (ev) SPS_HIP(hipEventRecord(ev[0], st));
             return;
         }
-        launch_fdmt_init(eng.init_mode, in, in_u8, ones, in_pitch, nm, eng.nchans, nsamps, d_grid.as<int32_t>(),
-                         d_grid_off.as<int64_t>(), d_chan_row_off.as<int64_t>(), sa, p, st, dmax);
+        if (!aux) ensure_trees(st);
+        // the first staged pass builds state 0 itself when its tables exist
+        const bool fused0 = !aux && !ones && in != nullptr && staged && d_stage0_init != nullptr;
+        if (!fused0)
+            launch_fdmt_init(eng.init_mode, in, in_u8, ones, in_pitch, nm, eng.nchans, nsamps, d_grid.as<int32_t>(),
+                             d_grid_off.as<int64_t>(), d_chan_row_off.as<int64_t>(), sa, p, st, dmax,
+                             d_live0.as<uint8_t>());
         if (ev) SPS_HIP(hipEventRecord(ev[0], st));
         if (aux) {
             // the mask transform stays on single-iteration launches (a short,
@@ -362,12 +530,10 @@ struct sps_fdmt {
                 const bool last = (i == eng.niters);
                 float* dst = last ? out : ((i % 2 == 0) ? sa : sb);
                 launch_fdmt_merge(src, p, dst, last ? out_pitch : p, d_rows[(size_t)i]->as<RowDescD>(),
-                                  (int64_t)eng.iter[(size_t)i].size(), nsamps, last ? eng.out_rows : INT64_MAX, st,
-                                  nullptr, true);
+                                  d_rows_n[(size_t)i], nsamps, last ? eng.out_rows : INT64_MAX, st, nullptr, true);
             }
             return;
         }
-        ensure_trees(st);
         int i0 = 1;
         for (size_t k = 0; k < schedule.size(); ++k) {
             const int L = schedule[k];
@@ -375,13 +541,17 @@ struct sps_fdmt {
             const float* src = (k % 2 == 0) ? sa : sb;
             const bool last = (top == eng.niters);
             float* dst = last ? out : ((k % 2 == 0) ? sb : sa);
-            if (staged) {
+            if (k == 0 && fused0) {
+                const StagedPass& sp = *d_stage0_init;
+                launch_fdmt_group_init(L, in, in_u8, in_pitch, nm, d_grid.as<int32_t>(), dst, p, sp.groups.p,
+                                       sp.ngroups, sp.leaves.p, sp.rows.p, sp.lds_floats, nsamps, st);
+            } else if (pass_staged(k)) {
                 const StagedPass& sp = *d_staged[k];
                 launch_fdmt_group(L, src, p, dst, last ? out_pitch : p, sp.groups.p, sp.ngroups, sp.leaves.p,
-                                  sp.rows.p, sp.lds_floats, nsamps, last ? eng.out_rows : INT64_MAX, st,
-                                  last ? snr : nullptr);
+                                  sp.rows.p, sp.unit_leaf.as<uint8_t>(), sp.lds_floats, group_tiles, nsamps,
+                                  last ? eng.out_rows : INT64_MAX, st, last ? snr : nullptr);
             } else {
-                launch_fdmt_fused(L, src, p, dst, last ? out_pitch : p, d_trees[k]->p, eng.rows[(size_t)top], nsamps,
+                launch_fdmt_fused(L, src, p, dst, last ? out_pitch : p, d_trees[k]->p, tree_rows[k], nsamps,
                                   last ? eng.out_rows : INT64_MAX, st, last ? snr : nullptr);
             }
             if (ev) SPS_HIP(hipEventRecord(ev[k + 1], st));
@@ -1073,14 +1243,28 @@ SPS_API sps_status sps_fdmt_get_schedule(const sps_fdmt* h, int32_t* h_depths, i
 SPS_API sps_status sps_fdmt_set_staged(sps_fdmt* h, int staged) {
     return guarded([&] {
         SPS_REQUIRE(h, "null handle");
-        h->set_staged(staged != 0);
+        h->set_staged(staged);
     });
 }
 
 SPS_API sps_status sps_search_set_staged(sps_search* s, int staged) {
     return guarded([&] {
         SPS_REQUIRE(s, "null handle");
-        s->f->set_staged(staged != 0);
+        s->f->set_staged(staged);
+    });
+}
+
+SPS_API sps_status sps_fdmt_set_group_shape(sps_fdmt* h, int lds_floats, int max_rows, int tiles_per_block) {
+    return guarded([&] {
+        SPS_REQUIRE(h, "null handle");
+        h->set_group_shape(lds_floats, max_rows, tiles_per_block);
+    });
+}
+
+SPS_API sps_status sps_search_set_group_shape(sps_search* s, int lds_floats, int max_rows, int tiles_per_block) {
+    return guarded([&] {
+        SPS_REQUIRE(s, "null handle");
+        s->f->set_group_shape(lds_floats, max_rows, tiles_per_block);
     });
 }
 
@@ -1372,11 +1556,21 @@ SPS_API sps_status sps_batch_set_schedule(sps_batch* b, int64_t beam, const int3
     });
 }
 
+SPS_API sps_status sps_batch_set_group_shape(sps_batch* b, int64_t beam, int lds_floats, int max_rows, int tiles_per_block) {
+    return guarded([&] {
+        SPS_REQUIRE(b, "null handle");
+        SPS_REQUIRE(beam >= 0 && beam < (int64_t)b->beams.size(), "beam index out of range");
+        b->beams[(size_t)beam].s->f->set_group_shape(lds_floats, max_rows, tiles_per_block);
+        b->drop_graph();
+        b->prepared = false;
+    });
+}
+
 SPS_API sps_status sps_batch_set_staged(sps_batch* b, int64_t beam, int staged) {
     return guarded([&] {
         SPS_REQUIRE(b, "null handle");
         SPS_REQUIRE(beam >= 0 && beam < (int64_t)b->beams.size(), "beam index out of range");
-        b->beams[(size_t)beam].s->f->set_staged(staged != 0);
+        b->beams[(size_t)beam].s->f->set_staged(staged);
         b->drop_graph();
         b->prepared = false;
     });

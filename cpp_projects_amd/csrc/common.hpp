@@ -59,6 +59,24 @@ struct DevBuf {
 
 __host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Correctly rounded x / y given r = 1.0f / y (the reciprocal rounded to
+// nearest), for divisors shared by many quotients: q0 = x * r is within one
+// ulp of x / y, the FMA residual x - q0 * y is exact, and RN(q0 + r * residual)
+// is the correctly rounded quotient (Markstein, IBM J. Res. Dev. 34(1), 1990).
+// tests/test_division.py checks it bit for bit against x / y over every x
+// mantissa for integer divisors and random float divisors.  Operands outside
+// [2^-60, 2^60] in magnitude (zero, tiny, huge, infinite, NaN) take the
+// division, so no intermediate under- or overflows.
+__device__ __forceinline__ float div_rn(float x, float y, float r) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    if (ax >= 0x1p-60f && ax <= 0x1p60f && ay >= 0x1p-60f && ay <= 0x1p60f) {
+        const float q0 = x * r;
+        const float e = fmaf(-q0, y, x);
+        return fmaf(e, r, q0);
+    }
+    return x / y;
+}
+
 // Launch geometry: the merge/init kernels tile rows of up to this many samples.
 constexpr int kThreads = 256;
 

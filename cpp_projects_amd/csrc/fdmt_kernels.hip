@@ -57,16 +57,19 @@ __global__ void __launch_bounds__(kThreads) k_merge(const float* __restrict__ in
     float* __restrict__ o = out + d.out_row * out_pitch;
     const int64_t off = d.offset;
     const int64_t base = tile * kMergeTile;
-    float dk = 1.0f;
+    float dk = 1.0f, rdk = 1.0f;
     const float* nrow = nullptr;
     if (SNR) {
         dk = sqrtf((float)((double)(float)epi.stats[4] + 1.0E-8));
+        rdk = 1.0f / dk;
         nrow = epi.nrm + d.out_row * epi.nrm_len;
     }
+    // beyond the mask transform's extent the divisor is the same for every
+    // sample, so its reciprocal is shared (correctly rounded, div_rn)
     auto div = [&](float v, int64_t t) -> float {
         if (!SNR) return v;
-        const float dd = (t < epi.nrm_len) ? sqrtf((float)((double)nrow[t] + 1.0E-8)) : dk;
-        return v / dd;
+        if (t < epi.nrm_len) return v / sqrtf((float)((double)nrow[t] + 1.0E-8));
+        return div_rn(v, dk, rdk);
     };
     if (VEC) {
 #pragma unroll
@@ -188,16 +191,19 @@ __global__ void __launch_bounds__(kThreads) k_merge_fused(const float* __restric
     if (d.out_row >= out_rows_limit) return;
     float* __restrict__ o = out + d.out_row * out_pitch;
     const int64_t base = tile * kMergeTile;
-    float dk = 1.0f;
+    float dk = 1.0f, rdk = 1.0f;
     const float* nrow = nullptr;
     if (SNR) {
         dk = sqrtf((float)((double)(float)epi.stats[4] + 1.0E-8));
+        rdk = 1.0f / dk;
         nrow = epi.nrm + d.out_row * epi.nrm_len;
     }
+    // beyond the mask transform's extent the divisor is the same for every
+    // sample, so its reciprocal is shared (correctly rounded, div_rn)
     auto div = [&](float v, int64_t t) -> float {
         if (!SNR) return v;
-        const float dd = (t < epi.nrm_len) ? sqrtf((float)((double)nrow[t] + 1.0E-8)) : dk;
-        return v / dd;
+        if (t < epi.nrm_len) return v / sqrtf((float)((double)nrow[t] + 1.0E-8));
+        return div_rn(v, dk, rdk);
     };
     if (VEC) {
 #pragma unroll
@@ -247,70 +253,127 @@ struct LdsTree {
     }
 };
 
-constexpr int kStageLeavesPerRound = 4;  // leaves a wave stages per round
-constexpr int kStageUnitsPerLeaf = 3;    // float4 units per lane per leaf (<= 192 units = 768 floats)
+// the same tree when every merge adds its head (t >= maxshift): u = t - T
+template <int L, int K>
+struct LdsTreeFast {
+    __device__ __forceinline__ static float eval(const float* __restrict__ lds, const GRowDesc<L>& d, int u) {
+        if constexpr (K >= (1 << L)) {
+            return lds[d.lbase[K - (1 << L)] + u];
+        } else {
+            float v = LdsTreeFast<L, 2 * K>::eval(lds, d, u);
+            const int32_t off = d.off[K];
+            if (off >= 0) v = v + LdsTreeFast<L, 2 * K + 1>::eval(lds, d, u - off);
+            return v;
+        }
+    }
+};
+
+// evaluates the trees of every row of group g for times [T, T + kGroupTile)
+template <int L, bool SNR>
+__device__ __forceinline__ void group_trees(const float* __restrict__ lds, const GroupDescD& g,
+                                            const GRowDesc<L>* __restrict__ rows, float* __restrict__ out,
+                                            int64_t out_pitch, int64_t T, int64_t nsamps, int64_t out_rows_limit,
+                                            const SnrEpi& epi) {
+    float dk = 1.0f, rdk = 1.0f;
+    if (SNR) {
+        dk = sqrtf((float)((double)(float)epi.stats[4] + 1.0E-8));
+        rdk = 1.0f / dk;
+    }
+    for (int i = 0; i < g.nrows; ++i) {
+        const GRowDesc<L> d = rows[g.row0 + i];
+        if (d.out_row >= out_rows_limit) continue;
+        float* __restrict__ o = out + d.out_row * out_pitch;
+        const float* nrow = SNR ? epi.nrm + d.out_row * epi.nrm_len : nullptr;
+        if (T >= d.maxshift && T + kGroupTile <= nsamps && (!SNR || T >= epi.nrm_len)) {
+            // the whole tile is past every shift: no per-sample tests
+#pragma unroll
+            for (int j = 0; j < kGroupTile / kThreads; ++j) {
+                const int u = j * kThreads + threadIdx.x;
+                float v = LdsTreeFast<L, 1>::eval(lds, d, u);
+                if (SNR) v = div_rn(v, dk, rdk);
+                o[T + u] = v;
+            }
+            continue;
+        }
+#pragma unroll
+        for (int j = 0; j < kGroupTile / kThreads; ++j) {
+            const int64_t t = T + j * kThreads + threadIdx.x;
+            if (t < nsamps) {
+                float v = LdsTree<L, 1>::eval(lds, d, t, T);
+                if (SNR)
+                    v = (t < epi.nrm_len) ? v / sqrtf((float)((double)nrow[t] + 1.0E-8)) : div_rn(v, dk, rdk);
+                o[t] = v;
+            }
+        }
+    }
+}
+
+// One block = one group x tiles_per_block consecutive tiles.  Every float4
+// unit of the group's LDS windows is a fixed (leaf row, offset) pair, so each
+// thread works out its unit slots once; per tile it issues the loads of the
+// next tile into registers, evaluates the current tile from one LDS buffer,
+// then commits the registers to the other buffer (one barrier per tile).
+constexpr int kStageSlots = kGroupMaxUnits / kThreads;
 
 template <int L, bool SNR>
 __global__ void __launch_bounds__(kThreads) k_merge_group(const float* __restrict__ in, int64_t in_pitch,
                                                           float* __restrict__ out, int64_t out_pitch,
                                                           const GroupDescD* __restrict__ groups, int64_t ngroups,
                                                           const LeafDescD* __restrict__ leaves,
-                                                          const GRowDesc<L>* __restrict__ rows, int64_t nsamps,
-                                                          int64_t out_rows_limit, SnrEpi epi) {
+                                                          const GRowDesc<L>* __restrict__ rows,
+                                                          const uint8_t* __restrict__ unit_leaf, int tiles_per_block,
+                                                          int lds_stride, int64_t nsamps, int64_t out_rows_limit,
+                                                          SnrEpi epi) {
     extern __shared__ float lds[];
     const uint32_t logical = xcd_remap(blockIdx.x, gridDim.x);
     const int64_t gi = (int64_t)(logical % (uint64_t)ngroups);
-    const int64_t T = (int64_t)(logical / (uint64_t)ngroups) * kGroupTile;
+    const int64_t chunk = (int64_t)(logical / (uint64_t)ngroups);
     const GroupDescD g = groups[gi];
-    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    constexpr int kWaves = kThreads / 64;
-    for (int l0 = wid * kStageLeavesPerRound; l0 < g.nleaves; l0 += kWaves * kStageLeavesPerRound) {
-        float4a buf[kStageLeavesPerRound][kStageUnitsPerLeaf];
+    const int64_t ntiles = ceil_div(nsamps, kGroupTile);
+    const int64_t k0 = chunk * tiles_per_block;
+    const int64_t k1 = min(ntiles, k0 + tiles_per_block);
+    if (k0 >= k1) return;  // block-uniform
+    int64_t src[kStageSlots];  // element offset of the slot's unit at T = 0
+    int32_t tk[kStageSlots];   // its time at T = 0; INT32_MIN for an unused slot
 #pragma unroll
-        for (int a = 0; a < kStageLeavesPerRound; ++a) {
-            const int l = l0 + a;
-            LeafDescD lf{};
-            if (l < g.nleaves) lf = leaves[g.leaf0 + l];
-            const float* src = in + lf.row * in_pitch;
-#pragma unroll
-            for (int b = 0; b < kStageUnitsPerLeaf; ++b) {
-                const int k = lane + 64 * b;
-                const int64_t t = T - lf.smax4 + 4 * (int64_t)k;
-                float4a v = {0.0f, 0.0f, 0.0f, 0.0f};
-                if (l < g.nleaves && k < lf.nunits && t >= 0 && t + 4 <= in_pitch)
-                    v = *reinterpret_cast<const float4a*>(src + t);
-                buf[a][b] = v;
-            }
-        }
-#pragma unroll
-        for (int a = 0; a < kStageLeavesPerRound; ++a) {
-            const int l = l0 + a;
-            if (l >= g.nleaves) continue;
-            const LeafDescD lf = leaves[g.leaf0 + l];
-#pragma unroll
-            for (int b = 0; b < kStageUnitsPerLeaf; ++b) {
-                const int k = lane + 64 * b;
-                if (k < lf.nunits) *reinterpret_cast<float4a*>(lds + lf.lds_off + 4 * k) = buf[a][b];
-            }
+    for (int j = 0; j < kStageSlots; ++j) {
+        const int f = (int)threadIdx.x + j * kThreads;
+        src[j] = 0;
+        tk[j] = INT32_MIN;
+        if (f < g.nunits) {
+            const LeafDescD lf = leaves[g.leaf0 + unit_leaf[g.unit0 + f]];
+            tk[j] = 4 * (f - lf.lds_off / 4) - lf.smax4;
+            src[j] = lf.row * in_pitch + tk[j];
         }
     }
-    __syncthreads();
-    float dk = 1.0f;
-    if (SNR) dk = sqrtf((float)((double)(float)epi.stats[4] + 1.0E-8));
-    for (int i = 0; i < g.nrows; ++i) {
-        const GRowDesc<L> d = rows[g.out_row0 + i];
-        if (d.out_row >= out_rows_limit) continue;
-        float* __restrict__ o = out + d.out_row * out_pitch;
-        const float* nrow = SNR ? epi.nrm + d.out_row * epi.nrm_len : nullptr;
+    float4a buf[kStageSlots];
+    auto issue = [&](int64_t T) {
 #pragma unroll
-        for (int j = 0; j < kGroupTile / kThreads; ++j) {
-            const int64_t t = T + j * kThreads + threadIdx.x;
-            if (t < nsamps) {
-                float v = LdsTree<L, 1>::eval(lds, d, t, T);
-                if (SNR) v = v / ((t < epi.nrm_len) ? sqrtf((float)((double)nrow[t] + 1.0E-8)) : dk);
-                o[t] = v;
-            }
+        for (int j = 0; j < kStageSlots; ++j) {
+            const int64_t t = T + tk[j];
+            float4a v = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (tk[j] != INT32_MIN && t >= 0 && t + 4 <= in_pitch) v = *reinterpret_cast<const float4a*>(in + src[j] + T);
+            buf[j] = v;
         }
+    };
+    auto commit = [&](float* dst) {
+#pragma unroll
+        for (int j = 0; j < kStageSlots; ++j) {
+            const int f = (int)threadIdx.x + j * kThreads;
+            if (f < g.nunits) *reinterpret_cast<float4a*>(dst + 4 * f) = buf[j];
+        }
+    };
+    issue(k0 * kGroupTile);
+    commit(lds);
+    __syncthreads();
+    for (int64_t k = k0; k < k1; ++k) {
+        float* cur = lds + ((k - k0) & 1) * lds_stride;
+        float* nxt = lds + ((k - k0 + 1) & 1) * lds_stride;
+        const bool more = k + 1 < k1;
+        if (more) issue((k + 1) * kGroupTile);
+        group_trees<L, SNR>(cur, g, rows, out, out_pitch, k * kGroupTile, nsamps, out_rows_limit, epi);
+        if (more) commit(nxt);
+        __syncthreads();
     }
 }
 
@@ -329,10 +392,12 @@ __global__ void __launch_bounds__(kThreads) k_init(const void* __restrict__ in, 
                                                    int64_t nsamps, const int32_t* __restrict__ grid,
                                                    const int64_t* __restrict__ grid_off,
                                                    const int64_t* __restrict__ chan_row_off,
-                                                   float* __restrict__ state, int64_t pitch) {
+                                                   float* __restrict__ state, int64_t pitch,
+                                                   const uint8_t* __restrict__ live) {
     const int64_t c = blockIdx.y;
     const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     if (t >= nsamps) return;
+    const uint8_t* lv = live ? live + chan_row_off[c] : nullptr;
     const int64_t g0 = grid_off[c];
     const int K = (int)(grid_off[c + 1] - g0);
     const int32_t* g = grid + g0;
@@ -361,7 +426,7 @@ __global__ void __launch_bounds__(kThreads) k_init(const void* __restrict__ in, 
             prev = sum / (float)(d0 + 1);
         }
     }
-    row[t] = prev;
+    if (!lv || lv[0]) row[t] = prev;
     for (int k = 1; k < K; ++k) {
         const int64_t dc = g[k], dp = g[k - 1];
         float v = 0.0f;
@@ -377,16 +442,16 @@ __global__ void __launch_bounds__(kThreads) k_init(const void* __restrict__ in, 
                 v = (prev * (float)dc + X(t - dc)) / (float)(dc + 1);
             }
         }
-        row[(int64_t)k * pitch + t] = v;
+        if (!lv || lv[k]) row[(int64_t)k * pitch + t] = v;
         prev = v;
     }
 }
 
 // LDS-staged init: one block per (channel, 4096-sample tile).  The tile plus
 // its history of dmax samples is read once (16-byte loads when aligned),
-// normalised once into LDS, and every thread then produces 4 groups of 4
-// consecutive samples of each of the channel's rows with the same operation
-// order as k_init, storing float4 per row.
+// normalised once into LDS, and every thread then produces 16 lane-strided
+// samples of each of the channel's rows with the same operation order as
+// k_init (rows that no output depends on are computed but not stored).
 constexpr int kInitTile = 4096;
 
 template <int IN_KIND>
@@ -441,11 +506,13 @@ __global__ void __launch_bounds__(kThreads) k_init_lds(const void* __restrict__ 
                                                        const int64_t* __restrict__ grid_off,
                                                        const int64_t* __restrict__ chan_row_off,
                                                        float* __restrict__ state, int64_t pitch, int dmax_all,
-                                                       int aligned_in, int aligned_out) {
+                                                       int aligned_in, int aligned_out,
+                                                       const uint8_t* __restrict__ live) {
     extern __shared__ float xs[];
     const int64_t c = blockIdx.y;
     const int64_t T = (int64_t)blockIdx.x * kInitTile;
     const int64_t g0 = grid_off[c];
+    const uint8_t* lv = live ? live + chan_row_off[c] : nullptr;
     const int K = (int)(grid_off[c + 1] - g0);
     const int32_t* g = grid + g0;
     const int dmax = g[K - 1];
@@ -457,72 +524,132 @@ __global__ void __launch_bounds__(kThreads) k_init_lds(const void* __restrict__ 
     if (NORM) {
         const bool keep = norm.keep[c] != 0;
         const float mu = norm.mu[c], sg = norm.sigma[c];
-        for (int i = threadIdx.x; i < n; i += kThreads) xs[i] = keep ? (xs[i] - mu) / sg : 0.0f;
+        const float rsg = 1.0f / sg;
+        for (int i = threadIdx.x; i < n; i += kThreads) xs[i] = keep ? div_rn(xs[i] - mu, sg, rsg) : 0.0f;
         __syncthreads();
     }
     (void)dmax_all;
+    (void)aligned_out;
+    // rows outer, the thread's 16 samples inner (lane-consecutive, so the LDS
+    // reads are conflict-free and each store instruction writes 256 bytes);
+    // the running value of every sample stays in a register between rows
+    constexpr int kPer = kInitTile / kThreads;
     float* rowbase = state + chan_row_off[c] * pitch;
-    for (int grp = 0; grp < kInitTile / (kThreads * 4); ++grp) {
-    const int lt0 = grp * kThreads * 4 + threadIdx.x * 4;
-    if (lt0 >= n_t) return;
-    float v[4], prev[4];
-    const int64_t d0 = g[0];
+    float prev[kPer];
+    {
+        const int64_t d0 = g[0];
+        const bool keep_row = !lv || lv[0];
+        const float y0 = (float)(d0 + 1), r0 = 1.0f / y0;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        const int64_t t = T + lt0 + e;
-        const int xi = lt0 + e + dmax;  // LDS index of sample t
-        float p = 0.0f;
-        if (t < nsamps && t >= d0) {
-            if (MODE == kInitCFdmtD) {
-                p = xs[xi];
-            } else {
-                float sum = 0.0f;
-                for (int i = xi - (int)d0; i <= xi; ++i) sum += xs[i];
-                p = sum / (float)(d0 + 1);
+        for (int j = 0; j < kPer; ++j) {
+            const int lt = j * kThreads + threadIdx.x;
+            const int64_t t = T + lt;
+            const int xi = lt + dmax;  // LDS index of sample t
+            float p = 0.0f;
+            if (lt < n_t && t >= d0) {
+                if (MODE == kInitCFdmtD) {
+                    p = xs[xi];
+                } else {
+                    float sum = 0.0f;
+                    for (int i = xi - (int)d0; i <= xi; ++i) sum += xs[i];
+                    p = div_rn(sum, y0, r0);
+                }
             }
+            prev[j] = p;
+            if (keep_row && lt < n_t) rowbase[t] = p;
         }
-        prev[e] = p;
     }
-    auto store = [&](float* row, const float* val) {
-        const int64_t t = T + lt0;
-        if (aligned_out && t + 4 <= nsamps) {
-            float4a a;
-            a.x = val[0];
-            a.y = val[1];
-            a.z = val[2];
-            a.w = val[3];
-            *reinterpret_cast<float4a*>(row + t) = a;
-        } else {
-            for (int e = 0; e < 4; ++e)
-                if (t + e < nsamps) row[t + e] = val[e];
-        }
-    };
-    store(rowbase, prev);
     for (int k = 1; k < K; ++k) {
         const int64_t dc = g[k], dp = g[k - 1];
+        const bool keep_row = !lv || lv[k];
+        float* row = rowbase + (int64_t)k * pitch;
+        const float yk = (float)dc + 1.0f, rk = 1.0f / yk;  // == (float)(dc + 1)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int64_t t = T + lt0 + e;
-            const int xi = lt0 + e + dmax;
+        for (int j = 0; j < kPer; ++j) {
+            const int lt = j * kThreads + threadIdx.x;
+            const int64_t t = T + lt;
+            const int xi = lt + dmax;
             float r = 0.0f;
-            if (t < nsamps && t >= dc) {
+            if (lt < n_t && t >= dc) {
                 if (MODE == kInitMeanD) {
                     float sum = 0.0f;
                     for (int i = xi - (int)dc; i < xi - (int)dp; ++i) sum += xs[i];
-                    r = (prev[e] * ((float)dp + 1.0f) + sum) / ((float)dc + 1.0f);
+                    r = div_rn(prev[j] * ((float)dp + 1.0f) + sum, yk, rk);
                 } else if (MODE == kInitRestrictD) {
                     const float sum = 0.0f + xs[xi - (int)dc];
-                    r = (prev[e] * (((float)dp + 1.0f) + sum)) / ((float)dc + 1.0f);
+                    r = div_rn(prev[j] * (((float)dp + 1.0f) + sum), yk, rk);
                 } else {
-                    r = (prev[e] * (float)dc + xs[xi - (int)dc]) / (float)(dc + 1);
+                    r = div_rn(prev[j] * (float)dc + xs[xi - (int)dc], yk, rk);
                 }
             }
-            v[e] = r;
-            prev[e] = r;
+            prev[j] = r;
+            if (keep_row && lt < n_t) row[t] = r;
         }
-        store(rowbase + (int64_t)k * pitch, v);
     }
-    }  // grp
+}
+
+// First staged pass with the mean init fused in: every init row a group's
+// trees read is built in LDS straight from the waterfall, with the operations
+// of k_init in the same order (normalised samples (x - mu) / sigma, running
+// means), so state 0 never reaches HBM.
+template <int L, int IN_KIND, bool NORM>
+__global__ void __launch_bounds__(kThreads) k_merge_group_init(const void* __restrict__ in, int64_t in_pitch,
+                                                               ChanNorm norm, const int32_t* __restrict__ grid,
+                                                               float* __restrict__ out, int64_t out_pitch,
+                                                               const GroupDescD* __restrict__ groups, int64_t ngroups,
+                                                               const ChanWinD* __restrict__ chans,
+                                                               const GRowDesc<L>* __restrict__ rows, int64_t nsamps) {
+    extern __shared__ float lds[];
+    const uint32_t logical = xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t gi = (int64_t)(logical % (uint64_t)ngroups);
+    const int64_t T = (int64_t)(logical / (uint64_t)ngroups) * kGroupTile;
+    const GroupDescD g = groups[gi];
+    for (int ci = 0; ci < g.nleaves; ++ci) {
+        const ChanWinD cw = chans[g.leaf0 + ci];
+        const int32_t* __restrict__ gr = grid + cw.grid0;
+        const int64_t xb = cw.chan * in_pitch;
+        float mu = 0.f, sg = 1.f, rsg = 1.f;
+        bool keep = true;
+        if (NORM) {
+            keep = norm.keep[cw.chan] != 0;
+            mu = norm.mu[cw.chan];
+            sg = norm.sigma[cw.chan];
+            rsg = 1.0f / sg;
+        }
+        auto X = [&](int64_t i) -> float {
+            float v = load_in<IN_KIND>(in, xb + i);
+            if (NORM) v = keep ? div_rn(v - mu, sg, rsg) : 0.0f;
+            return v;
+        };
+        const int n = 4 * cw.units;
+        float* __restrict__ rb = lds + cw.row_off;
+        const int64_t d0 = gr[0];
+        const float y0 = (float)(d0 + 1), r0 = 1.0f / y0;
+        for (int u = threadIdx.x; u < n; u += kThreads) {
+            const int64_t t = T - cw.s4 + u;
+            float prev = 0.0f;
+            if (t >= d0 && t < nsamps) {
+                float sum = 0.0f;
+                for (int64_t i = t - d0; i <= t; ++i) sum += X(i);
+                prev = div_rn(sum, y0, r0);
+            }
+            rb[u] = prev;
+            for (int k = 1; k < cw.nrows; ++k) {
+                const int64_t dc = gr[k], dp = gr[k - 1];
+                float v = 0.0f;
+                if (t >= dc && t < nsamps) {
+                    float sum = 0.0f;
+                    for (int64_t i = t - dc; i < t - dp; ++i) sum += X(i);
+                    const float yk = (float)dc + 1.0f;
+                    v = div_rn(prev * ((float)dp + 1.0f) + sum, yk, 1.0f / yk);
+                }
+                rb[k * n + u] = v;
+                prev = v;
+            }
+        }
+    }
+    __syncthreads();
+    group_trees<L, false>(lds, g, rows, out, out_pitch, T, nsamps, INT64_MAX, SnrEpi{nullptr, 0, nullptr});
 }
 
 // ---------------------------------------------------------------------------
@@ -662,50 +789,144 @@ void launch_fdmt_fused(int L, const float* in, int64_t in_pitch, float* out, int
     SPS_HIP(hipGetLastError());
 }
 
+struct GroupLaunch {
+    uint32_t nb;
+    size_t lds;
+    const float* in;
+    int64_t in_pitch;
+    float* out;
+    int64_t out_pitch;
+    const void* groups;
+    int64_t ngroups;
+    const void* leaves;
+    const void* rows;
+    const uint8_t* unit_leaf;
+    int tiles_per_block;
+    int lds_stride;
+    int64_t nsamps;
+    int64_t limit;
+    SnrEpi epi;
+};
+
 template <int L>
-static void group_dispatch(bool snr, uint32_t nb, size_t lds, const float* in, int64_t in_pitch, float* out,
-                           int64_t out_pitch, const void* g, int64_t ng, const void* lv, const void* rw,
-                           int64_t nsamps, int64_t limit, SnrEpi epi, hipStream_t st) {
-    const GroupDescD* gp = static_cast<const GroupDescD*>(g);
-    const LeafDescD* lp = static_cast<const LeafDescD*>(lv);
-    const GRowDesc<L>* rp = static_cast<const GRowDesc<L>*>(rw);
+static void group_dispatch(bool snr, const GroupLaunch& a, hipStream_t st) {
+    const GroupDescD* gp = static_cast<const GroupDescD*>(a.groups);
+    const LeafDescD* lp = static_cast<const LeafDescD*>(a.leaves);
+    const GRowDesc<L>* rp = static_cast<const GRowDesc<L>*>(a.rows);
     if (snr)
-        hipLaunchKernelGGL((k_merge_group<L, true>), dim3(nb), dim3(kThreads), lds, st, in, in_pitch, out, out_pitch,
-                           gp, ng, lp, rp, nsamps, limit, epi);
+        hipLaunchKernelGGL((k_merge_group<L, true>), dim3(a.nb), dim3(kThreads), a.lds, st, a.in, a.in_pitch, a.out,
+                           a.out_pitch, gp, a.ngroups, lp, rp, a.unit_leaf, a.tiles_per_block, a.lds_stride, a.nsamps,
+                           a.limit, a.epi);
     else
-        hipLaunchKernelGGL((k_merge_group<L, false>), dim3(nb), dim3(kThreads), lds, st, in, in_pitch, out, out_pitch,
-                           gp, ng, lp, rp, nsamps, limit, epi);
+        hipLaunchKernelGGL((k_merge_group<L, false>), dim3(a.nb), dim3(kThreads), a.lds, st, a.in, a.in_pitch, a.out,
+                           a.out_pitch, gp, a.ngroups, lp, rp, a.unit_leaf, a.tiles_per_block, a.lds_stride, a.nsamps,
+                           a.limit, a.epi);
 }
 
 void launch_fdmt_group(int L, const float* in, int64_t in_pitch, float* out, int64_t out_pitch, const void* d_groups,
-                       int64_t ngroups, const void* d_leaves, const void* d_rows, int lds_floats_max, int64_t nsamps,
-                       int64_t out_rows_limit, hipStream_t st, const MergeEpilogue* snr) {
+                       int64_t ngroups, const void* d_leaves, const void* d_rows, const uint8_t* d_unit_leaf,
+                       int lds_floats_max, int tiles_per_block, int64_t nsamps, int64_t out_rows_limit,
+                       hipStream_t st, const MergeEpilogue* snr) {
     if (ngroups <= 0 || nsamps <= 0) return;
     if (in_pitch % 4 != 0 || reinterpret_cast<uintptr_t>(in) % 16 != 0)
         throw Error(SPS_ERR_INVALID_ARGUMENT, "staged pass needs a 16-byte aligned input state");
+    if (lds_floats_max <= 0 || lds_floats_max > 4 * kGroupMaxUnits)
+        throw Error(SPS_ERR_INVALID_ARGUMENT, "staged pass LDS window out of range");
+    if (tiles_per_block < 1) throw Error(SPS_ERR_INVALID_ARGUMENT, "tiles per block must be positive");
+    const int64_t nb = ngroups * ceil_div(ceil_div(nsamps, kGroupTile), tiles_per_block);
+    if (nb > 0x7fffffffLL) throw Error(SPS_ERR_INVALID_ARGUMENT, "merge grid too large");
+    GroupLaunch a{};
+    a.nb = (uint32_t)nb;
+    a.lds = sizeof(float) * 2 * (size_t)lds_floats_max;  // two tile buffers
+    a.in = in;
+    a.in_pitch = in_pitch;
+    a.out = out;
+    a.out_pitch = out_pitch;
+    a.groups = d_groups;
+    a.ngroups = ngroups;
+    a.leaves = d_leaves;
+    a.rows = d_rows;
+    a.unit_leaf = d_unit_leaf;
+    a.tiles_per_block = tiles_per_block;
+    a.lds_stride = lds_floats_max;
+    a.nsamps = nsamps;
+    a.limit = out_rows_limit;
+    a.epi = SnrEpi{nullptr, 0, nullptr};
+    if (snr) a.epi = SnrEpi{snr->nrm, snr->nrm_len, snr->stats};
+    const bool s = snr != nullptr;
+    switch (L) {
+        case 1:
+            group_dispatch<1>(s, a, st);
+            break;
+        case 2:
+            group_dispatch<2>(s, a, st);
+            break;
+        case 3:
+            group_dispatch<3>(s, a, st);
+            break;
+        case 4:
+            group_dispatch<4>(s, a, st);
+            break;
+        default:
+            throw Error(SPS_ERR_INVALID_ARGUMENT, "staged fusion depth must be 1..4");
+    }
+    SPS_HIP(hipGetLastError());
+}
+
+template <int L, int IN_KIND>
+static void group_init_dispatch(bool nrm, uint32_t nb, size_t lds, const void* in, int64_t in_pitch, ChanNorm n,
+                                const int32_t* grid, float* out, int64_t out_pitch, const void* g, int64_t ng,
+                                const void* cw, const void* rw, int64_t nsamps, hipStream_t st) {
+    const GroupDescD* gp = static_cast<const GroupDescD*>(g);
+    const ChanWinD* cp = static_cast<const ChanWinD*>(cw);
+    const GRowDesc<L>* rp = static_cast<const GRowDesc<L>*>(rw);
+    if (nrm)
+        hipLaunchKernelGGL((k_merge_group_init<L, IN_KIND, true>), dim3(nb), dim3(kThreads), lds, st, in, in_pitch,
+                           n, grid, out, out_pitch, gp, ng, cp, rp, nsamps);
+    else
+        hipLaunchKernelGGL((k_merge_group_init<L, IN_KIND, false>), dim3(nb), dim3(kThreads), lds, st, in, in_pitch,
+                           n, grid, out, out_pitch, gp, ng, cp, rp, nsamps);
+}
+
+template <int L>
+static void group_init_kind(bool u8, bool nrm, uint32_t nb, size_t lds, const void* in, int64_t in_pitch,
+                            ChanNorm n, const int32_t* grid, float* out, int64_t out_pitch, const void* g, int64_t ng,
+                            const void* cw, const void* rw, int64_t nsamps, hipStream_t st) {
+    if (u8)
+        group_init_dispatch<L, 1>(nrm, nb, lds, in, in_pitch, n, grid, out, out_pitch, g, ng, cw, rw, nsamps, st);
+    else
+        group_init_dispatch<L, 0>(nrm, nb, lds, in, in_pitch, n, grid, out, out_pitch, g, ng, cw, rw, nsamps, st);
+}
+
+void launch_fdmt_group_init(int L, const void* in, int in_u8, int64_t in_pitch, const ChanNorm* norm,
+                            const int32_t* d_grid, float* out, int64_t out_pitch, const void* d_groups,
+                            int64_t ngroups, const void* d_chans, const void* d_rows, int lds_floats_max,
+                            int64_t nsamps, hipStream_t st) {
+    if (ngroups <= 0 || nsamps <= 0) return;
     if (lds_floats_max <= 0 || lds_floats_max > 40 * 1024)
         throw Error(SPS_ERR_INVALID_ARGUMENT, "staged pass LDS window out of range");
     const int64_t nb = ngroups * ceil_div(nsamps, kGroupTile);
     if (nb > 0x7fffffffLL) throw Error(SPS_ERR_INVALID_ARGUMENT, "merge grid too large");
-    SnrEpi epi{nullptr, 0, nullptr};
-    if (snr) epi = SnrEpi{snr->nrm, snr->nrm_len, snr->stats};
+    ChanNorm n{};
+    if (norm) n = *norm;
     const size_t lds = sizeof(float) * (size_t)lds_floats_max;
+    const bool u8 = in_u8 != 0, nrm = norm != nullptr;
     switch (L) {
         case 1:
-            group_dispatch<1>(snr, (uint32_t)nb, lds, in, in_pitch, out, out_pitch, d_groups, ngroups, d_leaves,
-                              d_rows, nsamps, out_rows_limit, epi, st);
+            group_init_kind<1>(u8, nrm, (uint32_t)nb, lds, in, in_pitch, n, d_grid, out, out_pitch, d_groups, ngroups,
+                               d_chans, d_rows, nsamps, st);
             break;
         case 2:
-            group_dispatch<2>(snr, (uint32_t)nb, lds, in, in_pitch, out, out_pitch, d_groups, ngroups, d_leaves,
-                              d_rows, nsamps, out_rows_limit, epi, st);
+            group_init_kind<2>(u8, nrm, (uint32_t)nb, lds, in, in_pitch, n, d_grid, out, out_pitch, d_groups, ngroups,
+                               d_chans, d_rows, nsamps, st);
             break;
         case 3:
-            group_dispatch<3>(snr, (uint32_t)nb, lds, in, in_pitch, out, out_pitch, d_groups, ngroups, d_leaves,
-                              d_rows, nsamps, out_rows_limit, epi, st);
+            group_init_kind<3>(u8, nrm, (uint32_t)nb, lds, in, in_pitch, n, d_grid, out, out_pitch, d_groups, ngroups,
+                               d_chans, d_rows, nsamps, st);
             break;
         case 4:
-            group_dispatch<4>(snr, (uint32_t)nb, lds, in, in_pitch, out, out_pitch, d_groups, ngroups, d_leaves,
-                              d_rows, nsamps, out_rows_limit, epi, st);
+            group_init_kind<4>(u8, nrm, (uint32_t)nb, lds, in, in_pitch, n, d_grid, out, out_pitch, d_groups, ngroups,
+                               d_chans, d_rows, nsamps, st);
             break;
         default:
             throw Error(SPS_ERR_INVALID_ARGUMENT, "staged fusion depth must be 1..4");
@@ -716,7 +937,7 @@ void launch_fdmt_group(int L, const float* in, int64_t in_pitch, float* out, int
 template <int MODE, int IN_KIND>
 static void init_dispatch_norm(const void* in, int64_t in_pitch, const ChanNorm* norm, int64_t nchans,
                                int64_t nsamps, const int32_t* g, const int64_t* go, const int64_t* cro, float* state,
-                               int64_t pitch, int dmax_all, hipStream_t st) {
+                               int64_t pitch, int dmax_all, const uint8_t* live, hipStream_t st) {
     ChanNorm n{};
     if (norm) n = *norm;
     if (dmax_all >= 0 && dmax_all <= 16384) {
@@ -729,52 +950,54 @@ static void init_dispatch_norm(const void* in, int64_t in_pitch, const ChanNorm*
         const size_t lds = sizeof(float) * (size_t)(kInitTile + dmax_all);
         if (norm)
             hipLaunchKernelGGL((k_init_lds<MODE, IN_KIND, true>), grid, dim3(kThreads), lds, st, in, in_pitch, n,
-                               nsamps, g, go, cro, state, pitch, dmax_all, al_in, al_out);
+                               nsamps, g, go, cro, state, pitch, dmax_all, al_in, al_out, live);
         else
             hipLaunchKernelGGL((k_init_lds<MODE, IN_KIND, false>), grid, dim3(kThreads), lds, st, in, in_pitch, n,
-                               nsamps, g, go, cro, state, pitch, dmax_all, al_in, al_out);
+                               nsamps, g, go, cro, state, pitch, dmax_all, al_in, al_out, live);
         return;
     }
     dim3 grid((uint32_t)ceil_div(nsamps, kThreads), (uint32_t)nchans);
     if (norm) {
         n = *norm;
         hipLaunchKernelGGL((k_init<MODE, IN_KIND, true>), grid, dim3(kThreads), 0, st, in, in_pitch, n, nsamps, g, go,
-                           cro, state, pitch);
+                           cro, state, pitch, live);
     } else {
         hipLaunchKernelGGL((k_init<MODE, IN_KIND, false>), grid, dim3(kThreads), 0, st, in, in_pitch, n, nsamps, g, go,
-                           cro, state, pitch);
+                           cro, state, pitch, live);
     }
 }
 
 template <int MODE>
 static void init_dispatch_kind(const void* in, int in_u8, int ones, int64_t in_pitch, const ChanNorm* norm,
                                int64_t nchans, int64_t nsamps, const int32_t* g, const int64_t* go,
-                               const int64_t* cro, float* state, int64_t pitch, int dmax_all, hipStream_t st) {
+                               const int64_t* cro, float* state, int64_t pitch, int dmax_all, const uint8_t* live,
+                               hipStream_t st) {
     if (ones)
-        init_dispatch_norm<MODE, 2>(in, in_pitch, norm, nchans, nsamps, g, go, cro, state, pitch, dmax_all, st);
+        init_dispatch_norm<MODE, 2>(in, in_pitch, norm, nchans, nsamps, g, go, cro, state, pitch, dmax_all, live, st);
     else if (in_u8)
-        init_dispatch_norm<MODE, 1>(in, in_pitch, norm, nchans, nsamps, g, go, cro, state, pitch, dmax_all, st);
+        init_dispatch_norm<MODE, 1>(in, in_pitch, norm, nchans, nsamps, g, go, cro, state, pitch, dmax_all, live, st);
     else
-        init_dispatch_norm<MODE, 0>(in, in_pitch, norm, nchans, nsamps, g, go, cro, state, pitch, dmax_all, st);
+        init_dispatch_norm<MODE, 0>(in, in_pitch, norm, nchans, nsamps, g, go, cro, state, pitch, dmax_all, live, st);
 }
 
 void launch_fdmt_init(int mode, const void* in, int in_u8, int ones, int64_t in_pitch, const ChanNorm* norm,
                       int64_t nchans, int64_t nsamps, const int32_t* d_grid, const int64_t* d_grid_off,
-                      const int64_t* d_chan_row_off, float* state, int64_t pitch, hipStream_t st, int dmax_all) {
+                      const int64_t* d_chan_row_off, float* state, int64_t pitch, hipStream_t st, int dmax_all,
+                      const uint8_t* d_live) {
     if (nchans <= 0 || nsamps <= 0) return;
     if (nchans > 65535) throw Error(SPS_ERR_INVALID_ARGUMENT, "more than 65535 channels");
     switch (mode) {
         case kInitMeanD:
             init_dispatch_kind<kInitMeanD>(in, in_u8, ones, in_pitch, norm, nchans, nsamps, d_grid, d_grid_off,
-                                           d_chan_row_off, state, pitch, dmax_all, st);
+                                           d_chan_row_off, state, pitch, dmax_all, d_live, st);
             break;
         case kInitRestrictD:
             init_dispatch_kind<kInitRestrictD>(in, in_u8, ones, in_pitch, norm, nchans, nsamps, d_grid, d_grid_off,
-                                               d_chan_row_off, state, pitch, dmax_all, st);
+                                               d_chan_row_off, state, pitch, dmax_all, d_live, st);
             break;
         case kInitCFdmtD:
             init_dispatch_kind<kInitCFdmtD>(in, in_u8, ones, in_pitch, norm, nchans, nsamps, d_grid, d_grid_off,
-                                            d_chan_row_off, state, pitch, dmax_all, st);
+                                            d_chan_row_off, state, pitch, dmax_all, d_live, st);
             break;
         default:
             throw Error(SPS_ERR_INVALID_ARGUMENT, "unknown init mode");

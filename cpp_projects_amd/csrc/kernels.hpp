@@ -32,10 +32,12 @@ struct ChanNorm {
 // FDMT init: rows of state 0 for every channel.
 //   in: nchans x nsamps (f32 or u8; ignored when ones), in_pitch elements
 //   norm: nullptr, or per-channel normalisation
+//   d_live: nullptr, or per state-0 row flags; rows flagged 0 are not stored
 void launch_fdmt_init(int mode, const void* in, int in_u8, int ones, int64_t in_pitch, const ChanNorm* norm,
                       int64_t nchans, int64_t nsamps, const int32_t* d_grid, const int64_t* d_grid_off,
                       const int64_t* d_chan_row_off, float* state, int64_t pitch, hipStream_t st,
-                      int dmax_all = -1);  // largest init dt of any channel; < 0 selects the unstaged kernel
+                      int dmax_all = -1,  // largest init dt of any channel; < 0 selects the unstaged kernel
+                      const uint8_t* d_live = nullptr);
 
 // One output row of a pass that fuses L iterations: the merge tree in heap
 // order (node 1 = the row; node k has tail child 2k and head child 2k+1 with
@@ -57,13 +59,15 @@ constexpr int kMaxFuse = 3;
 constexpr int kGroupTile = 512;
 constexpr int kMaxGroupFuse = 4;
 struct GroupDescD {
-    int64_t out_row0;  // first output row (rows are consecutive)
+    int64_t row0;      // first GRowDesc of the group (live output rows, in order)
     int64_t leaf0;     // first LeafDescD of the group
+    int64_t unit0;     // first entry of the group in the unit -> leaf table
     int32_t nrows;
     int32_t nleaves;
     int32_t lds_floats;
-    int32_t pad;
+    int32_t nunits;    // float4 units of all its leaf windows (lds_floats / 4)
 };
+constexpr int kGroupMaxUnits = 2304;  // per group: 9 float4 per thread (a lone depth-4 tree needs <= 2080)
 struct LeafDescD {
     int64_t row;       // input state row
     int32_t smax4;     // largest shift it is read with, rounded up to 4
@@ -76,8 +80,23 @@ struct LeafDescD {
 template <int L>
 struct GRowDesc {
     int64_t out_row;
+    int32_t maxshift;  // largest composite shift of a leaf: for t >= maxshift every merge adds its head
+    int32_t pad;
     int32_t lbase[1 << L];
     int32_t off[1 << L];
+};
+
+// First staged pass that computes state 0 itself (mean init only): for every
+// channel whose init rows are leaves of the group, rows 0..nrows-1 are built
+// from the input over [T - s4, T - s4 + 4*units) at LDS row_off + 4*units*k.
+// GroupDescD.leaf0 / nleaves then index ChanWinD entries.
+struct ChanWinD {
+    int64_t chan;
+    int64_t grid0;    // first init dt of the channel in the grid table
+    int32_t nrows;    // init rows of the channel
+    int32_t s4;       // largest leaf shift, rounded up to 4
+    int32_t units;    // row window length / 4
+    int32_t row_off;  // LDS float offset of row 0
 };
 
 // Optional epilogue of the last merge: divide by sqrtf(n + 1e-8) with n from
@@ -100,10 +119,21 @@ void launch_fdmt_fused(int L, const float* in, int64_t in_pitch, float* out, int
                        const MergeEpilogue* snr = nullptr);
 
 // A staged pass (1 <= L <= kMaxGroupFuse) over ngroups GroupDescD; in must be
-// 16-byte aligned with in_pitch % 4 == 0 (the internal states are).
+// 16-byte aligned with in_pitch % 4 == 0 (the internal states are).  Each
+// block streams tiles_per_block consecutive tiles of one group through two
+// LDS windows; d_unit_leaf maps every float4 unit of a group's windows to
+// its leaf (uint8, GroupDescD.unit0 based).
 void launch_fdmt_group(int L, const float* in, int64_t in_pitch, float* out, int64_t out_pitch, const void* d_groups,
-                       int64_t ngroups, const void* d_leaves, const void* d_rows, int lds_floats_max, int64_t nsamps,
-                       int64_t out_rows_limit, hipStream_t st, const MergeEpilogue* snr = nullptr);
+                       int64_t ngroups, const void* d_leaves, const void* d_rows, const uint8_t* d_unit_leaf,
+                       int lds_floats_max, int tiles_per_block, int64_t nsamps, int64_t out_rows_limit,
+                       hipStream_t st, const MergeEpilogue* snr = nullptr);
+
+// First staged pass with the (mean) init fused in: reads the waterfall (f32 or
+// u8, optional per-channel normalisation) instead of state 0.
+void launch_fdmt_group_init(int L, const void* in, int in_u8, int64_t in_pitch, const ChanNorm* norm,
+                            const int32_t* d_grid, float* out, int64_t out_pitch, const void* d_groups,
+                            int64_t ngroups, const void* d_chans, const void* d_rows, int lds_floats_max,
+                            int64_t nsamps, hipStream_t st);
 
 // Adjoint of one merge: g_prev[p][t] = sum over (o, s) in csr[p] of g_out[o][t+s] (t+s < nsamps).
 void launch_fdmt_merge_adjoint(const float* g_out, int64_t out_pitch, float* g_prev, int64_t prev_pitch,

@@ -263,33 +263,63 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_all(const float* __restrict
 //    largest flagged width) and decided exactly as the reference does.
 constexpr int kPowCells = 2048;            // cells per block (8 per thread)
 constexpr int kPowPer = kPowCells / kThreads;
+constexpr int kMaxPowQ = 13;
 
-__device__ __forceinline__ double gamma_n(double n) {
-    const double u = 0x1p-24;
-    return n * u / (1.0 - n * u);
+struct BoxErr {
+    double e[kMaxPowQ + 1];
+};
+
+__host__ __device__ constexpr int round_up4(int v) { return (v + 3) & ~3; }
+
+// Largest float not above the filter limit of width 2^k:
+// thr*sqrtf(w)*(1 - 2^-22) - e_k*M, e_k = (gamma_{w-1} + gamma_k)*w*(1 + 2^-20)
+// (computed on the host).  For a float P, P > this float <=> P > the limit;
+// NaN thresholds or NaN/infinite samples flag everything.
+__device__ __forceinline__ float level_limit(float thr, int k, double e_k, float M) {
+    if (!(thr > 0.0f)) return -INFINITY;
+    const double lim = (double)thr * (double)sqrtf((float)(1 << k)) * (1.0 - 0x1p-22) - e_k * (double)M;
+    return __double2float_rd(lim);
 }
 
 __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restrict__ img, int64_t cols, int64_t pitch,
                                                           const float* __restrict__ d_thr, float h_thr, int q,
-                                                          CandD* __restrict__ out, int64_t cap,
+                                                          BoxErr err, CandD* __restrict__ out, int64_t cap,
                                                           unsigned long long* __restrict__ count) {
     extern __shared__ float lds[];
     __shared__ float wmaxabs[kThreads / 64];
     const int W = 1 << q;
-    const int span_max = kPowCells + W - 1;
+    const int span_max = round_up4(kPowCells + W - 1);
     float* x = lds;                  // level 0 (the samples)
     float* la = lds + span_max;      // ping
     float* lb = la + span_max;       // pong
     const int64_t r = blockIdx.y;
     const int64_t T = (int64_t)blockIdx.x * kPowCells;
-    const int span = (int)min((int64_t)span_max, cols - T);
+    const int span = (int)min((int64_t)(kPowCells + W - 1), cols - T);
     const float* row = img + r * pitch;
     float m = 0.0f;
-    for (int i = threadIdx.x; i < span; i += kThreads) {
-        const float v = row[T + i];
-        x[i] = v;
-        m = fmaxf(m, fabsf(v));
-        if (v != v) m = __int_as_float(0x7f800000);  // NaN: disable the filter
+    if ((pitch & 3) == 0 && (reinterpret_cast<uintptr_t>(img) & 15) == 0) {
+        // 16-byte loads (T is a multiple of 4); the last partial vector scalar
+        const int n4 = span >> 2;
+        const float4* row4 = reinterpret_cast<const float4*>(row + T);
+        for (int i = threadIdx.x; i < n4; i += kThreads) {
+            const float4 v = row4[i];
+            reinterpret_cast<float4*>(x)[i] = v;
+            m = fmaxf(m, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+            if (v.x != v.x || v.y != v.y || v.z != v.z || v.w != v.w) m = __int_as_float(0x7f800000);
+        }
+        for (int i = 4 * n4 + threadIdx.x; i < span; i += kThreads) {
+            const float v = row[T + i];
+            x[i] = v;
+            m = fmaxf(m, fabsf(v));
+            if (v != v) m = __int_as_float(0x7f800000);
+        }
+    } else {
+        for (int i = threadIdx.x; i < span; i += kThreads) {
+            const float v = row[T + i];
+            x[i] = v;
+            m = fmaxf(m, fabsf(v));
+            if (v != v) m = __int_as_float(0x7f800000);  // NaN: disable the filter
+        }
     }
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
     if ((threadIdx.x & 63) == 0) wmaxabs[threadIdx.x >> 6] = m;
@@ -298,35 +328,46 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
     for (int i = 1; i < kThreads / 64; ++i) M = fmaxf(M, wmaxabs[i]);
     const float thr = d_thr ? *d_thr : h_thr;
 
+    // width 1 from the samples
     uint32_t flags[kPowPer];
-#pragma unroll
-    for (int j = 0; j < kPowPer; ++j) flags[j] = 0;
-    const float* cur = x;
-    for (int k = 0; k <= q; ++k) {
-        const int w = 1 << k;
-        if (k > 0) {
-            // level k from level k-1: valid for i + w <= span
-            float* dst = (k & 1) ? la : lb;
-            const int h = w >> 1;
-            for (int i = threadIdx.x; i + w <= span; i += kThreads) dst[i] = cur[i] + cur[i + h];
-            __syncthreads();
-            cur = dst;
-        }
-        const float c = sqrtf((float)w);
-        double lim = -INFINITY;
-        if (thr > 0.0f)
-            lim = (double)thr * (double)c * (1.0 - 0x1p-22) -
-                  (gamma_n((double)(w - 1)) + gamma_n((double)k)) * (double)w * (double)M * (1.0 + 0x1p-20);
+    {
+        const float lf = level_limit(thr, 0, err.e[0], M);
 #pragma unroll
         for (int j = 0; j < kPowPer; ++j) {
             const int lt = j * kThreads + threadIdx.x;
-            if (T + lt + w <= cols && lt + w <= span) {
-                const float P = cur[lt];
-                if (!((double)P <= lim)) flags[j] |= 1u << k;
+            flags[j] = (T + lt + 1 <= cols && !(x[lt] <= lf)) ? 1u : 0u;
+        }
+    }
+    // Two dyadic levels per round: from level k0 (cur), level k0+1 at i is
+    // cur[i] + cur[i+h] and level k0+2 is that plus cur[i+2h] + cur[i+3h]
+    // (h = 2^k0), the pairwise sums of the separate levels.  Only level
+    // k0+2 is stored; each round reads one buffer and writes the other.
+    const float* cur = x;
+    for (int k0 = 0; k0 < q; k0 += 2) {
+        const int k1 = k0 + 1, k2 = k0 + 2;
+        const int h = 1 << k0;
+        const float l1 = level_limit(thr, k1, err.e[k1], M);
+        const bool two = k2 <= q;
+        const float l2 = two ? level_limit(thr, k2, err.e[k2], M) : 0.0f;
+        float* dst = ((k0 >> 1) & 1) ? lb : la;
+#pragma unroll
+        for (int j = 0; j < kPowPer; ++j) {  // the thread's own cells
+            const int i = j * kThreads + threadIdx.x;
+            if (i + 2 * h > span) continue;  // level k1 invalid here
+            const float p1 = cur[i] + cur[i + h];
+            if (T + i + 2 * h <= cols && !(p1 <= l1)) flags[j] |= 1u << k1;
+            if (two && i + 4 * h <= span) {
+                const float p2 = p1 + (cur[i + 2 * h] + cur[i + 3 * h]);
+                dst[i] = p2;
+                if (T + i + 4 * h <= cols && !(p2 <= l2)) flags[j] |= 1u << k2;
             }
         }
-        // the next level overwrites the buffer read two levels back only
+        if (two) {  // the halo: stored for the next rounds only
+            for (int i = kPowCells + threadIdx.x; i + 4 * h <= span; i += kThreads)
+                dst[i] = (cur[i] + cur[i + h]) + (cur[i + 2 * h] + cur[i + 3 * h]);
+        }
         __syncthreads();
+        cur = dst;
     }
     // Compact the flagged cells into an LDS queue (cell << 14 | width flags;
     // the level buffers are free now), so the exact recompute runs one chain
@@ -369,15 +410,28 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
         int top = fl ? 31 - __clz(fl) : -1;
         for (int off = 32; off > 0; off >>= 1) top = max(top, __shfl_xor(top, off));
         if (top < 0) continue;  // wave-uniform
+        // segment p adds samples [2^(p-1), 2^p) to the running sum (sample 0
+        // for p = 0), then width 2^p is decided; a lane stops adding once no
+        // flag at or above p is left, so it never reads past its largest
+        // flagged window
         float s = 0.0f;
-        int p = 0, next = 1;
-        for (int iw = 0; iw < (1 << top); ++iw) {
-            if (fl >> p) s += x[lt + iw];
-            if (iw + 1 != next) continue;
+        const float* xs = x + lt;
+        for (int p = 0; p <= top; ++p) {
+            const int lo = p ? (1 << (p - 1)) : 0, hi = 1 << p;
+            if (fl >> p) {
+                int iw = lo;
+                for (; iw + 4 <= hi; iw += 4) {
+                    s += xs[iw];
+                    s += xs[iw + 1];
+                    s += xs[iw + 2];
+                    s += xs[iw + 3];
+                }
+                for (; iw < hi; ++iw) s += xs[iw];
+            }
             float snr = 0.0f;
             bool pred = false;
             if ((fl >> p) & 1u) {
-                snr = s / sqrtf((float)next);
+                snr = s / sqrtf((float)hi);
                 pred = snr > thr;
             }
             const uint64_t m = __ballot(pred);
@@ -391,8 +445,6 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
                     if ((int64_t)g < cap) out[g] = CandD{(int32_t)(T + lt), (int32_t)r, (int32_t)p, snr};
                 }
             }
-            next *= 2;
-            ++p;
         }
     }
 }
@@ -551,10 +603,18 @@ void launch_boxcar(const float* img, int64_t rows, int64_t cols, int64_t pitch, 
         while ((1 << q) < wmax) ++q;
         if ((1 << q) != wmax || q > 13) throw Error(SPS_ERR_INVALID_ARGUMENT, "pow2 window must be 2^q, q <= 13");
         const dim3 grid((uint32_t)ceil_div(cols, kPowCells), (uint32_t)rows);
-        const size_t lds = sizeof(float) * 3 * (size_t)(kPowCells + wmax - 1);
+        const size_t lds = sizeof(float) * 3 * (size_t)round_up4(kPowCells + wmax - 1);
         if (lds > 150 * 1024) throw Error(SPS_ERR_INVALID_ARGUMENT, "boxcar window too wide for LDS");
-        hipLaunchKernelGGL(k_boxcar_pow2, grid, dim3(kThreads), lds, st, img, cols, pitch, d_thr, h_thr, q, out, cap,
-                           d_count);
+        // error terms of the filter: (gamma_{w-1} + gamma_k) * w * (1 + 2^-20)
+        BoxErr err{};
+        const double u = 0x1p-24;
+        auto gam = [&](double n) { return n * u / (1.0 - n * u); };
+        for (int k = 0; k <= kMaxPowQ; ++k) {
+            const double w = std::ldexp(1.0, k);
+            err.e[k] = (gam(w - 1.0) + gam((double)k)) * w * (1.0 + 0x1p-20);
+        }
+        hipLaunchKernelGGL(k_boxcar_pow2, grid, dim3(kThreads), lds, st, img, cols, pitch, d_thr, h_thr, q, err, out,
+                           cap, d_count);
     } else {
         if (wmax > 8192) throw Error(SPS_ERR_INVALID_ARGUMENT, "window wider than 8192 samples");
         const dim3 grid((uint32_t)ceil_div(cols, kBoxCells), (uint32_t)rows);

@@ -81,10 +81,15 @@ class _Handle:
         arr = (C.c_int32 * max(len(depths), 1))(*depths)
         call("sps_fdmt_set_schedule", self._h, arr, len(depths))
 
-    def set_staged(self, on=True):
-        """LDS-staged passes (grouped output rows, each input row read once
-        per time tile)."""
-        call("sps_fdmt_set_staged", self._h, int(on))
+    def set_staged(self, mode=1):
+        """0: tree passes; 1: LDS-staged passes (grouped output rows, each
+        input row read once per time tile), the final pass on the tree kernel
+        when its depth allows; 2: as 1 with the mean init fused into the first
+        pass; 3, 4: as 1, 2 with the final pass staged too."""
+        call("sps_fdmt_set_staged", self._h, int(mode))
+
+    def set_group_shape(self, lds_floats=8192, max_rows=16, tiles_per_block=8):
+        call("sps_fdmt_set_group_shape", self._h, int(lds_floats), int(max_rows), int(tiles_per_block))
 
     def get_schedule(self):
         n = C.c_int64()

@@ -187,8 +187,12 @@ class SingleBeamSearch:
         arr = (C.c_int32 * max(len(depths), 1))(*depths)
         call("sps_search_set_schedule", self._h, arr, len(depths))
 
-    def set_staged(self, on=True):
-        call("sps_search_set_staged", self._h, int(on))
+    def set_staged(self, mode=1):
+        """Staging mode of the merge passes (see FDMTGPU.set_staged)."""
+        call("sps_search_set_staged", self._h, int(mode))
+
+    def set_group_shape(self, lds_floats=8192, max_rows=16, tiles_per_block=8):
+        call("sps_search_set_group_shape", self._h, int(lds_floats), int(max_rows), int(tiles_per_block))
 
     def get_schedule(self):
         n = C.c_int64()

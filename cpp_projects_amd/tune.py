@@ -1,7 +1,10 @@
 """Autotuner for the merge passes: which kernel family runs them (direct tree
-passes, or LDS-staged group passes) and how many FDMT iterations each pass
-fuses.  Candidates are timed with the stage events of the search handle on
-the real input; the fastest is kept on the handle."""
+passes, or LDS-staged group passes with or without a staged final pass), the
+staged group shape (LDS window, rows per group, tiles per block), and how many
+FDMT iterations each pass fuses.  Candidates are timed with the stage events
+of the search handle on the real input; the fastest is kept on the handle."""
+
+DEFAULT_SHAPES = ((6144, 16, 16), (5120, 16, 16), (7168, 16, 16), (6144, 16, 32))
 
 
 def candidate_schedules(n, max_depth=3):
@@ -25,30 +28,38 @@ def candidate_schedules(n, max_depth=3):
     return sorted(s for s in c if sum(s) == n and all(1 <= d <= max_depth for d in s) and s.count(1) <= 2)
 
 
-def tune_search(search, run_once, reps=2, log=None, modes=(False, True)):
-    """Times every (staged, schedule) candidate on `search` (a SingleBeamSearch)
-    with run_once() issuing one search step; keeps the fastest on the handle
-    and returns ({"staged": bool, "schedule": list}, results)."""
+def tune_search(search, run_once, reps=2, log=None, modes=(0, 1, 3), shapes=DEFAULT_SHAPES):
+    """Times every (staging mode, group shape, schedule) candidate on
+    `search` (a SingleBeamSearch) with run_once() issuing one search step;
+    keeps the fastest on the handle and returns
+    ({"staged": mode, "shape": shape or None, "schedule": list}, results).
+    Modes: 0 tree passes, 1 staged passes with a tree final pass, 2 as 1
+    with the init fused into the first pass, 3 all passes staged."""
     n = sum(search.get_schedule())
     results = []
     search.enable_timing(True)
     try:
-        for staged in modes:
-            search.set_staged(staged)
-            for s in candidate_schedules(n, 4 if staged else 3):
-                search.set_schedule(list(s))
-                run_once()  # builds the pass tables
-                tot = 0.0
-                for _ in range(reps):
-                    run_once()
-                    tot += sum(search.stage_times_ms())
-                results.append((tot / reps, staged, s))
-                if log:
-                    log(f"[tune] {'staged' if staged else 'tree'} {list(s)}: {tot / reps:.3f} ms")
+        for mode in modes:
+            for shape in ((None,) if mode == 0 else shapes):
+                search.set_staged(mode)
+                if shape is not None:
+                    search.set_group_shape(*shape)
+                for s in candidate_schedules(n, 4 if mode in (3, 4) else 3):
+                    search.set_schedule(list(s))
+                    run_once()  # builds the pass tables
+                    tot = 0.0
+                    for _ in range(reps):
+                        run_once()
+                        tot += sum(search.stage_times_ms())
+                    results.append((tot / reps, mode, shape, s))
+                    if log:
+                        log(f"[tune] mode {mode} shape {shape} {list(s)}: {tot / reps:.3f} ms")
     finally:
         search.enable_timing(False)
-    results.sort()
-    _, staged, sched = results[0]
-    search.set_staged(staged)
+    results.sort(key=lambda r: r[0])
+    _, mode, shape, sched = results[0]
+    search.set_staged(mode)
+    if shape is not None:
+        search.set_group_shape(*shape)
     search.set_schedule(list(sched))
-    return {"staged": staged, "schedule": list(sched)}, results
+    return {"staged": mode, "shape": shape, "schedule": list(sched)}, results

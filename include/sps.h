@@ -140,6 +140,10 @@ SPS_API sps_status sps_fdmt_get_schedule(const sps_fdmt* h, int32_t* h_depths, i
  * passes fuse 1..4 iterations; turning staging off resets a schedule with
  * depth-4 passes to the default. */
 SPS_API sps_status sps_fdmt_set_staged(sps_fdmt* h, int staged);
+/* Shape of the staged passes: LDS window per tile buffer (2048..9216 floats,
+ * default 8192; a block holds two), output rows per group (1..64, default
+ * 16) and consecutive 512-sample tiles streamed per block (1..512, default 8). */
+SPS_API sps_status sps_fdmt_set_group_shape(sps_fdmt* h, int lds_floats, int max_rows, int tiles_per_block);
 
 /* FDMT::initialise (fdmt_base.hpp:57): the iteration-0 state, rows0 x nsamps */
 SPS_API sps_status sps_fdmt_initialise(sps_fdmt* h, const void* d_waterfall, int dtype, float* d_state,
@@ -241,6 +245,7 @@ SPS_API sps_status sps_search_sort_candidates(sps_search* s, sps_cand* d_cands, 
 SPS_API sps_status sps_search_get_ndt(const sps_search* s, int64_t* h_ndt);
 /* staged passes of the search's data transform (see sps_fdmt_set_staged) */
 SPS_API sps_status sps_search_set_staged(sps_search* s, int staged);
+SPS_API sps_status sps_search_set_group_shape(sps_search* s, int lds_floats, int max_rows, int tiles_per_block);
 /* merge schedule of the search's data transform (see sps_fdmt_set_schedule) */
 SPS_API sps_status sps_search_set_schedule(sps_search* s, const int32_t* h_depths, int64_t n);
 SPS_API sps_status sps_search_get_schedule(const sps_search* s, int32_t* h_depths, int64_t cap, int64_t* h_n);
@@ -299,6 +304,8 @@ SPS_API sps_status sps_batch_get_beam_info(const sps_batch* b, int64_t beam, int
                                            int64_t* h_niters);
 /* staged passes of one beam (see sps_fdmt_set_staged); drops the graph */
 SPS_API sps_status sps_batch_set_staged(sps_batch* b, int64_t beam, int staged);
+SPS_API sps_status sps_batch_set_group_shape(sps_batch* b, int64_t beam, int lds_floats, int max_rows,
+                                           int tiles_per_block);
 /* merge schedule of one beam (see sps_fdmt_set_schedule); drops the graph */
 SPS_API sps_status sps_batch_set_schedule(sps_batch* b, int64_t beam, const int32_t* h_depths, int64_t n);
 SPS_API sps_status sps_batch_get_schedule(const sps_batch* b, int64_t beam, int32_t* h_depths, int64_t cap,

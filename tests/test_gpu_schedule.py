@@ -234,3 +234,101 @@ def test_search_staged_same_candidates(torch_cuda):
         n = s.sort_candidates(c, cnt)
         outs.append((sha(dmt.cpu().numpy()), cands_to_numpy(c, n).tobytes()))
     assert all(o == outs[0] for o in outs)
+
+
+def test_fused_init_on_golden(torch_cuda, golden):
+    """Staging mode 2: the first pass builds the mean-init rows itself."""
+    from cpp_projects_amd.fdmt import FDMTGPU
+
+    man, arr = golden
+    for name in ("c1000odd", "c1024", "c3000", "c256step2", "c512file", "c64"):
+        e = man["fdmt"][name]
+        x = make_input(e["input"], e["nchans"], e["nsamps"], e["seed"], arr)
+        f = FDMTGPU(e["fmin"], e["fmax"], e["nchans"], e["nsamps"], 1.0, e["dt_max"], e["dt_step"], e["dt_min"])
+        f.set_staged(2)
+        n = e["niters"]
+        for first in (1, 2, 3, 4):
+            rest = n - first
+            s = [first] + [3] * (rest // 3) + [rest % 3] * (rest % 3 > 0)
+            s = [d for d in s if d > 0]
+            f.set_schedule(s)
+            print("case fused init", name, s, flush=True)
+            assert sha(run(torch_cuda, f, x)) == e["out_sha256"], (name, s)
+
+
+def test_fused_init_u8_and_restrict_fallback(torch_cuda):
+    from cpp_projects_amd.fdmt import SPS_INIT_RESTRICT, FDMTGPU
+
+    torch = torch_cuda
+    rng = np.random.default_rng(23)
+    nch, ns, dtm = 300, 4099, 299
+    x = rng.integers(0, 256, (nch, ns), dtype=np.uint8)
+    p = orc.Plan(1200.0, 1600.0, nch, ns, 1.0, dtm)
+    ref = p.execute(x.astype(np.float32))
+    f = FDMTGPU(1200.0, 1600.0, nch, ns, 1.0, dtm)
+    f.set_staged(2)
+    n = p.niters
+    for s in ([3] * (n // 3) + [n % 3] * (n % 3 > 0), [4, 4] + [1] * (n - 8)):
+        s = [d for d in s if d > 0]
+        f.set_schedule(s)
+        d_in = torch.from_numpy(x).cuda()
+        d_out = torch.empty((dtm + 1, ns), dtype=torch.float32, device="cuda")
+        f.execute(d_in, d_in.numel(), d_out, d_out.numel())
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(d_out.cpu().numpy().view(np.uint32), ref.view(np.uint32), err_msg=str(s))
+    # RESTRICT init keeps the separate init kernel under mode 2
+    xf = rng.standard_normal((nch, ns)).astype(np.float32)
+    rr = orc.Plan(1200.0, 1600.0, nch, ns, 1.0, dtm).execute(xf, init_mode=1)
+    g = FDMTGPU(1200.0, 1600.0, nch, ns, 1.0, dtm, init_mode=SPS_INIT_RESTRICT)
+    g.set_staged(2)
+    g.set_schedule([3] * (n // 3) + [n % 3] * (n % 3 > 0) if n % 3 else [3] * (n // 3))
+    np.testing.assert_array_equal(run(torch, g, xf).view(np.uint32), rr.view(np.uint32))
+
+
+def test_search_fused_init_vs_oracle_chain(torch_cuda):
+    from cpp_projects_amd.search import SingleBeamSearch, cands_to_numpy
+
+    torch = torch_cuda
+    rng = np.random.default_rng(29)
+    nch, ns, dtm = 256, 6000, 255
+    x = np.clip(np.round(rng.normal(128, 16, (nch, ns))), 0, 255).astype(np.uint8)
+    x[17] = np.clip(np.round(rng.normal(128, 70, ns)), 0, 255).astype(np.uint8)  # RFI row
+    x[:, 2000:2003] = 210
+    s = SingleBeamSearch(1200.0, 1600.0, nch, ns, dtm, in_dtype="u8", box_q=5, threshold=5.0, cand_capacity=1 << 17)
+    s.set_staged(2)
+    s.set_schedule([4, 4])
+    d_in = torch.from_numpy(x).cuda()
+    dmt = torch.empty((s.ndt, ns), dtype=torch.float32, device="cuda")
+    c = torch.empty((1 << 17, 4), dtype=torch.int32, device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    s.run(d_in, dmt, c, cnt)
+    k = s.sort_candidates(c, cnt)
+    xn, mask, (_, _, keep, _) = orc.normalise(x, 1)
+    assert keep[17] == 0
+    p = orc.Plan(1200.0, 1600.0, nch, ns, 1.0, dtm)
+    snr = orc.fdmt_normalization(p.execute(xn), p.execute(mask))
+    np.testing.assert_array_equal(dmt.cpu().numpy().view(np.uint32), snr.view(np.uint32))
+    ref = orc.boxcar_pow2(snr, 5.0, 5)
+    assert k == len(ref) and k > 0
+    np.testing.assert_array_equal(cands_to_numpy(c, k).view(np.int32), ref.view(np.int32))
+
+
+def test_staged_group_shapes(torch_cuda, golden):
+    """Group windows from 2048 to 9216 floats, 1 to 64 rows per group and 1 to
+    64 tiles per block give the same bits (plain staged and fused init)."""
+    from cpp_projects_amd.fdmt import FDMTGPU
+
+    man, arr = golden
+    e = man["fdmt"]["c3000"]
+    x = make_input(e["input"], e["nchans"], e["nsamps"], e["seed"], arr)
+    f = FDMTGPU(e["fmin"], e["fmax"], e["nchans"], e["nsamps"], 1.0, e["dt_max"], e["dt_step"], e["dt_min"])
+    n = e["niters"]
+    for mode in (1, 2):
+        f.set_staged(mode)
+        for floats, rows, tiles in ((2048, 1, 1), (2048, 64, 3), (4096, 4, 8), (9216, 64, 64), (8192, 32, 5)):
+            f.set_group_shape(floats, rows, tiles)
+            for s in ([3] * (n // 3) + [n % 3] * (n % 3 > 0), [4] * (n // 4) + [n % 4] * (n % 4 > 0)):
+                s = [d for d in s if d > 0]
+                f.set_schedule(s)
+                print("case shape", mode, floats, rows, tiles, s, flush=True)
+                assert sha(run(torch_cuda, f, x)) == e["out_sha256"], (mode, floats, rows, tiles, s)
