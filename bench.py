@@ -266,13 +266,16 @@ def plan_rows_for(nch, ns, dt_max):
     return [int(f.state_shape(i)[3]) for i in range(f.get_niters() + 1)]
 
 
-def load_traffic():
+def load_traffic(kernel):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 counters
+    (profiles/merge_traffic.json), or None when the profile covers another
+    kernel."""
     path = os.path.join(ROOT, "profiles", "merge_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
-    return d.get("traffic_bytes_per_launch")
+    return d.get("traffic_bytes_per_launch") if d.get("kernel") == kernel else None
 
 
 def main():
@@ -282,14 +285,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-cols", type=int, default=1 << 17)
-    ap.add_argument("--schedule", type=str, default="",
-                    help="comma-separated merge fusion depths (default: autotuned during warmup)")
-    ap.add_argument("--no-tune", action="store_true", help="keep the library's default schedule")
-    ap.add_argument("--staged", type=int, default=0,
-                    help="with --schedule/--no-tune: staging mode 0..4 (0 tree passes, 1 staged passes with a "
-                         "tree final pass, 3 all staged; 2, 4 add the fused init)")
-    ap.add_argument("--group-shape", type=str, default="6144,16,16",
-                    help="with --schedule and --staged > 0: LDS floats, rows per group, tiles per block")
+    # the defaults are the autotuner's choice for this workload on MI355X
+    ap.add_argument("--schedule", type=str, default="3,3,2,2,2", help="comma-separated merge fusion depths")
+    ap.add_argument("--staged", type=int, default=1,
+                    help="staging mode 0..4 (0 tree passes, 1 staged passes with a tree final pass, 3 all "
+                         "staged; 2, 4 add the fused init)")
+    ap.add_argument("--group-shape", type=str, default="5120,16,16",
+                    help="staged passes: LDS floats per tile buffer, rows per group, tiles per block")
+    ap.add_argument("--tune", action="store_true", help="autotune mode, shape and schedule during warmup instead")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-beam measurement")
     ap.add_argument("--batch-beams", type=int, default=BATCH["nbeams"])
     ap.add_argument("--batch-steps", type=int, default=3)
@@ -333,22 +336,23 @@ def main():
         s.run(x, dmt, cands, count)
     torch.cuda.synchronize()
     shape = None
-    if args.schedule:
-        s.set_staged(args.staged)
-        if args.staged:
-            shape = [int(v) for v in args.group_shape.split(",")]
-            s.set_group_shape(*shape)
-        s.set_schedule([int(v) for v in args.schedule.split(",")])
-    elif not args.no_tune:
+    if args.tune:
         from cpp_projects_amd.tune import tune_search
 
         best, _ = tune_search(s, lambda: s.run(x, dmt, cands, count), reps=2, log=log)
         shape = best["shape"]
+        staged = int(best["staged"])
         log(f"[bench] tuned schedule {best}")
+    else:
+        staged = int(args.staged)
+        s.set_staged(staged)
+        if staged:
+            shape = [int(v) for v in args.group_shape.split(",")]
+            s.set_group_shape(*shape)
+        s.set_schedule([int(v) for v in args.schedule.split(",")])
     s.run(x, dmt, cands, count)
     torch.cuda.synchronize()
     schedule = s.get_schedule()
-    staged = int(args.staged) if (args.schedule or args.no_tune) else int(best["staged"])
     log(f"[bench] timing {args.steps} steps")
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -412,6 +416,7 @@ def main():
     avg_bytes = sum(alg[k] for k in ks) / len(ks)
     avg_io = sum(io[k] for k in ks) / len(ks)
     achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+    traffic = load_traffic(dom)
     names = (["stats", "mask_fdmt", "init"] + [f"pass{k}(L={schedule[k]})" for k in range(npass - 1)] +
              [f"pass{npass - 1}(L={schedule[-1]})+snr"] + ["snr_norm_separate", "boxcar"])
     stages = {n: round(v, 4) for n, v in zip(names, stage_ms)}
@@ -446,12 +451,17 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": load_traffic(),
+            "traffic": traffic,
+            "traffic_GBs": traffic / (avg_ms * 1e-3) / 1e9 if traffic else None,
+            "traffic_frac": traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic else None,
             "alg_bytes_per_launch": avg_bytes,
             "avg_launch_ms": avg_ms,
             "launches_per_step": len(ks),
             "compulsory_io_bytes_per_launch": avg_io,
             "compulsory_io_GBs": avg_io / (avg_ms * 1e-3) / 1e9,
+            "note": "achieved credits the unfused byte model (every state row read and written once per "
+                    "iteration); traffic is the counter-measured HBM bytes of one launch (rocprofv3, "
+                    "profiles/merge_traffic.json)",
         },
         "stages_ms": stages,
         "effective_GBs_whole_step": total_alg / (sum(stage_ms) * 1e-3) / 1e9,
