@@ -328,20 +328,25 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
     for (int i = 1; i < kThreads / 64; ++i) M = fmaxf(M, wmaxabs[i]);
     const float thr = d_thr ? *d_thr : h_thr;
 
-    // width 1 from the samples
+    // width 1 from the samples (span <= cols - T, so a window that fits the
+    // span fits the row)
     uint32_t flags[kPowPer];
     {
         const float lf = level_limit(thr, 0, err.e[0], M);
 #pragma unroll
         for (int j = 0; j < kPowPer; ++j) {
             const int lt = j * kThreads + threadIdx.x;
-            flags[j] = (T + lt + 1 <= cols && !(x[lt] <= lf)) ? 1u : 0u;
+            flags[j] = (lt < span && !(x[lt] <= lf)) ? 1u : 0u;
         }
     }
     // Two dyadic levels per round: from level k0 (cur), level k0+1 at i is
     // cur[i] + cur[i+h] and level k0+2 is that plus cur[i+2h] + cur[i+3h]
     // (h = 2^k0), the pairwise sums of the separate levels.  Only level
     // k0+2 is stored; each round reads one buffer and writes the other.
+    // All reads of a round are issued before any is used (no branches between
+    // them): indexes stay below span_max (i + 3h <= 2047 + 3W/4 when two
+    // levels are formed, i + h <= 2047 + W/2 otherwise) and entries at or
+    // past a level's valid span are computed but never flagged or consumed.
     const float* cur = x;
     for (int k0 = 0; k0 < q; k0 += 2) {
         const int k1 = k0 + 1, k2 = k0 + 2;
@@ -350,16 +355,37 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
         const bool two = k2 <= q;
         const float l2 = two ? level_limit(thr, k2, err.e[k2], M) : 0.0f;
         float* dst = ((k0 >> 1) & 1) ? lb : la;
+        float va[kPowPer], vb[kPowPer], vc[kPowPer], vd[kPowPer];
+        if (two) {
 #pragma unroll
-        for (int j = 0; j < kPowPer; ++j) {  // the thread's own cells
-            const int i = j * kThreads + threadIdx.x;
-            if (i + 2 * h > span) continue;  // level k1 invalid here
-            const float p1 = cur[i] + cur[i + h];
-            if (T + i + 2 * h <= cols && !(p1 <= l1)) flags[j] |= 1u << k1;
-            if (two && i + 4 * h <= span) {
-                const float p2 = p1 + (cur[i + 2 * h] + cur[i + 3 * h]);
+            for (int j = 0; j < kPowPer; ++j) {
+                const int i = j * kThreads + threadIdx.x;
+                va[j] = cur[i];
+                vb[j] = cur[i + h];
+                vc[j] = cur[i + 2 * h];
+                vd[j] = cur[i + 3 * h];
+            }
+#pragma unroll
+            for (int j = 0; j < kPowPer; ++j) {  // the thread's own cells
+                const int i = j * kThreads + threadIdx.x;
+                const float p1 = va[j] + vb[j];
+                const float p2 = p1 + (vc[j] + vd[j]);
                 dst[i] = p2;
-                if (T + i + 4 * h <= cols && !(p2 <= l2)) flags[j] |= 1u << k2;
+                if (i + 2 * h <= span && !(p1 <= l1)) flags[j] |= 1u << k1;
+                if (i + 4 * h <= span && !(p2 <= l2)) flags[j] |= 1u << k2;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < kPowPer; ++j) {
+                const int i = j * kThreads + threadIdx.x;
+                va[j] = cur[i];
+                vb[j] = cur[i + h];
+            }
+#pragma unroll
+            for (int j = 0; j < kPowPer; ++j) {
+                const int i = j * kThreads + threadIdx.x;
+                const float p1 = va[j] + vb[j];
+                if (i + 2 * h <= span && !(p1 <= l1)) flags[j] |= 1u << k1;
             }
         }
         if (two) {  // the halo: stored for the next rounds only
@@ -370,46 +396,72 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
         cur = dst;
     }
     // Compact the flagged cells into an LDS queue (cell << 14 | width flags;
-    // the level buffers are free now), so the exact recompute runs one chain
-    // per lane over flagged cells only.
+    // the level buffers are free now) in cell order: slot j of every thread
+    // holds cell j*256 + tid, so entries are ranked by (j, wave, lane) with
+    // one ballot per slot.  The exact recompute then runs one chain per lane
+    // over flagged cells only.
     uint32_t* queue = reinterpret_cast<uint32_t*>(la);
-    __shared__ uint32_t wave_tot[kThreads / 64];
+    __shared__ uint32_t slot_cnt[kPowPer][kThreads / 64];
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
-    uint32_t mine = 0;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint64_t mk[kPowPer];
 #pragma unroll
-    for (int j = 0; j < kPowPer; ++j) mine += flags[j] ? 1u : 0u;
-    uint32_t incl = mine;
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t v = __shfl_up(incl, d);
-        if (lane >= d) incl += v;
+    for (int j = 0; j < kPowPer; ++j) {
+        mk[j] = __ballot(flags[j] != 0);
+        if (lane == 0) slot_cnt[j][wid] = (uint32_t)__popcll(mk[j]);
     }
-    if (lane == 63) wave_tot[wid] = incl;
     __syncthreads();
-    uint32_t pos = incl - mine, total = 0;
-    for (int k = 0; k < kThreads / 64; ++k) {
-        if (k < wid) pos += wave_tot[k];
-        total += wave_tot[k];
+    uint32_t total = 0;
+#pragma unroll
+    for (int j = 0; j < kPowPer; ++j) {
+        uint32_t pos = total;
+        for (int k = 0; k < kThreads / 64; ++k) {
+            if (k < wid) pos += slot_cnt[j][k];
+            total += slot_cnt[j][k];
+        }
+        if (flags[j])
+            queue[pos + (uint32_t)__popcll(mk[j] & below)] = ((uint32_t)(j * kThreads + threadIdx.x) << 14) | flags[j];
     }
     if (total == 0) return;  // block-uniform
-#pragma unroll
-    for (int j = 0; j < kPowPer; ++j)
-        if (flags[j]) queue[pos++] = ((uint32_t)(j * kThreads + threadIdx.x) << 14) | flags[j];
     __syncthreads();
 
     // Exact recompute: lane i of a round takes queue entry i and runs the
-    // left-to-right running sum of its cell up to its largest flagged width;
-    // the loop bound is wave-uniform, so the candidates of each width are
-    // emitted with one ballot and one global atomic per wave.
-    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (uint32_t q0 = 0; q0 < total; q0 += kThreads) {
+    // left-to-right running sum of its cell up to its largest flagged width.
+    // The queue is in cell order, so the lanes of a wave read neighbouring
+    // samples (no LDS bank conflicts).  The loop bound is wave-uniform.
+    // Candidates go to a per-wave LDS stage (cell << 4 | width index, S/N
+    // bits) in the free pong buffer; a wave flushes its stage with ONE global
+    // atomic when the next ballot would not fit and once at the end, so the
+    // shared counter sees one atomic per stage-full instead of one per ballot.
+    const uint64_t lt_mask = below;
+    const uint32_t wcap = (uint32_t)span_max / 8u;  // uint2 entries per wave (>= 256 >= 64)
+    uint2* wst = reinterpret_cast<uint2*>(lb) + (size_t)wid * wcap;
+    uint32_t wn = 0;  // wave-uniform stage fill
+    auto flush = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(count, (unsigned long long)wn);
+        base = __shfl(base, 0);
+        for (uint32_t i = (uint32_t)lane; i < wn; i += 64) {
+            const uint2 v = wst[i];
+            const unsigned long long g = base + i;
+            if ((int64_t)g < cap)
+                out[g] = CandD{(int32_t)(T + (v.x >> 4)), (int32_t)r, (int32_t)(v.x & 15u), __uint_as_float(v.y)};
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        wn = 0;
+    };
+    for (uint32_t q0 = 0; q0 < total; q0 += kThreads) {  // block-uniform
         const uint32_t qi = q0 + threadIdx.x;
         const uint32_t e = qi < total ? queue[qi] : 0u;
         const int lt = (int)(e >> 14);
         const uint32_t fl = e & 0x3fffu;
         int top = fl ? 31 - __clz(fl) : -1;
         for (int off = 32; off > 0; off >>= 1) top = max(top, __shfl_xor(top, off));
-        if (top < 0) continue;  // wave-uniform
+        if (top < 0) continue;  // wave-uniform; no barrier in this loop
         // segment p adds samples [2^(p-1), 2^p) to the running sum (sample 0
         // for p = 0), then width 2^p is decided; a lane stops adding once no
         // flag at or above p is left, so it never reads past its largest
@@ -420,11 +472,12 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
             const int lo = p ? (1 << (p - 1)) : 0, hi = 1 << p;
             if (fl >> p) {
                 int iw = lo;
-                for (; iw + 4 <= hi; iw += 4) {
-                    s += xs[iw];
-                    s += xs[iw + 1];
-                    s += xs[iw + 2];
-                    s += xs[iw + 3];
+                for (; iw + 8 <= hi; iw += 8) {  // eight reads in flight, then the ordered adds
+                    float v[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) v[u] = xs[iw + u];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) s += v[u];
                 }
                 for (; iw < hi; ++iw) s += xs[iw];
             }
@@ -436,17 +489,16 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
             }
             const uint64_t m = __ballot(pred);
             if (m) {
-                const int leader = __ffsll((unsigned long long)m) - 1;
-                unsigned long long base = 0;
-                if (lane == leader) base = atomicAdd(count, (unsigned long long)__popcll(m));
-                base = __shfl(base, leader);
-                if (pred) {
-                    const unsigned long long g = base + (unsigned long long)__popcll(m & lt_mask);
-                    if ((int64_t)g < cap) out[g] = CandD{(int32_t)(T + lt), (int32_t)r, (int32_t)p, snr};
-                }
+                const uint32_t n = (uint32_t)__popcll(m);
+                if (wn + n > wcap) flush();
+                if (pred)
+                    wst[wn + (uint32_t)__popcll(m & lt_mask)] =
+                        make_uint2(((uint32_t)lt << 4) | (uint32_t)p, __float_as_uint(snr));
+                wn += n;
             }
         }
     }
+    if (wn) flush();  // wave-uniform
 }
 
 __global__ void __launch_bounds__(kThreads) k_boxcar_snr(const float* __restrict__ img, int64_t cols, int w,
