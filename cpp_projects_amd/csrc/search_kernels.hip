@@ -281,72 +281,66 @@ __device__ __forceinline__ float level_limit(float thr, int k, double e_k, float
     return __double2float_rd(lim);
 }
 
-__global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restrict__ img, int64_t cols, int64_t pitch,
-                                                          const float* __restrict__ d_thr, float h_thr, int q,
-                                                          BoxErr err, CandD* __restrict__ out, int64_t cap,
-                                                          unsigned long long* __restrict__ count) {
-    extern __shared__ float lds[];
-    __shared__ float wmaxabs[kThreads / 64];
-    const int W = 1 << q;
-    const int span_max = round_up4(kPowCells + W - 1);
-    float* x = lds;                  // level 0 (the samples)
-    float* la = lds + span_max;      // ping
-    float* lb = la + span_max;       // pong
-    const int64_t r = blockIdx.y;
-    const int64_t T = (int64_t)blockIdx.x * kPowCells;
-    const int span = (int)min((int64_t)(kPowCells + W - 1), cols - T);
-    const float* row = img + r * pitch;
+constexpr int kPowStage = 128;  // candidate stage entries (uint2) per wave
+
+// Copies samples [0, span) of a window into LDS; returns the thread's max
+// |x| (+inf once it saw a NaN, which disables the filter).  vec: 16-byte
+// loads (the window start is 16-byte aligned), the last partial vector scalar.
+__device__ __forceinline__ float pow2_load_window(float* x, const float* src, int span, bool vec) {
     float m = 0.0f;
-    if ((pitch & 3) == 0 && (reinterpret_cast<uintptr_t>(img) & 15) == 0) {
-        // 16-byte loads (T is a multiple of 4); the last partial vector scalar
+    if (vec) {
         const int n4 = span >> 2;
-        const float4* row4 = reinterpret_cast<const float4*>(row + T);
+        const float4* src4 = reinterpret_cast<const float4*>(src);
         for (int i = threadIdx.x; i < n4; i += kThreads) {
-            const float4 v = row4[i];
+            const float4 v = src4[i];
             reinterpret_cast<float4*>(x)[i] = v;
             m = fmaxf(m, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
             if (v.x != v.x || v.y != v.y || v.z != v.z || v.w != v.w) m = __int_as_float(0x7f800000);
         }
         for (int i = 4 * n4 + threadIdx.x; i < span; i += kThreads) {
-            const float v = row[T + i];
+            const float v = src[i];
             x[i] = v;
             m = fmaxf(m, fabsf(v));
             if (v != v) m = __int_as_float(0x7f800000);
         }
     } else {
         for (int i = threadIdx.x; i < span; i += kThreads) {
-            const float v = row[T + i];
+            const float v = src[i];
             x[i] = v;
             m = fmaxf(m, fabsf(v));
-            if (v != v) m = __int_as_float(0x7f800000);  // NaN: disable the filter
+            if (v != v) m = __int_as_float(0x7f800000);
         }
     }
-    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-    if ((threadIdx.x & 63) == 0) wmaxabs[threadIdx.x >> 6] = m;
-    __syncthreads();
-    float M = wmaxabs[0];
-    for (int i = 1; i < kThreads / 64; ++i) M = fmaxf(M, wmaxabs[i]);
-    const float thr = d_thr ? *d_thr : h_thr;
+    return m;
+}
 
-    // width 1 from the samples (span <= cols - T, so a window that fits the
-    // span fits the row)
-    uint32_t flags[kPowPer];
+// Width 1 from the samples, then two dyadic levels per round: from level k0
+// (cur), level k0+1 at i is cur[i] + cur[i+h] and level k0+2 is that plus
+// cur[i+2h] + cur[i+3h] (h = 2^k0), the pairwise sums of the separate
+// levels.  Only level k0+2 is stored; each round reads one buffer and writes
+// the other.  All reads of a round are issued before any is used (no
+// branches between them): indexes stay below span_max (i + 3h <= 2047 + 3W/4
+// when two levels are formed, i + h <= 2047 + W/2 otherwise) and entries at
+// or past a level's valid span are computed but never flagged or consumed.
+// flags[j] = f with f <- 2f + (level k flagged) for k = 0..q.  EDGE: the
+// window is cut by the row end, so a level is valid at i only while
+// i + 2^k <= span (span <= cols - T: a window inside the span is inside the
+// row); otherwise span = 2048 + W - 1 and every level is valid at every own
+// cell.
+template <bool EDGE>
+__device__ __forceinline__ void pow2_flag_rounds(float* x, float* la, int q, int span, float thr, const BoxErr& err,
+                                                 float M, uint32_t (&flags)[kPowPer]) {
     {
         const float lf = level_limit(thr, 0, err.e[0], M);
+        float v[kPowPer];
+#pragma unroll
+        for (int j = 0; j < kPowPer; ++j) v[j] = x[j * kThreads + threadIdx.x];
 #pragma unroll
         for (int j = 0; j < kPowPer; ++j) {
             const int lt = j * kThreads + threadIdx.x;
-            flags[j] = (lt < span && !(x[lt] <= lf)) ? 1u : 0u;
+            flags[j] = ((!EDGE || lt < span) && !(v[j] <= lf)) ? 1u : 0u;
         }
     }
-    // Two dyadic levels per round: from level k0 (cur), level k0+1 at i is
-    // cur[i] + cur[i+h] and level k0+2 is that plus cur[i+2h] + cur[i+3h]
-    // (h = 2^k0), the pairwise sums of the separate levels.  Only level
-    // k0+2 is stored; each round reads one buffer and writes the other.
-    // All reads of a round are issued before any is used (no branches between
-    // them): indexes stay below span_max (i + 3h <= 2047 + 3W/4 when two
-    // levels are formed, i + h <= 2047 + W/2 otherwise) and entries at or
-    // past a level's valid span are computed but never flagged or consumed.
     const float* cur = x;
     for (int k0 = 0; k0 < q; k0 += 2) {
         const int k1 = k0 + 1, k2 = k0 + 2;
@@ -354,7 +348,9 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
         const float l1 = level_limit(thr, k1, err.e[k1], M);
         const bool two = k2 <= q;
         const float l2 = two ? level_limit(thr, k2, err.e[k2], M) : 0.0f;
-        float* dst = ((k0 >> 1) & 1) ? lb : la;
+        // ping-pong: round 0 reads the samples and writes la, round 1 reads
+        // la and writes over the samples (read by round 0 only), and so on
+        float* dst = ((k0 >> 1) & 1) ? x : la;
         float va[kPowPer], vb[kPowPer], vc[kPowPer], vd[kPowPer];
         if (two) {
 #pragma unroll
@@ -371,8 +367,10 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
                 const float p1 = va[j] + vb[j];
                 const float p2 = p1 + (vc[j] + vd[j]);
                 dst[i] = p2;
-                if (i + 2 * h <= span && !(p1 <= l1)) flags[j] |= 1u << k1;
-                if (i + 4 * h <= span && !(p2 <= l2)) flags[j] |= 1u << k2;
+                const bool c1 = (!EDGE || i + 2 * h <= span) && !(p1 <= l1);
+                const bool c2 = (!EDGE || i + 4 * h <= span) && !(p2 <= l2);
+                flags[j] = flags[j] + flags[j] + (uint32_t)c1;
+                flags[j] = flags[j] + flags[j] + (uint32_t)c2;
             }
         } else {
 #pragma unroll
@@ -385,7 +383,8 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
             for (int j = 0; j < kPowPer; ++j) {
                 const int i = j * kThreads + threadIdx.x;
                 const float p1 = va[j] + vb[j];
-                if (i + 2 * h <= span && !(p1 <= l1)) flags[j] |= 1u << k1;
+                const bool c1 = (!EDGE || i + 2 * h <= span) && !(p1 <= l1);
+                flags[j] = flags[j] + flags[j] + (uint32_t)c1;
             }
         }
         if (two) {  // the halo: stored for the next rounds only
@@ -395,6 +394,43 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
         __syncthreads();
         cur = dst;
     }
+}
+
+__global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restrict__ img, int64_t cols, int64_t pitch,
+                                                          const float* __restrict__ d_thr, float h_thr, int q,
+                                                          BoxErr err, CandD* __restrict__ out, int64_t cap,
+                                                          unsigned long long* __restrict__ count) {
+    extern __shared__ float lds[];
+    __shared__ float wmaxabs[kThreads / 64];
+    const int W = 1 << q;
+    const int span_max = round_up4(kPowCells + W - 1);
+    float* x = lds;                  // the samples, then every odd round's level
+    float* la = lds + span_max;      // every even round's level, then the queue
+    uint2* stage = reinterpret_cast<uint2*>(la + span_max);  // kPowStage per wave
+    const int64_t r = blockIdx.y;
+    const int64_t T = (int64_t)blockIdx.x * kPowCells;
+    const int span = (int)min((int64_t)(kPowCells + W - 1), cols - T);
+    const float* row = img + r * pitch;
+    const bool vec = (pitch & 3) == 0 && (reinterpret_cast<uintptr_t>(img) & 15) == 0;
+    float m = pow2_load_window(x, row + T, span, vec);
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) wmaxabs[threadIdx.x >> 6] = m;
+    __syncthreads();
+    float M = wmaxabs[0];
+    for (int i = 1; i < kThreads / 64; ++i) M = fmaxf(M, wmaxabs[i]);
+    const float thr = d_thr ? *d_thr : h_thr;
+
+    // Flag phase.  Flags are shifted in one bit per level, so level k ends at
+    // bit q - k; reversing the word puts it at bit k.  Only the last tile of
+    // a row has a window cut by the row end (span < 2048 + W - 1) and checks
+    // each level's valid span.
+    uint32_t flags[kPowPer];
+    if (span < kPowCells + W - 1)
+        pow2_flag_rounds<true>(x, la, q, span, thr, err, M, flags);
+    else
+        pow2_flag_rounds<false>(x, la, q, span, thr, err, M, flags);
+#pragma unroll
+    for (int j = 0; j < kPowPer; ++j) flags[j] = __brev(flags[j]) >> (31 - q);
     // Compact the flagged cells into an LDS queue (cell << 14 | width flags;
     // the level buffers are free now) in cell order: slot j of every thread
     // holds cell j*256 + tid, so entries are ranked by (j, wave, lane) with
@@ -424,6 +460,9 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
             queue[pos + (uint32_t)__popcll(mk[j] & below)] = ((uint32_t)(j * kThreads + threadIdx.x) << 14) | flags[j];
     }
     if (total == 0) return;  // block-uniform
+    // from q = 4 on the second round stored its level over the samples:
+    // reload them (the block's window was just read, so this mostly hits L2)
+    if (q >= 4) pow2_load_window(x, row + T, span, vec);
     __syncthreads();
 
     // Exact recompute: lane i of a round takes queue entry i and runs the
@@ -431,12 +470,12 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
     // The queue is in cell order, so the lanes of a wave read neighbouring
     // samples (no LDS bank conflicts).  The loop bound is wave-uniform.
     // Candidates go to a per-wave LDS stage (cell << 4 | width index, S/N
-    // bits) in the free pong buffer; a wave flushes its stage with ONE global
-    // atomic when the next ballot would not fit and once at the end, so the
-    // shared counter sees one atomic per stage-full instead of one per ballot.
+    // bits); a wave flushes its stage with ONE global atomic when the next
+    // ballot would not fit and once at the end, so the shared counter sees
+    // one atomic per stage-full instead of one per ballot.
     const uint64_t lt_mask = below;
-    const uint32_t wcap = (uint32_t)span_max / 8u;  // uint2 entries per wave (>= 256 >= 64)
-    uint2* wst = reinterpret_cast<uint2*>(lb) + (size_t)wid * wcap;
+    const uint32_t wcap = kPowStage;
+    uint2* wst = stage + (size_t)wid * wcap;
     uint32_t wn = 0;  // wave-uniform stage fill
     auto flush = [&]() {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -655,7 +694,8 @@ void launch_boxcar(const float* img, int64_t rows, int64_t cols, int64_t pitch, 
         while ((1 << q) < wmax) ++q;
         if ((1 << q) != wmax || q > 13) throw Error(SPS_ERR_INVALID_ARGUMENT, "pow2 window must be 2^q, q <= 13");
         const dim3 grid((uint32_t)ceil_div(cols, kPowCells), (uint32_t)rows);
-        const size_t lds = sizeof(float) * 3 * (size_t)round_up4(kPowCells + wmax - 1);
+        const size_t lds = sizeof(float) * 2 * (size_t)round_up4(kPowCells + wmax - 1) +
+                           sizeof(uint2) * (size_t)kPowStage * (kThreads / 64);
         if (lds > 150 * 1024) throw Error(SPS_ERR_INVALID_ARGUMENT, "boxcar window too wide for LDS");
         // error terms of the filter: (gamma_{w-1} + gamma_k) * w * (1 + 2^-20)
         BoxErr err{};

@@ -73,6 +73,18 @@ def test_nonpositive_threshold_and_nan(torch_cuda):
     check(torch_cuda, img, 0.5, 5)
 
 
+def test_every_depth_interior_and_row_end(torch_cuda):
+    # rows of 3 full tiles plus a cut one, a broad bump (dense candidates:
+    # many recompute chains and stage flushes) and sparse spikes
+    rng = np.random.default_rng(5)
+    cols = 3 * 2048 + 777
+    img = rng.standard_normal((2, cols)).astype(np.float32)
+    img[0, 1500:2600] += 1.5
+    img[1, 5::301] += 9.0
+    for q in range(14):
+        assert check(torch_cuda, img, 3.0, q) > 0
+
+
 def test_many_candidates_overflow_stage(torch_cuda):
     img = np.full((4, 10000), 5.0, np.float32)
     assert check(torch_cuda, img, 1.0, 4) > 4 * 10000 * 4
