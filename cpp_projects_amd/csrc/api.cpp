@@ -77,6 +77,7 @@ struct sps_fdmt {
     DevBuf state_a, state_b;
     int64_t rows_a = 0, rows_b = 0;    // capacity of the ping-pong states in rows
     int dmax = 0;                      // largest init dt of any channel
+    int grid_consec = 0;               // every channel's init dts are 0, 1, ..., K-1 with K <= 5
     // adjoint tables
     bool adj_built = false;
     std::vector<std::unique_ptr<DevBuf>> csr_off, csr_row, csr_shift;
@@ -107,6 +108,13 @@ struct sps_fdmt {
     void init_layout() {
         pitch = round_up(eng.nsamps, 4);
         dmax = eng.max_init_dt();
+        grid_consec = 1;
+        for (int64_t c = 0; c < eng.nchans && grid_consec; ++c) {
+            const int64_t g0 = eng.grid_off[(size_t)c], K = eng.grid_off[(size_t)c + 1] - g0;
+            if (K < 1 || K > 5) grid_consec = 0;
+            for (int64_t k = 0; k < K && grid_consec; ++k)
+                if (eng.grid[(size_t)(g0 + k)] != k) grid_consec = 0;
+        }
         // state 0 in A; pass outputs alternate B, A, B, ... and rows only
         // shrink with the iteration, so A needs rows[0] and B rows[1]
         rows_a = eng.rows[0];
@@ -510,7 +518,8 @@ struct sps_fdmt {
         const ChanNorm* nm = norm;
         if (eng.niters == 0) {
             launch_fdmt_init(eng.init_mode, in, in_u8, ones, in_pitch, nm, eng.nchans, nsamps, d_grid.as<int32_t>(),
-                             d_grid_off.as<int64_t>(), d_chan_row_off.as<int64_t>(), out, out_pitch, st, dmax);
+                             d_grid_off.as<int64_t>(), d_chan_row_off.as<int64_t>(), out, out_pitch, st, dmax,
+                             nullptr, grid_consec);
             if (ev) SPS_HIP(hipEventRecord(ev[0], st));
             return;
         }
@@ -520,7 +529,7 @@ struct sps_fdmt {
         if (!fused0)
             launch_fdmt_init(eng.init_mode, in, in_u8, ones, in_pitch, nm, eng.nchans, nsamps, d_grid.as<int32_t>(),
                              d_grid_off.as<int64_t>(), d_chan_row_off.as<int64_t>(), sa, p, st, dmax,
-                             d_live0.as<uint8_t>());
+                             d_live0.as<uint8_t>(), grid_consec);
         if (ev) SPS_HIP(hipEventRecord(ev[0], st));
         if (aux) {
             // the mask transform stays on single-iteration launches (a short,
@@ -794,7 +803,7 @@ SPS_API sps_status sps_fdmt_initialise(sps_fdmt* h, const void* d_wf, int dtype,
         h->ensure_device(st);
         launch_fdmt_init(e.init_mode, d_wf, dtype == SPS_U8, 0, e.nsamps, nullptr, e.nchans, e.nsamps,
                          h->d_grid.as<int32_t>(), h->d_grid_off.as<int64_t>(), h->d_chan_row_off.as<int64_t>(), d_state,
-                         e.nsamps, st, h->dmax);
+                         e.nsamps, st, h->dmax, nullptr, h->grid_consec);
     });
 }
 

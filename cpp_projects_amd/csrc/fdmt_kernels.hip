@@ -588,6 +588,102 @@ __global__ void __launch_bounds__(kThreads) k_init_lds(const void* __restrict__ 
     }
 }
 
+// Mean-init fast path for grids whose init dts are 0, 1, ..., K-1 on every
+// channel (K <= kInitConsecMax): row 0 is (0 + x[t]) / 1 = 0 + x[t] and row k
+// adds the single sample x[t - k]: v_k = (v_{k-1} * k + (0 + x[t - k])) / (k + 1),
+// the operations of k_init_lds for such a grid, in the same order.  One
+// block = (channel, 4096-sample tile); a thread owns 4 chunks of 4
+// consecutive samples (chunk j * 256 + tid): one 4-byte (u8) or 16-byte
+// (f32) load per chunk, normalised once in registers; the 4 samples before a
+// chunk (x[t - k], k <= 4) come through one aligned 16-byte LDS read, and
+// each row is written as float4, so every store instruction covers 1 KB.
+constexpr int kInitConsecMax = 5;
+
+template <int IN_KIND, bool NORM>
+__global__ void __launch_bounds__(kThreads) k_init_consec(const void* __restrict__ in, int64_t in_pitch, ChanNorm norm,
+                                                          int64_t nsamps, const int64_t* __restrict__ grid_off,
+                                                          const int64_t* __restrict__ chan_row_off,
+                                                          float* __restrict__ state, int64_t pitch,
+                                                          const uint8_t* __restrict__ live) {
+    static_assert(IN_KIND == 0 || IN_KIND == 1, "f32 or u8 input");
+    __shared__ float4a xs[1 + kInitTile / 4];  // xs[0]: samples T-4 .. T-1; xs[1 + q]: chunk q
+    const int64_t c = blockIdx.y;
+    const int64_t T = (int64_t)blockIdx.x * kInitTile;
+    const int K = (int)(grid_off[c + 1] - grid_off[c]);
+    const uint8_t* lv = live ? live + chan_row_off[c] : nullptr;
+    float mu = 0.0f, sg = 1.0f, rsg = 1.0f;
+    bool keep = true;
+    if (NORM) {
+        keep = norm.keep[c] != 0;
+        mu = norm.mu[c];
+        sg = norm.sigma[c];
+        rsg = 1.0f / sg;
+    }
+    auto nrm = [&](float v) -> float { return NORM ? (keep ? div_rn(v - mu, sg, rsg) : 0.0f) : v; };
+    const int64_t xb = c * in_pitch;
+    float4a own[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int q = j * kThreads + (int)threadIdx.x;
+        const int64_t t0 = T + 4 * (int64_t)q;
+        float4a v = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (t0 < nsamps) {  // nsamps % 4 == 0: the chunk is whole
+            if (IN_KIND == 1) {
+                const uint32_t w = *reinterpret_cast<const uint32_t*>(static_cast<const uint8_t*>(in) + xb + t0);
+                v.x = (float)(w & 0xffu);
+                v.y = (float)((w >> 8) & 0xffu);
+                v.z = (float)((w >> 16) & 0xffu);
+                v.w = (float)(w >> 24);
+            } else {
+                v = *reinterpret_cast<const float4a*>(static_cast<const float*>(in) + xb + t0);
+            }
+            v.x = nrm(v.x);
+            v.y = nrm(v.y);
+            v.z = nrm(v.z);
+            v.w = nrm(v.w);
+        }
+        own[j] = v;
+        xs[1 + q] = v;
+    }
+    if (threadIdx.x < 4) {
+        const int64_t t = T - 4 + (int64_t)threadIdx.x;
+        reinterpret_cast<float*>(xs)[threadIdx.x] = t >= 0 ? nrm(load_in<IN_KIND>(in, xb + t)) : 0.0f;
+    }
+    __syncthreads();
+    float* __restrict__ rowbase = state + chan_row_off[c] * pitch;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int q = j * kThreads + (int)threadIdx.x;
+        const int64_t t0 = T + 4 * (int64_t)q;
+        if (t0 >= nsamps) continue;
+        const float4a h = xs[q];  // samples t0 - 4 .. t0 - 1
+        const float w[8] = {h.x, h.y, h.z, h.w, own[j].x, own[j].y, own[j].z, own[j].w};  // w[4 + e - k] = x[t0 + e - k]
+        float prev[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) prev[e] = 0.0f + w[4 + e];
+        if (!lv || lv[0]) {
+            float4a o = {prev[0], prev[1], prev[2], prev[3]};
+            *reinterpret_cast<float4a*>(rowbase + t0) = o;
+        }
+#pragma unroll
+        for (int k = 1; k < kInitConsecMax; ++k) {
+            if (k >= K) break;  // block-uniform
+            const float yk = (float)k + 1.0f, rk = 1.0f / yk;
+            const float mk = (float)(k - 1) + 1.0f;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                float v = 0.0f;
+                if (t0 + e >= k) v = div_rn(prev[e] * mk + (0.0f + w[4 + e - k]), yk, rk);
+                prev[e] = v;
+            }
+            if (!lv || lv[k]) {
+                float4a o = {prev[0], prev[1], prev[2], prev[3]};
+                *reinterpret_cast<float4a*>(rowbase + (int64_t)k * pitch + t0) = o;
+            }
+        }
+    }
+}
+
 // First staged pass with the mean init fused in: every init row a group's
 // trees read is built in LDS straight from the waterfall, with the operations
 // of k_init in the same order (normalised samples (x - mu) / sigma, running
@@ -937,9 +1033,24 @@ void launch_fdmt_group_init(int L, const void* in, int in_u8, int64_t in_pitch, 
 template <int MODE, int IN_KIND>
 static void init_dispatch_norm(const void* in, int64_t in_pitch, const ChanNorm* norm, int64_t nchans,
                                int64_t nsamps, const int32_t* g, const int64_t* go, const int64_t* cro, float* state,
-                               int64_t pitch, int dmax_all, const uint8_t* live, hipStream_t st) {
+                               int64_t pitch, int dmax_all, const uint8_t* live, int consec, hipStream_t st) {
     ChanNorm n{};
     if (norm) n = *norm;
+    if constexpr (MODE == kInitMeanD && IN_KIND != 2) {
+        const uintptr_t ib = reinterpret_cast<uintptr_t>(in);
+        const bool al_in = IN_KIND == 1 ? (ib % 4 == 0 && in_pitch % 4 == 0) : (ib % 16 == 0 && in_pitch % 4 == 0);
+        const bool al_out = (reinterpret_cast<uintptr_t>(state) % 16 == 0) && (pitch % 4 == 0);
+        if (consec && al_in && al_out && nsamps % 4 == 0) {
+            const dim3 grid((uint32_t)ceil_div(nsamps, kInitTile), (uint32_t)nchans);
+            if (norm)
+                hipLaunchKernelGGL((k_init_consec<IN_KIND, true>), grid, dim3(kThreads), 0, st, in, in_pitch, n,
+                                   nsamps, go, cro, state, pitch, live);
+            else
+                hipLaunchKernelGGL((k_init_consec<IN_KIND, false>), grid, dim3(kThreads), 0, st, in, in_pitch, n,
+                                   nsamps, go, cro, state, pitch, live);
+            return;
+        }
+    }
     if (dmax_all >= 0 && dmax_all <= 16384) {
         const uintptr_t ib = reinterpret_cast<uintptr_t>(in);
         int al_in = 0;
@@ -971,33 +1082,36 @@ template <int MODE>
 static void init_dispatch_kind(const void* in, int in_u8, int ones, int64_t in_pitch, const ChanNorm* norm,
                                int64_t nchans, int64_t nsamps, const int32_t* g, const int64_t* go,
                                const int64_t* cro, float* state, int64_t pitch, int dmax_all, const uint8_t* live,
-                               hipStream_t st) {
+                               int consec, hipStream_t st) {
     if (ones)
-        init_dispatch_norm<MODE, 2>(in, in_pitch, norm, nchans, nsamps, g, go, cro, state, pitch, dmax_all, live, st);
+        init_dispatch_norm<MODE, 2>(in, in_pitch, norm, nchans, nsamps, g, go, cro, state, pitch, dmax_all, live,
+                                    consec, st);
     else if (in_u8)
-        init_dispatch_norm<MODE, 1>(in, in_pitch, norm, nchans, nsamps, g, go, cro, state, pitch, dmax_all, live, st);
+        init_dispatch_norm<MODE, 1>(in, in_pitch, norm, nchans, nsamps, g, go, cro, state, pitch, dmax_all, live,
+                                    consec, st);
     else
-        init_dispatch_norm<MODE, 0>(in, in_pitch, norm, nchans, nsamps, g, go, cro, state, pitch, dmax_all, live, st);
+        init_dispatch_norm<MODE, 0>(in, in_pitch, norm, nchans, nsamps, g, go, cro, state, pitch, dmax_all, live,
+                                    consec, st);
 }
 
 void launch_fdmt_init(int mode, const void* in, int in_u8, int ones, int64_t in_pitch, const ChanNorm* norm,
                       int64_t nchans, int64_t nsamps, const int32_t* d_grid, const int64_t* d_grid_off,
                       const int64_t* d_chan_row_off, float* state, int64_t pitch, hipStream_t st, int dmax_all,
-                      const uint8_t* d_live) {
+                      const uint8_t* d_live, int grid_consec) {
     if (nchans <= 0 || nsamps <= 0) return;
     if (nchans > 65535) throw Error(SPS_ERR_INVALID_ARGUMENT, "more than 65535 channels");
     switch (mode) {
         case kInitMeanD:
             init_dispatch_kind<kInitMeanD>(in, in_u8, ones, in_pitch, norm, nchans, nsamps, d_grid, d_grid_off,
-                                           d_chan_row_off, state, pitch, dmax_all, d_live, st);
+                                           d_chan_row_off, state, pitch, dmax_all, d_live, grid_consec, st);
             break;
         case kInitRestrictD:
             init_dispatch_kind<kInitRestrictD>(in, in_u8, ones, in_pitch, norm, nchans, nsamps, d_grid, d_grid_off,
-                                               d_chan_row_off, state, pitch, dmax_all, d_live, st);
+                                               d_chan_row_off, state, pitch, dmax_all, d_live, grid_consec, st);
             break;
         case kInitCFdmtD:
             init_dispatch_kind<kInitCFdmtD>(in, in_u8, ones, in_pitch, norm, nchans, nsamps, d_grid, d_grid_off,
-                                            d_chan_row_off, state, pitch, dmax_all, d_live, st);
+                                            d_chan_row_off, state, pitch, dmax_all, d_live, grid_consec, st);
             break;
         default:
             throw Error(SPS_ERR_INVALID_ARGUMENT, "unknown init mode");

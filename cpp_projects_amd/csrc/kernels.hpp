@@ -33,11 +33,13 @@ struct ChanNorm {
 //   in: nchans x nsamps (f32 or u8; ignored when ones), in_pitch elements
 //   norm: nullptr, or per-channel normalisation
 //   d_live: nullptr, or per state-0 row flags; rows flagged 0 are not stored
+//   grid_consec: every channel's init dts are 0, 1, ..., K-1 with K <= 5
+//     (enables the register fast path of the mean init)
 void launch_fdmt_init(int mode, const void* in, int in_u8, int ones, int64_t in_pitch, const ChanNorm* norm,
                       int64_t nchans, int64_t nsamps, const int32_t* d_grid, const int64_t* d_grid_off,
                       const int64_t* d_chan_row_off, float* state, int64_t pitch, hipStream_t st,
                       int dmax_all = -1,  // largest init dt of any channel; < 0 selects the unstaged kernel
-                      const uint8_t* d_live = nullptr);
+                      const uint8_t* d_live = nullptr, int grid_consec = 0);
 
 // One output row of a pass that fuses L iterations: the merge tree in heap
 // order (node 1 = the row; node k has tail child 2k and head child 2k+1 with
