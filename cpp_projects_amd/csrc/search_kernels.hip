@@ -222,6 +222,16 @@ __device__ __forceinline__ void stage_flush(CandStage& st, CandD* __restrict__ o
 // All widths 1..wmax: the running left-to-right sum is exact for every width.
 constexpr int kBoxCells = 1024;  // cells per block (4 per thread)
 
+// Time tile of a (tile, row) grid, rotated by the row.  The hardware deals
+// consecutive block ids round-robin to the 8 XCDs, so with a multiple of 8
+// tiles per row, tile t of every row would run on XCD t % 8; candidate-dense
+// blocks follow tracks across rows (a pulse's arrival drifts slowly with DM)
+// and would pile onto one XCD while the others idle.  Rotating by the row
+// (a bijection within each row) spreads such a track over all XCDs.
+__device__ __forceinline__ int64_t rotated_tile() {
+    return ((int64_t)blockIdx.x + (int64_t)blockIdx.y) % (int64_t)gridDim.x;
+}
+
 __global__ void __launch_bounds__(kThreads) k_boxcar_all(const float* __restrict__ img, int64_t cols, int64_t pitch,
                                                          const float* __restrict__ d_thr, float h_thr, int wmax,
                                                          CandD* __restrict__ out, int64_t cap,
@@ -230,7 +240,7 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_all(const float* __restrict
     __shared__ CandStage st;
     if (threadIdx.x == 0) st.n = 0;
     const int64_t r = blockIdx.y;
-    const int64_t T = (int64_t)blockIdx.x * kBoxCells;
+    const int64_t T = rotated_tile() * kBoxCells;
     const int64_t span = min((int64_t)kBoxCells + wmax - 1, cols - T);
     const float* row = img + r * pitch;
     for (int64_t i = threadIdx.x; i < span; i += kThreads) tile[i] = row[T + i];
@@ -408,7 +418,7 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
     float* la = lds + span_max;      // every even round's level, then the queue
     uint2* stage = reinterpret_cast<uint2*>(la + span_max);  // kPowStage per wave
     const int64_t r = blockIdx.y;
-    const int64_t T = (int64_t)blockIdx.x * kPowCells;
+    const int64_t T = rotated_tile() * kPowCells;
     const int span = (int)min((int64_t)(kPowCells + W - 1), cols - T);
     const float* row = img + r * pitch;
     const bool vec = (pitch & 3) == 0 && (reinterpret_cast<uintptr_t>(img) & 15) == 0;
@@ -470,9 +480,10 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
     // The queue is in cell order, so the lanes of a wave read neighbouring
     // samples (no LDS bank conflicts).  The loop bound is wave-uniform.
     // Candidates go to a per-wave LDS stage (cell << 4 | width index, S/N
-    // bits); a wave flushes its stage with ONE global atomic when the next
-    // ballot would not fit and once at the end, so the shared counter sees
-    // one atomic per stage-full instead of one per ballot.
+    // bits).  A wave whose stage would overflow flushes it with one global
+    // atomic; at the end the block adds its four stage fills and takes ONE
+    // atomic for all of them (device-scope atomics on the shared counter
+    // serialise, so their number, not the candidate count, sets the cost).
     const uint64_t lt_mask = below;
     const uint32_t wcap = kPowStage;
     uint2* wst = stage + (size_t)wid * wcap;
@@ -500,30 +511,20 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
         const uint32_t fl = e & 0x3fffu;
         int top = fl ? 31 - __clz(fl) : -1;
         for (int off = 32; off > 0; off >>= 1) top = max(top, __shfl_xor(top, off));
+        top = __builtin_amdgcn_readfirstlane(top);  // equal on every lane: a scalar loop bound
         if (top < 0) continue;  // wave-uniform; no barrier in this loop
-        // segment p adds samples [2^(p-1), 2^p) to the running sum (sample 0
-        // for p = 0), then width 2^p is decided; a lane stops adding once no
-        // flag at or above p is left, so it never reads past its largest
-        // flagged window
+        // The running sum s over samples lt, lt+1, ... is taken to the
+        // wave's largest flagged width 2^top by every lane (uniform control
+        // flow), and width 2^p is decided when s covers 2^p samples.  A lane
+        // reads at most to lt + 2^top - 1 <= 2047 + W - 1 < span_max; samples
+        // past its own flagged windows only enter widths it does not decide.
         float s = 0.0f;
         const float* xs = x + lt;
-        for (int p = 0; p <= top; ++p) {
-            const int lo = p ? (1 << (p - 1)) : 0, hi = 1 << p;
-            if (fl >> p) {
-                int iw = lo;
-                for (; iw + 8 <= hi; iw += 8) {  // eight reads in flight, then the ordered adds
-                    float v[8];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) v[u] = xs[iw + u];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) s += v[u];
-                }
-                for (; iw < hi; ++iw) s += xs[iw];
-            }
+        auto decide = [&](int p) {
             float snr = 0.0f;
             bool pred = false;
             if ((fl >> p) & 1u) {
-                snr = s / sqrtf((float)hi);
+                snr = s / sqrtf((float)(1 << p));
                 pred = snr > thr;
             }
             const uint64_t m = __ballot(pred);
@@ -535,9 +536,76 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
                         make_uint2(((uint32_t)lt << 4) | (uint32_t)p, __float_as_uint(snr));
                 wn += n;
             }
+        };
+        // widths 1 .. 16 (p <= 4): samples 0 .. 15
+        {
+            float v[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) v[u] = (u < (1 << top)) ? xs[u] : 0.0f;
+            s += v[0];
+            decide(0);
+            if (top >= 1) {
+                s += v[1];
+                decide(1);
+            }
+            if (top >= 2) {
+                s += v[2];
+                s += v[3];
+                decide(2);
+            }
+            if (top >= 3) {
+#pragma unroll
+                for (int u = 4; u < 8; ++u) s += v[u];
+                decide(3);
+            }
+            if (top >= 4) {
+#pragma unroll
+                for (int u = 8; u < 16; ++u) s += v[u];
+                decide(4);
+            }
+        }
+        // widths 32 .. 2^top: blocks of 16 samples, the next block's reads
+        // issued before the current block is added
+        if (top >= 5) {
+            const int end = 1 << top;
+            float nx[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) nx[u] = xs[16 + u];
+            for (int b = 16; b < end; b += 16) {  // wave-uniform
+                float cur[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) cur[u] = nx[u];
+                if (b + 16 < end) {
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) nx[u] = xs[b + 16 + u];
+                }
+#pragma unroll
+                for (int u = 0; u < 16; ++u) s += cur[u];
+                const int w = b + 16;
+                if ((w & (w - 1)) == 0) decide(31 - __clz(w));
+            }
         }
     }
-    if (wn) flush();  // wave-uniform
+    // block-level final flush (the recompute loop is block-uniform, so every
+    // thread reaches the barriers)
+    __shared__ uint32_t wfill[kThreads / 64];
+    __shared__ unsigned long long bbase;
+    if (lane == 0) wfill[wid] = wn;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int k = 0; k < kThreads / 64; ++k) tot += wfill[k];
+        bbase = tot ? atomicAdd(count, (unsigned long long)tot) : 0ull;
+    }
+    __syncthreads();
+    unsigned long long base = bbase;
+    for (int k = 0; k < wid; ++k) base += wfill[k];
+    for (uint32_t i = (uint32_t)lane; i < wn; i += 64) {
+        const uint2 v = wst[i];
+        const unsigned long long g = base + i;
+        if ((int64_t)g < cap)
+            out[g] = CandD{(int32_t)(T + (v.x >> 4)), (int32_t)r, (int32_t)(v.x & 15u), __uint_as_float(v.y)};
+    }
 }
 
 __global__ void __launch_bounds__(kThreads) k_boxcar_snr(const float* __restrict__ img, int64_t cols, int w,
