@@ -375,7 +375,7 @@ struct sps_fdmt {
                 int64_t ms = 0;
                 for (int j = 0; j < NL; ++j) {
                     const int64_t lr = leaf[(size_t)rr * NL + j];
-                    d.lbase[j] = lr >= 0 ? base[lr] : -1;
+                    d.lbase[j] = lr >= 0 ? base[lr] : 0;  // under a missing head: read, never added
                     if (lr >= 0) ms = std::max(ms, shift[(size_t)rr * NL + j]);
                 }
                 SPS_REQUIRE(ms <= INT32_MAX, "shift too large");

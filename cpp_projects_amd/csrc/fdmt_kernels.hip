@@ -253,17 +253,21 @@ struct LdsTree {
     }
 };
 
-// the same tree when every merge adds its head (t >= maxshift): u = t - T
+// the same tree when every merge adds its head (t >= maxshift): u = t - T.
+// Branch-free, so all 2^L leaf reads can be in flight at once: a missing
+// head (a copied row, off < 0) is still read, at offset 0 from leaves whose
+// base is 0, and dropped by the select; the kept additions are those of the
+// separate iterations.
 template <int L, int K>
 struct LdsTreeFast {
     __device__ __forceinline__ static float eval(const float* __restrict__ lds, const GRowDesc<L>& d, int u) {
         if constexpr (K >= (1 << L)) {
             return lds[d.lbase[K - (1 << L)] + u];
         } else {
-            float v = LdsTreeFast<L, 2 * K>::eval(lds, d, u);
             const int32_t off = d.off[K];
-            if (off >= 0) v = v + LdsTreeFast<L, 2 * K + 1>::eval(lds, d, u - off);
-            return v;
+            const float a = LdsTreeFast<L, 2 * K>::eval(lds, d, u);
+            const float b = LdsTreeFast<L, 2 * K + 1>::eval(lds, d, u - (off >= 0 ? off : 0));
+            return off >= 0 ? a + b : a;
         }
     }
 };

@@ -78,7 +78,8 @@ struct LeafDescD {
     int32_t pad;
 };
 // tree of one output row with leaves as LDS bases: the leaf read at time u
-// is lds[lbase + u - T] (lbase = lds_off + smax4), -1 for a missing leaf
+// is lds[lbase + u - T] (lbase = lds_off + smax4); a leaf under a missing
+// head has lbase 0 (a valid address whose value is never added)
 template <int L>
 struct GRowDesc {
     int64_t out_row;
