@@ -328,7 +328,7 @@ __global__ void __launch_bounds__(kThreads) k_merge_group(const float* __restric
                                                           const uint8_t* __restrict__ unit_leaf, int tiles_per_block,
                                                           int lds_stride, int64_t nsamps, int64_t out_rows_limit,
                                                           SnrEpi epi) {
-    extern __shared__ float lds[];
+    extern __shared__ __attribute__((aligned(16))) float lds[];
     const uint32_t logical = xcd_remap(blockIdx.x, gridDim.x);
     const int64_t gi = (int64_t)(logical % (uint64_t)ngroups);
     const int64_t chunk = (int64_t)(logical / (uint64_t)ngroups);
@@ -512,7 +512,7 @@ __global__ void __launch_bounds__(kThreads) k_init_lds(const void* __restrict__ 
                                                        float* __restrict__ state, int64_t pitch, int dmax_all,
                                                        int aligned_in, int aligned_out,
                                                        const uint8_t* __restrict__ live) {
-    extern __shared__ float xs[];
+    extern __shared__ __attribute__((aligned(16))) float xs[];
     const int64_t c = blockIdx.y;
     const int64_t T = (int64_t)blockIdx.x * kInitTile;
     const int64_t g0 = grid_off[c];
@@ -699,7 +699,7 @@ __global__ void __launch_bounds__(kThreads) k_merge_group_init(const void* __res
                                                                const GroupDescD* __restrict__ groups, int64_t ngroups,
                                                                const ChanWinD* __restrict__ chans,
                                                                const GRowDesc<L>* __restrict__ rows, int64_t nsamps) {
-    extern __shared__ float lds[];
+    extern __shared__ __attribute__((aligned(16))) float lds[];
     const uint32_t logical = xcd_remap(blockIdx.x, gridDim.x);
     const int64_t gi = (int64_t)(logical % (uint64_t)ngroups);
     const int64_t T = (int64_t)(logical / (uint64_t)ngroups) * kGroupTile;

@@ -236,7 +236,7 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_all(const float* __restrict
                                                          const float* __restrict__ d_thr, float h_thr, int wmax,
                                                          CandD* __restrict__ out, int64_t cap,
                                                          unsigned long long* __restrict__ count) {
-    extern __shared__ float tile[];
+    extern __shared__ __attribute__((aligned(16))) float tile[];
     __shared__ CandStage st;
     if (threadIdx.x == 0) st.n = 0;
     const int64_t r = blockIdx.y;
@@ -406,11 +406,99 @@ __device__ __forceinline__ void pow2_flag_rounds(float* x, float* la, int q, int
     }
 }
 
+__device__ __forceinline__ float max4(const float4& v) { return fmaxf(fmaxf(v.x, v.y), fmaxf(v.z, v.w)); }
+
+// The same rounds in the chunk layout, for interior windows (span = 2048 +
+// W - 1, so every level is valid at every own cell) whose samples are finite
+// with W * M < 2^125 (no level sum is NaN or overflows).  Thread t owns chunks
+// t and t + 256 of four consecutive cells; every LDS access is 16 bytes.  A
+// level is flagged for a chunk when the largest of its four sums exceeds the
+// limit, and the chunk's four cells take those flags (a superset of the
+// per-cell flags: the recompute decides each cell exactly).  The level sums
+// are those of pow2_flag_rounds, in the same order.  Reads stay below
+// span_max: chunk 511 + 3h/4 (cell 2047 + 3h <= 2047 + 3W/4) when two levels
+// are formed, 511 + h/4 otherwise, and chunk 512 (cells 2048..2051 <
+// span_max for W >= 2) in the first round.
+__device__ __forceinline__ void pow2_flag_chunks(float* x, float* la, int q, float thr, const BoxErr& err, float M,
+                                                 uint32_t (&cf)[2]) {
+    const int span = kPowCells + (1 << q) - 1;
+    {
+        const float lf = level_limit(thr, 0, err.e[0], M);
+        const float4* x4 = reinterpret_cast<const float4*>(x);
+#pragma unroll
+        for (int s = 0; s < 2; ++s) cf[s] = !(max4(x4[s * kThreads + threadIdx.x]) <= lf) ? 1u : 0u;
+    }
+    const float* cur = x;
+    for (int k0 = 0; k0 < q; k0 += 2) {
+        const int k1 = k0 + 1, k2 = k0 + 2;
+        const int h = 1 << k0;
+        const float l1 = level_limit(thr, k1, err.e[k1], M);
+        const bool two = k2 <= q;
+        const float l2 = two ? level_limit(thr, k2, err.e[k2], M) : 0.0f;
+        float* dst = ((k0 >> 1) & 1) ? x : la;
+        const float4* c4 = reinterpret_cast<const float4*>(cur);
+        float4* d4 = reinterpret_cast<float4*>(dst);
+        float4 p1[2], p2[2];
+        if (k0 == 0) {
+            // h = 1: cells 4c .. 4c+7 cover both levels of chunk c
+            float4 w0[2], w1[2];
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int c = s * kThreads + threadIdx.x;
+                w0[s] = c4[c];
+                w1[s] = c4[c + 1];
+            }
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const float4 a = w0[s], b = w1[s];
+                p1[s] = make_float4(a.x + a.y, a.y + a.z, a.z + a.w, a.w + b.x);
+                p2[s] = make_float4(p1[s].x + (a.z + a.w), p1[s].y + (a.w + b.x), p1[s].z + (b.x + b.y),
+                                    p1[s].w + (b.y + b.z));
+            }
+        } else {
+            // h a multiple of 4: the four terms are whole chunks h/4 apart
+            const int hq = h >> 2;
+            float4 A[2], B[2], C[2], D[2];
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int c = s * kThreads + threadIdx.x;
+                A[s] = c4[c];
+                B[s] = c4[c + hq];
+                if (two) {
+                    C[s] = c4[c + 2 * hq];
+                    D[s] = c4[c + 3 * hq];
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                p1[s] = make_float4(A[s].x + B[s].x, A[s].y + B[s].y, A[s].z + B[s].z, A[s].w + B[s].w);
+                if (two)
+                    p2[s] = make_float4(p1[s].x + (C[s].x + D[s].x), p1[s].y + (C[s].y + D[s].y),
+                                        p1[s].z + (C[s].z + D[s].z), p1[s].w + (C[s].w + D[s].w));
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            cf[s] = cf[s] + cf[s] + (!(max4(p1[s]) <= l1) ? 1u : 0u);
+            if (two) {
+                d4[s * kThreads + threadIdx.x] = p2[s];
+                cf[s] = cf[s] + cf[s] + (!(max4(p2[s]) <= l2) ? 1u : 0u);
+            }
+        }
+        if (two) {  // the halo: stored for the next rounds only
+            for (int i = kPowCells + threadIdx.x; i + 4 * h <= span; i += kThreads)
+                dst[i] = (cur[i] + cur[i + h]) + (cur[i + 2 * h] + cur[i + 3 * h]);
+        }
+        __syncthreads();
+        cur = dst;
+    }
+}
+
 __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restrict__ img, int64_t cols, int64_t pitch,
                                                           const float* __restrict__ d_thr, float h_thr, int q,
                                                           BoxErr err, CandD* __restrict__ out, int64_t cap,
                                                           unsigned long long* __restrict__ count) {
-    extern __shared__ float lds[];
+    extern __shared__ __attribute__((aligned(16))) float lds[];
     __shared__ float wmaxabs[kThreads / 64];
     const int W = 1 << q;
     const int span_max = round_up4(kPowCells + W - 1);
@@ -431,43 +519,76 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
     const float thr = d_thr ? *d_thr : h_thr;
 
     // Flag phase.  Flags are shifted in one bit per level, so level k ends at
-    // bit q - k; reversing the word puts it at bit k.  Only the last tile of
-    // a row has a window cut by the row end (span < 2048 + W - 1) and checks
-    // each level's valid span.
-    uint32_t flags[kPowPer];
-    if (span < kPowCells + W - 1)
-        pow2_flag_rounds<true>(x, la, q, span, thr, err, M, flags);
-    else
-        pow2_flag_rounds<false>(x, la, q, span, thr, err, M, flags);
-#pragma unroll
-    for (int j = 0; j < kPowPer; ++j) flags[j] = __brev(flags[j]) >> (31 - q);
-    // Compact the flagged cells into an LDS queue (cell << 14 | width flags;
-    // the level buffers are free now) in cell order: slot j of every thread
-    // holds cell j*256 + tid, so entries are ranked by (j, wave, lane) with
-    // one ballot per slot.  The exact recompute then runs one chain per lane
-    // over flagged cells only.
+    // bit q - k; reversing the word puts it at bit k.  The flagged cells are
+    // then compacted into an LDS queue (cell << 14 | width flags; the level
+    // buffers are free by then) in cell order, ranked by (slot, wave, lane)
+    // with one ballot per slot; the exact recompute runs one chain per lane
+    // over the queue.
     uint32_t* queue = reinterpret_cast<uint32_t*>(la);
     __shared__ uint32_t slot_cnt[kPowPer][kThreads / 64];
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    uint64_t mk[kPowPer];
-#pragma unroll
-    for (int j = 0; j < kPowPer; ++j) {
-        mk[j] = __ballot(flags[j] != 0);
-        if (lane == 0) slot_cnt[j][wid] = (uint32_t)__popcll(mk[j]);
-    }
-    __syncthreads();
     uint32_t total = 0;
+    // Per-cell flags where exactness needs them: the last tile of a row (its
+    // window is cut by the row end, so level spans are checked) and windows
+    // with NaN/inf samples or magnitudes where a level sum could overflow.
+    const bool careful = span < kPowCells + W - 1 || !(M * (float)W < 0x1p125f);
+    if (!careful) {
+        // chunk layout: slot s of thread t is chunk s*256 + t (4 cells); a
+        // flagged chunk queues its four cells
+        uint32_t cf[2];
+        pow2_flag_chunks(x, la, q, thr, err, M, cf);
+        uint64_t mk[2];
 #pragma unroll
-    for (int j = 0; j < kPowPer; ++j) {
-        uint32_t pos = total;
-        for (int k = 0; k < kThreads / 64; ++k) {
-            if (k < wid) pos += slot_cnt[j][k];
-            total += slot_cnt[j][k];
+        for (int s = 0; s < 2; ++s) {
+            cf[s] = __brev(cf[s]) >> (31 - q);
+            mk[s] = __ballot(cf[s] != 0);
+            if (lane == 0) slot_cnt[s][wid] = (uint32_t)__popcll(mk[s]);
         }
-        if (flags[j])
-            queue[pos + (uint32_t)__popcll(mk[j] & below)] = ((uint32_t)(j * kThreads + threadIdx.x) << 14) | flags[j];
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            uint32_t pos = total;
+            for (int k = 0; k < kThreads / 64; ++k) {
+                if (k < wid) pos += slot_cnt[s][k];
+                total += slot_cnt[s][k];
+            }
+            if (cf[s]) {
+                const uint32_t p = pos + (uint32_t)__popcll(mk[s] & below);
+                const uint32_t c0 = 4u * (uint32_t)(s * kThreads + threadIdx.x);
+                reinterpret_cast<uint4*>(queue)[p] =
+                    make_uint4((c0 << 14) | cf[s], ((c0 + 1) << 14) | cf[s], ((c0 + 2) << 14) | cf[s],
+                               ((c0 + 3) << 14) | cf[s]);
+            }
+        }
+        total *= 4;
+    } else {
+        // strided layout: slot j of thread t is cell j*256 + t
+        uint32_t flags[kPowPer];
+        if (span < kPowCells + W - 1)
+            pow2_flag_rounds<true>(x, la, q, span, thr, err, M, flags);
+        else
+            pow2_flag_rounds<false>(x, la, q, span, thr, err, M, flags);
+        uint64_t mk[kPowPer];
+#pragma unroll
+        for (int j = 0; j < kPowPer; ++j) {
+            flags[j] = __brev(flags[j]) >> (31 - q);
+            mk[j] = __ballot(flags[j] != 0);
+            if (lane == 0) slot_cnt[j][wid] = (uint32_t)__popcll(mk[j]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kPowPer; ++j) {
+            uint32_t pos = total;
+            for (int k = 0; k < kThreads / 64; ++k) {
+                if (k < wid) pos += slot_cnt[j][k];
+                total += slot_cnt[j][k];
+            }
+            if (flags[j])
+                queue[pos + (uint32_t)__popcll(mk[j] & below)] =
+                    ((uint32_t)(j * kThreads + threadIdx.x) << 14) | flags[j];
+        }
     }
     if (total == 0) return;  // block-uniform
     // from q = 4 on the second round stored its level over the samples:
