@@ -338,10 +338,10 @@ __device__ __forceinline__ float pow2_load_window(float* x, const float* src, in
 // row); otherwise span = 2048 + W - 1 and every level is valid at every own
 // cell.
 template <bool EDGE>
-__device__ __forceinline__ void pow2_flag_rounds(float* x, float* la, int q, int span, float thr, const BoxErr& err,
-                                                 float M, uint32_t (&flags)[kPowPer]) {
+__device__ __forceinline__ void pow2_flag_rounds(float* x, float* la, int q, int span, const float* lim,
+                                                 uint32_t (&flags)[kPowPer]) {
     {
-        const float lf = level_limit(thr, 0, err.e[0], M);
+        const float lf = lim[0];
         float v[kPowPer];
 #pragma unroll
         for (int j = 0; j < kPowPer; ++j) v[j] = x[j * kThreads + threadIdx.x];
@@ -355,9 +355,9 @@ __device__ __forceinline__ void pow2_flag_rounds(float* x, float* la, int q, int
     for (int k0 = 0; k0 < q; k0 += 2) {
         const int k1 = k0 + 1, k2 = k0 + 2;
         const int h = 1 << k0;
-        const float l1 = level_limit(thr, k1, err.e[k1], M);
+        const float l1 = lim[k1];
         const bool two = k2 <= q;
-        const float l2 = two ? level_limit(thr, k2, err.e[k2], M) : 0.0f;
+        const float l2 = two ? lim[k2] : 0.0f;
         // ping-pong: round 0 reads the samples and writes la, round 1 reads
         // la and writes over the samples (read by round 0 only), and so on
         float* dst = ((k0 >> 1) & 1) ? x : la;
@@ -419,11 +419,10 @@ __device__ __forceinline__ float max4(const float4& v) { return fmaxf(fmaxf(v.x,
 // span_max: chunk 511 + 3h/4 (cell 2047 + 3h <= 2047 + 3W/4) when two levels
 // are formed, 511 + h/4 otherwise, and chunk 512 (cells 2048..2051 <
 // span_max for W >= 2) in the first round.
-__device__ __forceinline__ void pow2_flag_chunks(float* x, float* la, int q, float thr, const BoxErr& err, float M,
-                                                 uint32_t (&cf)[2]) {
+__device__ __forceinline__ void pow2_flag_chunks(float* x, float* la, int q, const float* lim, uint32_t (&cf)[2]) {
     const int span = kPowCells + (1 << q) - 1;
     {
-        const float lf = level_limit(thr, 0, err.e[0], M);
+        const float lf = lim[0];
         const float4* x4 = reinterpret_cast<const float4*>(x);
 #pragma unroll
         for (int s = 0; s < 2; ++s) cf[s] = !(max4(x4[s * kThreads + threadIdx.x]) <= lf) ? 1u : 0u;
@@ -432,9 +431,9 @@ __device__ __forceinline__ void pow2_flag_chunks(float* x, float* la, int q, flo
     for (int k0 = 0; k0 < q; k0 += 2) {
         const int k1 = k0 + 1, k2 = k0 + 2;
         const int h = 1 << k0;
-        const float l1 = level_limit(thr, k1, err.e[k1], M);
+        const float l1 = lim[k1];
         const bool two = k2 <= q;
-        const float l2 = two ? level_limit(thr, k2, err.e[k2], M) : 0.0f;
+        const float l2 = two ? lim[k2] : 0.0f;
         float* dst = ((k0 >> 1) & 1) ? x : la;
         const float4* c4 = reinterpret_cast<const float4*>(cur);
         float4* d4 = reinterpret_cast<float4*>(dst);
@@ -517,6 +516,11 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
     float M = wmaxabs[0];
     for (int i = 1; i < kThreads / 64; ++i) M = fmaxf(M, wmaxabs[i]);
     const float thr = d_thr ? *d_thr : h_thr;
+    // the filter limits are block-uniform: thread k computes level k's once
+    // (each is double-precision work that every wave would otherwise repeat)
+    __shared__ float lim[kMaxPowQ + 1];
+    if ((int)threadIdx.x <= q) lim[threadIdx.x] = level_limit(thr, (int)threadIdx.x, err.e[threadIdx.x], M);
+    __syncthreads();
 
     // Flag phase.  Flags are shifted in one bit per level, so level k ends at
     // bit q - k; reversing the word puts it at bit k.  The flagged cells are
@@ -538,7 +542,7 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
         // chunk layout: slot s of thread t is chunk s*256 + t (4 cells); a
         // flagged chunk queues its four cells
         uint32_t cf[2];
-        pow2_flag_chunks(x, la, q, thr, err, M, cf);
+        pow2_flag_chunks(x, la, q, lim, cf);
         uint64_t mk[2];
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
@@ -567,9 +571,9 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
         // strided layout: slot j of thread t is cell j*256 + t
         uint32_t flags[kPowPer];
         if (span < kPowCells + W - 1)
-            pow2_flag_rounds<true>(x, la, q, span, thr, err, M, flags);
+            pow2_flag_rounds<true>(x, la, q, span, lim, flags);
         else
-            pow2_flag_rounds<false>(x, la, q, span, thr, err, M, flags);
+            pow2_flag_rounds<false>(x, la, q, span, lim, flags);
         uint64_t mk[kPowPer];
 #pragma unroll
         for (int j = 0; j < kPowPer; ++j) {
