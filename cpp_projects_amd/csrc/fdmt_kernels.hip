@@ -272,36 +272,42 @@ struct LdsTreeFast {
     }
 };
 
-// evaluates the trees of every row of group g for times [T, T + kGroupTile)
+// evaluates the trees of every row of group g for times [T, T + kGroupTile):
+// wave w takes rows w, w + 4, ... over the whole tile (8 samples per lane), so
+// the per-row scalar work (descriptor, output address, path test) is done
+// once per row rather than once per wave
 template <int L, bool SNR>
 __device__ __forceinline__ void group_trees(const float* __restrict__ lds, const GroupDescD& g,
                                             const GRowDesc<L>* __restrict__ rows, float* __restrict__ out,
                                             int64_t out_pitch, int64_t T, int64_t nsamps, int64_t out_rows_limit,
                                             const SnrEpi& epi) {
+    constexpr int kWaves = kThreads / 64;
+    const int lane = (int)threadIdx.x & 63;
+    const int wid = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     float dk = 1.0f, rdk = 1.0f;
     if (SNR) {
         dk = sqrtf((float)((double)(float)epi.stats[4] + 1.0E-8));
         rdk = 1.0f / dk;
     }
-    for (int i = 0; i < g.nrows; ++i) {
+    for (int i = wid; i < g.nrows; i += kWaves) {  // wave-uniform
         const GRowDesc<L> d = rows[g.row0 + i];
         if (d.out_row >= out_rows_limit) continue;
         float* __restrict__ o = out + d.out_row * out_pitch;
         const float* nrow = SNR ? epi.nrm + d.out_row * epi.nrm_len : nullptr;
         if (T >= d.maxshift && T + kGroupTile <= nsamps && (!SNR || T >= epi.nrm_len)) {
             // the whole tile is past every shift: no per-sample tests
-#pragma unroll
-            for (int j = 0; j < kGroupTile / kThreads; ++j) {
-                const int u = j * kThreads + threadIdx.x;
+#pragma unroll 4
+            for (int j = 0; j < kGroupTile / 64; ++j) {
+                const int u = j * 64 + lane;
                 float v = LdsTreeFast<L, 1>::eval(lds, d, u);
                 if (SNR) v = div_rn(v, dk, rdk);
                 o[T + u] = v;
             }
             continue;
         }
-#pragma unroll
-        for (int j = 0; j < kGroupTile / kThreads; ++j) {
-            const int64_t t = T + j * kThreads + threadIdx.x;
+#pragma unroll 4
+        for (int j = 0; j < kGroupTile / 64; ++j) {
+            const int64_t t = T + j * 64 + lane;
             if (t < nsamps) {
                 float v = LdsTree<L, 1>::eval(lds, d, t, T);
                 if (SNR)
