@@ -294,14 +294,51 @@ __device__ __forceinline__ void group_trees(const float* __restrict__ lds, const
         if (d.out_row >= out_rows_limit) continue;
         float* __restrict__ o = out + d.out_row * out_pitch;
         const float* nrow = SNR ? epi.nrm + d.out_row * epi.nrm_len : nullptr;
-        if (T >= d.maxshift && T + kGroupTile <= nsamps && (!SNR || T >= epi.nrm_len)) {
+        if (T >= d.maxshift && T + kGroupTile <= nsamps) {
             // the whole tile is past every shift: no per-sample tests
+            if (SNR && T < epi.nrm_len) {
+                // within the mask transform's extent: per-sample divisor there
 #pragma unroll 4
-            for (int j = 0; j < kGroupTile / 64; ++j) {
-                const int u = j * 64 + lane;
-                float v = LdsTreeFast<L, 1>::eval(lds, d, u);
-                if (SNR) v = div_rn(v, dk, rdk);
-                o[T + u] = v;
+                for (int j = 0; j < kGroupTile / 64; ++j) {
+                    const int u = j * 64 + lane;
+                    const int64_t t = T + u;
+                    const float v = LdsTreeFast<L, 1>::eval(lds, d, u);
+                    o[t] = (t < epi.nrm_len) ? v / sqrtf((float)((double)nrow[t] + 1.0E-8)) : div_rn(v, dk, rdk);
+                }
+            } else if constexpr (SNR) {
+                // div_rn(v, dk, rdk) four samples at a time: dk = sqrtf(kept +
+                // 1e-8) lies in [1e-4, 256], so only the dividends need the
+                // range test, and the IEEE division runs in one rare branch
+#pragma unroll
+                for (int j0 = 0; j0 < kGroupTile / 64; j0 += 4) {
+                    float v[4], qv[4];
+                    bool slow = false;
+#pragma unroll
+                    for (int jj = 0; jj < 4; ++jj) v[jj] = LdsTreeFast<L, 1>::eval(lds, d, (j0 + jj) * 64 + lane);
+#pragma unroll
+                    for (int jj = 0; jj < 4; ++jj) {
+                        const float ax = fabsf(v[jj]);
+                        slow = slow || !(ax >= 0x1p-60f && ax <= 0x1p60f);
+                        const float q0 = v[jj] * rdk;
+                        const float e = fmaf(-q0, dk, v[jj]);
+                        qv[jj] = fmaf(e, rdk, q0);
+                    }
+                    if (slow) {
+#pragma unroll
+                        for (int jj = 0; jj < 4; ++jj) {
+                            const float ax = fabsf(v[jj]);
+                            if (!(ax >= 0x1p-60f && ax <= 0x1p60f)) qv[jj] = v[jj] / dk;
+                        }
+                    }
+#pragma unroll
+                    for (int jj = 0; jj < 4; ++jj) o[T + (j0 + jj) * 64 + lane] = qv[jj];
+                }
+            } else {
+#pragma unroll 4
+                for (int j = 0; j < kGroupTile / 64; ++j) {
+                    const int u = j * 64 + lane;
+                    o[T + u] = LdsTreeFast<L, 1>::eval(lds, d, u);
+                }
             }
             continue;
         }
