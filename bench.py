@@ -286,11 +286,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-cols", type=int, default=1 << 17)
     # the defaults are the autotuner's choice for this workload on MI355X
-    ap.add_argument("--schedule", type=str, default="3,3,2,2,2", help="comma-separated merge fusion depths")
-    ap.add_argument("--staged", type=int, default=1,
+    ap.add_argument("--schedule", type=str, default="3,3,3,3", help="comma-separated merge fusion depths")
+    ap.add_argument("--staged", type=int, default=3,
                     help="staging mode 0..4 (0 tree passes, 1 staged passes with a tree final pass, 3 all "
                          "staged; 2, 4 add the fused init)")
-    ap.add_argument("--group-shape", type=str, default="5120,16,16",
+    ap.add_argument("--group-shape", type=str, default="6144,16,32",
                     help="staged passes: LDS floats per tile buffer, rows per group, tiles per block")
     ap.add_argument("--tune", action="store_true", help="autotune mode, shape and schedule during warmup instead")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-beam measurement")

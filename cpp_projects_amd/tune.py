@@ -4,7 +4,8 @@ staged group shape (LDS window, rows per group, tiles per block), and how many
 FDMT iterations each pass fuses.  Candidates are timed with the stage events
 of the search handle on the real input; the fastest is kept on the handle."""
 
-DEFAULT_SHAPES = ((6144, 16, 16), (5120, 16, 16), (7168, 16, 16), (6144, 16, 32))
+DEFAULT_SHAPES = ((6144, 16, 16), (5120, 16, 16), (7168, 16, 16), (6144, 16, 32), (6144, 16, 64), (7168, 16, 32),
+                  (5120, 16, 32), (6144, 32, 32))
 
 
 def candidate_schedules(n, max_depth=3):
