@@ -89,6 +89,7 @@ SIGNATURES = {
     "sps_cluster": [_vp, _i64, _vp, _vp, _vp, _pi64, _vp],
     "sps_detect_max": [_vp, _vp, _i64, _i64, _int, _vp, _vp],
     "sps_boxcar_snr": [_vp, _i64, _i64, _int, _vp, _vp],
+    "sps_digitize": [_vp, _i64, _vp],
     "sps_boxcar_snr_adjoint": [_vp, _i64, _i64, _int, _vp, _vp],
     "sps_search_create": [C.POINTER(SearchParams), C.POINTER(_vp)],
     "sps_search_destroy": [_vp],

@@ -1003,6 +1003,13 @@ SPS_API sps_status sps_boxcar_snr(const float* d_img, int64_t rows, int64_t cols
     });
 }
 
+SPS_API sps_status sps_digitize(float* d_x, int64_t n, void* stream) {
+    return guarded([&] {
+        SPS_REQUIRE(n >= 0 && (d_x || n == 0), "invalid argument");
+        launch_digitize(d_x, n, as_stream(stream));
+    });
+}
+
 SPS_API sps_status sps_boxcar_snr_adjoint(const float* d_g, int64_t rows, int64_t cols, int w, float* d_gi,
                                           void* stream) {
     return guarded([&] {

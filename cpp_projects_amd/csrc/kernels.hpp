@@ -177,6 +177,8 @@ void launch_boxcar(const float* img, int64_t rows, int64_t cols, int64_t pitch, 
                    int wmax, int pow2, CandD* out, int64_t cap, unsigned long long* d_count, hipStream_t st);
 void launch_boxcar_snr(const float* img, int64_t rows, int64_t cols, int w, float* snr, hipStream_t st);
 void launch_boxcar_snr_adjoint(const float* g, int64_t rows, int64_t cols, int w, float* gi, hipStream_t st);
+// x[i] = floorf(x[i]) for i < n
+void launch_digitize(float* x, int64_t n, hipStream_t st);
 void launch_detect_max(const float* a, const float* n, int64_t rows, int64_t cols, int wnd, void* d_block_best,
                        int64_t nblocks_cap, void* d_out, hipStream_t st);
 int64_t detect_max_blocks(int64_t rows, int64_t cols);

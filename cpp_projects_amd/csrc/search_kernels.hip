@@ -733,6 +733,12 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
     }
 }
 
+// digitize_kernel (Clusterization.cu:206-214): x = floorf(x) in place
+__global__ void __launch_bounds__(kThreads) k_digitize(float* __restrict__ x, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+        x[i] = floorf(x[i]);
+}
+
 __global__ void __launch_bounds__(kThreads) k_boxcar_snr(const float* __restrict__ img, int64_t cols, int w,
                                                          float* __restrict__ snr) {
     const int64_t r = blockIdx.y;
@@ -915,6 +921,13 @@ void launch_boxcar_snr(const float* img, int64_t rows, int64_t cols, int w, floa
     if (rows > 65535) throw Error(SPS_ERR_INVALID_ARGUMENT, "more than 65535 rows");
     hipLaunchKernelGGL(k_boxcar_snr, dim3((uint32_t)ceil_div(cols, kThreads), (uint32_t)rows), dim3(kThreads), 0, st,
                        img, cols, w, snr);
+    SPS_HIP(hipGetLastError());
+}
+
+void launch_digitize(float* x, int64_t n, hipStream_t st) {
+    if (n <= 0) return;
+    const int64_t nb = std::min<int64_t>(ceil_div(n, kThreads), 65536);
+    hipLaunchKernelGGL(k_digitize, dim3((uint32_t)nb), dim3(kThreads), 0, st, x, n);
     SPS_HIP(hipGetLastError());
 }
 

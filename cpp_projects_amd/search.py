@@ -75,6 +75,15 @@ class clusterization:
         return n > 0, c, n
 
     @staticmethod
+    def digitize_kernel(d_arrfdmt, len_arr=None):
+        """Clusterization.cu:206-214: d_arrfdmt[i] = floorf(d_arrfdmt[i]) in
+        place for i < len_arr (default: the whole contiguous f32 tensor)."""
+        n = d_arrfdmt.numel() if len_arr is None else int(len_arr)
+        if n > d_arrfdmt.numel():
+            raise ValueError("len_arr exceeds the tensor")
+        call("sps_digitize", _dptr(d_arrfdmt), n, _stream())
+
+    @staticmethod
     def cluster(d_cands, n, d_bins):
         """Three-key gap clustering + delegates (Clusterization.cu:92-166).
         Returns (delegates int32 device tensor sorted by mt, group sizes int64)."""

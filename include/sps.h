@@ -204,6 +204,11 @@ SPS_API sps_status sps_cluster(const sps_cand* d_cands, int64_t n, const int32_t
 SPS_API sps_status sps_detect_max(const float* d_arr, const float* d_norm, int64_t rows, int64_t cols, int wnd,
                                   sps_detection* d_out, void* stream);
 
+/* clusterization::digitize_kernel (Clusterization.cuh:34-35,
+ * Clusterization.cu:206-214): d_x[i] = floorf(d_x[i]) in place, i < n (the
+ * reference test harness uses it to make integer-valued images). */
+SPS_API sps_status sps_digitize(float* d_x, int64_t n, void* stream);
+
 /* Boxcar S/N map and its adjoint for the differentiable path (no reference
  * counterpart): d_snr[r][t] = sum_{j<w} d_img[r][t+j] / sqrtf(w) for t+w <= cols
  * (0 elsewhere); adjoint d_grad_img[r][t'] = sum_{t<=t'<t+w} d_grad_snr[r][t]/sqrtf(w). */

@@ -76,6 +76,13 @@ inline void write_log(const Cand* d_parrCand, const int QUantCandidates, const s
 
 namespace clusterization {
 
+// Clusterization.cuh:34-35 (__global__ digitize_kernel, launched <<<1,1024>>>
+// by the reference harness): the same in-place floorf over LEnarr floats,
+// as a host call on the given stream.
+inline void digitize_kernel(float* d_arrfdmt, const int LEnarr, hipStream_t stream = nullptr) {
+    sps_detail::check(sps_digitize(d_arrfdmt, LEnarr, stream));
+}
+
 // Clusterization.cuh:22-23: widths 1,2,..,2^QUantPower2_Wnd, mwidth = log2(w).
 // *d_arrCand is hipMalloc'ed here and owned by the caller (hipFree).
 inline void gather_candidates_in_dynamicalArray(const float* d_arr_fdmt, const int Rows, const int Cols,

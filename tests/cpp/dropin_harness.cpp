@@ -75,6 +75,7 @@ int main(int argc, char** argv) {
         hipMalloc(&d_thr, 4);
         hipMalloc(&d_bins, 12);
         hipMemcpy(d_img, img.data(), 48 * 4, hipMemcpyHostToDevice);
+        clusterization::digitize_kernel(d_img, 48);  // as clusterization_test.cu does (integer image: unchanged)
         const float thr = 4.0f;
         const int bins[3] = {2, 1, 2};
         hipMemcpy(d_thr, &thr, 4, hipMemcpyHostToDevice);
