@@ -639,23 +639,29 @@ __global__ void __launch_bounds__(kThreads) k_init_lds(const void* __restrict__ 
 // channel (K <= kInitConsecMax): row 0 is (0 + x[t]) / 1 = 0 + x[t] and row k
 // adds the single sample x[t - k]: v_k = (v_{k-1} * k + (0 + x[t - k])) / (k + 1),
 // the operations of k_init_lds for such a grid, in the same order.  One
-// block = (channel, 4096-sample tile); a thread owns 4 chunks of 4
-// consecutive samples (chunk j * 256 + tid): one 4-byte (u8) or 16-byte
-// (f32) load per chunk, normalised once in registers; the 4 samples before a
-// chunk (x[t - k], k <= 4) come through one aligned 16-byte LDS read, and
-// each row is written as float4, so every store instruction covers 1 KB.
+// block = one channel x tiles_per_block consecutive 4096-sample tiles; a
+// thread owns 4 chunks of 4 consecutive samples of a tile (chunk j * 256 +
+// tid): one 4-byte (u8) or 16-byte (f32) load per chunk, issued for the next
+// tile before the current one is computed, normalised once in registers; the
+// 4 samples before a chunk (x[t - k], k <= 4) come through one aligned
+// 16-byte LDS read (for a tile's first chunk: the previous tile's last chunk,
+// or the input before the block's first tile), and each row is written as
+// float4, so every store instruction covers 1 KB.
 constexpr int kInitConsecMax = 5;
+constexpr int kInitTilesPerBlock = 8;
 
 template <int IN_KIND, bool NORM>
 __global__ void __launch_bounds__(kThreads) k_init_consec(const void* __restrict__ in, int64_t in_pitch, ChanNorm norm,
                                                           int64_t nsamps, const int64_t* __restrict__ grid_off,
                                                           const int64_t* __restrict__ chan_row_off,
                                                           float* __restrict__ state, int64_t pitch,
-                                                          const uint8_t* __restrict__ live) {
+                                                          const uint8_t* __restrict__ live, int tiles_per_block) {
     static_assert(IN_KIND == 0 || IN_KIND == 1, "f32 or u8 input");
     __shared__ float4a xs[1 + kInitTile / 4];  // xs[0]: samples T-4 .. T-1; xs[1 + q]: chunk q
     const int64_t c = blockIdx.y;
-    const int64_t T = (int64_t)blockIdx.x * kInitTile;
+    const int64_t ntiles = ceil_div(nsamps, (int64_t)kInitTile);
+    const int64_t k0 = (int64_t)blockIdx.x * tiles_per_block;
+    const int64_t k1 = min(k0 + (int64_t)tiles_per_block, ntiles);
     const int K = (int)(grid_off[c + 1] - grid_off[c]);
     const uint8_t* lv = live ? live + chan_row_off[c] : nullptr;
     float mu = 0.0f, sg = 1.0f, rsg = 1.0f;
@@ -668,66 +674,92 @@ __global__ void __launch_bounds__(kThreads) k_init_consec(const void* __restrict
     }
     auto nrm = [&](float v) -> float { return NORM ? (keep ? div_rn(v - mu, sg, rsg) : 0.0f) : v; };
     const int64_t xb = c * in_pitch;
-    float4a own[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int q = j * kThreads + (int)threadIdx.x;
-        const int64_t t0 = T + 4 * (int64_t)q;
-        float4a v = {0.0f, 0.0f, 0.0f, 0.0f};
-        if (t0 < nsamps) {  // nsamps % 4 == 0: the chunk is whole
-            if (IN_KIND == 1) {
-                const uint32_t w = *reinterpret_cast<const uint32_t*>(static_cast<const uint8_t*>(in) + xb + t0);
-                v.x = (float)(w & 0xffu);
-                v.y = (float)((w >> 8) & 0xffu);
-                v.z = (float)((w >> 16) & 0xffu);
-                v.w = (float)(w >> 24);
-            } else {
-                v = *reinterpret_cast<const float4a*>(static_cast<const float*>(in) + xb + t0);
-            }
-            v.x = nrm(v.x);
-            v.y = nrm(v.y);
-            v.z = nrm(v.z);
-            v.w = nrm(v.w);
-        }
-        own[j] = v;
-        xs[1 + q] = v;
-    }
-    if (threadIdx.x < 4) {
-        const int64_t t = T - 4 + (int64_t)threadIdx.x;
-        reinterpret_cast<float*>(xs)[threadIdx.x] = t >= 0 ? nrm(load_in<IN_KIND>(in, xb + t)) : 0.0f;
-    }
-    __syncthreads();
     float* __restrict__ rowbase = state + chan_row_off[c] * pitch;
+    uint32_t raw8[4] = {0u, 0u, 0u, 0u};  // u8: the chunk's 4 bytes
+    float4a rawf[4];                      // f32: the chunk
+    auto issue = [&](int64_t T) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int q = j * kThreads + (int)threadIdx.x;
-        const int64_t t0 = T + 4 * (int64_t)q;
-        if (t0 >= nsamps) continue;
-        const float4a h = xs[q];  // samples t0 - 4 .. t0 - 1
-        const float w[8] = {h.x, h.y, h.z, h.w, own[j].x, own[j].y, own[j].z, own[j].w};  // w[4 + e - k] = x[t0 + e - k]
-        float prev[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) prev[e] = 0.0f + w[4 + e];
-        if (!lv || lv[0]) {
-            float4a o = {prev[0], prev[1], prev[2], prev[3]};
-            *reinterpret_cast<float4a*>(rowbase + t0) = o;
-        }
-#pragma unroll
-        for (int k = 1; k < kInitConsecMax; ++k) {
-            if (k >= K) break;  // block-uniform
-            const float yk = (float)k + 1.0f, rk = 1.0f / yk;
-            const float mk = (float)(k - 1) + 1.0f;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                float v = 0.0f;
-                if (t0 + e >= k) v = div_rn(prev[e] * mk + (0.0f + w[4 + e - k]), yk, rk);
-                prev[e] = v;
+        for (int j = 0; j < 4; ++j) {
+            const int64_t t0 = T + 4 * (int64_t)(j * kThreads + (int)threadIdx.x);
+            if (t0 < nsamps) {  // nsamps % 4 == 0: the chunk is whole
+                if (IN_KIND == 1)
+                    raw8[j] = *reinterpret_cast<const uint32_t*>(static_cast<const uint8_t*>(in) + xb + t0);
+                else
+                    rawf[j] = *reinterpret_cast<const float4a*>(static_cast<const float*>(in) + xb + t0);
             }
-            if (!lv || lv[k]) {
+        }
+    };
+    float4a prev_last = {0.0f, 0.0f, 0.0f, 0.0f};  // thread kThreads-1: the previous tile's last chunk
+    if (k0 < k1) issue(k0 * kInitTile);
+    for (int64_t k = k0; k < k1; ++k) {  // block-uniform
+        const int64_t T = k * kInitTile;
+        float4a own[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int64_t t0 = T + 4 * (int64_t)(j * kThreads + (int)threadIdx.x);
+            float4a v = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (t0 < nsamps) {
+                if (IN_KIND == 1) {
+                    const uint32_t w = raw8[j];
+                    v.x = (float)(w & 0xffu);
+                    v.y = (float)((w >> 8) & 0xffu);
+                    v.z = (float)((w >> 16) & 0xffu);
+                    v.w = (float)(w >> 24);
+                } else {
+                    v = rawf[j];
+                }
+                v.x = nrm(v.x);
+                v.y = nrm(v.y);
+                v.z = nrm(v.z);
+                v.w = nrm(v.w);
+            }
+            own[j] = v;
+        }
+        if (k + 1 < k1) issue(T + kInitTile);
+        __syncthreads();  // the previous tile's reads of xs are done
+#pragma unroll
+        for (int j = 0; j < 4; ++j) xs[1 + j * kThreads + (int)threadIdx.x] = own[j];
+        if (k == k0) {
+            if (threadIdx.x < 4) {
+                const int64_t t = T - 4 + (int64_t)threadIdx.x;
+                reinterpret_cast<float*>(xs)[threadIdx.x] = t >= 0 ? nrm(load_in<IN_KIND>(in, xb + t)) : 0.0f;
+            }
+        } else if (threadIdx.x == kThreads - 1) {
+            xs[0] = prev_last;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int q = j * kThreads + (int)threadIdx.x;
+            const int64_t t0 = T + 4 * (int64_t)q;
+            if (t0 >= nsamps) continue;
+            const float4a h = xs[q];  // samples t0 - 4 .. t0 - 1
+            const float w[8] = {h.x, h.y, h.z, h.w, own[j].x, own[j].y, own[j].z, own[j].w};  // w[4 + e - k] = x[t0 + e - k]
+            float prev[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) prev[e] = 0.0f + w[4 + e];
+            if (!lv || lv[0]) {
                 float4a o = {prev[0], prev[1], prev[2], prev[3]};
-                *reinterpret_cast<float4a*>(rowbase + (int64_t)k * pitch + t0) = o;
+                *reinterpret_cast<float4a*>(rowbase + t0) = o;
+            }
+#pragma unroll
+            for (int kk = 1; kk < kInitConsecMax; ++kk) {
+                if (kk >= K) break;  // block-uniform
+                const float yk = (float)kk + 1.0f, rk = 1.0f / yk;
+                const float mk = (float)(kk - 1) + 1.0f;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    float v = 0.0f;
+                    if (t0 + e >= kk) v = div_rn(prev[e] * mk + (0.0f + w[4 + e - kk]), yk, rk);
+                    prev[e] = v;
+                }
+                if (!lv || lv[kk]) {
+                    float4a o = {prev[0], prev[1], prev[2], prev[3]};
+                    *reinterpret_cast<float4a*>(rowbase + (int64_t)kk * pitch + t0) = o;
+                }
             }
         }
+        if (threadIdx.x == kThreads - 1) prev_last = own[3];
     }
 }
 
@@ -1088,13 +1120,14 @@ static void init_dispatch_norm(const void* in, int64_t in_pitch, const ChanNorm*
         const bool al_in = IN_KIND == 1 ? (ib % 4 == 0 && in_pitch % 4 == 0) : (ib % 16 == 0 && in_pitch % 4 == 0);
         const bool al_out = (reinterpret_cast<uintptr_t>(state) % 16 == 0) && (pitch % 4 == 0);
         if (consec && al_in && al_out && nsamps % 4 == 0) {
-            const dim3 grid((uint32_t)ceil_div(nsamps, kInitTile), (uint32_t)nchans);
+            const int tpb = kInitTilesPerBlock;
+            const dim3 grid((uint32_t)ceil_div(ceil_div(nsamps, (int64_t)kInitTile), (int64_t)tpb), (uint32_t)nchans);
             if (norm)
                 hipLaunchKernelGGL((k_init_consec<IN_KIND, true>), grid, dim3(kThreads), 0, st, in, in_pitch, n,
-                                   nsamps, go, cro, state, pitch, live);
+                                   nsamps, go, cro, state, pitch, live, tpb);
             else
                 hipLaunchKernelGGL((k_init_consec<IN_KIND, false>), grid, dim3(kThreads), 0, st, in, in_pitch, n,
-                                   nsamps, go, cro, state, pitch, live);
+                                   nsamps, go, cro, state, pitch, live, tpb);
             return;
         }
     }
