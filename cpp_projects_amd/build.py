@@ -28,6 +28,10 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-I", os.path.join(ROOT, "include")]
 
 HIP_SOURCES = ["fdmt_kernels.hip", "search_kernels.hip", "sort_kernels.hip", "fused_kernels.hip"]
+# per-source additions: the boxcar's multi-tile kernel loops over tiles, and
+# machine LICM hoists constants of the tile body out of that loop into
+# registers (112 VGPRs instead of 90: four waves per SIMD instead of five)
+HIP_SOURCE_FLAGS = {"search_kernels.hip": ["-mllvm", "-disable-machine-licm"]}
 HIPCC_HOST_SOURCES = ["api.cpp", "batch.cpp"]
 GXX_SOURCES = ["plan.cpp"]
 
@@ -46,7 +50,7 @@ def build(verbose=False):
         src = os.path.join(CSRC, s)
         if not os.path.exists(src):
             continue
-        jobs.append([HIPCC, *HIP_FLAGS, "-c", src, "-o", os.path.join(BUILD, s + ".o")])
+        jobs.append([HIPCC, *HIP_FLAGS, *HIP_SOURCE_FLAGS.get(s, []), "-c", src, "-o", os.path.join(BUILD, s + ".o")])
     for s in HIPCC_HOST_SOURCES:
         src = os.path.join(CSRC, s)
         if not os.path.exists(src):

@@ -279,11 +279,24 @@ struct BoxErr {
     double e[kMaxPowQ + 1];
 };
 
+// threadIdx.x through an empty asm statement: the compiler cannot hoist
+// what is computed from it out of a loop.  The boxcar tile body uses it, so
+// that a block looping over tiles recomputes its per-thread LDS addresses
+// per tile instead of holding them in registers across the loop.
+__device__ __forceinline__ unsigned opaque_tid() {
+    unsigned t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
 __host__ __device__ constexpr int round_up4(int v) { return (v + 3) & ~3; }
 
 // Largest float not above the filter limit of width 2^k:
-// thr*sqrtf(w)*(1 - 2^-22) - e_k*M, e_k = (gamma_{w-1} + gamma_k)*w*(1 + 2^-20)
-// (computed on the host).  For a float P, P > this float <=> P > the limit;
+// thr*sqrtf(w)*(1 - 2^-22) - e_k*M, e_k = (gamma_{w-1}*w + E_k)*(1 + 2^-20)
+// (computed on the host): gamma_{w-1}*w*M bounds the error of the reference's
+// left-to-right sum, E_k*M that of the value the filter compares (gamma_k*w
+// for the pairwise level sums, gamma_{k+6}*(w + 16) for the segment bound of
+// widths >= 16).  For a float P, P > this float <=> P > the limit;
 // NaN thresholds or NaN/infinite samples flag everything.
 __device__ __forceinline__ float level_limit(float thr, int k, double e_k, float M) {
     if (!(thr > 0.0f)) return -INFINITY;
@@ -297,24 +310,25 @@ constexpr int kPowStage = 128;  // candidate stage entries (uint2) per wave
 // |x| (+inf once it saw a NaN, which disables the filter).  vec: 16-byte
 // loads (the window start is 16-byte aligned), the last partial vector scalar.
 __device__ __forceinline__ float pow2_load_window(float* x, const float* src, int span, bool vec) {
+    const unsigned tid = opaque_tid();
     float m = 0.0f;
     if (vec) {
         const int n4 = span >> 2;
         const float4* src4 = reinterpret_cast<const float4*>(src);
-        for (int i = threadIdx.x; i < n4; i += kThreads) {
+        for (int i = tid; i < n4; i += kThreads) {
             const float4 v = src4[i];
             reinterpret_cast<float4*>(x)[i] = v;
             m = fmaxf(m, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
             if (v.x != v.x || v.y != v.y || v.z != v.z || v.w != v.w) m = __int_as_float(0x7f800000);
         }
-        for (int i = 4 * n4 + threadIdx.x; i < span; i += kThreads) {
+        for (int i = 4 * n4 + tid; i < span; i += kThreads) {
             const float v = src[i];
             x[i] = v;
             m = fmaxf(m, fabsf(v));
             if (v != v) m = __int_as_float(0x7f800000);
         }
     } else {
-        for (int i = threadIdx.x; i < span; i += kThreads) {
+        for (int i = tid; i < span; i += kThreads) {
             const float v = src[i];
             x[i] = v;
             m = fmaxf(m, fabsf(v));
@@ -340,14 +354,15 @@ __device__ __forceinline__ float pow2_load_window(float* x, const float* src, in
 template <bool EDGE>
 __device__ __forceinline__ void pow2_flag_rounds(float* x, float* la, int q, int span, const float* lim,
                                                  uint32_t (&flags)[kPowPer]) {
+    const unsigned tid = opaque_tid();
     {
         const float lf = lim[0];
         float v[kPowPer];
 #pragma unroll
-        for (int j = 0; j < kPowPer; ++j) v[j] = x[j * kThreads + threadIdx.x];
+        for (int j = 0; j < kPowPer; ++j) v[j] = x[j * kThreads + tid];
 #pragma unroll
         for (int j = 0; j < kPowPer; ++j) {
-            const int lt = j * kThreads + threadIdx.x;
+            const int lt = j * kThreads + tid;
             flags[j] = ((!EDGE || lt < span) && !(v[j] <= lf)) ? 1u : 0u;
         }
     }
@@ -365,7 +380,7 @@ __device__ __forceinline__ void pow2_flag_rounds(float* x, float* la, int q, int
         if (two) {
 #pragma unroll
             for (int j = 0; j < kPowPer; ++j) {
-                const int i = j * kThreads + threadIdx.x;
+                const int i = j * kThreads + tid;
                 va[j] = cur[i];
                 vb[j] = cur[i + h];
                 vc[j] = cur[i + 2 * h];
@@ -373,7 +388,7 @@ __device__ __forceinline__ void pow2_flag_rounds(float* x, float* la, int q, int
             }
 #pragma unroll
             for (int j = 0; j < kPowPer; ++j) {  // the thread's own cells
-                const int i = j * kThreads + threadIdx.x;
+                const int i = j * kThreads + tid;
                 const float p1 = va[j] + vb[j];
                 const float p2 = p1 + (vc[j] + vd[j]);
                 dst[i] = p2;
@@ -385,20 +400,20 @@ __device__ __forceinline__ void pow2_flag_rounds(float* x, float* la, int q, int
         } else {
 #pragma unroll
             for (int j = 0; j < kPowPer; ++j) {
-                const int i = j * kThreads + threadIdx.x;
+                const int i = j * kThreads + tid;
                 va[j] = cur[i];
                 vb[j] = cur[i + h];
             }
 #pragma unroll
             for (int j = 0; j < kPowPer; ++j) {
-                const int i = j * kThreads + threadIdx.x;
+                const int i = j * kThreads + tid;
                 const float p1 = va[j] + vb[j];
                 const bool c1 = (!EDGE || i + 2 * h <= span) && !(p1 <= l1);
                 flags[j] = flags[j] + flags[j] + (uint32_t)c1;
             }
         }
         if (two) {  // the halo: stored for the next rounds only
-            for (int i = kPowCells + threadIdx.x; i + 4 * h <= span; i += kThreads)
+            for (int i = kPowCells + tid; i + 4 * h <= span; i += kThreads)
                 dst[i] = (cur[i] + cur[i + h]) + (cur[i + 2 * h] + cur[i + 3 * h]);
         }
         __syncthreads();
@@ -419,15 +434,22 @@ __device__ __forceinline__ float max4(const float4& v) { return fmaxf(fmaxf(v.x,
 // span_max: chunk 511 + 3h/4 (cell 2047 + 3h <= 2047 + 3W/4) when two levels
 // are formed, 511 + h/4 otherwise, and chunk 512 (cells 2048..2051 <
 // span_max for W >= 2) in the first round.
-__device__ __forceinline__ void pow2_flag_chunks(float* x, float* la, int q, const float* lim, uint32_t (&cf)[2]) {
-    const int span = kPowCells + (1 << q) - 1;
+// Q is a template parameter so that the rounds unroll: every round's
+// branches (first round, one or two levels) resolve at compile time and no
+// values are carried across rounds.
+template <int Q>
+__device__ __forceinline__ void pow2_flag_chunks(float* x, float* la, const float* lim, uint32_t (&cf)[2]) {
+    const unsigned tid = opaque_tid();
+    constexpr int q = Q;
+    constexpr int span = kPowCells + (1 << q) - 1;
     {
         const float lf = lim[0];
         const float4* x4 = reinterpret_cast<const float4*>(x);
 #pragma unroll
-        for (int s = 0; s < 2; ++s) cf[s] = !(max4(x4[s * kThreads + threadIdx.x]) <= lf) ? 1u : 0u;
+        for (int s = 0; s < 2; ++s) cf[s] = !(max4(x4[s * kThreads + tid]) <= lf) ? 1u : 0u;
     }
     const float* cur = x;
+#pragma unroll
     for (int k0 = 0; k0 < q; k0 += 2) {
         const int k1 = k0 + 1, k2 = k0 + 2;
         const int h = 1 << k0;
@@ -443,7 +465,7 @@ __device__ __forceinline__ void pow2_flag_chunks(float* x, float* la, int q, con
             float4 w0[2], w1[2];
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
-                const int c = s * kThreads + threadIdx.x;
+                const int c = s * kThreads + tid;
                 w0[s] = c4[c];
                 w1[s] = c4[c + 1];
             }
@@ -460,7 +482,7 @@ __device__ __forceinline__ void pow2_flag_chunks(float* x, float* la, int q, con
             float4 A[2], B[2], C[2], D[2];
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
-                const int c = s * kThreads + threadIdx.x;
+                const int c = s * kThreads + tid;
                 A[s] = c4[c];
                 B[s] = c4[c + hq];
                 if (two) {
@@ -480,12 +502,12 @@ __device__ __forceinline__ void pow2_flag_chunks(float* x, float* la, int q, con
         for (int s = 0; s < 2; ++s) {
             cf[s] = cf[s] + cf[s] + (!(max4(p1[s]) <= l1) ? 1u : 0u);
             if (two) {
-                d4[s * kThreads + threadIdx.x] = p2[s];
+                d4[s * kThreads + tid] = p2[s];
                 cf[s] = cf[s] + cf[s] + (!(max4(p2[s]) <= l2) ? 1u : 0u);
             }
         }
         if (two) {  // the halo: stored for the next rounds only
-            for (int i = kPowCells + threadIdx.x; i + 4 * h <= span; i += kThreads)
+            for (int i = kPowCells + tid; i + 4 * h <= span; i += kThreads)
                 dst[i] = (cur[i] + cur[i + h]) + (cur[i + 2 * h] + cur[i + 3 * h]);
         }
         __syncthreads();
@@ -493,33 +515,171 @@ __device__ __forceinline__ void pow2_flag_chunks(float* x, float* la, int q, con
     }
 }
 
-__global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restrict__ img, int64_t cols, int64_t pitch,
-                                                          const float* __restrict__ d_thr, float h_thr, int q,
-                                                          BoxErr err, CandD* __restrict__ out, int64_t cap,
-                                                          unsigned long long* __restrict__ count) {
-    extern __shared__ __attribute__((aligned(16))) float lds[];
+// Runs pow2_flag_chunks<q> for a runtime q in [Q, QMAX].
+template <int Q, int QMAX>
+__device__ __forceinline__ void pow2_flag_chunks_at(int q, float* x, float* la, const float* lim, uint32_t (&cf)[2]) {
+    if (q == Q) {
+        pow2_flag_chunks<Q>(x, la, lim, cf);
+    } else {
+        if constexpr (Q < QMAX) pow2_flag_chunks_at<Q + 1, QMAX>(q, x, la, lim, cf);
+    }
+}
+
+constexpr int kPowPfMaxQ = 8;  // widest window of the segment filter and the multi-tile kernel
+constexpr int kPowPfSpan = round_up4(kPowCells + (1 << kPowPfMaxQ) - 1);
+
+// max of four values (the chunk filter's flag test), as in pow2_flag_chunks
+__device__ __forceinline__ uint32_t chunk_flag(float a, float b, float c, float d, float l) {
+    return !(fmaxf(fmaxf(a, b), fmaxf(c, d)) <= l) ? 1u : 0u;
+}
+
+// The flag phase of an interior window (as pow2_flag_chunks: span = 2048 +
+// W - 1, finite samples, W * M < 2^125) with thread t owning segment t =
+// cells 8t .. 8t+7, i.e. chunks 2t (cf[0]) and 2t + 1 (cf[1]).
+//  * Widths 1 .. 8: the per-cell pairwise level sums of pow2_flag_rounds
+//    (P_2w(i) = P_w(i) + P_w(i + w)), from the thread's own cells and the
+//    next 7; a chunk is flagged when its largest sum exceeds the limit.
+//  * Widths w = 2^k >= 16: a window starting at cell 8g + m (m < 8) covers
+//    segments g .. g + w/8 - 1 minus the first m cells of g, plus the first
+//    m cells of segment g' = g + w/8.  With pre_g(m) the sum of segment g's
+//    first m cells (pre_g(0) = 0) and D(g) the sum of segments g ..
+//    g + w/8 - 1, it is D(g) - pre_g(m) + pre_g'(m) <= D(g) + max_m pre_g'(m)
+//    - min_m pre_g(m) = U(g), so both chunks of segment g are flagged at level
+//    k when U(g) exceeds the limit (the host's error term for k >= 4 covers
+//    the rounding of U: segment sums of 8, D a depth-(k-3) tree of them,
+//    prefixes of <= 7, two more operations).  D doubles over segment sums
+//    (D_2n(g) = D_n(g) + D_n(g + n)), one barrier per width.
+// Unlike the level rounds it keeps the samples; LDS traffic per cell is a
+// few floats instead of about two per level.  aux: 5 (256 + W/8) floats.
+template <int Q>
+__device__ __forceinline__ void pow2_flag_segments(const float* x, float* aux, const float* lim, uint32_t (&cf)[2]) {
+    static_assert(Q <= kPowPfMaxQ, "segment filter window");
+    const unsigned tid = opaque_tid();
+    const float4* x4 = reinterpret_cast<const float4*>(x);
+    float c[16];
+    {
+        const float4 v0 = x4[2 * tid], v1 = x4[2 * tid + 1];
+        c[0] = v0.x, c[1] = v0.y, c[2] = v0.z, c[3] = v0.w, c[4] = v1.x, c[5] = v1.y, c[6] = v1.z, c[7] = v1.w;
+        if constexpr (Q >= 1) {  // cells 8t + 8 .. 8t + 15 (< 2056 <= kPowPfSpan); past the window only for
+                                 // sums of levels above Q, which flag nothing
+            const float4 v2 = x4[2 * tid + 2], v3 = x4[2 * tid + 3];
+            c[8] = v2.x, c[9] = v2.y, c[10] = v2.z, c[11] = v2.w, c[12] = v3.x, c[13] = v3.y, c[14] = v3.z,
+            c[15] = v3.w;
+        }
+    }
+    auto flag8 = [&](const float* v, float l) {
+        cf[0] = cf[0] + cf[0] + chunk_flag(v[0], v[1], v[2], v[3], l);
+        cf[1] = cf[1] + cf[1] + chunk_flag(v[4], v[5], v[6], v[7], l);
+    };
+    cf[0] = 0u;
+    cf[1] = 0u;
+    flag8(c, lim[0]);
+    if constexpr (Q >= 1) {
+        float p1[14], p2[12], p3[8];
+#pragma unroll
+        for (int i = 0; i < 14; ++i) p1[i] = c[i] + c[i + 1];
+        flag8(p1, lim[1]);
+        if constexpr (Q >= 2) {
+#pragma unroll
+            for (int i = 0; i < 12; ++i) p2[i] = p1[i] + p1[i + 2];
+            flag8(p2, lim[2]);
+        }
+        if constexpr (Q >= 3) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) p3[i] = p2[i] + p2[i + 4];
+            flag8(p3, lim[3]);
+        }
+    }
+    if constexpr (Q >= 4) {
+        constexpr int NQ = (1 << Q) / 8;  // segments of the widest window
+        constexpr int G = kPowCells / 8 + NQ;
+        float* S = aux;
+        float* MN = aux + G;
+        float* MX = aux + 2 * G;
+        float* DB[2] = {aux + 3 * G, aux + 4 * G};
+        auto stats = [&](const float* v, int g) {
+            float pre = v[0];
+            float mn = fminf(0.0f, pre), mx = fmaxf(0.0f, pre);
+#pragma unroll
+            for (int m = 1; m < 7; ++m) {
+                pre = pre + v[m];
+                mn = fminf(mn, pre);
+                mx = fmaxf(mx, pre);
+            }
+            S[g] = pre + v[7];
+            MN[g] = mn;
+            MX[g] = mx;
+        };
+        stats(c, (int)tid);
+        if ((int)tid < NQ) {  // halo segments 256 .. 255 + NQ (cells < 2048 + W <= kPowPfSpan)
+            const int g = kPowCells / 8 + (int)tid;
+            const float4 v0 = x4[2 * g], v1 = x4[2 * g + 1];
+            const float h[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+            stats(h, g);
+        }
+        __syncthreads();
+        const float mn_own = MN[tid];
+        // D_n(g) is kept for g = t and, while needed by wider levels, for
+        // g = t + 256 (g <= 255 + NQ - n)
+        float d0 = S[tid] + S[tid + 1];
+        float d1 = ((int)tid <= NQ - 3) ? S[tid + 256] + S[tid + 257] : 0.0f;
+#pragma unroll
+        for (int k = 4; k <= Q; ++k) {
+            const int n = 1 << (k - 3);
+            if (k > 4) {
+                const float* prev = DB[k & 1];
+                d0 = d0 + prev[tid + n / 2];
+                if ((int)tid <= NQ - n - 1) d1 = d1 + prev[tid + 256 + n / 2];
+            }
+            const float u = (d0 + MX[tid + n]) - mn_own;
+            const uint32_t f = !(u <= lim[k]) ? 1u : 0u;
+            cf[0] = cf[0] + cf[0] + f;
+            cf[1] = cf[1] + cf[1] + f;
+            if (k < Q) {
+                float* nxt = DB[(k + 1) & 1];
+                nxt[tid] = d0;
+                if ((int)tid <= NQ - n - 1) nxt[tid + 256] = d1;
+                __syncthreads();
+            }
+        }
+    }
+}
+
+// Runs pow2_flag_segments<q> for a runtime q in [Q, kPowPfMaxQ].
+template <int Q>
+__device__ __forceinline__ void pow2_flag_segments_at(int q, const float* x, float* aux, const float* lim,
+                                                      uint32_t (&cf)[2]) {
+    if (q == Q) {
+        pow2_flag_segments<Q>(x, aux, lim, cf);
+    } else {
+        if constexpr (Q < kPowPfMaxQ) pow2_flag_segments_at<Q + 1>(q, x, aux, lim, cf);
+    }
+}
+
+// One tile: the window [T, T + span) of row r is in x (la: a second buffer of
+// the same size, stage: kPowStage entries per wave), m is this thread's
+// max |x| over its share of the window, err_e the filter's error terms
+// (BoxErr::e, in LDS).  Finds the tile's candidates and
+// appends them to out.  Every thread of the block calls it; it ends with no
+// barrier after its last LDS access, so a caller reusing x, la or stage
+// synchronises first.
+template <int QMAX, bool SEG>
+__device__ __forceinline__ void pow2_tile(float* x, float* la, uint2* stage, float m, const float* row, int64_t T,
+                                          int span, bool vec, int q, float thr, const double* err_e, int64_t r,
+                                          CandD* __restrict__ out, int64_t cap,
+                                          unsigned long long* __restrict__ count) {
+    const unsigned tid = opaque_tid();
     __shared__ float wmaxabs[kThreads / 64];
     const int W = 1 << q;
-    const int span_max = round_up4(kPowCells + W - 1);
-    float* x = lds;                  // the samples, then every odd round's level
-    float* la = lds + span_max;      // every even round's level, then the queue
-    uint2* stage = reinterpret_cast<uint2*>(la + span_max);  // kPowStage per wave
-    const int64_t r = blockIdx.y;
-    const int64_t T = rotated_tile() * kPowCells;
-    const int span = (int)min((int64_t)(kPowCells + W - 1), cols - T);
-    const float* row = img + r * pitch;
-    const bool vec = (pitch & 3) == 0 && (reinterpret_cast<uintptr_t>(img) & 15) == 0;
-    float m = pow2_load_window(x, row + T, span, vec);
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-    if ((threadIdx.x & 63) == 0) wmaxabs[threadIdx.x >> 6] = m;
+    if ((tid & 63) == 0) wmaxabs[tid >> 6] = m;
     __syncthreads();
     float M = wmaxabs[0];
     for (int i = 1; i < kThreads / 64; ++i) M = fmaxf(M, wmaxabs[i]);
-    const float thr = d_thr ? *d_thr : h_thr;
     // the filter limits are block-uniform: thread k computes level k's once
     // (each is double-precision work that every wave would otherwise repeat)
     __shared__ float lim[kMaxPowQ + 1];
-    if ((int)threadIdx.x <= q) lim[threadIdx.x] = level_limit(thr, (int)threadIdx.x, err.e[threadIdx.x], M);
+    if ((int)tid <= q) lim[tid] = level_limit(thr, (int)tid, err_e[tid], M);
     __syncthreads();
 
     // Flag phase.  Flags are shifted in one bit per level, so level k ends at
@@ -530,19 +690,46 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
     // over the queue.
     uint32_t* queue = reinterpret_cast<uint32_t*>(la);
     __shared__ uint32_t slot_cnt[kPowPer][kThreads / 64];
-    const int lane = threadIdx.x & 63;
-    const int wid = threadIdx.x >> 6;
+    const int lane = tid & 63;
+    const int wid = tid >> 6;
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint32_t total = 0;
     // Per-cell flags where exactness needs them: the last tile of a row (its
     // window is cut by the row end, so level spans are checked) and windows
     // with NaN/inf samples or magnitudes where a level sum could overflow.
     const bool careful = span < kPowCells + W - 1 || !(M * (float)W < 0x1p125f);
-    if (!careful) {
+    if (SEG && !careful) {
+        // segment layout: thread t holds chunks 2t and 2t + 1 (cells 8t ..
+        // 8t + 7); a flagged chunk queues its four cells, ranked by (wave,
+        // lane, chunk)
+        uint32_t cf[2];
+        pow2_flag_segments_at<0>(q, x, la, lim, cf);
+        cf[0] = __brev(cf[0]) >> (31 - q);
+        cf[1] = __brev(cf[1]) >> (31 - q);
+        const uint64_t b0 = __ballot(cf[0] != 0), b1 = __ballot(cf[1] != 0);
+        if (lane == 0) slot_cnt[0][wid] = (uint32_t)(__popcll(b0) + __popcll(b1));
+        __syncthreads();
+        uint32_t pos = (uint32_t)(__popcll(b0 & below) + __popcll(b1 & below));
+        for (int k = 0; k < kThreads / 64; ++k) {
+            if (k < wid) pos += slot_cnt[0][k];
+            total += slot_cnt[0][k];
+        }
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            if (cf[s]) {
+                const uint32_t c0 = 8u * tid + 4u * (uint32_t)s;
+                reinterpret_cast<uint4*>(queue)[pos] =
+                    make_uint4((c0 << 14) | cf[s], ((c0 + 1) << 14) | cf[s], ((c0 + 2) << 14) | cf[s],
+                               ((c0 + 3) << 14) | cf[s]);
+                ++pos;
+            }
+        }
+        total *= 4;
+    } else if (!careful) {
         // chunk layout: slot s of thread t is chunk s*256 + t (4 cells); a
         // flagged chunk queues its four cells
         uint32_t cf[2];
-        pow2_flag_chunks(x, la, q, lim, cf);
+        pow2_flag_chunks_at<0, QMAX>(q, x, la, lim, cf);
         uint64_t mk[2];
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
@@ -560,7 +747,7 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
             }
             if (cf[s]) {
                 const uint32_t p = pos + (uint32_t)__popcll(mk[s] & below);
-                const uint32_t c0 = 4u * (uint32_t)(s * kThreads + threadIdx.x);
+                const uint32_t c0 = 4u * (uint32_t)(s * kThreads + tid);
                 reinterpret_cast<uint4*>(queue)[p] =
                     make_uint4((c0 << 14) | cf[s], ((c0 + 1) << 14) | cf[s], ((c0 + 2) << 14) | cf[s],
                                ((c0 + 3) << 14) | cf[s]);
@@ -591,13 +778,14 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
             }
             if (flags[j])
                 queue[pos + (uint32_t)__popcll(mk[j] & below)] =
-                    ((uint32_t)(j * kThreads + threadIdx.x) << 14) | flags[j];
+                    ((uint32_t)(j * kThreads + tid) << 14) | flags[j];
         }
     }
     if (total == 0) return;  // block-uniform
-    // from q = 4 on the second round stored its level over the samples:
-    // reload them (the block's window was just read, so this mostly hits L2)
-    if (q >= 4) pow2_load_window(x, row + T, span, vec);
+    // from q = 4 on the level rounds' second round stored its level over the
+    // samples: reload them (the block's window was just read, so this mostly
+    // hits L2); the segment filter leaves them in place
+    if (q >= 4 && (!SEG || careful)) pow2_load_window(x, row + T, span, vec);
     __syncthreads();
 
     // Exact recompute: lane i of a round takes queue entry i and runs the
@@ -630,7 +818,7 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
         wn = 0;
     };
     for (uint32_t q0 = 0; q0 < total; q0 += kThreads) {  // block-uniform
-        const uint32_t qi = q0 + threadIdx.x;
+        const uint32_t qi = q0 + tid;
         const uint32_t e = qi < total ? queue[qi] : 0u;
         const int lt = (int)(e >> 14);
         const uint32_t fl = e & 0x3fffu;
@@ -717,7 +905,7 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
     __shared__ unsigned long long bbase;
     if (lane == 0) wfill[wid] = wn;
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
         uint32_t tot = 0;
         for (int k = 0; k < kThreads / 64; ++k) tot += wfill[k];
         bbase = tot ? atomicAdd(count, (unsigned long long)tot) : 0ull;
@@ -730,6 +918,125 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
         const unsigned long long g = base + i;
         if ((int64_t)g < cap)
             out[g] = CandD{(int32_t)(T + (v.x >> 4)), (int32_t)r, (int32_t)(v.x & 15u), __uint_as_float(v.y)};
+    }
+}
+
+// One block = one 2048-cell tile of one row, any q <= kMaxPowQ (window in
+// dynamic LDS: samples, level buffer, candidate stage).
+__global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restrict__ img, int64_t cols, int64_t pitch,
+                                                          const float* __restrict__ d_thr, float h_thr, int q,
+                                                          BoxErr err, CandD* __restrict__ out, int64_t cap,
+                                                          unsigned long long* __restrict__ count) {
+    extern __shared__ __attribute__((aligned(16))) float lds[];
+    const int W = 1 << q;
+    const int span_max = round_up4(kPowCells + W - 1);
+    float* x = lds;                  // the samples, then every odd round's level
+    float* la = lds + span_max;      // every even round's level, then the queue
+    uint2* stage = reinterpret_cast<uint2*>(la + span_max);  // kPowStage per wave
+    const int64_t r = blockIdx.y;
+    const int64_t T = rotated_tile() * kPowCells;
+    const int span = (int)min((int64_t)(kPowCells + W - 1), cols - T);
+    const float* row = img + r * pitch;
+    const bool vec = (pitch & 3) == 0 && (reinterpret_cast<uintptr_t>(img) & 15) == 0;
+    __shared__ double err_e[kMaxPowQ + 1];
+    if ((int)threadIdx.x <= q) err_e[threadIdx.x] = err.e[threadIdx.x];  // read after pow2_tile's first barrier
+    const float m = pow2_load_window(x, row + T, span, vec);
+    pow2_tile<kMaxPowQ, false>(x, la, stage, m, row, T, span, vec, q, d_thr ? *d_thr : h_thr, err_e, r, out, cap,
+                               count);
+}
+
+// Windows up to q = kPowPfMaxQ in static LDS: two sample buffers, so that the
+// next tile's window is fetched while the current one is searched.
+constexpr int kPowPfTiles = 4;  // consecutive tiles of a row per block
+
+// Starts copying floats [0, span4) of a window (span4 % 4 == 0, src 16-byte
+// aligned, all of it inside the row) to LDS with global_load_lds_dwordx4:
+// the memory unit writes lane l's 16 bytes at M0 + 16 l (M0: the wave's LDS
+// byte address), so no registers are held while the loads are in flight.
+// Issued as inline asm, so the compiler does not count these LDS writes:
+// its barriers and LDS accesses then wait for nothing of them (the builtin
+// would make every later barrier wait for the whole window).  The caller
+// waits itself: complete once this wave's vmcnt is zero, visible to the
+// block after a barrier that follows, and nothing reads or writes dst
+// before then.
+__device__ __forceinline__ void pow2_issue_window(float* dst, const float* src, int span4) {
+    const unsigned tid = opaque_tid();
+    const int n4 = span4 >> 2;
+    const int wbase = (int)(tid & ~63u);
+    for (int j0 = 0; j0 < n4; j0 += kThreads) {  // block-uniform
+        const int i = j0 + (int)tid;
+        const uint32_t m0 = __builtin_amdgcn_readfirstlane(
+            (uint32_t)(uintptr_t)(__attribute__((address_space(3))) float*)(dst + 4 * (j0 + wbase)));
+        const float* g = src + 4 * i;
+        // M0 is saved and restored around the load; one wait state between
+        // the SALU write of M0 and the load that reads it
+        uint32_t saved;
+        if (i < n4)
+            asm volatile("s_mov_b32 %0, m0\n\ts_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %2, off\n\t"
+                         "s_mov_b32 m0, %0"
+                         : "=&s"(saved)
+                         : "s"(m0), "v"(g));
+    }
+}
+
+// This thread's max |x| over float4 units tid, tid + 256, ... of x[0, span4)
+// (+inf once it saw a NaN), as pow2_load_window reports it.
+__device__ __forceinline__ float pow2_window_max(const float* x, int span4) {
+    const unsigned tid = opaque_tid();
+    float m = 0.0f;
+    for (int i = tid; i < (span4 >> 2); i += kThreads) {
+        const float4 v = reinterpret_cast<const float4*>(x)[i];
+        m = fmaxf(m, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+        if (v.x != v.x || v.y != v.y || v.z != v.z || v.w != v.w) m = __int_as_float(0x7f800000);
+    }
+    return m;
+}
+
+// One block = kPowPfTiles consecutive tiles of one row (q <= kPowPfMaxQ).  A
+// tile whose whole rounded window lies inside a 16-byte aligned row is
+// fetched by global_load_lds, issued when the previous tile starts; the row's
+// last tiles are loaded directly.  M may then include the one sample after
+// the window (a larger bound: the filter only gets more conservative).
+__global__ void __launch_bounds__(kThreads) k_boxcar_pow2_pf(const float* __restrict__ img, int64_t cols,
+                                                             int64_t pitch, const float* __restrict__ d_thr,
+                                                             float h_thr, int q, BoxErr err,
+                                                             CandD* __restrict__ out, int64_t cap,
+                                                             unsigned long long* __restrict__ count) {
+    __shared__ __attribute__((aligned(16))) float xa[kPowPfSpan];
+    __shared__ __attribute__((aligned(16))) float xb[kPowPfSpan];
+    __shared__ __attribute__((aligned(16))) float la[kPowPfSpan];
+    __shared__ __attribute__((aligned(16))) uint2 stage[kPowStage * (kThreads / 64)];
+    __shared__ double err_e[kPowPfMaxQ + 1];
+    if ((int)threadIdx.x <= q) err_e[threadIdx.x] = err.e[threadIdx.x];  // read after the first tile's barriers
+    const int full = kPowCells + (1 << q) - 1;
+    const int full4 = round_up4(full);
+    const int64_t r = blockIdx.y;
+    const int64_t ntiles = ceil_div(cols, (int64_t)kPowCells);
+    const int64_t k0 = rotated_tile() * kPowPfTiles;
+    const int64_t k1 = min(ntiles, k0 + (int64_t)kPowPfTiles);
+    const float* row = img + r * pitch;
+    const bool vec = (pitch & 3) == 0 && (reinterpret_cast<uintptr_t>(img) & 15) == 0;
+    const float thr = d_thr ? *d_thr : h_thr;
+    auto dma = [&](int64_t k) { return vec && k * kPowCells + full4 <= cols; };  // monotone in k
+    auto step = [&](float* cur, float* nxt, int64_t k) {
+        const int64_t T = k * kPowCells;
+        const int span = (int)min((int64_t)full, cols - T);
+        float m;
+        if (dma(k)) {
+            asm volatile("s_waitcnt vmcnt(0)" : : : "memory");  // this wave's part of the window has landed
+            __syncthreads();
+            if (k + 1 < k1 && dma(k + 1)) pow2_issue_window(nxt, row + T + kPowCells, full4);
+            m = pow2_window_max(cur, full4);
+        } else {
+            __syncthreads();
+            m = pow2_load_window(cur, row + T, span, vec);
+        }
+        pow2_tile<kPowPfMaxQ, true>(cur, la, stage, m, row, T, span, vec, q, thr, err_e, r, out, cap, count);
+    };
+    if (k0 < k1 && dma(k0)) pow2_issue_window(xa, row + k0 * kPowCells, full4);
+    for (int64_t k = k0; k < k1; ++k) {  // block-uniform
+        const bool odd = ((k - k0) & 1) != 0;
+        step(odd ? xb : xa, odd ? xa : xb, k);
     }
 }
 
@@ -891,7 +1198,7 @@ void launch_boxcar(const float* img, int64_t rows, int64_t cols, int64_t pitch, 
     if (pow2) {
         int q = 0;
         while ((1 << q) < wmax) ++q;
-        if ((1 << q) != wmax || q > 13) throw Error(SPS_ERR_INVALID_ARGUMENT, "pow2 window must be 2^q, q <= 13");
+        if ((1 << q) != wmax || q > kMaxPowQ) throw Error(SPS_ERR_INVALID_ARGUMENT, "pow2 window must be 2^q, q <= 13");
         const dim3 grid((uint32_t)ceil_div(cols, kPowCells), (uint32_t)rows);
         const size_t lds = sizeof(float) * 2 * (size_t)round_up4(kPowCells + wmax - 1) +
                            sizeof(uint2) * (size_t)kPowStage * (kThreads / 64);
@@ -902,10 +1209,20 @@ void launch_boxcar(const float* img, int64_t rows, int64_t cols, int64_t pitch, 
         auto gam = [&](double n) { return n * u / (1.0 - n * u); };
         for (int k = 0; k <= kMaxPowQ; ++k) {
             const double w = std::ldexp(1.0, k);
-            err.e[k] = (gam(w - 1.0) + gam((double)k)) * w * (1.0 + 0x1p-20);
+            // pairwise level sums: gamma_k w; the segment bound U of widths
+            // >= 16 (segment sums of 8 under a depth k-3 tree, two prefix
+            // extremes of <= 7 terms, two more roundings): gamma_{k+6} (w + 16)
+            const double ours = k >= 4 ? gam((double)k + 6.0) * (w + 16.0) : gam((double)k) * w;
+            err.e[k] = (gam(w - 1.0) * w + ours) * (1.0 + 0x1p-20);
         }
-        hipLaunchKernelGGL(k_boxcar_pow2, grid, dim3(kThreads), lds, st, img, cols, pitch, d_thr, h_thr, q, err, out,
-                           cap, d_count);
+        if (q <= kPowPfMaxQ) {
+            const dim3 gpf((uint32_t)ceil_div(ceil_div(cols, kPowCells), (int64_t)kPowPfTiles), (uint32_t)rows);
+            hipLaunchKernelGGL(k_boxcar_pow2_pf, gpf, dim3(kThreads), 0, st, img, cols, pitch, d_thr, h_thr, q, err,
+                               out, cap, d_count);
+        } else {
+            hipLaunchKernelGGL(k_boxcar_pow2, grid, dim3(kThreads), lds, st, img, cols, pitch, d_thr, h_thr, q, err,
+                               out, cap, d_count);
+        }
     } else {
         if (wmax > 8192) throw Error(SPS_ERR_INVALID_ARGUMENT, "window wider than 8192 samples");
         const dim3 grid((uint32_t)ceil_div(cols, kBoxCells), (uint32_t)rows);
