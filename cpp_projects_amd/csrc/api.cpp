@@ -381,6 +381,9 @@ struct sps_fdmt {
                 SPS_REQUIRE(ms <= INT32_MAX, "shift too large");
                 d.maxshift = (int32_t)ms;
                 for (int k = 0; k < NL; ++k) d.off[k] = offs[(size_t)rr * NL + k];
+                d.full = 1;
+                for (int k = 1; k < NL; ++k)
+                    if (d.off[k] < 0) d.full = 0;
                 rows.push_back(d);
             }
         }

@@ -8,6 +8,8 @@
 // one contiguous range of (time tile, row) work, row fastest.  Rows that
 // share an input row (neighbouring dt of one sub-band) then run at the same
 // time on the same XCD and the shared row is served from that XCD's L2.
+#include <type_traits>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -257,17 +259,22 @@ struct LdsTree {
 // Branch-free, so all 2^L leaf reads can be in flight at once: a missing
 // head (a copied row, off < 0) is still read, at offset 0 from leaves whose
 // base is 0, and dropped by the select; the kept additions are those of the
-// separate iterations.
-template <int L, int K>
+// separate iterations.  FULL: the row has no missing head (GRowDesc::full),
+// so no select is needed.
+template <int L, int K, bool FULL = false>
 struct LdsTreeFast {
     __device__ __forceinline__ static float eval(const float* __restrict__ lds, const GRowDesc<L>& d, int u) {
         if constexpr (K >= (1 << L)) {
             return lds[d.lbase[K - (1 << L)] + u];
         } else {
             const int32_t off = d.off[K];
-            const float a = LdsTreeFast<L, 2 * K>::eval(lds, d, u);
-            const float b = LdsTreeFast<L, 2 * K + 1>::eval(lds, d, u - (off >= 0 ? off : 0));
-            return off >= 0 ? a + b : a;
+            const float a = LdsTreeFast<L, 2 * K, FULL>::eval(lds, d, u);
+            if constexpr (FULL) {
+                return a + LdsTreeFast<L, 2 * K + 1, FULL>::eval(lds, d, u - off);
+            } else {
+                const float b = LdsTreeFast<L, 2 * K + 1, FULL>::eval(lds, d, u - (off >= 0 ? off : 0));
+                return off >= 0 ? a + b : a;
+            }
         }
     }
 };
@@ -296,50 +303,57 @@ __device__ __forceinline__ void group_trees(const float* __restrict__ lds, const
         const float* nrow = SNR ? epi.nrm + d.out_row * epi.nrm_len : nullptr;
         if (T >= d.maxshift && T + kGroupTile <= nsamps) {
             // the whole tile is past every shift: no per-sample tests
-            if (SNR && T < epi.nrm_len) {
-                // within the mask transform's extent: per-sample divisor there
+            auto fast = [&](auto full_tag) {
+                constexpr bool FULL = decltype(full_tag)::value;
+                if (SNR && T < epi.nrm_len) {
+                    // within the mask transform's extent: per-sample divisor there
 #pragma unroll 4
-                for (int j = 0; j < kGroupTile / 64; ++j) {
-                    const int u = j * 64 + lane;
-                    const int64_t t = T + u;
-                    const float v = LdsTreeFast<L, 1>::eval(lds, d, u);
-                    o[t] = (t < epi.nrm_len) ? v / sqrtf((float)((double)nrow[t] + 1.0E-8)) : div_rn(v, dk, rdk);
-                }
-            } else if constexpr (SNR) {
-                // div_rn(v, dk, rdk) four samples at a time: dk = sqrtf(kept +
-                // 1e-8) lies in [1e-4, 256], so only the dividends need the
-                // range test, and the IEEE division runs in one rare branch
-#pragma unroll
-                for (int j0 = 0; j0 < kGroupTile / 64; j0 += 4) {
-                    float v[4], qv[4];
-                    bool slow = false;
-#pragma unroll
-                    for (int jj = 0; jj < 4; ++jj) v[jj] = LdsTreeFast<L, 1>::eval(lds, d, (j0 + jj) * 64 + lane);
-#pragma unroll
-                    for (int jj = 0; jj < 4; ++jj) {
-                        const float ax = fabsf(v[jj]);
-                        slow = slow || !(ax >= 0x1p-60f && ax <= 0x1p60f);
-                        const float q0 = v[jj] * rdk;
-                        const float e = fmaf(-q0, dk, v[jj]);
-                        qv[jj] = fmaf(e, rdk, q0);
+                    for (int j = 0; j < kGroupTile / 64; ++j) {
+                        const int u = j * 64 + lane;
+                        const int64_t t = T + u;
+                        const float v = LdsTreeFast<L, 1, FULL>::eval(lds, d, u);
+                        o[t] = (t < epi.nrm_len) ? v / sqrtf((float)((double)nrow[t] + 1.0E-8)) : div_rn(v, dk, rdk);
                     }
-                    if (slow) {
+                } else if constexpr (SNR) {
+                    // div_rn(v, dk, rdk) four samples at a time: dk = sqrtf(kept +
+                    // 1e-8) lies in [1e-4, 256], so only the dividends need the
+                    // range test, and the IEEE division runs in one rare branch
+#pragma unroll
+                    for (int j0 = 0; j0 < kGroupTile / 64; j0 += 4) {
+                        float v[4], qv[4];
+                        bool slow = false;
+#pragma unroll
+                        for (int jj = 0; jj < 4; ++jj) v[jj] = LdsTreeFast<L, 1, FULL>::eval(lds, d, (j0 + jj) * 64 + lane);
 #pragma unroll
                         for (int jj = 0; jj < 4; ++jj) {
                             const float ax = fabsf(v[jj]);
-                            if (!(ax >= 0x1p-60f && ax <= 0x1p60f)) qv[jj] = v[jj] / dk;
+                            slow = slow || !(ax >= 0x1p-60f && ax <= 0x1p60f);
+                            const float q0 = v[jj] * rdk;
+                            const float e = fmaf(-q0, dk, v[jj]);
+                            qv[jj] = fmaf(e, rdk, q0);
                         }
-                    }
+                        if (slow) {
 #pragma unroll
-                    for (int jj = 0; jj < 4; ++jj) o[T + (j0 + jj) * 64 + lane] = qv[jj];
-                }
-            } else {
+                            for (int jj = 0; jj < 4; ++jj) {
+                                const float ax = fabsf(v[jj]);
+                                if (!(ax >= 0x1p-60f && ax <= 0x1p60f)) qv[jj] = v[jj] / dk;
+                            }
+                        }
+#pragma unroll
+                        for (int jj = 0; jj < 4; ++jj) o[T + (j0 + jj) * 64 + lane] = qv[jj];
+                    }
+                } else {
 #pragma unroll 4
-                for (int j = 0; j < kGroupTile / 64; ++j) {
-                    const int u = j * 64 + lane;
-                    o[T + u] = LdsTreeFast<L, 1>::eval(lds, d, u);
+                    for (int j = 0; j < kGroupTile / 64; ++j) {
+                        const int u = j * 64 + lane;
+                        o[T + u] = LdsTreeFast<L, 1, FULL>::eval(lds, d, u);
+                    }
                 }
-            }
+            };
+            if (d.full)
+                fast(std::true_type{});
+            else
+                fast(std::false_type{});
             continue;
         }
 #pragma unroll 4

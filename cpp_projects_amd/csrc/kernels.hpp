@@ -84,7 +84,7 @@ template <int L>
 struct GRowDesc {
     int64_t out_row;
     int32_t maxshift;  // largest composite shift of a leaf: for t >= maxshift every merge adds its head
-    int32_t pad;
+    int32_t full;      // 1: no missing head (off[k] >= 0 for k = 1 .. 2^L - 1)
     int32_t lbase[1 << L];
     int32_t off[1 << L];
 };
