@@ -100,6 +100,7 @@ struct sps_fdmt {
         DevBuf groups, leaves, rows, unit_leaf;
         int64_t ngroups = 0;
         int lds_floats = 0;
+        int64_t max_shift = 0;  // largest GRowDesc::maxshift
         bool init = false;     // leaves are ChanWinD (init fused)
     };
     std::vector<std::unique_ptr<StagedPass>> d_staged;  // per pass
@@ -266,6 +267,7 @@ struct sps_fdmt {
             if (live[(size_t)top][(size_t)r]) order.push_back(r);
         const int64_t nr = (int64_t)order.size();
         int maxf = 0;
+        int64_t max_shift = 0;
         int64_t q = 0;
         while (q < nr) {
             std::map<int64_t, std::pair<int64_t, int64_t>> S;  // leaf row -> (smin, smax)
@@ -380,6 +382,7 @@ struct sps_fdmt {
                 }
                 SPS_REQUIRE(ms <= INT32_MAX, "shift too large");
                 d.maxshift = (int32_t)ms;
+                max_shift = std::max(max_shift, ms);
                 for (int k = 0; k < NL; ++k) d.off[k] = offs[(size_t)rr * NL + k];
                 d.full = 1;
                 for (int k = 1; k < NL; ++k)
@@ -396,6 +399,7 @@ struct sps_fdmt {
         upload(sp.unit_leaf, unit_leaf, st);
         sp.ngroups = (int64_t)groups.size();
         sp.lds_floats = maxf;
+        sp.max_shift = max_shift;
         sp.init = init_leaves;
         return true;
     }
@@ -561,7 +565,7 @@ struct sps_fdmt {
                 const StagedPass& sp = *d_staged[k];
                 launch_fdmt_group(L, src, p, dst, last ? out_pitch : p, sp.groups.p, sp.ngroups, sp.leaves.p,
                                   sp.rows.p, sp.unit_leaf.as<uint8_t>(), sp.lds_floats, group_tiles, nsamps,
-                                  last ? eng.out_rows : INT64_MAX, st, last ? snr : nullptr);
+                                  last ? eng.out_rows : INT64_MAX, sp.max_shift, st, last ? snr : nullptr);
             } else {
                 launch_fdmt_fused(L, src, p, dst, last ? out_pitch : p, d_trees[k]->p, tree_rows[k], nsamps,
                                   last ? eng.out_rows : INT64_MAX, st, last ? snr : nullptr);

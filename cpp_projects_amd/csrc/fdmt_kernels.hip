@@ -283,7 +283,11 @@ struct LdsTreeFast {
 // wave w takes rows w, w + 4, ... over the whole tile (8 samples per lane), so
 // the per-row scalar work (descriptor, output address, path test) is done
 // once per row rather than once per wave
-template <int L, bool SNR>
+// FAST: the caller guarantees that the tile is whole and past every row's
+// shifts (T >= maxshift) and, with SNR, past the mask transform's extent, so
+// only the branch-free path is compiled (the general one's registers made the
+// hot loop spill).
+template <int L, bool SNR, bool FAST = false>
 __device__ __forceinline__ void group_trees(const float* __restrict__ lds, const GroupDescD& g,
                                             const GRowDesc<L>* __restrict__ rows, float* __restrict__ out,
                                             int64_t out_pitch, int64_t T, int64_t nsamps, int64_t out_rows_limit,
@@ -301,11 +305,11 @@ __device__ __forceinline__ void group_trees(const float* __restrict__ lds, const
         if (d.out_row >= out_rows_limit) continue;
         float* __restrict__ o = out + d.out_row * out_pitch;
         const float* nrow = SNR ? epi.nrm + d.out_row * epi.nrm_len : nullptr;
-        if (T >= d.maxshift && T + kGroupTile <= nsamps) {
+        if (FAST || (T >= d.maxshift && T + kGroupTile <= nsamps)) {
             // the whole tile is past every shift: no per-sample tests
             auto fast = [&](auto full_tag) {
                 constexpr bool FULL = decltype(full_tag)::value;
-                if (SNR && T < epi.nrm_len) {
+                if (!FAST && SNR && T < epi.nrm_len) {
                     // within the mask transform's extent: per-sample divisor there
 #pragma unroll 4
                     for (int j = 0; j < kGroupTile / 64; ++j) {
@@ -356,27 +360,31 @@ __device__ __forceinline__ void group_trees(const float* __restrict__ lds, const
                 fast(std::false_type{});
             continue;
         }
+        if constexpr (!FAST) {
 #pragma unroll 4
-        for (int j = 0; j < kGroupTile / 64; ++j) {
-            const int64_t t = T + j * 64 + lane;
-            if (t < nsamps) {
-                float v = LdsTree<L, 1>::eval(lds, d, t, T);
-                if (SNR)
-                    v = (t < epi.nrm_len) ? v / sqrtf((float)((double)nrow[t] + 1.0E-8)) : div_rn(v, dk, rdk);
-                o[t] = v;
+            for (int j = 0; j < kGroupTile / 64; ++j) {
+                const int64_t t = T + j * 64 + lane;
+                if (t < nsamps) {
+                    float v = LdsTree<L, 1>::eval(lds, d, t, T);
+                    if (SNR)
+                        v = (t < epi.nrm_len) ? v / sqrtf((float)((double)nrow[t] + 1.0E-8)) : div_rn(v, dk, rdk);
+                    o[t] = v;
+                }
             }
         }
     }
 }
 
-// One block = one group x tiles_per_block consecutive tiles.  Every float4
+// One block = one group x tiles_per_block consecutive tiles of [tile_lo,
+// tile_hi) (FAST: tiles where group_trees may take its fast path for every
+// row, see there).  Every float4
 // unit of the group's LDS windows is a fixed (leaf row, offset) pair, so each
 // thread works out its unit slots once; per tile it issues the loads of the
 // next tile into registers, evaluates the current tile from one LDS buffer,
 // then commits the registers to the other buffer (one barrier per tile).
 constexpr int kStageSlots = kGroupMaxUnits / kThreads;
 
-template <int L, bool SNR>
+template <int L, bool SNR, bool FAST>
 __global__ void __launch_bounds__(kThreads) k_merge_group(const float* __restrict__ in, int64_t in_pitch,
                                                           float* __restrict__ out, int64_t out_pitch,
                                                           const GroupDescD* __restrict__ groups, int64_t ngroups,
@@ -384,15 +392,14 @@ __global__ void __launch_bounds__(kThreads) k_merge_group(const float* __restric
                                                           const GRowDesc<L>* __restrict__ rows,
                                                           const uint8_t* __restrict__ unit_leaf, int tiles_per_block,
                                                           int lds_stride, int64_t nsamps, int64_t out_rows_limit,
-                                                          SnrEpi epi) {
+                                                          SnrEpi epi, int64_t tile_lo, int64_t tile_hi) {
     extern __shared__ __attribute__((aligned(16))) float lds[];
     const uint32_t logical = xcd_remap(blockIdx.x, gridDim.x);
     const int64_t gi = (int64_t)(logical % (uint64_t)ngroups);
     const int64_t chunk = (int64_t)(logical / (uint64_t)ngroups);
     const GroupDescD g = groups[gi];
-    const int64_t ntiles = ceil_div(nsamps, kGroupTile);
-    const int64_t k0 = chunk * tiles_per_block;
-    const int64_t k1 = min(ntiles, k0 + tiles_per_block);
+    const int64_t k0 = tile_lo + chunk * tiles_per_block;
+    const int64_t k1 = min(tile_hi, k0 + tiles_per_block);
     if (k0 >= k1) return;  // block-uniform
     int64_t src[kStageSlots];  // element offset of the slot's unit at T = 0
     int32_t tk[kStageSlots];   // its time at T = 0; INT32_MIN for an unused slot
@@ -432,7 +439,7 @@ __global__ void __launch_bounds__(kThreads) k_merge_group(const float* __restric
         float* nxt = lds + ((k - k0 + 1) & 1) * lds_stride;
         const bool more = k + 1 < k1;
         if (more) issue((k + 1) * kGroupTile);
-        group_trees<L, SNR>(cur, g, rows, out, out_pitch, k * kGroupTile, nsamps, out_rows_limit, epi);
+        group_trees<L, SNR, FAST>(cur, g, rows, out, out_pitch, k * kGroupTile, nsamps, out_rows_limit, epi);
         if (more) commit(nxt);
         __syncthreads();
     }
@@ -979,7 +986,6 @@ void launch_fdmt_fused(int L, const float* in, int64_t in_pitch, float* out, int
 }
 
 struct GroupLaunch {
-    uint32_t nb;
     size_t lds;
     const float* in;
     int64_t in_pitch;
@@ -997,35 +1003,46 @@ struct GroupLaunch {
     SnrEpi epi;
 };
 
-template <int L>
-static void group_dispatch(bool snr, const GroupLaunch& a, hipStream_t st) {
-    const GroupDescD* gp = static_cast<const GroupDescD*>(a.groups);
-    const LeafDescD* lp = static_cast<const LeafDescD*>(a.leaves);
-    const GRowDesc<L>* rp = static_cast<const GRowDesc<L>*>(a.rows);
-    if (snr)
-        hipLaunchKernelGGL((k_merge_group<L, true>), dim3(a.nb), dim3(kThreads), a.lds, st, a.in, a.in_pitch, a.out,
-                           a.out_pitch, gp, a.ngroups, lp, rp, a.unit_leaf, a.tiles_per_block, a.lds_stride, a.nsamps,
-                           a.limit, a.epi);
-    else
-        hipLaunchKernelGGL((k_merge_group<L, false>), dim3(a.nb), dim3(kThreads), a.lds, st, a.in, a.in_pitch, a.out,
-                           a.out_pitch, gp, a.ngroups, lp, rp, a.unit_leaf, a.tiles_per_block, a.lds_stride, a.nsamps,
-                           a.limit, a.epi);
+// one launch over tiles [lo, hi) of every group
+template <int L, bool SNR, bool FAST>
+static void group_launch(const GroupLaunch& a, int64_t lo, int64_t hi, hipStream_t st) {
+    if (hi <= lo) return;
+    const int64_t nb = a.ngroups * ceil_div(hi - lo, (int64_t)a.tiles_per_block);
+    if (nb > 0x7fffffffLL) throw Error(SPS_ERR_INVALID_ARGUMENT, "merge grid too large");
+    hipLaunchKernelGGL((k_merge_group<L, SNR, FAST>), dim3((uint32_t)nb), dim3(kThreads), a.lds, st, a.in, a.in_pitch,
+                       a.out, a.out_pitch, static_cast<const GroupDescD*>(a.groups), a.ngroups,
+                       static_cast<const LeafDescD*>(a.leaves), static_cast<const GRowDesc<L>*>(a.rows), a.unit_leaf,
+                       a.tiles_per_block, a.lds_stride, a.nsamps, a.limit, a.epi, lo, hi);
+}
+
+// Tiles [fast_lo, full) take the fast-only kernel (fast_lo: the first tile at
+// or past every row's largest shift and, with the S/N epilogue, the mask
+// transform's extent; full: the whole tiles); the others the general one.
+template <int L, bool SNR>
+static void group_dispatch(const GroupLaunch& a, int64_t fast_lo, hipStream_t st) {
+    const int64_t ntiles = ceil_div(a.nsamps, kGroupTile);
+    const int64_t full = a.nsamps / kGroupTile;
+    if (fast_lo < full) {
+        group_launch<L, SNR, false>(a, 0, fast_lo, st);
+        group_launch<L, SNR, true>(a, fast_lo, full, st);
+        group_launch<L, SNR, false>(a, full, ntiles, st);
+    } else {
+        group_launch<L, SNR, false>(a, 0, ntiles, st);
+    }
 }
 
 void launch_fdmt_group(int L, const float* in, int64_t in_pitch, float* out, int64_t out_pitch, const void* d_groups,
                        int64_t ngroups, const void* d_leaves, const void* d_rows, const uint8_t* d_unit_leaf,
                        int lds_floats_max, int tiles_per_block, int64_t nsamps, int64_t out_rows_limit,
-                       hipStream_t st, const MergeEpilogue* snr) {
+                       int64_t max_shift, hipStream_t st, const MergeEpilogue* snr) {
     if (ngroups <= 0 || nsamps <= 0) return;
     if (in_pitch % 4 != 0 || reinterpret_cast<uintptr_t>(in) % 16 != 0)
         throw Error(SPS_ERR_INVALID_ARGUMENT, "staged pass needs a 16-byte aligned input state");
     if (lds_floats_max <= 0 || lds_floats_max > 4 * kGroupMaxUnits)
         throw Error(SPS_ERR_INVALID_ARGUMENT, "staged pass LDS window out of range");
     if (tiles_per_block < 1) throw Error(SPS_ERR_INVALID_ARGUMENT, "tiles per block must be positive");
-    const int64_t nb = ngroups * ceil_div(ceil_div(nsamps, kGroupTile), tiles_per_block);
-    if (nb > 0x7fffffffLL) throw Error(SPS_ERR_INVALID_ARGUMENT, "merge grid too large");
+    if (max_shift < 0) throw Error(SPS_ERR_INVALID_ARGUMENT, "negative row shift");
     GroupLaunch a{};
-    a.nb = (uint32_t)nb;
     a.lds = sizeof(float) * 2 * (size_t)lds_floats_max;  // two tile buffers
     a.in = in;
     a.in_pitch = in_pitch;
@@ -1042,19 +1059,20 @@ void launch_fdmt_group(int L, const float* in, int64_t in_pitch, float* out, int
     a.limit = out_rows_limit;
     a.epi = SnrEpi{nullptr, 0, nullptr};
     if (snr) a.epi = SnrEpi{snr->nrm, snr->nrm_len, snr->stats};
-    const bool s = snr != nullptr;
+    const int64_t ext = (snr && snr->nrm_len > max_shift) ? snr->nrm_len : max_shift;
+    const int64_t fast_lo = ceil_div(ext, (int64_t)kGroupTile);
     switch (L) {
         case 1:
-            group_dispatch<1>(s, a, st);
+            snr ? group_dispatch<1, true>(a, fast_lo, st) : group_dispatch<1, false>(a, fast_lo, st);
             break;
         case 2:
-            group_dispatch<2>(s, a, st);
+            snr ? group_dispatch<2, true>(a, fast_lo, st) : group_dispatch<2, false>(a, fast_lo, st);
             break;
         case 3:
-            group_dispatch<3>(s, a, st);
+            snr ? group_dispatch<3, true>(a, fast_lo, st) : group_dispatch<3, false>(a, fast_lo, st);
             break;
         case 4:
-            group_dispatch<4>(s, a, st);
+            snr ? group_dispatch<4, true>(a, fast_lo, st) : group_dispatch<4, false>(a, fast_lo, st);
             break;
         default:
             throw Error(SPS_ERR_INVALID_ARGUMENT, "staged fusion depth must be 1..4");
