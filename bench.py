@@ -400,7 +400,9 @@ def main():
         if k == 0 and fused_init:
             return f"k_merge_group_init<{L}, 1, true>"
         if staged and not (last and staged in (1, 2) and L <= 3):
-            return f"k_merge_group<{L}, {'true' if last else 'false'}>"
+            # a staged pass is two launches: <L, SNR, true> over the whole
+            # tiles past every row's shifts, <L, SNR, false> over the rest
+            return f"k_merge_group<{L}, {'true' if last else 'false'}, *>"
         return f"k_merge_fused<{L}, true, {'true' if last else 'false'}>"
 
     # dominant kernel: the non-final passes of the kernel instantiation with
@@ -461,7 +463,8 @@ def main():
             "compulsory_io_GBs": avg_io / (avg_ms * 1e-3) / 1e9,
             "note": "achieved credits the unfused byte model (every state row read and written once per "
                     "iteration); traffic is the counter-measured HBM bytes of one launch (rocprofv3, "
-                    "profiles/merge_traffic.json)",
+                    "profiles/merge_traffic.json); for a staged pass a 'launch' is its two kernel launches "
+                    "(fast-path tiles and the first tiles), timed together by the stage events",
         },
         "stages_ms": stages,
         "effective_GBs_whole_step": total_alg / (sum(stage_ms) * 1e-3) / 1e9,
