@@ -331,6 +331,11 @@ def main():
         torch.cuda.synchronize()
         log(f"[bench] calibration copy of {args.calib_copy_mib} MiB done")
         del a, b
+    if not args.tune:  # the measured configuration is also the warmed-up one
+        s.set_staged(int(args.staged))
+        if int(args.staged):
+            s.set_group_shape(*[int(v) for v in args.group_shape.split(",")])
+        s.set_schedule([int(v) for v in args.schedule.split(",")])
     log(f"[bench] ndt={ndt}; warmup {args.warmup}")
     for _ in range(args.warmup):
         s.run(x, dmt, cands, count)

@@ -1003,16 +1003,16 @@ struct GroupLaunch {
     SnrEpi epi;
 };
 
-// one launch over tiles [lo, hi) of every group
+// one launch over tiles [lo, hi) of every group, tpb consecutive tiles per block
 template <int L, bool SNR, bool FAST>
-static void group_launch(const GroupLaunch& a, int64_t lo, int64_t hi, hipStream_t st) {
+static void group_launch(const GroupLaunch& a, int64_t lo, int64_t hi, int tpb, hipStream_t st) {
     if (hi <= lo) return;
-    const int64_t nb = a.ngroups * ceil_div(hi - lo, (int64_t)a.tiles_per_block);
+    const int64_t nb = a.ngroups * ceil_div(hi - lo, (int64_t)tpb);
     if (nb > 0x7fffffffLL) throw Error(SPS_ERR_INVALID_ARGUMENT, "merge grid too large");
     hipLaunchKernelGGL((k_merge_group<L, SNR, FAST>), dim3((uint32_t)nb), dim3(kThreads), a.lds, st, a.in, a.in_pitch,
                        a.out, a.out_pitch, static_cast<const GroupDescD*>(a.groups), a.ngroups,
                        static_cast<const LeafDescD*>(a.leaves), static_cast<const GRowDesc<L>*>(a.rows), a.unit_leaf,
-                       a.tiles_per_block, a.lds_stride, a.nsamps, a.limit, a.epi, lo, hi);
+                       tpb, a.lds_stride, a.nsamps, a.limit, a.epi, lo, hi);
 }
 
 // Tiles [fast_lo, full) take the fast-only kernel (fast_lo: the first tile at
@@ -1023,11 +1023,13 @@ static void group_dispatch(const GroupLaunch& a, int64_t fast_lo, hipStream_t st
     const int64_t ntiles = ceil_div(a.nsamps, kGroupTile);
     const int64_t full = a.nsamps / kGroupTile;
     if (fast_lo < full) {
-        group_launch<L, SNR, false>(a, 0, fast_lo, st);
-        group_launch<L, SNR, true>(a, fast_lo, full, st);
-        group_launch<L, SNR, false>(a, full, ntiles, st);
+        // the few first and last tiles: one per block, so that they spread
+        // over the GPU instead of queueing in one block per group
+        group_launch<L, SNR, false>(a, 0, fast_lo, 1, st);
+        group_launch<L, SNR, true>(a, fast_lo, full, a.tiles_per_block, st);
+        group_launch<L, SNR, false>(a, full, ntiles, 1, st);
     } else {
-        group_launch<L, SNR, false>(a, 0, ntiles, st);
+        group_launch<L, SNR, false>(a, 0, ntiles, a.tiles_per_block, st);
     }
 }
 
