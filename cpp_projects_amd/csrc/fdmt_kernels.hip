@@ -306,7 +306,10 @@ __device__ __forceinline__ void group_trees(const float* __restrict__ lds, const
         float* __restrict__ o = out + d.out_row * out_pitch;
         const float* nrow = SNR ? epi.nrm + d.out_row * epi.nrm_len : nullptr;
         if (FAST || (T >= d.maxshift && T + kGroupTile <= nsamps)) {
-            // the whole tile is past every shift: no per-sample tests
+            // the whole tile is past every shift: no per-sample tests.  The
+            // outputs are stored non-temporally: a state is far larger than
+            // the caches and is read back only by the next pass (measured:
+            // 0.2 ms less per search step at 4096 x 2^18)
             auto fast = [&](auto full_tag) {
                 constexpr bool FULL = decltype(full_tag)::value;
                 if (!FAST && SNR && T < epi.nrm_len) {
@@ -344,13 +347,13 @@ __device__ __forceinline__ void group_trees(const float* __restrict__ lds, const
                             }
                         }
 #pragma unroll
-                        for (int jj = 0; jj < 4; ++jj) o[T + (j0 + jj) * 64 + lane] = qv[jj];
+                        for (int jj = 0; jj < 4; ++jj) __builtin_nontemporal_store(qv[jj], &o[T + (j0 + jj) * 64 + lane]);
                     }
                 } else {
 #pragma unroll 4
                     for (int j = 0; j < kGroupTile / 64; ++j) {
                         const int u = j * 64 + lane;
-                        o[T + u] = LdsTreeFast<L, 1, FULL>::eval(lds, d, u);
+                        __builtin_nontemporal_store(LdsTreeFast<L, 1, FULL>::eval(lds, d, u), &o[T + u]);
                     }
                 }
             };
