@@ -458,7 +458,10 @@ struct sps_fdmt {
                 else if (L == 2) build_groups_L<2>(i0, sp, st);
                 else if (L == 3) build_groups_L<3>(i0, sp, st);
                 else build_groups_L<4>(i0, sp, st);
-            } else {
+            }
+            // tree tables for every pass of depth <= 3: the tree passes use
+            // them, and the mask transform runs on them in any mode
+            if (L <= 3) {
                 std::vector<uint8_t> bytes;
                 if (L == 1) build_trees_L<1>(i0, bytes);
                 else if (L == 2) build_trees_L<2>(i0, bytes);
@@ -530,7 +533,7 @@ struct sps_fdmt {
             if (ev) SPS_HIP(hipEventRecord(ev[0], st));
             return;
         }
-        if (!aux) ensure_trees(st);
+        ensure_trees(st);
         // the first staged pass builds state 0 itself when its tables exist
         const bool fused0 = !aux && !ones && in != nullptr && staged && d_stage0_init != nullptr;
         if (!fused0)
@@ -538,9 +541,25 @@ struct sps_fdmt {
                              d_grid_off.as<int64_t>(), d_chan_row_off.as<int64_t>(), sa, p, st, dmax,
                              d_live0.as<uint8_t>(), grid_consec);
         if (ev) SPS_HIP(hipEventRecord(ev[0], st));
+        bool trees_all = true;
+        for (int L : schedule) trees_all = trees_all && L <= 3;
+        if (aux && trees_all) {
+            // the mask transform over its short extent: fused tree passes
+            // (the staged windows would stage mostly halo for so few tiles)
+            int i0 = 1;
+            for (size_t k = 0; k < schedule.size(); ++k) {
+                const int L = schedule[k];
+                const float* src = (k % 2 == 0) ? sa : sb;
+                const bool last = (i0 + L - 1 == eng.niters);
+                float* dst = last ? out : ((k % 2 == 0) ? sb : sa);
+                launch_fdmt_fused(L, src, p, dst, last ? out_pitch : p, d_trees[k]->p, tree_rows[k], nsamps,
+                                  last ? eng.out_rows : INT64_MAX, st, nullptr);
+                i0 += L;
+            }
+            return;
+        }
         if (aux) {
-            // the mask transform stays on single-iteration launches (a short,
-            // separately named kernel)
+            // schedules with deeper passes: single-iteration launches
             for (int i = 1; i <= eng.niters; ++i) {
                 const float* src = ((i - 1) % 2 == 0) ? sa : sb;
                 const bool last = (i == eng.niters);
