@@ -687,7 +687,11 @@ __global__ void __launch_bounds__(kThreads) k_init_consec(const void* __restrict
     const int64_t k0 = (int64_t)blockIdx.x * tiles_per_block;
     const int64_t k1 = min(k0 + (int64_t)tiles_per_block, ntiles);
     const int K = (int)(grid_off[c + 1] - grid_off[c]);
-    const uint8_t* lv = live ? live + chan_row_off[c] : nullptr;
+    // which rows are written, read once: a uint8_t load may alias the float
+    // stores below, so reading the flags per store would reload them each time
+    uint32_t wmask = 0u;
+    for (int kk = 0; kk < K && kk < kInitConsecMax; ++kk)
+        if (!live || live[chan_row_off[c] + kk]) wmask |= 1u << kk;
     float mu = 0.0f, sg = 1.0f, rsg = 1.0f;
     bool keep = true;
     if (NORM) {
@@ -762,7 +766,7 @@ __global__ void __launch_bounds__(kThreads) k_init_consec(const void* __restrict
             float prev[4];
 #pragma unroll
             for (int e = 0; e < 4; ++e) prev[e] = 0.0f + w[4 + e];
-            if (!lv || lv[0]) {
+            if (wmask & 1u) {
                 float4a o = {prev[0], prev[1], prev[2], prev[3]};
                 *reinterpret_cast<float4a*>(rowbase + t0) = o;
             }
@@ -777,7 +781,7 @@ __global__ void __launch_bounds__(kThreads) k_init_consec(const void* __restrict
                     if (t0 + e >= kk) v = div_rn(prev[e] * mk + (0.0f + w[4 + e - kk]), yk, rk);
                     prev[e] = v;
                 }
-                if (!lv || lv[kk]) {
+                if ((wmask >> kk) & 1u) {
                     float4a o = {prev[0], prev[1], prev[2], prev[3]};
                     *reinterpret_cast<float4a*>(rowbase + (int64_t)kk * pitch + t0) = o;
                 }
