@@ -307,6 +307,10 @@ struct BoxErr {
     double e[kMaxPowQ + 1];
 };
 
+// wave-wide max from the ROCm device library (DPP within rows, then across
+// rows): a short VALU chain instead of six LDS-routed shuffles
+extern "C" __device__ float __ockl_wfred_max_f32(float);
+
 // threadIdx.x through an empty asm statement: the compiler cannot hoist
 // what is computed from it out of a loop.  The boxcar tile body uses it, so
 // that a block looping over tiles recomputes its per-thread LDS addresses
@@ -699,7 +703,7 @@ __device__ __forceinline__ void pow2_tile(float* x, float* la, uint2* stage, flo
     const unsigned tid = opaque_tid();
     __shared__ float wmaxabs[kThreads / 64];
     const int W = 1 << q;
-    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    m = __ockl_wfred_max_f32(m);  // wave max (m is never NaN: a NaN sample made it +inf)
     if ((tid & 63) == 0) wmaxabs[tid >> 6] = m;
     __syncthreads();
     float M = wmaxabs[0];
