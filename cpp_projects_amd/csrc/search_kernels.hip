@@ -310,6 +310,7 @@ struct BoxErr {
 // wave-wide max from the ROCm device library (DPP within rows, then across
 // rows): a short VALU chain instead of six LDS-routed shuffles
 extern "C" __device__ float __ockl_wfred_max_f32(float);
+extern "C" __device__ int __ockl_wfred_max_i32(int);
 
 // threadIdx.x through an empty asm statement: the compiler cannot hoist
 // what is computed from it out of a loop.  The boxcar tile body uses it, so
@@ -855,8 +856,7 @@ __device__ __forceinline__ void pow2_tile(float* x, float* la, uint2* stage, flo
         const int lt = (int)(e >> 14);
         const uint32_t fl = e & 0x3fffu;
         int top = fl ? 31 - __clz(fl) : -1;
-        for (int off = 32; off > 0; off >>= 1) top = max(top, __shfl_xor(top, off));
-        top = __builtin_amdgcn_readfirstlane(top);  // equal on every lane: a scalar loop bound
+        top = __builtin_amdgcn_readfirstlane(__ockl_wfred_max_i32(top));  // wave max: a scalar loop bound
         if (top < 0) continue;  // wave-uniform; no barrier in this loop
         // The running sum s over samples lt, lt+1, ... is taken to the
         // wave's largest flagged width 2^top by every lane (uniform control
