@@ -670,7 +670,8 @@ __global__ void __launch_bounds__(kThreads) k_init_lds(const void* __restrict__ 
 // 4 samples before a chunk (x[t - k], k <= 4) come through one aligned
 // 16-byte LDS read (for a tile's first chunk: the previous tile's last chunk,
 // or the input before the block's first tile), and each row is written as
-// float4, so every store instruction covers 1 KB.
+// float4, so every store instruction covers 1 KB (non-temporally: state 0 is
+// read back only by the first merge pass).
 constexpr int kInitConsecMax = 5;
 constexpr int kInitTilesPerBlock = 8;
 
@@ -768,7 +769,7 @@ __global__ void __launch_bounds__(kThreads) k_init_consec(const void* __restrict
             for (int e = 0; e < 4; ++e) prev[e] = 0.0f + w[4 + e];
             if (wmask & 1u) {
                 float4a o = {prev[0], prev[1], prev[2], prev[3]};
-                *reinterpret_cast<float4a*>(rowbase + t0) = o;
+                __builtin_nontemporal_store(o, reinterpret_cast<float4a*>(rowbase + t0));
             }
 #pragma unroll
             for (int kk = 1; kk < kInitConsecMax; ++kk) {
@@ -783,7 +784,7 @@ __global__ void __launch_bounds__(kThreads) k_init_consec(const void* __restrict
                 }
                 if ((wmask >> kk) & 1u) {
                     float4a o = {prev[0], prev[1], prev[2], prev[3]};
-                    *reinterpret_cast<float4a*>(rowbase + (int64_t)kk * pitch + t0) = o;
+                    __builtin_nontemporal_store(o, reinterpret_cast<float4a*>(rowbase + (int64_t)kk * pitch + t0));
                 }
             }
         }
