@@ -313,13 +313,20 @@ __device__ __forceinline__ void group_trees(const float* __restrict__ lds, const
             auto fast = [&](auto full_tag) {
                 constexpr bool FULL = decltype(full_tag)::value;
                 if (!FAST && SNR && T < epi.nrm_len) {
-                    // within the mask transform's extent: per-sample divisor there
+                    // within the mask transform's extent: per-sample divisor
+                    // there, all of the row-tile's divisors loaded first
+                    float nv[kGroupTile / 64];
+#pragma unroll
+                    for (int j = 0; j < kGroupTile / 64; ++j) {
+                        const int64_t t = T + j * 64 + lane;
+                        nv[j] = (t < epi.nrm_len) ? nrow[t] : 0.0f;
+                    }
 #pragma unroll 4
                     for (int j = 0; j < kGroupTile / 64; ++j) {
                         const int u = j * 64 + lane;
                         const int64_t t = T + u;
                         const float v = LdsTreeFast<L, 1, FULL>::eval(lds, d, u);
-                        o[t] = (t < epi.nrm_len) ? v / sqrtf((float)((double)nrow[t] + 1.0E-8)) : div_rn(v, dk, rdk);
+                        o[t] = (t < epi.nrm_len) ? v / sqrtf((float)((double)nv[j] + 1.0E-8)) : div_rn(v, dk, rdk);
                     }
                 } else if constexpr (SNR) {
                     // div_rn(v, dk, rdk) four samples at a time: dk = sqrtf(kept +
@@ -364,13 +371,21 @@ __device__ __forceinline__ void group_trees(const float* __restrict__ lds, const
             continue;
         }
         if constexpr (!FAST) {
+            float nv[kGroupTile / 64];
+            if (SNR) {
+#pragma unroll
+                for (int j = 0; j < kGroupTile / 64; ++j) {
+                    const int64_t t = T + j * 64 + lane;
+                    nv[j] = (t < epi.nrm_len && t < nsamps) ? nrow[t] : 0.0f;
+                }
+            }
 #pragma unroll 4
             for (int j = 0; j < kGroupTile / 64; ++j) {
                 const int64_t t = T + j * 64 + lane;
                 if (t < nsamps) {
                     float v = LdsTree<L, 1>::eval(lds, d, t, T);
                     if (SNR)
-                        v = (t < epi.nrm_len) ? v / sqrtf((float)((double)nrow[t] + 1.0E-8)) : div_rn(v, dk, rdk);
+                        v = (t < epi.nrm_len) ? v / sqrtf((float)((double)nv[j] + 1.0E-8)) : div_rn(v, dk, rdk);
                     o[t] = v;
                 }
             }
