@@ -558,8 +558,7 @@ __device__ __forceinline__ void pow2_flag_chunks_at(int q, float* x, float* la, 
     }
 }
 
-constexpr int kPowPfMaxQ = 8;  // widest window of the segment filter and the multi-tile kernel
-constexpr int kPowPfSpan = round_up4(kPowCells + (1 << kPowPfMaxQ) - 1);
+constexpr int kPowPfMaxQ = 9;  // widest window of the segment filter and the multi-tile kernel
 
 // max of four values (the chunk filter's flag test), as in pow2_flag_chunks
 __device__ __forceinline__ uint32_t chunk_flag(float a, float b, float c, float d, float l) {
@@ -586,14 +585,14 @@ __device__ __forceinline__ uint32_t chunk_flag(float a, float b, float c, float 
 // few floats instead of about two per level.  aux: 5 (256 + W/8) floats.
 template <int Q>
 __device__ __forceinline__ void pow2_flag_segments(const float* x, float* aux, const float* lim, uint32_t (&cf)[2]) {
-    static_assert(Q <= kPowPfMaxQ, "segment filter window");
+    static_assert(Q <= kPowPfMaxQ, "segment filter window");  // the halo segments fit one wave
     const unsigned tid = opaque_tid();
     const float4* x4 = reinterpret_cast<const float4*>(x);
     float c[16];
     {
         const float4 v0 = x4[2 * tid], v1 = x4[2 * tid + 1];
         c[0] = v0.x, c[1] = v0.y, c[2] = v0.z, c[3] = v0.w, c[4] = v1.x, c[5] = v1.y, c[6] = v1.z, c[7] = v1.w;
-        if constexpr (Q >= 1) {  // cells 8t + 8 .. 8t + 15 (< 2056 <= kPowPfSpan); past the window only for
+        if constexpr (Q >= 1) {  // cells 8t + 8 .. 8t + 15 (< 2056, inside the buffer); past the window only for
                                  // sums of levels above Q, which flag nothing
             const float4 v2 = x4[2 * tid + 2], v3 = x4[2 * tid + 3];
             c[8] = v2.x, c[9] = v2.y, c[10] = v2.z, c[11] = v2.w, c[12] = v3.x, c[13] = v3.y, c[14] = v3.z,
@@ -644,7 +643,7 @@ __device__ __forceinline__ void pow2_flag_segments(const float* x, float* aux, c
             MX[g] = mx;
         };
         stats(c, (int)tid);
-        if ((int)tid < NQ) {  // halo segments 256 .. 255 + NQ (cells < 2048 + W <= kPowPfSpan)
+        if ((int)tid < NQ) {  // halo segments 256 .. 255 + NQ (cells < 2048 + W, inside the buffer)
             const int g = kPowCells / 8 + (int)tid;
             const float4 v0 = x4[2 * g], v1 = x4[2 * g + 1];
             const float h[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
@@ -678,14 +677,14 @@ __device__ __forceinline__ void pow2_flag_segments(const float* x, float* aux, c
     }
 }
 
-// Runs pow2_flag_segments<q> for a runtime q in [Q, kPowPfMaxQ].
-template <int Q>
+// Runs pow2_flag_segments<q> for a runtime q in [Q, QM].
+template <int Q, int QM>
 __device__ __forceinline__ void pow2_flag_segments_at(int q, const float* x, float* aux, const float* lim,
                                                       uint32_t (&cf)[2]) {
     if (q == Q) {
         pow2_flag_segments<Q>(x, aux, lim, cf);
     } else {
-        if constexpr (Q < kPowPfMaxQ) pow2_flag_segments_at<Q + 1>(q, x, aux, lim, cf);
+        if constexpr (Q < QM) pow2_flag_segments_at<Q + 1, QM>(q, x, aux, lim, cf);
     }
 }
 
@@ -736,7 +735,7 @@ __device__ __forceinline__ void pow2_tile(float* x, float* la, uint2* stage, flo
         // 8t + 7); a flagged chunk queues its four cells, ranked by (wave,
         // lane, chunk)
         uint32_t cf[2];
-        pow2_flag_segments_at<0>(q, x, la, lim, cf);
+        if constexpr (SEG) pow2_flag_segments_at<0, QMAX>(q, x, la, lim, cf);
         cf[0] = __brev(cf[0]) >> (31 - q);
         cf[1] = __brev(cf[1]) >> (31 - q);
         const uint64_t b0 = __ballot(cf[0] != 0), b1 = __ballot(cf[1] != 0);
@@ -762,7 +761,7 @@ __device__ __forceinline__ void pow2_tile(float* x, float* la, uint2* stage, flo
         // chunk layout: slot s of thread t is chunk s*256 + t (4 cells); a
         // flagged chunk queues its four cells
         uint32_t cf[2];
-        pow2_flag_chunks_at<0, QMAX>(q, x, la, lim, cf);
+        if constexpr (!SEG) pow2_flag_chunks_at<0, QMAX>(q, x, la, lim, cf);
         uint64_t mk[2];
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
@@ -977,8 +976,9 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
                                count);
 }
 
-// Windows up to q = kPowPfMaxQ in static LDS: two sample buffers, so that the
-// next tile's window is fetched while the current one is searched.
+// Windows up to q = QM in static LDS (QM = 8, or 9 for q = 9): two sample
+// buffers, so that the next tile's window is fetched while the current one
+// is searched.
 constexpr int kPowPfTiles = 4;  // consecutive tiles of a row per block
 
 // Starts copying floats [0, span4) of a window (span4 % 4 == 0, src 16-byte
@@ -1024,21 +1024,24 @@ __device__ __forceinline__ float pow2_window_max(const float* x, int span4) {
     return m;
 }
 
-// One block = kPowPfTiles consecutive tiles of one row (q <= kPowPfMaxQ).  A
+// One block = kPowPfTiles consecutive tiles of one row (q <= QM).  A
 // tile whose whole rounded window lies inside a 16-byte aligned row is
 // fetched by global_load_lds, issued when the previous tile starts; the row's
 // last tiles are loaded directly.  M may then include the one sample after
 // the window (a larger bound: the filter only gets more conservative).
+template <int QM>
 __global__ void __launch_bounds__(kThreads) k_boxcar_pow2_pf(const float* __restrict__ img, int64_t cols,
                                                              int64_t pitch, const float* __restrict__ d_thr,
                                                              float h_thr, int q, BoxErr err,
                                                              CandD* __restrict__ out, int64_t cap,
                                                              unsigned long long* __restrict__ count) {
-    __shared__ __attribute__((aligned(16))) float xa[kPowPfSpan];
-    __shared__ __attribute__((aligned(16))) float xb[kPowPfSpan];
-    __shared__ __attribute__((aligned(16))) float la[kPowPfSpan];
+    static_assert(QM <= kPowPfMaxQ, "segment filter window");
+    constexpr int kSpan = round_up4(kPowCells + (1 << QM) - 1);
+    __shared__ __attribute__((aligned(16))) float xa[kSpan];
+    __shared__ __attribute__((aligned(16))) float xb[kSpan];
+    __shared__ __attribute__((aligned(16))) float la[kSpan];
     __shared__ __attribute__((aligned(16))) uint2 stage[kPowStage * (kThreads / 64)];
-    __shared__ double err_e[kPowPfMaxQ + 1];
+    __shared__ double err_e[QM + 1];
     if ((int)threadIdx.x <= q) err_e[threadIdx.x] = err.e[threadIdx.x];  // read after the first tile's barriers
     const int full = kPowCells + (1 << q) - 1;
     const int full4 = round_up4(full);
@@ -1063,7 +1066,7 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2_pf(const float* __rest
             __syncthreads();
             m = pow2_load_window(cur, row + T, span, vec);
         }
-        pow2_tile<kPowPfMaxQ, true>(cur, la, stage, m, row, T, span, vec, q, thr, err_e, r, out, cap, count);
+        pow2_tile<QM, true>(cur, la, stage, m, row, T, span, vec, q, thr, err_e, r, out, cap, count);
     };
     if (k0 < k1 && dma(k0)) pow2_issue_window(xa, row + k0 * kPowCells, full4);
     for (int64_t k = k0; k < k1; ++k) {  // block-uniform
@@ -1248,9 +1251,14 @@ void launch_boxcar(const float* img, int64_t rows, int64_t cols, int64_t pitch, 
             err.e[k] = (gam(w - 1.0) * w + ours) * (1.0 + 0x1p-20);
         }
         if (q <= kPowPfMaxQ) {
+            // static LDS sized for q <= 8 (5 blocks per CU), or for q = 9
             const dim3 gpf((uint32_t)ceil_div(ceil_div(cols, kPowCells), (int64_t)kPowPfTiles), (uint32_t)rows);
-            hipLaunchKernelGGL(k_boxcar_pow2_pf, gpf, dim3(kThreads), 0, st, img, cols, pitch, d_thr, h_thr, q, err,
-                               out, cap, d_count);
+            if (q <= 8)
+                hipLaunchKernelGGL(k_boxcar_pow2_pf<8>, gpf, dim3(kThreads), 0, st, img, cols, pitch, d_thr, h_thr, q,
+                                   err, out, cap, d_count);
+            else
+                hipLaunchKernelGGL(k_boxcar_pow2_pf<kPowPfMaxQ>, gpf, dim3(kThreads), 0, st, img, cols, pitch, d_thr,
+                                   h_thr, q, err, out, cap, d_count);
         } else {
             hipLaunchKernelGGL(k_boxcar_pow2, grid, dim3(kThreads), lds, st, img, cols, pitch, d_thr, h_thr, q, err,
                                out, cap, d_count);

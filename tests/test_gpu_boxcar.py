@@ -117,7 +117,7 @@ def test_many_candidates_overflow_stage(torch_cuda):
 
 def test_prefetched_tiles_with_careful_windows(torch_cuda):
     # rows whose pitch is a multiple of 4: interior tiles are fetched ahead
-    # (4 tiles per block) and filtered by segments; a NaN and a huge sample
+    # (4 tiles per block, windows up to 512 samples) and filtered by segments; a NaN and a huge sample
     # force per-cell flags on single tiles inside such a chain, and the
     # last tiles of each row are loaded directly
     rng = np.random.default_rng(6)
@@ -127,5 +127,5 @@ def test_prefetched_tiles_with_careful_windows(torch_cuda):
     img[1, 5000] = np.nan
     img[2, 9000] = 3.0e37
     img[2, 12000:12020] += 4.0
-    for q in (0, 3, 5, 8):
+    for q in (0, 3, 5, 8, 9):
         check(torch_cuda, img, 2.5, q)
