@@ -680,7 +680,7 @@ __global__ void __launch_bounds__(kThreads) k_init_lds(const void* __restrict__ 
 // the operations of k_init_lds for such a grid, in the same order.  One
 // block = one channel x tiles_per_block consecutive 4096-sample tiles; a
 // thread owns 4 chunks of 4 consecutive samples of a tile (chunk j * 256 +
-// tid): one 4-byte (u8) or 16-byte (f32) load per chunk, issued for the next
+// tid): one 4-byte (u8) or 16-byte (f32) load per chunk (none for ones), issued for the next
 // tile before the current one is computed, normalised once in registers; the
 // 4 samples before a chunk (x[t - k], k <= 4) come through one aligned
 // 16-byte LDS read (for a tile's first chunk: the previous tile's last chunk,
@@ -696,7 +696,7 @@ __global__ void __launch_bounds__(kThreads) k_init_consec(const void* __restrict
                                                           const int64_t* __restrict__ chan_row_off,
                                                           float* __restrict__ state, int64_t pitch,
                                                           const uint8_t* __restrict__ live, int tiles_per_block) {
-    static_assert(IN_KIND == 0 || IN_KIND == 1, "f32 or u8 input");
+    static_assert(IN_KIND >= 0 && IN_KIND <= 2, "f32, u8 or ones input");
     __shared__ float4a xs[1 + kInitTile / 4];  // xs[0]: samples T-4 .. T-1; xs[1 + q]: chunk q
     const int64_t c = blockIdx.y;
     const int64_t ntiles = ceil_div(nsamps, (int64_t)kInitTile);
@@ -726,9 +726,9 @@ __global__ void __launch_bounds__(kThreads) k_init_consec(const void* __restrict
         for (int j = 0; j < 4; ++j) {
             const int64_t t0 = T + 4 * (int64_t)(j * kThreads + (int)threadIdx.x);
             if (t0 < nsamps) {  // nsamps % 4 == 0: the chunk is whole
-                if (IN_KIND == 1)
+                if constexpr (IN_KIND == 1)
                     raw8[j] = *reinterpret_cast<const uint32_t*>(static_cast<const uint8_t*>(in) + xb + t0);
-                else
+                else if constexpr (IN_KIND == 0)
                     rawf[j] = *reinterpret_cast<const float4a*>(static_cast<const float*>(in) + xb + t0);
             }
         }
@@ -743,14 +743,16 @@ __global__ void __launch_bounds__(kThreads) k_init_consec(const void* __restrict
             const int64_t t0 = T + 4 * (int64_t)(j * kThreads + (int)threadIdx.x);
             float4a v = {0.0f, 0.0f, 0.0f, 0.0f};
             if (t0 < nsamps) {
-                if (IN_KIND == 1) {
+                if constexpr (IN_KIND == 1) {
                     const uint32_t w = raw8[j];
                     v.x = (float)(w & 0xffu);
                     v.y = (float)((w >> 8) & 0xffu);
                     v.z = (float)((w >> 16) & 0xffu);
                     v.w = (float)(w >> 24);
-                } else {
+                } else if constexpr (IN_KIND == 0) {
                     v = rawf[j];
+                } else {
+                    v = {1.0f, 1.0f, 1.0f, 1.0f};  // ones: the mask transform's input
                 }
                 v.x = nrm(v.x);
                 v.y = nrm(v.y);
@@ -1172,9 +1174,10 @@ static void init_dispatch_norm(const void* in, int64_t in_pitch, const ChanNorm*
                                int64_t pitch, int dmax_all, const uint8_t* live, int consec, hipStream_t st) {
     ChanNorm n{};
     if (norm) n = *norm;
-    if constexpr (MODE == kInitMeanD && IN_KIND != 2) {
+    if constexpr (MODE == kInitMeanD) {
         const uintptr_t ib = reinterpret_cast<uintptr_t>(in);
-        const bool al_in = IN_KIND == 1 ? (ib % 4 == 0 && in_pitch % 4 == 0) : (ib % 16 == 0 && in_pitch % 4 == 0);
+        const bool al_in = IN_KIND == 2 || (IN_KIND == 1 ? (ib % 4 == 0 && in_pitch % 4 == 0)
+                                                         : (ib % 16 == 0 && in_pitch % 4 == 0));
         const bool al_out = (reinterpret_cast<uintptr_t>(state) % 16 == 0) && (pitch % 4 == 0);
         if (consec && al_in && al_out && nsamps % 4 == 0) {
             const int tpb = kInitTilesPerBlock;
