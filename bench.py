@@ -8,15 +8,18 @@ outlier-row cleaning, FDMT with dt_max 4095 (4096 DM trials), S/N
 normalisation by the transform of the channel mask, and boxcar S/N over
 widths 1,2,..,256 with a strict 6-sigma threshold and candidate compaction.
 
-Prints ONE JSON line: DM-trials x samples / s, the merge kernel's achieved
-HBM bandwidth against 8 TB/s (HIP events on the launch stream, algorithmic
-bytes 4*nsamps*(rows_{i-1}+rows_i) per iteration), and the CPU oracle timed
-on a bounded time slice of the same beam.
+Prints ONE JSON line: DM-trials x samples / s; the dominant merge pass's
+achieved HBM bandwidth against 8 TB/s (HIP events on the launch stream over
+its compulsory bytes: one read of the live rows of the pass's input state and
+one write of those of its output state, 4*nsamps bytes per row); a per-stage
+table of the same figures; and the CPU oracle chain timed on a bounded time
+slice of the same beam.  Every candidate buffer is sized from a sizing run of
+its input, and the line reports found and stored counts.
 
 The "batched" object measures a ragged batch of 64 float32 beams (channel
 counts 1024/2048/4096, 2^14..2^18 samples) replayed as one captured HIP graph
 per step, next to the same beams run one at a time through the single-beam
-search with a count readback after each.
+search with a count readback after each, overall and per channel count.
 """
 import argparse
 import json
@@ -30,8 +33,11 @@ sys.path.insert(0, ROOT)
 METRIC = "DM-trials×samples/s (FDMT+boxcar) and % of HBM peak at 1 GPU, batched vs plain"
 HBM_PEAK_GBS = 8000.0
 
-CFG = dict(f_min=1200.0, f_max=1600.0, nchans=4096, nsamps=1 << 18, dt_max=4095, box_q=8, threshold=6.0,
-           npulses=20, seed=2)
+from cpp_projects_amd import workloads  # noqa: E402
+
+CFG = dict(workloads.SINGLE_BEAM, f_min=workloads.F_MIN, f_max=workloads.F_MAX)
+BATCH = workloads.BATCH
+LARGE = workloads.LARGE
 
 
 def log(*a):
@@ -59,22 +65,8 @@ def cpu_baseline(x_u8, ndt, sample_cols, threads):
     return ndt * sample_cols / dt, dt, len(cands)
 
 
-BATCH = dict(nbeams=64, nch_choices=(1024, 2048, 4096), n_lo=1 << 14, n_hi=1 << 18, seed=3, box_q=8, threshold=6.0)
-
-
 def batch_specs(nbeams):
-    """Ragged batch: channel count from {1024, 2048, 4096}, length uniform in
-    2^14..2^18 samples, 0-5 pulses, beam b seeded 3 + b; dt_max = nch - 1."""
-    import numpy as np
-
-    rng = np.random.default_rng(BATCH["seed"])
-    specs = []
-    for b in range(nbeams):
-        nch = int(rng.choice(BATCH["nch_choices"]))
-        ns = int(rng.integers(BATCH["n_lo"], BATCH["n_hi"] + 1))
-        npulses = int(rng.integers(0, 6))
-        specs.append(dict(nch=nch, ns=ns, npulses=npulses, seed=BATCH["seed"] + b))
-    return specs
+    return workloads.batch_specs(nbeams)
 
 
 def live_rows(plan, out_rows):
@@ -103,84 +95,137 @@ def alg_bytes(nch, ns, rows):
     return 4.0 * ns * (nch + (nch + rows[0]) + merges + rows[-1])
 
 
+def sized_capacity(n):
+    """Candidate capacity for a count found by a sizing run (counts are
+    deterministic for a given input, so the exact count suffices; a little
+    slack keeps the buffer reusable)."""
+    return max(1024, int(n) + (int(n) >> 6) + 1024)
+
+
 def bench_batch(args):
     """Batched multi-beam search (one captured graph per step) against the
     same beams run one at a time through the single-beam search, each
-    followed by its candidate-count readback."""
+    followed by its candidate-count readback; plus the same comparison for
+    the beams of each channel count alone."""
     import torch
 
-    from cpp_projects_amd import synth
     from cpp_projects_amd.batch import BatchSearch, Beam
     from cpp_projects_amd.search import SingleBeamSearch
     from cpp_projects_amd.tune import tune_search
 
     specs = batch_specs(args.batch_beams)
-    rows = {nch: plan_rows(nch) for nch in {s["nch"] for s in specs}}
-    cap = 1 << 16
     log(f"[batch] generating {len(specs)} beams")
-    ins, cands, counts = [], [], []
+    ins = []
     for s in specs:
-        x, _ = synth.f32_filterbank(s["nch"], s["ns"], CFG["f_min"], CFG["f_max"], s["seed"], npulses=s["npulses"],
-                                    max_sweep=s["nch"] - 1, width_range=(1.0, 16.0))
-        ins.append(x)
-        cands.append(torch.empty((cap, 4), dtype=torch.int32, device="cuda"))
-        counts.append(torch.zeros(1, dtype=torch.int64, device="cuda"))
+        ins.append(workloads.batch_beam_input(s))
     torch.cuda.synchronize()
-    mk = dict(in_dtype="f32", box_q=BATCH["box_q"], threshold=BATCH["threshold"], cand_capacity=cap)
+    mk = dict(in_dtype="f32", box_q=BATCH["box_q"], threshold=BATCH["threshold"])
+    nchs = sorted({s["nch"] for s in specs})
     # schedule per channel count, tuned on the longest beam of that count
     sched = {}
-    for nch in sorted(rows):
+    tmp_c = torch.empty((1 << 16, 4), dtype=torch.int32, device="cuda")
+    tmp_n = torch.zeros(1, dtype=torch.int64, device="cuda")
+    for nch in nchs:
         i = max((k for k, s in enumerate(specs) if s["nch"] == nch), key=lambda k: specs[k]["ns"])
-        s = SingleBeamSearch(CFG["f_min"], CFG["f_max"], nch, specs[i]["ns"], nch - 1, **mk)
-        s.run(ins[i], None, cands[i], counts[i])
-        best, _ = tune_search(s, lambda: s.run(ins[i], None, cands[i], counts[i]), reps=1)
+        s = SingleBeamSearch(CFG["f_min"], CFG["f_max"], nch, specs[i]["ns"], nch - 1, cand_capacity=1 << 16, **mk)
+        s.run(ins[i], None, tmp_c, tmp_n)
+        best, _ = tune_search(s, lambda: s.run(ins[i], None, tmp_c, tmp_n), reps=1)
         sched[nch] = best
         log(f"[batch] nch {nch}: schedule {best}")
         del s
-    work = sum((s["nch"]) * s["ns"] for s in specs)  # DM trials x samples (ndt = nch)
-    tot_bytes = sum(alg_bytes(s["nch"], s["ns"], rows[s["nch"]]) for s in specs)
+    del tmp_c, tmp_n
 
-    # plain: one beam per launch sequence, synchronised per beam
-    plain_s, plain_counts = 0.0, []
+    # plain: one beam per launch sequence, synchronised per beam; the first
+    # (untimed) run of each beam sizes its candidate buffer
+    plain_ms, counts_found, cands, counts, comp_bytes = [], [], [], [], []
     for i, s in enumerate(specs):
-        sb = SingleBeamSearch(CFG["f_min"], CFG["f_max"], s["nch"], s["ns"], s["nch"] - 1, **mk)
-        sb.set_staged(sched[s["nch"]]["staged"])
-        if sched[s["nch"]]["shape"]:
-            sb.set_group_shape(*sched[s["nch"]]["shape"])
-        sb.set_schedule(sched[s["nch"]]["schedule"])
-        sb.run(ins[i], None, cands[i], counts[i])
+        sc = sched[s["nch"]]
+        cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+        probe = torch.empty((1 << 16, 4), dtype=torch.int32, device="cuda")
+        sb = SingleBeamSearch(CFG["f_min"], CFG["f_max"], s["nch"], s["ns"], s["nch"] - 1, cand_capacity=1 << 16, **mk)
+        sb.run(ins[i], None, probe, cnt)
+        cap = sized_capacity(int(cnt.item()))
+        del sb, probe
+        sb = SingleBeamSearch(CFG["f_min"], CFG["f_max"], s["nch"], s["ns"], s["nch"] - 1, cand_capacity=cap, **mk)
+        sb.set_staged(sc["staged"])
+        if sc["shape"]:
+            sb.set_group_shape(*sc["shape"])
+        sb.set_schedule(sc["schedule"])
+        cb = torch.empty((cap, 4), dtype=torch.int32, device="cuda")
+        sb.run(ins[i], None, cb, cnt)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        sb.run(ins[i], None, cands[i], counts[i])
-        plain_counts.append(int(counts[i].item()))
-        plain_s += time.perf_counter() - t0
+        sb.run(ins[i], None, cb, cnt)
+        counts_found.append(int(cnt.item()))
+        plain_ms.append((time.perf_counter() - t0) * 1e3)
+        comp_bytes.append(sum(sb.stage_bytes()))
+        cands.append(cb)
+        counts.append(cnt)
         del sb
+    plain_s = sum(plain_ms) / 1e3
+    caps = [int(c.shape[0]) for c in cands]
     log(f"[batch] plain: {plain_s * 1e3:.1f} ms")
 
-    beams = [Beam(CFG["f_min"], CFG["f_max"], s["nch"], s["ns"], s["nch"] - 1, ins[i], cands[i], counts[i])
-             for i, s in enumerate(specs)]
-    b = BatchSearch(beams, in_dtype="f32", box_q=BATCH["box_q"], threshold=BATCH["threshold"],
-                    nworkers=args.batch_workers)
-    for i, s in enumerate(specs):
-        b.set_staged(i, sched[s["nch"]]["staged"])
-        if sched[s["nch"]]["shape"]:
-            b.set_group_shape(i, *sched[s["nch"]]["shape"])
-        b.set_schedule(i, sched[s["nch"]]["schedule"])
-    b.launch()  # capture + first replay
-    torch.cuda.synchronize()
+    def make_batch(idx):
+        beams = [Beam(CFG["f_min"], CFG["f_max"], specs[i]["nch"], specs[i]["ns"], specs[i]["nch"] - 1, ins[i],
+                      cands[i], counts[i]) for i in idx]
+        b = BatchSearch(beams, in_dtype="f32", box_q=BATCH["box_q"], threshold=BATCH["threshold"],
+                        nworkers=args.batch_workers)
+        for k, i in enumerate(idx):
+            sc = sched[specs[i]["nch"]]
+            b.set_staged(k, sc["staged"])
+            if sc["shape"]:
+                b.set_group_shape(k, *sc["shape"])
+            b.set_schedule(k, sc["schedule"])
+        return b
+
+    def time_batch(b, steps):
+        b.launch()  # capture + first replay
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.launch()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    all_idx = list(range(len(specs)))
+    b = make_batch(all_idx)
+    batch_s = time_batch(b, args.batch_steps)
     batch_counts = [int(c.item()) for c in counts]
-    t0 = time.perf_counter()
-    for _ in range(args.batch_steps):
-        b.launch()
-    torch.cuda.synchronize()
-    batch_s = (time.perf_counter() - t0) / args.batch_steps
+    sorted_ok = True
+    for k in all_idx:
+        try:
+            b.sort_candidates(k)
+        except Exception:
+            sorted_ok = False
     log(f"[batch] graph: {batch_s * 1e3:.1f} ms per batch")
     t0 = time.perf_counter()
     b.run()
     torch.cuda.synchronize()
     eager_s = time.perf_counter() - t0
-    same = batch_counts == plain_counts
     del b
+    # per channel count: the beams of one count alone, batched vs plain
+    by_nch = {}
+    for nch in nchs:
+        idx = [i for i in all_idx if specs[i]["nch"] == nch]
+        bb = make_batch(idx)
+        t = time_batch(bb, args.batch_steps)
+        del bb
+        p = sum(plain_ms[i] for i in idx) / 1e3
+        by_nch[str(nch)] = {"beams": len(idx), "samples": sum(specs[i]["ns"] for i in idx),
+                            "batched_ms": t * 1e3, "plain_ms": p * 1e3, "speedup": p / t,
+                            "compulsory_frac": sum(comp_bytes[i] for i in idx) / t / 1e9 / HBM_PEAK_GBS}
+    # per length: plain time per beam against its compulsory bytes
+    short = [i for i in all_idx if specs[i]["ns"] < (1 << 16)]
+    long_ = [i for i in all_idx if specs[i]["ns"] >= (1 << 16)]
+
+    def plain_rate(idx):
+        if not idx:
+            return None
+        return sum(comp_bytes[i] for i in idx) / (sum(plain_ms[i] for i in idx) * 1e-3) / 1e9
+
+    work = sum(s["nch"] * s["ns"] for s in specs)  # DM trials x samples (ndt = nch)
+    tot_comp = sum(comp_bytes)
     return {
         "workload": f"{len(specs)} beams, nch in {{1024,2048,4096}}, nsamps U[2^14,2^18], f32 N(0,1) + 0-5 pulses, "
                     "dt_max nch-1, global norm + S/N norm, boxcar widths 1..256 pow2, thr 6",
@@ -192,18 +237,19 @@ def bench_batch(args):
         "plain_ms_per_batch": plain_s * 1e3,
         "eager_ms_per_batch": eager_s * 1e3,
         "speedup_vs_plain": plain_s / batch_s,
-        "alg_bytes_per_batch": tot_bytes,
-        "effective_GBs": tot_bytes / batch_s / 1e9,
-        "frac_of_hbm_peak": tot_bytes / batch_s / 1e9 / HBM_PEAK_GBS,
+        "compulsory_bytes_per_batch": tot_comp,
+        "compulsory_GBs": tot_comp / batch_s / 1e9,
+        "compulsory_frac": tot_comp / batch_s / 1e9 / HBM_PEAK_GBS,
+        "plain_compulsory_frac": tot_comp / plain_s / 1e9 / HBM_PEAK_GBS,
+        "by_nchans": by_nch,
+        "plain_GBs_short_beams_lt_2e16": plain_rate(short),
+        "plain_GBs_long_beams": plain_rate(long_),
         "workers": args.batch_workers,
-        "candidates": sum(batch_counts),
-        "candidate_counts_equal_plain": same,
+        "candidates_found": sum(batch_counts),
+        "cand_capacity_ok": all(n <= c for n, c in zip(batch_counts, caps)) and sorted_ok,
+        "candidate_counts_equal_plain": batch_counts == counts_found,
         "schedules": {str(k): v for k, v in sched.items()},
     }
-
-
-LARGE = dict(nchans=8192, nsamps=1 << 20, dt_max=8191, box_q=9, threshold=6.0, npulses=50, seed=5, rfi_frac=0.1,
-             rfi_sigma=5.0)
 
 
 def bench_large(args):
@@ -212,25 +258,29 @@ def bench_large(args):
     to 8192 samples, widths 1-512), dt_max 8191, widths 1..512."""
     import torch
 
-    from cpp_projects_amd import synth
     from cpp_projects_amd.search import SingleBeamSearch
     from cpp_projects_amd.tune import tune_search
 
     L = LARGE
     R, N = L["nchans"], L["nsamps"]
     log(f"[large] generating {R}x{N} f32 filterbank")
-    x, _ = synth.f32_filterbank(R, N, CFG["f_min"], CFG["f_max"], L["seed"], npulses=L["npulses"],
-                                max_sweep=L["dt_max"], width_range=(1.0, 512.0), rfi_frac=L["rfi_frac"],
-                                rfi_sigma=L["rfi_sigma"])
-    cap = 1 << 24
-    s = SingleBeamSearch(CFG["f_min"], CFG["f_max"], R, N, L["dt_max"], in_dtype="f32", box_q=L["box_q"],
-                         threshold=L["threshold"], cand_capacity=cap)
-    cands = torch.empty((cap, 4), dtype=torch.int32, device="cuda")
+    x, _ = workloads.large_input()
+    mk = dict(in_dtype="f32", box_q=L["box_q"], threshold=L["threshold"])
     count = torch.zeros(1, dtype=torch.int64, device="cuda")
-    s.run(x, None, cands, count)
+    probe = torch.empty((1 << 16, 4), dtype=torch.int32, device="cuda")
+    s = SingleBeamSearch(CFG["f_min"], CFG["f_max"], R, N, L["dt_max"], cand_capacity=1 << 16, **mk)
+    s.run(x, None, probe, count)
     torch.cuda.synchronize()
-    best, _ = tune_search(s, lambda: s.run(x, None, cands, count), reps=1)
+    best, _ = tune_search(s, lambda: s.run(x, None, probe, count), reps=1)
     log(f"[large] schedule {best}")
+    cap = sized_capacity(int(count.item()))
+    del s, probe
+    s = SingleBeamSearch(CFG["f_min"], CFG["f_max"], R, N, L["dt_max"], cand_capacity=cap, **mk)
+    s.set_staged(best["staged"])
+    if best["shape"]:
+        s.set_group_shape(*best["shape"])
+    s.set_schedule(best["schedule"])
+    cands = torch.empty((cap, 4), dtype=torch.int32, device="cuda")
     s.run(x, None, cands, count)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -239,9 +289,11 @@ def bench_large(args):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.large_steps
     ncand = int(count.item())
+    n_sorted = s.sort_candidates(cands, count)  # raises on overflow
     ndt = s.ndt
     rows = plan_rows_for(R, N, L["dt_max"])
     b = alg_bytes(R, N, rows)
+    comp = sum(s.stage_bytes())
     del s, cands, count, x
     return {
         "workload": "8192ch x 2^20 f32 (1200-1600 MHz), 10% channels N(0,25), 50 pulses, dt_max 8191, "
@@ -250,10 +302,14 @@ def bench_large(args):
         "unit": "DM-trials*samples/s",
         "ms_per_step": dt * 1e3,
         "steps": args.large_steps,
-        "alg_bytes_per_step": b,
-        "effective_GBs": b / dt / 1e9,
-        "frac_of_hbm_peak": b / dt / 1e9 / HBM_PEAK_GBS,
-        "candidates": ncand,
+        "compulsory_bytes_per_step": comp,
+        "compulsory_GBs": comp / dt / 1e9,
+        "compulsory_frac": comp / dt / 1e9 / HBM_PEAK_GBS,
+        "unfused_equiv_GBs": b / dt / 1e9,
+        "candidates_found": ncand,
+        "candidates_stored": n_sorted,
+        "cand_capacity": cap,
+        "cand_capacity_ok": ncand <= cap,
         "candidates_per_s": ncand / dt,
         "merge_schedule": best,
     }
@@ -286,11 +342,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-cols", type=int, default=1 << 17)
     # the defaults are the autotuner's choice for this workload on MI355X
-    ap.add_argument("--schedule", type=str, default="3,3,3,3", help="comma-separated merge fusion depths")
-    ap.add_argument("--staged", type=int, default=3,
+    ap.add_argument("--schedule", type=str, default=",".join(map(str, workloads.SINGLE_BEAM_TUNED["schedule"])),
+                    help="comma-separated merge fusion depths")
+    ap.add_argument("--staged", type=int, default=workloads.SINGLE_BEAM_TUNED["staged"],
                     help="staging mode 0..4 (0 tree passes, 1 staged passes with a tree final pass, 3 all "
                          "staged; 2, 4 add the fused init)")
-    ap.add_argument("--group-shape", type=str, default="6144,16,32",
+    ap.add_argument("--group-shape", type=str, default=",".join(map(str, workloads.SINGLE_BEAM_TUNED["shape"])),
                     help="staged passes: LDS floats per tile buffer, rows per group, tiles per block")
     ap.add_argument("--tune", action="store_true", help="autotune mode, shape and schedule during warmup instead")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-beam measurement")
@@ -308,20 +365,13 @@ def main():
 
     import torch
 
-    from cpp_projects_amd import synth
     from cpp_projects_amd.search import SingleBeamSearch
 
     c = CFG
     N, R = c["nsamps"], c["nchans"]
     log(f"[bench] generating {R}x{N} u8 filterbank")
-    x, pulses = synth.u8_filterbank(R, N, c["f_min"], c["f_max"], c["seed"], npulses=c["npulses"])
-    cap = 1 << 22
-    s = SingleBeamSearch(c["f_min"], c["f_max"], R, N, c["dt_max"], in_dtype="u8", box_q=c["box_q"],
-                         threshold=c["threshold"], cand_capacity=cap)
-    ndt = s.ndt
-    dmt = torch.empty((ndt, N), dtype=torch.float32, device="cuda")
-    cands = torch.empty((cap, 4), dtype=torch.int32, device="cuda")
-    count = torch.zeros(1, dtype=torch.int64, device="cuda")
+    x, pulses = workloads.single_beam_input()
+    mk = dict(in_dtype="u8", box_q=c["box_q"], threshold=c["threshold"])
     if args.calib_copy_mib > 0:
         nb = args.calib_copy_mib << 20
         a = torch.empty(nb // 4, dtype=torch.float32, device="cuda").fill_(1.0)
@@ -331,30 +381,39 @@ def main():
         torch.cuda.synchronize()
         log(f"[bench] calibration copy of {args.calib_copy_mib} MiB done")
         del a, b
+    # sizing run: the candidate count of this input fixes the buffer
+    count = torch.zeros(1, dtype=torch.int64, device="cuda")
+    probe = torch.empty((1 << 16, 4), dtype=torch.int32, device="cuda")
+    s = SingleBeamSearch(c["f_min"], c["f_max"], R, N, c["dt_max"], cand_capacity=1 << 16, **mk)
+    s.run(x, None, probe, count)
+    cap = sized_capacity(int(count.item()))
+    del s, probe
+    s = SingleBeamSearch(c["f_min"], c["f_max"], R, N, c["dt_max"], cand_capacity=cap, **mk)
+    ndt = s.ndt
+    dmt = torch.empty((ndt, N), dtype=torch.float32, device="cuda")
+    cands = torch.empty((cap, 4), dtype=torch.int32, device="cuda")
+
+    def configure(staged, shape, sched):
+        s.set_staged(int(staged))
+        if int(staged) and shape:
+            s.set_group_shape(*shape)
+        s.set_schedule(sched)
+
+    shape = [int(v) for v in args.group_shape.split(",")]
+    staged = int(args.staged)
     if not args.tune:  # the measured configuration is also the warmed-up one
-        s.set_staged(int(args.staged))
-        if int(args.staged):
-            s.set_group_shape(*[int(v) for v in args.group_shape.split(",")])
-        s.set_schedule([int(v) for v in args.schedule.split(",")])
-    log(f"[bench] ndt={ndt}; warmup {args.warmup}")
+        configure(staged, shape, [int(v) for v in args.schedule.split(",")])
+    log(f"[bench] ndt={ndt}; candidate capacity {cap}; warmup {args.warmup}")
     for _ in range(args.warmup):
         s.run(x, dmt, cands, count)
     torch.cuda.synchronize()
-    shape = None
     if args.tune:
         from cpp_projects_amd.tune import tune_search
 
         best, _ = tune_search(s, lambda: s.run(x, dmt, cands, count), reps=2, log=log)
-        shape = best["shape"]
-        staged = int(best["staged"])
+        shape, staged = best["shape"], int(best["staged"])
+        configure(staged, shape, best["schedule"])
         log(f"[bench] tuned schedule {best}")
-    else:
-        staged = int(args.staged)
-        s.set_staged(staged)
-        if staged:
-            shape = [int(v) for v in args.group_shape.split(",")]
-            s.set_group_shape(*shape)
-        s.set_schedule([int(v) for v in args.schedule.split(",")])
     s.run(x, dmt, cands, count)
     torch.cuda.synchronize()
     schedule = s.get_schedule()
@@ -367,8 +426,9 @@ def main():
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = ndt * N * args.steps / (t1 - t0)
     ncand = int(count.item())
+    n_sorted = s.sort_candidates(cands, count)  # raises CapacityError if the buffer overflowed
 
-    # stage timing with HIP events on the same stream (separate, untimed steps)
+    # stage timing with HIP events on the launch stream (separate, untimed steps)
     s.enable_timing(True)
     stage_sum = None
     reps = max(3, min(args.steps, 5))
@@ -378,40 +438,44 @@ def main():
         stage_sum = st if stage_sum is None else [a + b for a, b in zip(stage_sum, st)]
     s.enable_timing(False)
     stage_ms = [v / reps for v in stage_sum]
-    sbytes = s.stage_bytes()  # per pass: compulsory I/O, one read of its input state + one write
+    sbytes = s.stage_bytes()  # compulsory bytes per stage (live rows, actual input element size)
     npass = len(schedule)
-    # algorithmic bytes of the unfused model: 4*N*(R_{i-1} + R_i) per iteration,
-    # credited to the pass that performs iteration i
+    # unfused byte model credited to each pass (informational only: fusion and
+    # row pruning make it exceed the traffic)
     from cpp_projects_amd.fdmt import FDMTGPU
 
     plan = FDMTGPU(c["f_min"], c["f_max"], R, N, 1.0, c["dt_max"])
     rows = [int(plan.state_shape(i)[3]) for i in range(sum(schedule) + 1)]
     live = live_rows(plan, ndt)
-    alg, io, i0 = [], [], 1
+    unfused, i0 = [], 1
     for L in schedule:
-        alg.append(sum(4.0 * N * (rows[i - 1] + rows[i]) for i in range(i0, i0 + L)))
-        # compulsory traffic: the live rows of the pass's input and output states
-        io.append(4.0 * N * (live[i0 - 1] + live[i0 + L - 1]))
+        unfused.append(sum(4.0 * N * (rows[i - 1] + rows[i]) for i in range(i0, i0 + L)))
         i0 += L
-    pass_ms = stage_ms[3:3 + npass]
-    fused_init = staged in (2, 4) and npass >= 2 and stage_ms[2] < 0.05
-    if fused_init:
-        # the first pass also performs the init: u8 input read, no state 0
-        alg[0] += 4.0 * N * (R + rows[0])
-        io[0] = 1.0 * N * R + 4.0 * N * live[schedule[0]]
 
     def kname(k):
         L, last = schedule[k], k == npass - 1
-        if k == 0 and fused_init:
-            return f"k_merge_group_init<{L}, 1, true>"
         if staged and not (last and staged in (1, 2) and L <= 3):
             # a staged pass is two launches: <L, SNR, true> over the whole
             # tiles past every row's shifts, <L, SNR, false> over the rest
             return f"k_merge_group<{L}, {'true' if last else 'false'}, *>"
         return f"k_merge_fused<{L}, true, {'true' if last else 'false'}>"
 
+    stage_names = (["stats", "mask_transform", "init"] + [f"pass{k}(L={schedule[k]})" for k in range(npass - 1)] +
+                   [f"pass{npass - 1}(L={schedule[-1]})+snr"] + ["snr_norm_separate", "boxcar"])
+    stage_kernels = (["k_row_stats<true> + k_stats_finalize", "k_init_consec<2, true> + k_merge_fused<L, true, false>",
+                      "k_init_consec<1, true>"] + [kname(k) for k in range(npass)] +
+                     ["k_snr_normalise", "k_boxcar_pow2_pf<%d>" % min(max(c["box_q"], 8), 9)])
+    kernels = {}
+    for name, kn, ms, b in zip(stage_names, stage_kernels, stage_ms, sbytes):
+        if ms <= 0.0:
+            continue
+        kernels[name] = {"kernel": kn, "ms": round(ms, 4), "compulsory_bytes": b,
+                         "GBs": round(b / (ms * 1e-3) / 1e9, 1),
+                         "frac_of_peak": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     # dominant kernel: the non-final passes of the kernel instantiation with
     # the most time (the final pass carries the S/N division)
+    pass_ms = stage_ms[3:3 + npass]
+    pass_io = sbytes[3:3 + npass]
     by_kernel = {}
     for k in range(npass - 1):
         by_kernel.setdefault(kname(k), []).append(k)
@@ -420,14 +484,12 @@ def main():
     dom = max(by_kernel, key=lambda d: sum(pass_ms[k] for k in by_kernel[d]))
     ks = by_kernel[dom]
     avg_ms = sum(pass_ms[k] for k in ks) / len(ks)
-    avg_bytes = sum(alg[k] for k in ks) / len(ks)
-    avg_io = sum(io[k] for k in ks) / len(ks)
-    achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+    avg_io = sum(pass_io[k] for k in ks) / len(ks)
+    avg_unfused = sum(unfused[k] for k in ks) / len(ks)
+    achieved = avg_io / (avg_ms * 1e-3) / 1e9
     traffic = load_traffic(dom)
-    names = (["stats", "mask_fdmt", "init"] + [f"pass{k}(L={schedule[k]})" for k in range(npass - 1)] +
-             [f"pass{npass - 1}(L={schedule[-1]})+snr"] + ["snr_norm_separate", "boxcar"])
-    stages = {n: round(v, 4) for n, v in zip(names, stage_ms)}
-    total_alg = sum(sbytes[:2 if fused_init else 3]) + sum(alg) + sum(sbytes[3 + npass:])
+    step_comp = sum(sbytes)
+    step_ms = sum(stage_ms)
 
     out = {
         "metric": METRIC,
@@ -445,7 +507,9 @@ def main():
         "config": {
             "workload": "single-beam search 4096ch x 262144 u8 (1200-1600 MHz), dt_max 4095, "
                         "global norm + S/N norm, boxcar widths 1..256 pow2, thr 6",
-            "ndt": ndt, "nsamps": N, "nchans": R, "candidates": ncand, "path": "plain (one beam per launch sequence)",
+            "ndt": ndt, "nsamps": N, "nchans": R, "path": "plain (one beam per launch sequence)",
+            "candidates_found": ncand, "candidates_stored": n_sorted, "cand_capacity": cap,
+            "cand_capacity_ok": ncand <= cap,
             "merge_schedule": schedule,
             "staged_passes": staged,
             "group_shape": shape if staged else None,
@@ -461,31 +525,30 @@ def main():
             "traffic": traffic,
             "traffic_GBs": traffic / (avg_ms * 1e-3) / 1e9 if traffic else None,
             "traffic_frac": traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic else None,
-            "alg_bytes_per_launch": avg_bytes,
+            "compulsory_bytes_per_launch": avg_io,
             "avg_launch_ms": avg_ms,
             "launches_per_step": len(ks),
-            "compulsory_io_bytes_per_launch": avg_io,
-            "compulsory_io_GBs": avg_io / (avg_ms * 1e-3) / 1e9,
-            "note": "achieved credits the unfused byte model (every state row read and written once per "
-                    "iteration); traffic is the counter-measured HBM bytes of one launch (rocprofv3, "
-                    "profiles/merge_traffic.json); for a staged pass a 'launch' is its two kernel launches "
-                    "(fast-path tiles and the first tiles), timed together by the stage events",
+            "unfused_equiv_GBs": avg_unfused / (avg_ms * 1e-3) / 1e9,
+            "note": "achieved = compulsory bytes of one pass (one read of the live rows of its input state, "
+                    "one write of the live rows of its output state) / the pass's event-timed duration; "
+                    "traffic = counter-measured HBM bytes of one pass (rocprofv3, profiles/merge_traffic.json); "
+                    "unfused_equiv_GBs credits the unfused byte model (every row of every intermediate state "
+                    "read and written once) and is not a fraction of peak; a staged pass is two kernel "
+                    "launches (whole tiles past every row's shifts, and the first tiles), timed together",
         },
-        "stages_ms": stages,
-        "effective_GBs_whole_step": total_alg / (sum(stage_ms) * 1e-3) / 1e9,
+        "kernels": kernels,
+        "step_compulsory_bytes": step_comp,
+        "step_compulsory_frac": step_comp / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
     }
     xs = x[:, : args.cpu_sample_cols].cpu().numpy() if not args.no_cpu_baseline else None
+    del s, dmt, cands, count, x
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
     if not args.no_batch:
-        del s, dmt, cands, count, x
-        torch.cuda.synchronize()
-        torch.cuda.empty_cache()
         out["batched"] = bench_batch(args)
+        torch.cuda.synchronize()
         torch.cuda.empty_cache()
     if not args.no_large:
-        if args.no_batch:
-            del s, dmt, cands, count, x
-        torch.cuda.synchronize()
-        torch.cuda.empty_cache()
         out["large"] = bench_large(args)
         torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
@@ -493,9 +556,21 @@ def main():
         log(f"[bench] cpu baseline on {R}x{args.cpu_sample_cols} with {threads} threads")
         v, dt, nc = cpu_baseline(xs, ndt, args.cpu_sample_cols, threads)
         out["cpu_baseline"] = {"value": v, "unit": "DM-trials*samples/s", "cores": threads, "kind": "port",
+                               "cpu_model": cpu_model(), "nproc": os.cpu_count(),
                                "sample": f"{R}ch x {args.cpu_sample_cols} samples (first slice of the beam), "
                                          f"same chain, {dt:.2f} s"}
     print(json.dumps(out))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 if __name__ == "__main__":

@@ -95,6 +95,7 @@ SIGNATURES = {
     "sps_search_destroy": [_vp],
     "sps_search_run": [_vp, _vp, _vp, _vp, _vp, _vp],
     "sps_search_sort_candidates": [_vp, _vp, _vp, _pi64, _vp],
+    "sps_search_gather": [_vp, _vp, _vp, _i64, _vp, _pi64, _vp],
     "sps_search_get_ndt": [_vp, _pi64],
     "sps_fdmt_set_schedule": [_vp, C.POINTER(C.c_int32), _i64],
     "sps_fdmt_get_schedule": [_vp, C.POINTER(C.c_int32), _i64, _pi64],

@@ -10,38 +10,18 @@
 #include <string>
 #include <vector>
 
+#include "abi.hpp"
 #include "common.hpp"
 #include "kernels.hpp"
 #include "plan.hpp"
 
 using namespace sps;
 
-namespace {
-
+namespace sps {
 thread_local std::string g_last_error;
-
-template <class F>
-sps_status guarded(F&& f) {
-    try {
-        f();
-        return SPS_OK;
-    } catch (const sps::Error& e) {
-        g_last_error = e.what();
-        return e.status;
-    } catch (const std::invalid_argument& e) {
-        g_last_error = e.what();
-        return SPS_ERR_INVALID_ARGUMENT;
-    } catch (const std::bad_alloc&) {
-        g_last_error = "host allocation failed";
-        return SPS_ERR_NO_MEMORY;
-    } catch (const std::exception& e) {
-        g_last_error = e.what();
-        return SPS_ERR_RUNTIME;
-    } catch (...) {
-        g_last_error = "unknown error";
-        return SPS_ERR_RUNTIME;
-    }
 }
+
+namespace {
 
 template <class T>
 void upload(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
@@ -76,6 +56,7 @@ struct sps_fdmt {
     std::vector<int64_t> d_rows_n;                // per iteration: number of those rows
     DevBuf state_a, state_b;
     int64_t rows_a = 0, rows_b = 0;    // capacity of the ping-pong states in rows
+    DevBuf host_in, host_out;          // device copies for the host-pointer calls
     int dmax = 0;                      // largest init dt of any channel
     int grid_consec = 0;               // every channel's init dts are 0, 1, ..., K-1 with K <= 5
     // adjoint tables
@@ -833,6 +814,70 @@ SPS_API sps_status sps_fdmt_initialise(sps_fdmt* h, const void* d_wf, int dtype,
     });
 }
 
+SPS_API sps_status sps_fdmt_execute_host(sps_fdmt* h, const float* h_waterfall, size_t waterfall_size, float* h_dmt,
+                                         size_t dmt_size, int ones) {
+    return guarded([&] {
+        SPS_REQUIRE(h && h_dmt, "null argument");
+        const EnginePlan& e = h->eng;
+        if (!ones) {
+            SPS_REQUIRE(h_waterfall, "null waterfall");
+            if ((int64_t)waterfall_size != e.nchans * e.nsamps)
+                throw Error(SPS_ERR_INVALID_ARGUMENT, "Invalid size of waterfall");
+        }
+        if ((int64_t)dmt_size != e.out_rows * e.nsamps) throw Error(SPS_ERR_INVALID_ARGUMENT, "Invalid size of dmt");
+        hipStream_t st = nullptr;
+        h->ensure_device(st);
+        h->ensure_states();
+        const void* d_in = nullptr;
+        if (!ones) {
+            h->host_in.ensure(sizeof(float) * waterfall_size);
+            SPS_HIP(hipMemcpyAsync(h->host_in.p, h_waterfall, sizeof(float) * waterfall_size, hipMemcpyHostToDevice,
+                                   st));
+            d_in = h->host_in.p;
+        }
+        h->host_out.ensure(sizeof(float) * dmt_size);
+        h->run(d_in, 0, ones, e.nsamps, nullptr, e.nsamps, h->host_out.as<float>(), e.nsamps, h->state_a.as<float>(),
+               h->state_b.as<float>(), h->pitch, st);
+        SPS_HIP(hipMemcpyAsync(h_dmt, h->host_out.p, sizeof(float) * dmt_size, hipMemcpyDeviceToHost, st));
+        SPS_HIP(hipStreamSynchronize(st));
+    });
+}
+
+SPS_API sps_status sps_fdmt_initialise_host(sps_fdmt* h, const float* h_waterfall, float* h_state, size_t state_size) {
+    return guarded([&] {
+        SPS_REQUIRE(h && h_waterfall && h_state, "null argument");
+        const EnginePlan& e = h->eng;
+        SPS_REQUIRE((int64_t)state_size == e.rows[0] * e.nsamps, "Invalid size of state");
+        hipStream_t st = nullptr;
+        h->ensure_device(st);
+        const size_t nwf = (size_t)(e.nchans * e.nsamps);
+        h->host_in.ensure(sizeof(float) * nwf);
+        h->host_out.ensure(sizeof(float) * state_size);
+        SPS_HIP(hipMemcpyAsync(h->host_in.p, h_waterfall, sizeof(float) * nwf, hipMemcpyHostToDevice, st));
+        launch_fdmt_init(e.init_mode, h->host_in.p, 0, 0, e.nsamps, nullptr, e.nchans, e.nsamps,
+                         h->d_grid.as<int32_t>(), h->d_grid_off.as<int64_t>(), h->d_chan_row_off.as<int64_t>(),
+                         h->host_out.as<float>(), e.nsamps, st, h->dmax, nullptr, h->grid_consec);
+        SPS_HIP(hipMemcpyAsync(h_state, h->host_out.p, sizeof(float) * state_size, hipMemcpyDeviceToHost, st));
+        SPS_HIP(hipStreamSynchronize(st));
+    });
+}
+
+SPS_API sps_status sps_cfdmt_get_config(const sps_fdmt* h, int64_t iter, int32_t* h_quant, int32_t* h_rows_cumsum,
+                                        float* h_freq) {
+    return guarded([&] {
+        SPS_REQUIRE(h && h_quant, "null argument");
+        SPS_REQUIRE(h->cplan, "not a legacy CFdmt handle");
+        const CFdmtPlan& c = *h->cplan;
+        SPS_REQUIRE(iter >= 0 && iter <= c.niter, "iteration out of range");
+        const int q = c.quant[(size_t)iter];
+        *h_quant = q;
+        for (int k = 0; k <= q; ++k) {
+            if (h_rows_cumsum) h_rows_cumsum[k] = c.cumsum[(size_t)iter][(size_t)k];
+            if (h_freq) h_freq[k] = c.freq[(size_t)iter][(size_t)k];
+        }
+    });
+}
+
 SPS_API sps_status sps_fdmt_adjoint(sps_fdmt* h, const float* g_dmt, size_t dmt_size, float* g_wf, size_t wf_size,
                                     void* stream) {
     return guarded([&] {
@@ -1351,19 +1396,24 @@ SPS_API sps_status sps_search_get_timing(sps_search* s, double* h_ms, int64_t ca
 SPS_API sps_status sps_search_get_stage_bytes(const sps_search* s, double* h_b, int64_t cap, int64_t* h_n) {
     return guarded([&] {
         SPS_REQUIRE(s && h_n, "null argument");
-        const EnginePlan& e = s->f->eng;
-        const double N4 = 4.0 * (double)s->p.nsamps;
+        const sps_fdmt& f = *s->f;
+        const EnginePlan& e = f.eng;
+        const double N = (double)s->p.nsamps, N4 = 4.0 * N;
+        const double in_elem = s->p.in_dtype == SPS_U8 ? 1.0 : 4.0;
+        const double in_bytes = in_elem * N * (double)e.nchans;
         std::vector<double> b;
-        b.push_back(N4 * (double)e.nchans);                        // stats: one input read (fp32 figure)
-        b.push_back(0.0);                                          // mask transform (not counted)
-        b.push_back(N4 * (double)(e.nchans + e.rows[0]));          // init: input read + state 0 write
+        b.push_back(s->p.norm_mode != SPS_NORM_NONE ? in_bytes : 0.0);  // statistics: one input read
+        // mask transform: its output image (ndt rows over its extent), written
+        // once and read once by the final pass; its states stay in the caches
+        b.push_back(s->p.snr_norm ? 2.0 * 4.0 * (double)s->norm_len * (double)s->ndt : 0.0);
+        b.push_back(in_bytes + N4 * (double)f.live_rows(0));             // init: input read + live state-0 rows
         int i0 = 1;
-        for (int L : s->f->schedule) {  // a pass reads state i0-1 once and writes state i0+L-1 once
-            b.push_back(N4 * (double)(e.rows[(size_t)i0 - 1] + e.rows[(size_t)(i0 + L - 1)]));
+        for (int L : f.schedule) {  // a pass reads the live rows of state i0-1 and writes those of state i0+L-1
+            b.push_back(N4 * (double)(f.live_rows(i0 - 1) + f.live_rows(i0 + L - 1)));
             i0 += L;
         }
-        b.push_back(0.0);                                          // S/N normalisation (fused: not counted)
-        b.push_back(N4 * (double)e.rows.back());                   // boxcar: one read of the image
+        b.push_back(0.0);                                                // S/N normalisation (fused into the last pass)
+        b.push_back(N4 * (double)s->ndt);                                // boxcar: one read of the image
         *h_n = (int64_t)b.size();
         for (int64_t i = 0; i < (int64_t)b.size() && i < cap; ++i) h_b[i] = b[(size_t)i];
     });
@@ -1380,6 +1430,30 @@ SPS_API sps_status sps_search_sort_candidates(sps_search* s, sps_cand* d_cands, 
         const int64_t m = std::min<int64_t>((int64_t)n, s->p.cand_capacity);
         sort_cands(reinterpret_cast<CandD*>(d_cands), m, s->ndt, s->p.nsamps, s->p.box_q, st);
         *h_count = (int64_t)n;
+        // the kept prefix is whichever candidates won the counter first: the
+        // caller must re-gather into a buffer of h_count entries
+        if ((int64_t)n > s->p.cand_capacity)
+            throw Error(SPS_ERR_CAPACITY, "more candidates than cand_capacity: re-gather with sps_search_gather");
+    });
+}
+
+SPS_API sps_status sps_search_gather(sps_search* s, const float* d_dmt, sps_cand* d_cands, int64_t cap,
+                                     uint64_t* d_count, int64_t* h_count, void* stream) {
+    return guarded([&] {
+        SPS_REQUIRE(s && d_count && h_count && cap >= 0 && (cap == 0 || d_cands), "null argument");
+        const float* img = d_dmt ? d_dmt : s->dmt.as<float>();
+        SPS_REQUIRE(img, "no DM-time image: run the search first (or pass the image it wrote)");
+        hipStream_t st = as_stream(stream);
+        const int64_t N = s->p.nsamps;
+        launch_fill_u64(reinterpret_cast<unsigned long long*>(d_count), 1, 0ull, st);
+        launch_boxcar(img, s->ndt, N, N, nullptr, s->p.threshold, 1 << s->p.box_q, 1,
+                      reinterpret_cast<CandD*>(d_cands), cap, reinterpret_cast<unsigned long long*>(d_count), st);
+        unsigned long long n = 0;
+        SPS_HIP(hipMemcpyAsync(&n, d_count, sizeof(n), hipMemcpyDeviceToHost, st));
+        SPS_HIP(hipStreamSynchronize(st));
+        sort_cands(reinterpret_cast<CandD*>(d_cands), std::min<int64_t>((int64_t)n, cap), s->ndt, N, s->p.box_q, st);
+        *h_count = (int64_t)n;
+        if ((int64_t)n > cap) throw Error(SPS_ERR_CAPACITY, "more candidates than the gather's capacity");
     });
 }
 
@@ -1660,6 +1734,8 @@ SPS_API sps_status sps_batch_sort_candidates(sps_batch* b, int64_t beam, int64_t
         const int64_t m = std::min<int64_t>((int64_t)n, bm.io.cand_capacity);
         sort_cands(reinterpret_cast<CandD*>(bm.io.d_cands), m, bm.s->ndt, bm.s->p.nsamps, bm.s->p.box_q, st);
         *h_count = (int64_t)n;
+        if ((int64_t)n > bm.io.cand_capacity)
+            throw Error(SPS_ERR_CAPACITY, "more candidates than the beam's cand_capacity");
     });
 }
 

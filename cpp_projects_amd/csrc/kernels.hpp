@@ -184,6 +184,14 @@ void launch_digitize(float* x, int64_t n, hipStream_t st);
 void launch_detect_max(const float* a, const float* n, int64_t rows, int64_t cols, int wnd, void* d_block_best,
                        int64_t nblocks_cap, void* d_out, hipStream_t st);
 int64_t detect_max_blocks(int64_t rows, int64_t cols);
+// tiled detectors of the reference's per-block kernels (mode 1: boxcar S/N of
+// x / sqrtf(1e-7 + n) inside a tile; mode 2: energy ratio sqrt(sum x^2 / sum n^2));
+// per-tile results in the caller's arrays of rows * ceil(cols / tile) entries
+void launch_detect_tiles(int mode, const float* x, const float* n, int64_t rows, int64_t cols, int tile, int wnd,
+                         float* tile_val, int32_t* tile_pos, int32_t* tile_w, void* d_out, hipStream_t st);
+// zeroes the rows not kept, writes a byte mask and {row variances, mean, std of variances}
+void launch_clean_rows(float* x, uint8_t* mask, int64_t rows, int64_t cols, const uint8_t* d_keep,
+                       const double* d_row_var, const double* d_stats, float* d_buff, hipStream_t st);
 
 // ---- sorting / scans / clustering ----
 // stable LSD radix sort of (key, val) pairs over key bits [0, nbits); the
@@ -209,6 +217,9 @@ void launch_delegate_keys(const CandD* c, const unsigned long long* best, const 
                           int64_t n, uint64_t* keys, uint32_t* vals, hipStream_t st);
 void launch_delegate_out(const unsigned long long* best, const uint32_t* start, const uint32_t* order, int64_t ng,
                          int64_t n, int32_t* delegates, int64_t* sizes, hipStream_t st);
+// per caller group [begin[g], begin[g+1]) of idx: the index with the largest msnr (first on ties)
+void launch_select_delegates(const CandD* c, const int32_t* idx, int64_t nidx, const int32_t* begin, int64_t ngroups,
+                             int32_t* out, hipStream_t st);
 void launch_iota_u32(uint32_t* v, int64_t n, hipStream_t st);
 void launch_fill_u64(unsigned long long* v, int64_t n, unsigned long long val, hipStream_t st);
 

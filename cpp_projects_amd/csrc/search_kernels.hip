@@ -1171,6 +1171,158 @@ __global__ void __launch_bounds__(kThreads) k_detect_final(const Best* __restric
     }
 }
 
+// ---------------------------------------------------------------------------
+// tiled single-maximum detectors with the window arithmetic of the
+// reference's per-block kernels (detection/kernel.cu:42-226): a row is cut
+// into tiles of `tile` columns (the caller's block width); inside a tile the
+// values are prefix sums accumulated sequentially in fp32 from the tile's
+// first column, and a window's value is a difference of two prefix sums.
+//   mode 1 (multi_windowing_kernel_v1, :42-133): y = x / sqrtf(1e-7 + n),
+//     value = (Y[t+w] - Y[t]) / sqrtf(w); the window lies inside its tile.
+//   mode 2 (multi_windowing_kernel_v2, :136-226): value =
+//     sqrtf((X2[t+w] - X2[t]) / (N2[t+w] - N2[t])) with X2, N2 the prefix sums
+//     of x^2 and n^2 over the tile and the W samples after it, and N2[0] =
+//     1e-6 as the reference initialises it.
+// Windows must end inside the row.  Per tile the best window (largest value;
+// ties to the smaller width, then column) goes to the caller's per-tile
+// arrays {value, row*cols + column, width}; a second kernel reduces them
+// (ties to the smaller width, then row, then column).
+// ---------------------------------------------------------------------------
+constexpr int kDetWave = 64;
+
+__device__ __forceinline__ bool det_better(float v, int w, int64_t pos, float bv, int bw, int64_t bpos) {
+    if (!(v == v)) return false;  // NaN never wins
+    if (bw < 0 || !(bv == bv)) return true;
+    if (v != bv) return v > bv;
+    if (w != bw) return w < bw;
+    return pos < bpos;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(kDetWave) k_detect_tiles(const float* __restrict__ x, const float* __restrict__ n,
+                                                           int64_t cols, int tile, int wnd,
+                                                           float* __restrict__ tile_val,
+                                                           int32_t* __restrict__ tile_pos,
+                                                           int32_t* __restrict__ tile_w) {
+    extern __shared__ float det_lds[];
+    const int64_t r = blockIdx.y;
+    const int64_t c0 = (int64_t)blockIdx.x * tile;
+    // samples this tile's prefix sums cover
+    const int64_t span = MODE == 1 ? tile : (int64_t)tile + wnd;
+    const int64_t navail = cols - c0 < span ? cols - c0 : span;
+    float* A = det_lds;             // per-sample y (mode 1) or x^2 (mode 2), then prefix sums
+    float* B = det_lds + span + 1;  // n^2 then its prefix sums (mode 2)
+    const float* xr = x + r * cols + c0;
+    const float* nr = n ? n + r * cols + c0 : nullptr;
+    for (int64_t i = threadIdx.x; i < navail; i += kDetWave) {
+        const float xv = xr[i];
+        if (MODE == 1) {
+            A[i + 1] = nr ? xv / sqrtf((float)(1.0E-7 + (double)nr[i])) : xv;
+        } else {
+            const float nv = nr ? nr[i] : 1.0f;
+            A[i + 1] = xv * xv;
+            B[i + 1] = nv * nv;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float s = 0.0f, q = 0.0f;
+        A[0] = 0.0f;
+        if (MODE == 2) B[0] = 0.000001f;
+        for (int64_t i = 0; i < navail; ++i) {
+            s = s + A[i + 1];
+            A[i + 1] = s;
+            if (MODE == 2) {
+                q = q + B[i + 1];
+                B[i + 1] = q;
+            }
+        }
+    }
+    __syncthreads();
+    float bv = 0.0f;
+    int bw = -1;
+    int64_t bpos = 0;
+    const int64_t tmax = cols - c0 < tile ? cols - c0 : tile;
+    for (int64_t t = threadIdx.x; t < tmax; t += kDetWave) {
+        for (int w = 1; w <= wnd; ++w) {
+            if (t + w > navail) break;                  // past the row's end (or, mode 1, the tile's)
+            float v;
+            if (MODE == 1)
+                v = (A[t + w] - A[t]) / sqrtf((float)w);
+            else
+                v = sqrtf((A[t + w] - A[t]) / (B[t + w] - B[t]));
+            if (det_better(v, w, t, bv, bw, bpos)) {
+                bv = v;
+                bw = w;
+                bpos = t;
+            }
+        }
+    }
+    // wave reduction: (value desc, width asc, column asc)
+    for (int off = kDetWave / 2; off > 0; off >>= 1) {
+        const float ov = __shfl_down(bv, off);
+        const int ow = __shfl_down(bw, off);
+        const int64_t op = __shfl_down(bpos, off);
+        if (ow >= 0 && det_better(ov, ow, op, bv, bw, bpos)) {
+            bv = ov;
+            bw = ow;
+            bpos = op;
+        }
+    }
+    if (threadIdx.x == 0) {
+        const int64_t k = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+        tile_val[k] = bv;
+        tile_pos[k] = bw >= 0 ? (int32_t)(r * cols + c0 + bpos) : -1;
+        tile_w[k] = bw;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) k_detect_tiles_final(const float* __restrict__ tile_val,
+                                                                 const int32_t* __restrict__ tile_pos,
+                                                                 const int32_t* __restrict__ tile_w, int64_t ntiles,
+                                                                 int64_t cols, sps_detection* __restrict__ out) {
+    __shared__ Best sh[kThreads];
+    Best b{0.0f, -1, -1, -1};
+    for (int64_t i = threadIdx.x; i < ntiles; i += kThreads) {
+        if (tile_w[i] < 0) continue;
+        Best c{tile_val[i], tile_w[i], (int32_t)(tile_pos[i] / cols), (int32_t)(tile_pos[i] % cols)};
+        if (c.snr == c.snr && better(c, b)) b = c;
+    }
+    block_best(b, sh);
+    if (threadIdx.x == 0) {
+        out->iwidth = sh[0].w;
+        out->irow = sh[0].r;
+        out->icol = sh[0].t;
+        out->snr = sh[0].snr;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// row cleaning without normalisation (clean_fdmtInput_and_create_normInput,
+// CDMT/Statistical_preprocessing.cu:102-120,142-268): rows whose variance is
+// more than 4 sigma from the mean row variance are zeroed; the 0/1 mask is a
+// byte per sample; buff = {row variances[R], mean variance, std of variances}.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads) k_clean_rows(float* __restrict__ x, uint8_t* __restrict__ mask,
+                                                         int64_t cols, const uint8_t* __restrict__ keep) {
+    const int64_t r = blockIdx.y;
+    const bool k = keep[r] != 0;
+    for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < cols; t += (int64_t)gridDim.x * kThreads) {
+        const int64_t i = r * cols + t;
+        if (!k) x[i] = 0.0f;
+        if (mask) mask[i] = k ? 1 : 0;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) k_clean_buff(const double* __restrict__ rv, const double* __restrict__ stats,
+                                                         int64_t rows, float* __restrict__ buff) {
+    for (int64_t r = threadIdx.x; r < rows; r += kThreads) buff[r] = (float)rv[r];
+    if (threadIdx.x == 0) {
+        buff[rows] = (float)stats[2];
+        buff[rows + 1] = (float)stats[3];
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1297,6 +1449,38 @@ void launch_boxcar_snr_adjoint(const float* g, int64_t rows, int64_t cols, int w
 }
 
 int64_t detect_max_blocks(int64_t rows, int64_t cols) { return ceil_div(cols, kDetCells) * rows; }
+
+void launch_detect_tiles(int mode, const float* x, const float* n, int64_t rows, int64_t cols, int tile, int wnd,
+                         float* tile_val, int32_t* tile_pos, int32_t* tile_w, void* d_out, hipStream_t st) {
+    if (mode != 1 && mode != 2) throw Error(SPS_ERR_INVALID_ARGUMENT, "detector mode must be 1 or 2");
+    if (tile < 1 || tile > 4096 || wnd < 1 || wnd > 4096) throw Error(SPS_ERR_INVALID_ARGUMENT, "tile and window must be 1..4096");
+    if (rows < 1 || rows > 65535 || cols < 1) throw Error(SPS_ERR_INVALID_ARGUMENT, "rows must be 1..65535");
+    if (cols > 0x7fffffffLL / rows)
+        throw Error(SPS_ERR_INVALID_ARGUMENT, "the per-tile index row*cols+column is a 32-bit int");
+    const int64_t nt = ceil_div(cols, tile);
+    if (nt > 0x7fffffffLL) throw Error(SPS_ERR_INVALID_ARGUMENT, "too many tiles");
+    const int64_t span = mode == 1 ? tile : (int64_t)tile + wnd;
+    const size_t lds = sizeof(float) * (size_t)(2 * (span + 1));
+    const dim3 grid((uint32_t)nt, (uint32_t)rows);
+    if (mode == 1)
+        hipLaunchKernelGGL(k_detect_tiles<1>, grid, dim3(kDetWave), lds, st, x, n, cols, tile, wnd, tile_val, tile_pos,
+                           tile_w);
+    else
+        hipLaunchKernelGGL(k_detect_tiles<2>, grid, dim3(kDetWave), lds, st, x, n, cols, tile, wnd, tile_val, tile_pos,
+                           tile_w);
+    hipLaunchKernelGGL(k_detect_tiles_final, dim3(1), dim3(kThreads), 0, st, tile_val, tile_pos, tile_w, nt * rows,
+                       cols, static_cast<sps_detection*>(d_out));
+    SPS_HIP(hipGetLastError());
+}
+
+void launch_clean_rows(float* x, uint8_t* mask, int64_t rows, int64_t cols, const uint8_t* d_keep,
+                       const double* d_row_var, const double* d_stats, float* d_buff, hipStream_t st) {
+    if (rows <= 0 || cols <= 0) return;
+    if (rows > 65535) throw Error(SPS_ERR_INVALID_ARGUMENT, "more than 65535 rows");
+    hipLaunchKernelGGL(k_clean_rows, row_grid(rows, cols), dim3(kThreads), 0, st, x, mask, cols, d_keep);
+    if (d_buff) hipLaunchKernelGGL(k_clean_buff, dim3(1), dim3(kThreads), 0, st, d_row_var, d_stats, rows, d_buff);
+    SPS_HIP(hipGetLastError());
+}
 
 void launch_detect_max(const float* a, const float* n, int64_t rows, int64_t cols, int wnd, void* d_block_best,
                        int64_t nblocks_cap, void* d_out, hipStream_t st) {

@@ -252,6 +252,47 @@ __global__ void k_fill_u64(unsigned long long* v, int64_t n, unsigned long long 
 
 inline uint32_t grid1(int64_t n) { return (uint32_t)ceil_div(n > 0 ? n : 1, kThreads); }
 
+// select_delegates (select_delegates.cu:16-51,62-125) over caller-provided
+// groups: group g is positions [begin[g], begin[g+1]) (the last ends at
+// nidx) of the index array; its delegate is the candidate index there with
+// the largest msnr, the first such position on ties -- the rule of
+// thrust::max_element with a less-than comparator, which also keeps a NaN
+// found at the group's first position.  One wave per group.
+__global__ void __launch_bounds__(64) k_select_delegates(const CandD* __restrict__ c, const int32_t* __restrict__ idx,
+                                                         int64_t nidx, const int32_t* __restrict__ begin,
+                                                         int64_t ngroups, int32_t* __restrict__ out) {
+    const int64_t g = blockIdx.x;
+    const int64_t b = begin[g];
+    const int64_t e = g + 1 < ngroups ? (int64_t)begin[g + 1] : nidx;
+    if (b >= e) {
+        if (threadIdx.x == 0) out[g] = -1;
+        return;
+    }
+    const float first = c[idx[b]].msnr;
+    if (!(first == first)) {
+        if (threadIdx.x == 0) out[g] = idx[b];
+        return;
+    }
+    float bv = first;
+    int64_t bp = b;
+    for (int64_t p = b + threadIdx.x; p < e; p += 64) {
+        const float v = c[idx[p]].msnr;
+        if (v > bv || (v == bv && p < bp)) {
+            bv = v;
+            bp = p;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const float ov = __shfl_down(bv, off);
+        const int64_t op = __shfl_down(bp, off);
+        if (ov > bv || (ov == bv && op < bp)) {
+            bv = ov;
+            bp = op;
+        }
+    }
+    if (threadIdx.x == 0) out[g] = idx[bp];
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -376,6 +417,14 @@ void launch_iota_u32(uint32_t* v, int64_t n, hipStream_t st) {
 void launch_fill_u64(unsigned long long* v, int64_t n, unsigned long long val, hipStream_t st) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_fill_u64, dim3(grid1(n)), dim3(kThreads), 0, st, v, n, val);
+    SPS_HIP(hipGetLastError());
+}
+
+void launch_select_delegates(const CandD* c, const int32_t* idx, int64_t nidx, const int32_t* begin, int64_t ngroups,
+                             int32_t* out, hipStream_t st) {
+    if (ngroups <= 0) return;
+    if (ngroups > 0x7fffffffLL) throw Error(SPS_ERR_INVALID_ARGUMENT, "too many groups");
+    hipLaunchKernelGGL(k_select_delegates, dim3((uint32_t)ngroups), dim3(64), 0, st, c, idx, nidx, begin, ngroups, out);
     SPS_HIP(hipGetLastError());
 }
 

@@ -213,8 +213,11 @@ class SingleBeamSearch:
         call("sps_search_enable_timing", self._h, int(on))
 
     def stage_times_ms(self):
-        """Per-stage durations of the last run (HIP events on the run's stream):
-        [stats, init, merge_1..merge_n, mask transform, S/N norm, boxcar]."""
+        """Per-stage durations of the last run (HIP events on the run's
+        stream), in launch order: [statistics, mask transform, FDMT init,
+        pass 1 .. pass P (each pass fuses the iterations its schedule entry
+        names; the last carries the S/N division), separate S/N
+        normalisation (0 when fused), boxcar]."""
         n = C.c_int64()
         buf = (C.c_double * 64)()
         call("sps_search_get_timing", self._h, buf, 64, C.byref(n))
@@ -227,6 +230,39 @@ class SingleBeamSearch:
         return list(buf)[: n.value]
 
     def sort_candidates(self, d_cands, d_count):
+        """Sorts the run's candidates by (mdt, mt, mwidth) and returns their
+        count.  Raises CapacityError (with .count) when the run found more
+        than cand_capacity: the kept subset is then arbitrary and the caller
+        re-gathers with gather() into a buffer of .count entries."""
         n = C.c_int64()
-        call("sps_search_sort_candidates", self._h, _dptr(d_cands), _dptr(d_count), C.byref(n), _stream())
+        st = _lib.lib().sps_search_sort_candidates(self._h, _dptr(d_cands), _dptr(d_count), C.byref(n), _stream())
+        if st == _lib.SPS_ERR_CAPACITY:
+            e = _lib.CapacityError(st, _lib.lib().sps_last_error().decode(errors="replace"))
+            e.count = n.value
+            raise e
+        _lib.check(st)
         return n.value
+
+    def gather(self, d_dmt, d_cands, d_count):
+        """Re-runs only the boxcar gather of the last run over its S/N image
+        (d_dmt as passed to run(), None when the run kept it internally) into
+        d_cands ((cap, 4) int32), sorted; returns the count (CapacityError
+        when it exceeds cap)."""
+        n = C.c_int64()
+        call("sps_search_gather", self._h, _dptr(d_dmt), _dptr(d_cands), int(d_cands.shape[0]), _dptr(d_count),
+             C.byref(n), _stream())
+        return n.value
+
+    def candidates(self, d_dmt, d_cands, d_count):
+        """Sorted candidates of the last run: (tensor (n, 4), n).  When the run
+        overflowed d_cands, a buffer of exactly the count is allocated and
+        re-gathered, so no candidate is ever dropped."""
+        import torch
+
+        try:
+            n = self.sort_candidates(d_cands, d_count)
+            return d_cands[:n], n
+        except _lib.CapacityError as e:
+            big = torch.empty((e.count, 4), dtype=torch.int32, device=d_cands.device)
+            n = self.gather(d_dmt, big, d_count)
+            return big[:n], n

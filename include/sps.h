@@ -145,6 +145,29 @@ SPS_API sps_status sps_fdmt_set_staged(sps_fdmt* h, int staged);
  * 16) and consecutive 512-sample tiles streamed per block (1..512, default 8). */
 SPS_API sps_status sps_fdmt_set_group_shape(sps_fdmt* h, int lds_floats, int max_rows, int tiles_per_block);
 
+/* FDMTCPU::execute (FDMT_PRAVIR_RESTRICT/fdmt_cpu.hpp:5-20, fdmt_cpu.cpp:28-54)
+ * and CFdmtCpu::process_image (FDMT/FdmtCpu.h:6-34, FdmtCpu.cpp:197-249):
+ * the same transform as sps_fdmt_execute on HOST buffers (f32 waterfall,
+ * nchans x nsamps; dmt, output rows x nsamps).  The work runs on the GPU: the
+ * waterfall is copied in, transformed and the result copied back before the
+ * call returns (default stream, synchronous).  Sizes are checked like
+ * check_inputs.  ones != 0: transform of an all-ones input (h_waterfall may
+ * be NULL), the b_ones mode of process_image. */
+SPS_API sps_status sps_fdmt_execute_host(sps_fdmt* h, const float* h_waterfall, size_t waterfall_size, float* h_dmt,
+                                         size_t dmt_size, int ones);
+/* FDMTCPU::initialise (fdmt_cpu.hpp:13) on host buffers: the iteration-0
+ * state (rows0 x nsamps) of a host waterfall, computed on the GPU. */
+SPS_API sps_status sps_fdmt_initialise_host(sps_fdmt* h, const float* h_waterfall, float* h_state,
+                                            size_t state_size);
+/* Legacy plan configuration of a sps_cfdmt_create handle (CFdmtCpu::
+ * create_config, FDMT/FdmtCpu.cpp:143-194; the members m_parrQuantMtrx_h,
+ * m_pparrRowsCumSum_h, m_pparrFreq_h of FDMT/FdmtCpu.h:21-24): for iteration
+ * iter (0..niters), h_quant receives the sub-matrix count q, and
+ * h_rows_cumsum / h_freq (q + 1 entries each, may be NULL) the cumulative
+ * row offsets and the sub-band edge frequencies. */
+SPS_API sps_status sps_cfdmt_get_config(const sps_fdmt* h, int64_t iter, int32_t* h_quant, int32_t* h_rows_cumsum,
+                                        float* h_freq);
+
 /* FDMT::initialise (fdmt_base.hpp:57): the iteration-0 state, rows0 x nsamps */
 SPS_API sps_status sps_fdmt_initialise(sps_fdmt* h, const void* d_waterfall, int dtype, float* d_state,
                                        size_t state_size, void* stream);
@@ -204,6 +227,59 @@ SPS_API sps_status sps_cluster(const sps_cand* d_cands, int64_t n, const int32_t
 SPS_API sps_status sps_detect_max(const float* d_arr, const float* d_norm, int64_t rows, int64_t cols, int wnd,
                                   sps_detection* d_out, void* stream);
 
+/* select_delegates::select_delegates / select_delegates_kernel
+ * (select_delegates.cuh:4-14, select_delegates.cu:16-125) over groups the
+ * caller made: group g is positions [d_group_begin[g], d_group_begin[g+1])
+ * of d_indices (the last ends at nidx); d_delegates[g] receives the
+ * candidate index there with the largest msnr, the first such position on
+ * ties (thrust::max_element's rule; a NaN at a group's first position is
+ * kept as max_element keeps it); -1 for an empty group. */
+SPS_API sps_status sps_select_delegates(const sps_cand* d_cands, const int32_t* d_indices, int64_t nidx,
+                                        const int32_t* d_group_begin, int64_t ngroups, int32_t* d_delegates,
+                                        void* stream);
+
+/* select_delegates::write_log (select_delegates.cuh:15-25,
+ * select_delegates.cu:127-177): writes filename with the columns
+ * N, t bin, t sec (t_chunk_begin + mt * delt_bin), dedisp, width, SNR, num
+ * (each std::setw(10)) for the delegates h_delegates[0..ndel) (indices into
+ * the ncand device candidates) and their group sizes h_group_sizes. */
+SPS_API sps_status sps_write_log(const sps_cand* d_cands, int64_t ncand, const int32_t* h_delegates,
+                                 const int64_t* h_group_sizes, int64_t ndel, const char* filename, float delt_bin,
+                                 float t_chunk_begin);
+
+/* statistical_preprocessing::clean_fdmtInput_and_create_normInput
+ * (CDMT/Statistical_preprocessing.cuh:6-7, .cu:102-120,142-268): rows whose
+ * variance lies more than 4 standard deviations from the mean row variance
+ * are zeroed in place (values are not normalised); d_mask (may be NULL)
+ * receives one byte per sample, 1 where the row is kept, 0 where it was
+ * cleaned; d_buff (may be NULL, >= nrows + 2 floats) receives the row
+ * variances, their mean and their standard deviation.  The reference never
+ * writes its mask argument (and reads its device-resident mean and std on
+ * the host); this entry point fills the mask the function's name promises.
+ * Synchronises the stream. */
+SPS_API sps_status sps_clean_fdmt_input(float* d_arr, int64_t nrows, int64_t ncols, uint8_t* d_mask, float* d_buff,
+                                        void* stream);
+
+/* doDetectSignal_v1 / doDetectSignal_v2 (detection/kernel.cu:561-605) with
+ * the window arithmetic of their per-block kernels (:42-226): rows are cut
+ * into tiles of `tile` columns (the reference's block width), inside which
+ * fp32 prefix sums run sequentially from the tile's first column.
+ *   version 1: value = (Y[t+w] - Y[t]) / sqrtf(w), y = x / sqrtf(1e-7 + n)
+ *              (n may be NULL: y = x); windows lie inside their tile.
+ *   version 2: value = sqrtf((X2[t+w]-X2[t]) / (N2[t+w]-N2[t])), X2, N2
+ *              prefix sums of x^2 and n^2 over the tile and the wnd samples
+ *              after it, N2[0] = 1e-6 (n may be NULL: n = 1).
+ * Widths 1..wnd; windows end inside the row.  The per-tile best goes to the
+ * caller's arrays d_tile_val / d_tile_pos (row*cols + column) / d_tile_w
+ * (the reference's d_pAuxArray, d_pAuxNumArray, d_pWidthArray; at least
+ * rows * ceil(cols/tile) entries, ntiles_cap), and the global best to d_out.
+ * Ties: larger value, then smaller width, row, column (the reference's tie
+ * order follows its tree reductions and is not reproduced).  Allocates
+ * nothing. */
+SPS_API sps_status sps_detect_signal(int version, const float* d_arr, const float* d_norm, int64_t rows, int64_t cols,
+                                     int wnd, int tile, float* d_tile_val, int32_t* d_tile_pos, int32_t* d_tile_w,
+                                     int64_t ntiles_cap, sps_detection* d_out, void* stream);
+
 /* clusterization::digitize_kernel (Clusterization.cuh:34-35,
  * Clusterization.cu:206-214): d_x[i] = floorf(d_x[i]) in place, i < n (the
  * reference test harness uses it to make integer-valued images). */
@@ -244,9 +320,21 @@ SPS_API sps_status sps_search_destroy(sps_search* s);
 SPS_API sps_status sps_search_run(sps_search* s, const void* d_in, float* d_dmt, sps_cand* d_cands,
                                   uint64_t* d_count, void* stream);
 /* Sorts the first min(count, capacity) candidates by (mdt, mt, mwidth) and
- * returns the count (synchronises). */
+ * returns the count (synchronises).  A count above cand_capacity returns
+ * SPS_ERR_CAPACITY after sorting the kept prefix and setting h_count: which
+ * candidates were kept depends on the order the device counter was taken,
+ * so the caller re-gathers into a buffer of h_count entries with
+ * sps_search_gather (the reference's gather counts first and then
+ * allocates exactly, Clusterization.cu:56-88, so it never drops one). */
 SPS_API sps_status sps_search_sort_candidates(sps_search* s, sps_cand* d_cands, const uint64_t* d_count,
                                               int64_t* h_count, void* stream);
+/* Re-runs only the boxcar gather of the last sps_search_run over its S/N
+ * image (d_dmt: the image that run wrote, or NULL when the run kept it
+ * internally) into d_cands (cap entries), sorted by (mdt, mt, mwidth);
+ * d_count and h_count receive the total (synchronises); SPS_ERR_CAPACITY
+ * when it exceeds cap. */
+SPS_API sps_status sps_search_gather(sps_search* s, const float* d_dmt, sps_cand* d_cands, int64_t cap,
+                                     uint64_t* d_count, int64_t* h_count, void* stream);
 SPS_API sps_status sps_search_get_ndt(const sps_search* s, int64_t* h_ndt);
 /* staged passes of the search's data transform (see sps_fdmt_set_staged) */
 SPS_API sps_status sps_search_set_staged(sps_search* s, int staged);
@@ -263,10 +351,12 @@ SPS_API sps_status sps_search_get_schedule(const sps_search* s, int32_t* h_depth
  *  number of passes of the schedule). */
 SPS_API sps_status sps_search_enable_timing(sps_search* s, int enable);
 SPS_API sps_status sps_search_get_timing(sps_search* s, double* h_ms, int64_t cap, int64_t* h_n);
-/* Compulsory bytes of each stage in the order of sps_search_get_timing: one
- * read of a pass's input state plus one write of its output state,
- * 4*nsamps*(rows_in + rows_out); the input read of the statistics pass; the
- * input read and state-0 write of the init; the image read of the boxcar. */
+/* Compulsory HBM bytes of each stage in the order of sps_search_get_timing:
+ * the statistics' input read (at the input's element size); the mask
+ * transform's output image written and read once; the init's input read and
+ * state-0 write; each pass's read of its input state and write of its output
+ * state; the boxcar's read of the image.  States count only their live rows
+ * (rows some output row depends on; the library computes only those). */
 SPS_API sps_status sps_search_get_stage_bytes(const sps_search* s, double* h_bytes, int64_t cap, int64_t* h_n);
 
 /* ---------------------------------------------------------------------- */
@@ -321,7 +411,8 @@ SPS_API sps_status sps_batch_run(sps_batch* b, void* stream);
  * a schedule change). */
 SPS_API sps_status sps_batch_launch(sps_batch* b, void* stream);
 /* Sorts one beam's candidates by (mdt, mt, mwidth) and returns its count
- * (synchronises stream). */
+ * (synchronises stream); SPS_ERR_CAPACITY (count still set) when the beam
+ * found more candidates than its cand_capacity. */
 SPS_API sps_status sps_batch_sort_candidates(sps_batch* b, int64_t beam, int64_t* h_count, void* stream);
 
 #ifdef __cplusplus

@@ -1,16 +1,21 @@
 // sps/fdmt.hpp -- drop-in C++ FDMT classes over the libsps C ABI.
 //
 // Mirrors FDMT_PRAVIR_RESTRICT/fdmt_base.hpp:8-85 (FDMT, FDMTPlan and its
-// types), fdmt_gpu.cuh:68-89 (FDMTGPU) and FDMT/FdmtB.h:6-59 +
-// FDMT/FdmtGpu.cuh:5-57 (CFdmtB, CFdmtGpu).  Pointers passed to execute /
-// process_image are HIP device pointers; work is issued on the default stream
-// like the reference.  Errors throw the reference's exception types:
+// types), fdmt_cpu.hpp:5-20 (FDMTCPU), fdmt_gpu.cuh:68-89 (FDMTGPU) and
+// FDMT/FdmtB.h:6-59, FdmtCpu.h:6-34, FdmtGpu.cuh:5-57 (CFdmtB, CFdmtCpu,
+// CFdmtGpu).  FDMTGPU and CFdmtGpu take HIP device pointers and issue their
+// work on the default stream like the reference; FDMTCPU and CFdmtCpu take
+// host pointers and return when the result is in host memory (the transform
+// runs on the GPU either way).  Errors throw the reference's exception types:
 // std::invalid_argument for size/argument errors, std::runtime_error otherwise.
 #pragma once
 
 #include <array>
 #include <cstddef>
+#include <cstdlib>
+#include <cstring>
 #include <memory>
+#include <new>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -194,8 +199,37 @@ public:
     }
 };
 
+// FDMTCPU (FDMT_PRAVIR_RESTRICT/fdmt_cpu.hpp:5-20): host waterfall and host
+// dmt.  The transform itself runs on the GPU (copy in, transform, copy back,
+// synchronous); set_num_threads records the thread count a caller asks for,
+// which no host computation here uses.
+class FDMTCPU : public FDMT {
+public:
+    FDMTCPU(float f_min, float f_max, size_t nchans, size_t nsamps, float tsamp, size_t dt_max, size_t dt_step = 1,
+            size_t dt_min = 0, int init_mode = SPS_INIT_MEAN)
+        : FDMT(f_min, f_max, nchans, nsamps, tsamp, dt_max, dt_step, dt_min, init_mode) {}
+
+    static void set_num_threads(int nthreads) { threads_ref() = nthreads; }
+    static int get_num_threads() { return threads_ref(); }
+
+    void execute(const float* waterfall, size_t waterfall_size, float* dmt, size_t dmt_size) override {
+        check_inputs(waterfall_size, dmt_size);
+        sps_detail::check(sps_fdmt_execute_host(handle(), waterfall, waterfall_size, dmt, dmt_size, 0));
+    }
+    void initialise(const float* waterfall, float* state) override {
+        const auto& s = get_plan().state_shape[0];
+        sps_detail::check(sps_fdmt_initialise_host(handle(), waterfall, state, s[3] * s[4]));
+    }
+
+private:
+    static int& threads_ref() {
+        static int n = 0;
+        return n;
+    }
+};
+
 // ---------------------------------------------------------------------------
-// legacy CFdmt (FDMT/FdmtB.h:6-59, FDMT/FdmtGpu.cuh:5-57)
+// legacy CFdmt (FDMT/FdmtB.h:6-59, FDMT/FdmtCpu.h:6-34, FDMT/FdmtGpu.cuh:5-57)
 // ---------------------------------------------------------------------------
 typedef float fdmt_type_;  // Constants.h:16
 
@@ -215,58 +249,143 @@ public:
         (void)out;
         (void)b_ones;
     }
-    // FdmtB.cpp:144-156
+    // FdmtB.cpp:144-156.  The dispersion delay across the band of a pulse at
+    // val_DM_Max is td = 4148.8 * DM * (fmin^-2 - fmax^-2) seconds and one
+    // sample of a critically sampled band lasts ts = 1e-6 / (fmax - fmin)
+    // seconds.  With the pulse length L (seconds) the result is
+    //   td / (td / ts / (L / ts)^2) / L = L / ts
+    // evaluated in that order in fp32, truncated, and at least 1.
     static unsigned int calc_MaxDT(const float val_fmin_MHz, const float val_fmax_MHz, const float length_of_pulse,
                                    const float val_DM_Max, const int nchan) {
         (void)nchan;
-        float t0 = 4148.8 * (1.0 / (val_fmin_MHz * val_fmin_MHz) - 1.0 / (val_fmax_MHz * val_fmax_MHz));
-        (void)t0;
-        float td = 4148.8 * val_DM_Max * (1.0 / (val_fmin_MHz * val_fmin_MHz) - 1.0 / (val_fmax_MHz * val_fmax_MHz));
-        float t_tel = 1.0E-6 / (val_fmax_MHz - val_fmin_MHz);
-        float temp = length_of_pulse / t_tel;
-        float val_M = ((td / t_tel) / (temp * temp));
-        unsigned int r = (unsigned int)(td / val_M / length_of_pulse);
-        return (0 == r) ? 1 : r;
+        const double band = 1.0 / (val_fmin_MHz * val_fmin_MHz) - 1.0 / (val_fmax_MHz * val_fmax_MHz);
+        const float delay_s = (float)(4148.8 * val_DM_Max * band);
+        const float sample_s = (float)(1.0E-6 / (val_fmax_MHz - val_fmin_MHz));
+        const float pulse_samples = length_of_pulse / sample_s;
+        const float ratio = (delay_s / sample_s) / (pulse_samples * pulse_samples);
+        const unsigned int n = (unsigned int)(delay_s / ratio / length_of_pulse);
+        return n == 0 ? 1u : n;
     }
     size_t calc_size_input() const { return (size_t)m_cols * m_nchan * sizeof(fdmt_type_); }
     size_t calc_size_output() const { return (size_t)m_cols * m_imaxDt * sizeof(fdmt_type_); }
 };
 
-// CFdmtGpu: device in/out; process_image(nullptr, out, true) gives the
-// transform of an all-ones input (FDMT/ReadMe.txt:21-27).
-class CFdmtGpu : public CFdmtB {
+// CFdmtCpu (FDMT/FdmtCpu.h:6-34): host input and output.  The public
+// configuration members are those of the reference (malloc'ed tables, one
+// entry per iteration 0..m_iNumIter), deep-copied on copy.  process_image
+// runs the transform on the GPU (copy in, transform, copy back, synchronous);
+// process_image(nullptr, out, true) gives the transform of an all-ones input
+// (FDMT/ReadMe.txt:21-27).
+class CFdmtCpu : public CFdmtB {
+public:
+    CFdmtCpu() : CFdmtB() {}
+    CFdmtCpu(const float Fmin, const float Fmax, int nchan, const int cols, int imaxDt)
+        : CFdmtB(Fmin, Fmax, nchan, cols, imaxDt) {
+        create_handle();
+        create_config(m_pparrRowsCumSum_h, m_pparrFreq_h, &m_parrQuantMtrx_h, &m_iNumIter);
+    }
+    CFdmtCpu(const CFdmtCpu& r) : CFdmtB(r) { copy_from(r); }
+    CFdmtCpu& operator=(const CFdmtCpu& r) {
+        if (this == &r) return *this;
+        CFdmtB::operator=(r);
+        release_config();
+        m_h.reset();
+        copy_from(r);
+        return *this;
+    }
+    ~CFdmtCpu() override { release_config(); }
+
+    int** m_pparrRowsCumSum_h = nullptr;
+    float** m_pparrFreq_h = nullptr;
+    int* m_parrQuantMtrx_h = nullptr;
+    int m_iNumIter = 0;
+
+    void process_image(fdmt_type_* piarrImgInp, fdmt_type_* piarrImgOut, const bool b_ones) override {
+        if (!m_h) throw std::runtime_error("CFdmtCpu is not configured");
+        sps_detail::check(sps_fdmt_execute_host(m_h.get(), piarrImgInp, (size_t)m_nchan * m_cols, piarrImgOut,
+                                                (size_t)m_imaxDt * m_cols, b_ones ? 1 : 0));
+    }
+
+    // FdmtCpu.cpp:143-194: malloc'ed tables, owned by the caller
+    virtual void create_config(int**& pparrRowsCumSum, float**& pparrFreq, int** pparrQuantMtrx, int* piNumIter) {
+        if (!m_h) throw std::runtime_error("CFdmtCpu is not configured");
+        int64_t n = 0;
+        sps_detail::check(sps_fdmt_get_niters(m_h.get(), &n));
+        *piNumIter = (int)n;
+        *pparrQuantMtrx = (int*)malloc(sizeof(int) * (size_t)(n + 1));
+        pparrRowsCumSum = (int**)malloc(sizeof(int*) * (size_t)(n + 1));
+        pparrFreq = (float**)malloc(sizeof(float*) * (size_t)(n + 1));
+        if (!*pparrQuantMtrx || !pparrRowsCumSum || !pparrFreq) throw std::bad_alloc();
+        for (int64_t i = 0; i <= n; ++i) {
+            int32_t q = 0;
+            sps_detail::check(sps_cfdmt_get_config(m_h.get(), i, &q, nullptr, nullptr));
+            (*pparrQuantMtrx)[i] = q;
+            pparrRowsCumSum[i] = (int*)malloc(sizeof(int) * (size_t)(q + 1));
+            pparrFreq[i] = (float*)malloc(sizeof(float) * (size_t)(q + 1));
+            if (!pparrRowsCumSum[i] || !pparrFreq[i]) throw std::bad_alloc();
+            sps_detail::check(sps_cfdmt_get_config(m_h.get(), i, &q, pparrRowsCumSum[i], pparrFreq[i]));
+        }
+    }
+
+    sps_fdmt* handle() const { return m_h.get(); }
+
+protected:
+    std::shared_ptr<sps_fdmt> m_h;
+
+    void create_handle() {
+        sps_fdmt* h = nullptr;
+        sps_detail::check(sps_cfdmt_create(m_Fmin, m_Fmax, m_nchan, m_cols, m_imaxDt, &h));
+        m_h.reset(h, [](sps_fdmt* p) { sps_fdmt_destroy(p); });
+    }
+
+private:
+    void release_config() {
+        for (int i = 0; i <= m_iNumIter; ++i) {
+            if (m_pparrRowsCumSum_h) free(m_pparrRowsCumSum_h[i]);
+            if (m_pparrFreq_h) free(m_pparrFreq_h[i]);
+        }
+        free(m_pparrRowsCumSum_h);
+        free(m_pparrFreq_h);
+        free(m_parrQuantMtrx_h);
+        m_pparrRowsCumSum_h = nullptr;
+        m_pparrFreq_h = nullptr;
+        m_parrQuantMtrx_h = nullptr;
+        m_iNumIter = 0;
+    }
+    void copy_from(const CFdmtCpu& r) {
+        if (!r.m_h) return;
+        create_handle();  // the device state is never shared between copies
+        m_iNumIter = r.m_iNumIter;
+        const size_t n = (size_t)m_iNumIter + 1;
+        m_parrQuantMtrx_h = (int*)malloc(sizeof(int) * n);
+        m_pparrRowsCumSum_h = (int**)calloc(n, sizeof(int*));
+        m_pparrFreq_h = (float**)calloc(n, sizeof(float*));
+        if (!m_parrQuantMtrx_h || !m_pparrRowsCumSum_h || !m_pparrFreq_h) throw std::bad_alloc();
+        std::memcpy(m_parrQuantMtrx_h, r.m_parrQuantMtrx_h, sizeof(int) * n);
+        for (size_t i = 0; i < n; ++i) {
+            const size_t q = (size_t)m_parrQuantMtrx_h[i] + 1;
+            m_pparrRowsCumSum_h[i] = (int*)malloc(sizeof(int) * q);
+            m_pparrFreq_h[i] = (float*)malloc(sizeof(float) * q);
+            if (!m_pparrRowsCumSum_h[i] || !m_pparrFreq_h[i]) throw std::bad_alloc();
+            std::memcpy(m_pparrRowsCumSum_h[i], r.m_pparrRowsCumSum_h[i], sizeof(int) * q);
+            std::memcpy(m_pparrFreq_h[i], r.m_pparrFreq_h[i], sizeof(float) * q);
+        }
+    }
+};
+
+// CFdmtGpu (FDMT/FdmtGpu.cuh:5-57): a CFdmtCpu whose process_image takes
+// device input and output and is ordered on the default stream.
+class CFdmtGpu : public CFdmtCpu {
 public:
     CFdmtGpu() = default;
     CFdmtGpu(const float Fmin, const float Fmax, int nchan, const int cols, int imaxDt)
-        : CFdmtB(Fmin, Fmax, nchan, cols, imaxDt) {
-        create();
-    }
-    CFdmtGpu(const CFdmtGpu& r) : CFdmtB(r) {
-        if (r.m_h) create();
-    }
-    CFdmtGpu& operator=(const CFdmtGpu& r) {
-        if (this == &r) return *this;
-        CFdmtB::operator=(r);
-        m_h.reset();
-        if (r.m_h) create();
-        return *this;
-    }
+        : CFdmtCpu(Fmin, Fmax, nchan, cols, imaxDt) {}
+    CFdmtGpu(const CFdmtGpu& r) = default;
+    CFdmtGpu& operator=(const CFdmtGpu& r) = default;
+
     void process_image(fdmt_type_* d_in, fdmt_type_* d_out, const bool b_ones) override {
         if (!m_h) throw std::runtime_error("CFdmtGpu is not configured");
         sps_detail::check(sps_fdmt_execute(m_h.get(), d_in, SPS_F32, (size_t)m_nchan * m_cols, d_out,
                                            (size_t)m_imaxDt * m_cols, b_ones ? 1 : 0, nullptr));
-    }
-    int m_iNumIter() const {
-        int64_t n = 0;
-        sps_detail::check(sps_fdmt_get_niters(m_h.get(), &n));
-        return (int)n;
-    }
-
-private:
-    std::shared_ptr<sps_fdmt> m_h;
-    void create() {
-        sps_fdmt* h = nullptr;
-        sps_detail::check(sps_cfdmt_create(m_Fmin, m_Fmax, m_nchan, m_cols, m_imaxDt, &h));
-        m_h.reset(h, [](sps_fdmt* p) { sps_fdmt_destroy(p); });
     }
 };

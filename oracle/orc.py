@@ -69,6 +69,9 @@ def lib():
         L.orc_fdmt_normalization.argtypes = [_vp, _vp, _i64, _vp]
         L.orc_boxcar_pow2.restype = _i64
         L.orc_boxcar_pow2.argtypes = [_vp, _i64, _i64, _f32, C.c_int, _vp, _i64]
+        L.orc_boxcar_pow2_alloc.restype = _vp
+        L.orc_boxcar_pow2_alloc.argtypes = [_vp, _i64, _i64, _f32, C.c_int, C.POINTER(_i64)]
+        L.orc_free.argtypes = [_vp]
         L.orc_boxcar_all.restype = _i64
         L.orc_boxcar_all.argtypes = [_vp, _i64, _i64, _f32, C.c_int, _vp, _i64]
         L.orc_cluster.restype = _i64
@@ -221,6 +224,25 @@ def normalise(x, mode=1):
     return out, mask, (rm, rv, keep, st)
 
 
+def normalise_with_stats(x, mode, st):
+    """calc_fdmt_inputs' normalisation of x with statistics st = (row means,
+    row variances, keep flags, stats) computed elsewhere (for instance over
+    the whole of a beam of which x is a leading time slice)."""
+    rm, rv, keep, stv = st
+    R, N = x.shape
+    out = np.zeros((R, N), np.float32)
+    mask = np.zeros((R, N), np.float32)
+    if x.dtype == np.uint8:
+        xa = np.ascontiguousarray(x)
+        lib().orc_normalise(None, _ptr(xa), R, N, mode, _ptr(rm), _ptr(rv), _ptr(keep), _ptr(stv), _ptr(out),
+                            _ptr(mask))
+    else:
+        xa = np.ascontiguousarray(x, np.float32)
+        lib().orc_normalise(_ptr(xa), None, R, N, mode, _ptr(rm), _ptr(rv), _ptr(keep), _ptr(stv), _ptr(out),
+                            _ptr(mask))
+    return out, mask
+
+
 def fdmt_normalization(a, n):
     a = np.ascontiguousarray(a, np.float32)
     n = np.ascontiguousarray(n, np.float32)
@@ -229,13 +251,16 @@ def fdmt_normalization(a, n):
     return out
 
 
-def boxcar_pow2(img, thr, q, cap=1 << 20):
+def boxcar_pow2(img, thr, q):
+    """Power-of-two boxcar candidates (widths 1..2^q) in (mdt, mt, mwidth) order."""
     img = np.ascontiguousarray(img, np.float32)
-    out = np.zeros(cap, CAND_DTYPE)
-    n = lib().orc_boxcar_pow2(_ptr(img), img.shape[0], img.shape[1], thr, q, _ptr(out), cap)
-    if n > cap:
-        return boxcar_pow2(img, thr, q, cap=int(n))
-    return out[:n].copy()
+    n = _i64()
+    p = lib().orc_boxcar_pow2_alloc(_ptr(img), img.shape[0], img.shape[1], thr, q, C.byref(n))
+    out = np.zeros(n.value, CAND_DTYPE)
+    if n.value:
+        C.memmove(out.ctypes.data, p, out.nbytes)
+        lib().orc_free(p)
+    return out
 
 
 def boxcar_all(img, thr, w, cap=1 << 20):

@@ -121,6 +121,10 @@ void ref_cfdmt_config(void* h, int i, int* cumsum, float* freq) {
     std::memcpy(cumsum, f->m_pparrRowsCumSum_h[i], sizeof(int) * (size_t)(q + 1));
     std::memcpy(freq, f->m_pparrFreq_h[i], sizeof(float) * (size_t)(q + 1));
 }
+// CFdmtB::calc_MaxDT (FdmtB.h:33-34, FdmtB.cpp:144-156)
+unsigned int ref_calc_maxdt(float fmin, float fmax, float length_of_pulse, float dm_max, int nchan) {
+    return CFdmtB::calc_MaxDT(fmin, fmax, length_of_pulse, dm_max, nchan);
+}
 // CFdmtCpu::process_image(in, out, b_ones); in may be NULL with b_ones.
 void ref_cfdmt_process(void* h, float* x, float* out, int b_ones) {
     static_cast<CFdmtCpu*>(h)->process_image(x, out, b_ones != 0);

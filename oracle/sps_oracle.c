@@ -869,6 +869,30 @@ ORC_EXPORT int64_t orc_boxcar_pow2(const float* img, int64_t R, int64_t N, float
     return n;
 }
 
+/* As orc_boxcar_pow2 with an output sized by its own count pass: returns a
+ * malloc'd array of *n candidates (release with orc_free); NULL with *n = 0
+ * when there are none. */
+ORC_EXPORT orc_cand* orc_boxcar_pow2_alloc(const float* img, int64_t R, int64_t N, float thr, int q, int64_t* n) {
+    int64_t* cnt = (int64_t*)calloc((size_t)R + 1, sizeof(int64_t));
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t r = 0; r < R; ++r) cnt[r + 1] = orc_boxcar_pow2_row(img + r * N, r, N, thr, q, NULL, 0);
+    for (int64_t r = 0; r < R; ++r) cnt[r + 1] += cnt[r];
+    *n = cnt[R];
+    orc_cand* out = NULL;
+    if (cnt[R] > 0) {
+        out = (orc_cand*)malloc(sizeof(orc_cand) * (size_t)cnt[R]);
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int64_t r = 0; r < R; ++r) {
+            if (cnt[r + 1] == cnt[r]) continue;
+            orc_boxcar_pow2_row(img + r * N, r, N, thr, q, out + cnt[r], cnt[r + 1] - cnt[r]);
+        }
+    }
+    free(cnt);
+    return out;
+}
+
+ORC_EXPORT void orc_free(void* p) { free(p); }
+
 /* do_plan_kernel + gather_candidates_in_heap_kernel (Clusterization.cu:
  * 591-671): widths 1..W, each window summed from scratch left to right,
  * mwidth = w, row-major cell order then width. */
