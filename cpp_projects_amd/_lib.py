@@ -82,6 +82,13 @@ SIGNATURES = {
     "sps_fdmt_get_rows": [_vp, _i64, _pi64, _pi64],
     "sps_fdmt_execute": [_vp, _vp, _int, C.c_size_t, _vp, C.c_size_t, _int, _vp],
     "sps_fdmt_initialise": [_vp, _vp, _int, _vp, C.c_size_t, _vp],
+    "sps_fdmt_execute_host": [_vp, _vp, C.c_size_t, _vp, C.c_size_t, _int],
+    "sps_fdmt_initialise_host": [_vp, _vp, _vp, C.c_size_t],
+    "sps_cfdmt_get_config": [_vp, _i64, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _pf32],
+    "sps_select_delegates": [_vp, _vp, _i64, _vp, _i64, _vp, _vp],
+    "sps_write_log": [_vp, _i64, _vp, _vp, _i64, C.c_char_p, _f32, _f32],
+    "sps_clean_fdmt_input": [_vp, _i64, _i64, _vp, _vp, _vp],
+    "sps_detect_signal": [_int, _vp, _vp, _i64, _i64, _int, _int, _vp, _vp, _vp, _i64, _vp, _vp],
     "sps_fdmt_adjoint": [_vp, _vp, C.c_size_t, _vp, C.c_size_t, _vp],
     "sps_calc_fdmt_inputs": [_vp, _i64, _i64, _vp, _vp, _int, _vp],
     "sps_fdmt_normalization": [_vp, _vp, _i64, _vp, _vp],

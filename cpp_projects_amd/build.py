@@ -75,10 +75,12 @@ HARNESS_BIN = os.path.join(ROOT, "tests", "cpp", "dropin_harness")
 
 
 def build_harness():
-    """Compiles the C++ drop-in harness (tests/cpp) against include/sps/*.hpp and libsps.so."""
-    _run([CXX, "-std=c++17", "-O1", "-Wno-unused-result", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "include"),
-          "-I", "/opt/rocm/include", HARNESS_SRC, "-o", HARNESS_BIN, "-L", PKG, "-lsps", "-L", "/opt/rocm/lib",
-          "-lamdhip64", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/../../cpp_projects_amd"])
+    """Compiles the C++ drop-in harness (tests/cpp) against include/sps/*.hpp and
+    libsps.so with hipcc (its thrust::device_vector calls are rocThrust's)."""
+    _run([HIPCC, f"--offload-arch={ARCH}", "-std=c++17", "-O1", "-Wno-unused-result", "-x", "hip",
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", os.path.join(ROOT, "include"),
+          HARNESS_SRC, "-o", HARNESS_BIN, "-L", PKG, "-lsps", f"-Wl,-rpath,{PKG}",
+          "-Wl,-rpath,$ORIGIN/../../cpp_projects_amd"])
     return HARNESS_BIN
 
 

@@ -8,6 +8,7 @@
 //   cross-row  : row r goes to lane r % 256, same tree.
 // For integer (u8) input every partial sum is an exact integer in FP64.
 #include "common.hpp"
+#include "detect.hpp"
 #include "kernels.hpp"
 
 namespace sps {
@@ -1108,67 +1109,18 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_snr_adjoint(const float* __
 }
 
 // ---------------------------------------------------------------------------
-// single-maximum detector
+// single-maximum detector (Best, better, detect_windows, k_detect_final in
+// detect.hpp)
 // ---------------------------------------------------------------------------
-struct Best {
-    float snr;
-    int32_t w, r, t;
-};
-
-__device__ __forceinline__ bool better(const Best& a, const Best& b) {
-    if (a.w < 0) return false;
-    if (b.w < 0) return true;
-    if (a.snr != b.snr) return a.snr > b.snr;
-    if (a.w != b.w) return a.w < b.w;
-    if (a.r != b.r) return a.r < b.r;
-    return a.t < b.t;
-}
-
-__device__ void block_best(Best v, Best* sh) {
-    sh[threadIdx.x] = v;
-    for (int s = kThreads / 2; s > 0; s >>= 1) {
-        __syncthreads();
-        if ((int)threadIdx.x < s && better(sh[threadIdx.x + s], sh[threadIdx.x])) sh[threadIdx.x] = sh[threadIdx.x + s];
-    }
-    __syncthreads();
-}
-
-constexpr int kDetCells = 256;
-
 __global__ void __launch_bounds__(kThreads) k_detect_blocks(const float* __restrict__ a, const float* __restrict__ n,
                                                             int64_t cols, int wnd, Best* __restrict__ blocks) {
     __shared__ Best sh[kThreads];
     const int64_t r = blockIdx.y;
     const int64_t t = (int64_t)blockIdx.x * kDetCells + threadIdx.x;
     Best b{0.0f, -1, -1, -1};
-    if (t < cols) {
-        float s = 0.0f;
-        for (int w = 1; w <= wnd && t + w <= cols; ++w) {
-            float v = a[r * cols + t + w - 1];
-            if (n) v = v / sqrtf((float)(1.0E-7 + (double)n[r * cols + t + w - 1]));
-            s += v;
-            const float val = s / sqrtf((float)w);
-            Best c{val, w, (int32_t)r, (int32_t)t};
-            if (better(c, b)) b = c;
-        }
-    }
+    if (t < cols) detect_windows(a, n, r, t, cols, wnd, b);
     block_best(b, sh);
     if (threadIdx.x == 0) blocks[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
-}
-
-__global__ void __launch_bounds__(kThreads) k_detect_final(const Best* __restrict__ blocks, int64_t nb,
-                                                           sps_detection* __restrict__ out) {
-    __shared__ Best sh[kThreads];
-    Best b{0.0f, -1, -1, -1};
-    for (int64_t i = threadIdx.x; i < nb; i += kThreads)
-        if (better(blocks[i], b)) b = blocks[i];
-    block_best(b, sh);
-    if (threadIdx.x == 0) {
-        out->iwidth = sh[0].w;
-        out->irow = sh[0].r;
-        out->icol = sh[0].t;
-        out->snr = sh[0].snr;
-    }
 }
 
 // ---------------------------------------------------------------------------

@@ -1,9 +1,12 @@
 """Python mirror of the reference FDMT classes over the C ABI.
 
 FDMTGPU mirrors FDMT/FDMTGPU (FDMT_PRAVIR_RESTRICT/fdmt_base.hpp:37-85,
-fdmt_gpu.cuh:68-89); CFdmtGpu mirrors the legacy CFdmtGpu
-(FDMT/FdmtGpu.cuh:5-57).  Device buffers are torch tensors on the HIP
-device; work runs on torch's current stream.  Plan queries need no GPU.
+fdmt_gpu.cuh:68-89) and FDMTCPU (fdmt_cpu.hpp:5-20); CFdmtGpu and CFdmtCpu
+mirror the legacy classes (FDMT/FdmtGpu.cuh:5-57, FdmtCpu.h:6-34).  The GPU
+classes take torch tensors on the HIP device and run on torch's current
+stream; the CPU classes take numpy arrays and return when the result is in
+host memory (the transform runs on the GPU in both).  Plan queries need no
+GPU.
 """
 import ctypes as C
 
@@ -199,3 +202,64 @@ class CFdmtGpu(_Handle):
         """CFdmtGpu::process_image (FdmtGpu.cuh:50-53); d_in may be None when b_ones."""
         n_in = self.m_nchan * self.m_cols
         self._execute(d_in, n_in, d_out, d_out.numel(), ones=b_ones)
+
+
+def _host_f32(a, n, what):
+    if a is None:
+        return None
+    if not isinstance(a, np.ndarray) or a.dtype != np.float32 or not a.flags["C_CONTIGUOUS"]:
+        raise ValueError(f"{what} must be a C-contiguous float32 numpy array")
+    if a.size != n:
+        raise _lib.InvalidArgument(_lib.SPS_ERR_INVALID_ARGUMENT, f"Invalid size of {what}")
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class FDMTCPU(FDMTGPU):
+    """FDMTCPU(f_min, f_max, nchans, nsamps, tsamp, dt_max, dt_step=1, dt_min=0)
+    (fdmt_cpu.hpp:5-20): execute / initialise take host (numpy float32)
+    arrays.  set_num_threads records the thread count a caller asks for; no
+    host computation here uses it."""
+
+    _threads = 0
+
+    @staticmethod
+    def set_num_threads(nthreads):
+        FDMTCPU._threads = int(nthreads)
+
+    def execute(self, waterfall, waterfall_size, dmt, dmt_size):
+        if int(waterfall_size) != self.nchans * self.nsamps:
+            raise _lib.InvalidArgument(_lib.SPS_ERR_INVALID_ARGUMENT, "Invalid size of waterfall")
+        call("sps_fdmt_execute_host", self._h, _host_f32(waterfall, int(waterfall_size), "waterfall"),
+             int(waterfall_size), _host_f32(dmt, int(dmt_size), "dmt"), int(dmt_size), 0)
+
+    def initialise(self, waterfall, state):
+        call("sps_fdmt_initialise_host", self._h, _host_f32(waterfall, self.nchans * self.nsamps, "waterfall"),
+             _host_f32(state, state.size, "state"), state.size)
+
+
+class CFdmtCpu(CFdmtGpu):
+    """CFdmtCpu(Fmin, Fmax, nchan, cols, imaxDt) (FdmtCpu.h:6-34): host
+    (numpy float32) input and output; m_iNumIter, m_parrQuantMtrx_h,
+    m_pparrRowsCumSum_h and m_pparrFreq_h hold the legacy configuration."""
+
+    def __init__(self, f_min, f_max, nchan, cols, imaxdt):
+        super().__init__(f_min, f_max, nchan, cols, imaxdt)
+        self.m_iNumIter = self.get_niters()
+        self.m_parrQuantMtrx_h, self.m_pparrRowsCumSum_h, self.m_pparrFreq_h = [], [], []
+        for i in range(self.m_iNumIter + 1):
+            q = C.c_int32()
+            call("sps_cfdmt_get_config", self._h, i, C.byref(q), None, None)
+            cs = np.zeros(q.value + 1, np.int32)
+            fr = np.zeros(q.value + 1, np.float32)
+            call("sps_cfdmt_get_config", self._h, i, C.byref(q), cs.ctypes.data_as(C.POINTER(C.c_int32)),
+                 fr.ctypes.data_as(C.POINTER(C.c_float)))
+            self.m_parrQuantMtrx_h.append(q.value)
+            self.m_pparrRowsCumSum_h.append(cs)
+            self.m_pparrFreq_h.append(fr)
+
+    def process_image(self, h_in, h_out, b_ones):
+        """CFdmtCpu::process_image (FdmtCpu.cpp:197-249); h_in may be None when b_ones."""
+        n_in = self.m_nchan * self.m_cols
+        call("sps_fdmt_execute_host", self._h, None if b_ones and h_in is None else _host_f32(h_in, n_in, "input"),
+             n_in, _host_f32(h_out, self.m_imaxDt * self.m_cols, "output"), self.m_imaxDt * self.m_cols,
+             int(bool(b_ones)))

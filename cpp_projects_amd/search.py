@@ -38,6 +38,25 @@ class statistical_preprocessing:
         call("sps_calc_fdmt_inputs", _dptr(d_in), int(nrows), int(ncols), _dptr(d_buff), _dptr(d_norm_input),
              int(mode), _stream())
 
+    @staticmethod
+    def clean_fdmtInput_and_create_normInput(d_arr, nrows, ncols, d_pbarr_norm, d_buff):
+        """statistical_preprocessing::clean_fdmtInput_and_create_normInput
+        (Statistical_preprocessing.cuh:6-7): rows whose variance is more than
+        4 sigma from the mean row variance are zeroed in place; d_pbarr_norm
+        (a bool tensor of nrows*ncols, or None) receives the keep mask;
+        d_buff (>= nrows + 2 floats, or None) the row variances, their mean
+        and standard deviation."""
+        call("sps_clean_fdmt_input", _dptr(d_arr), int(nrows), int(ncols), _dptr(d_pbarr_norm), _dptr(d_buff),
+             _stream())
+
+    @staticmethod
+    def calcThreadsForMean_and_Disp(n_cols):
+        """Statistical_preprocessing.cuh:12-17: the power of two >= n_cols, at most 1024."""
+        k = 0
+        while (1 << k) < n_cols and k < 10:
+            k += 1
+        return 1 << k
+
 
 def fdmt_normalization(d_arr, d_norm, length, d_out):
     """fdmt_normalization (CDMT/Chunk_gpu.cu:841-852)."""
@@ -99,12 +118,12 @@ class clusterization:
 
     @staticmethod
     def complete_clusterization(d_cands, n, d_bins, filename, delt_bin, t_chunk_begin, bwrite=True):
-        """Clusterization.cuh:25-32."""
+        """Clusterization.cuh:25-32: gap clustering by (mt, mdt, mwidth) with
+        d_bins, delegate selection, delegates sorted by mt and the log.  Like
+        the reference it returns nothing; cluster() returns the delegates."""
         dele, gsz = clusterization.cluster(d_cands, n, d_bins)
         if bwrite:
-            select_delegates.write_log(cands_to_numpy(d_cands, n), dele.cpu().numpy(), gsz.cpu().numpy(), filename,
-                                       delt_bin, t_chunk_begin)
-        return dele, gsz
+            _write_delegates(d_cands, n, dele.cpu().numpy(), gsz.cpu().numpy(), filename, delt_bin, t_chunk_begin)
 
     @staticmethod
     def clusterization_main(d_fdmt, rows, cols, d_val_thresh, wnd_width, d_bins, filename, delt_bin, t_chunk_begin,
@@ -126,21 +145,77 @@ class clusterization:
                 f.write(f"{int(c['mt'])},{int(c['mdt'])},{int(c['mwidth'])},{float(c['msnr']):g}\n")
 
 
+def _write_delegates(d_cands, n, delegates, group_sizes, filename, delt_bin, t_chunk_begin):
+    dl = np.ascontiguousarray(delegates, np.int32)
+    gs = np.ascontiguousarray(group_sizes, np.int64)
+    call("sps_write_log", _dptr(d_cands) if n else None, int(n), dl.ctypes.data_as(C.c_void_p),
+         gs.ctypes.data_as(C.c_void_p), len(dl), str(filename).encode(), float(delt_bin), float(t_chunk_begin))
+
+
 class select_delegates:
     @staticmethod
-    def write_log(h_cands, delegates, group_sizes, filename, delt_bin, t_chunk_begin):
-        """select_delegates::write_log (select_delegates.cu:127-177): columns
-        N, t bin, t sec, dedisp, width, SNR, num, each 10 wide; num is the
-        delegate's own group size."""
-        with open(filename, "w") as f:
-            hdr = ["N", "t, bin", "t, sec", "dedisp", "width", "SNR", "num"]
-            f.write("".join(f"{h:>10}" for h in hdr) + "\n")
-            for i, (j, num) in enumerate(zip(delegates, group_sizes)):
-                c = h_cands[int(j)]
-                tsec = np.float32(t_chunk_begin) + np.float32(int(c["mt"])) * np.float32(delt_bin)
-                cols = [str(i + 1), str(int(c["mt"])), f"{float(tsec):g}", str(int(c["mdt"])), str(int(c["mwidth"])),
-                        f"{float(c['msnr']):g}", str(int(num))]
-                f.write("".join(f"{v:>10}" for v in cols) + "\n")
+    def select_delegates(d_cands, d_indices, d_group_begin, d_out=None):
+        """select_delegates::select_delegates (select_delegates.cu:16-51):
+        for group g = positions [begin[g], begin[g+1]) of d_indices (the last
+        ends at len(d_indices)), the candidate index with the largest msnr,
+        the first position on ties.  Returns an int32 device tensor."""
+        import torch
+
+        ng = int(d_group_begin.numel())
+        out = d_out if d_out is not None else torch.empty(max(ng, 1), dtype=torch.int32, device=d_cands.device)
+        call("sps_select_delegates", _dptr(d_cands), _dptr(d_indices), int(d_indices.numel()), _dptr(d_group_begin),
+             ng, _dptr(out), _stream())
+        return out[:ng]
+
+    @staticmethod
+    def select_delegates_kernel(d_cands, d_indices, quant_cand, d_group_begin, d_num_delegates, quant_groups):
+        """select_delegates_kernel (select_delegates.cu:62-125), launched by the
+        reference with one block per group: quant_cand ends the last group."""
+        call("sps_select_delegates", _dptr(d_cands), _dptr(d_indices), int(quant_cand), _dptr(d_group_begin),
+             int(quant_groups), _dptr(d_num_delegates), _stream())
+
+    @staticmethod
+    def write_log(d_cands, quant_candidates, d_group_begin, d_indices, d_num_delegates, filename, delt_bin,
+                  t_chunk_begin):
+        """select_delegates::write_log (select_delegates.cu:127-177): delegate
+        i is d_num_delegates[i] and its num column is the size of group i."""
+        begin = d_group_begin.cpu().numpy().astype(np.int64)
+        ends = np.append(begin[1:], int(d_indices.numel()))
+        _write_delegates(d_cands, quant_candidates, d_num_delegates.cpu().numpy(), ends - begin, filename,
+                         delt_bin, t_chunk_begin)
+
+
+def doDetectSignal_v1(d_arr, d_norm, rows, cols, wnd, thr, grid, block, sz, d_aux, d_aux_num, d_width, out,
+                      version=1):
+    """doDetectSignal_v1 (detection/kernel.cu:561-582): block[0] columns per
+    tile, grid (ceil(cols/block[0]), rows); d_aux (float), d_aux_num and
+    d_width (int32) hold one entry per tile; out a (4,) int32 device tensor
+    receiving {iwidth, irow, icol, snr bits}.  Returns snr >= thr."""
+    tile = int(block[0])
+    if int(grid[0]) != (cols + tile - 1) // tile or int(grid[1]) != rows:
+        raise _lib.InvalidArgument(_lib.SPS_ERR_INVALID_ARGUMENT, "grid must be (ceil(cols/block.x), rows)")
+    call("sps_detect_signal", int(version), _dptr(d_arr), _dptr(d_norm), int(rows), int(cols), int(wnd), tile,
+         _dptr(d_aux), _dptr(d_aux_num), _dptr(d_width), int(d_aux.numel()), _dptr(out), _stream())
+    o = out.cpu()
+    import torch
+
+    return float(o[3:4].view(torch.float32)[0]) >= thr
+
+
+def doDetectSignal_v2(d_arr, d_norm, rows, cols, wnd, thr, grid, block, sz, d_aux, d_aux_num, d_width, out):
+    """doDetectSignal_v2 (detection/kernel.cu:584-605): energy ratio
+    sqrt(sum x^2 / sum n^2) over windows of 1..wnd samples."""
+    return doDetectSignal_v1(d_arr, d_norm, rows, cols, wnd, thr, grid, block, sz, d_aux, d_aux_num, d_width, out,
+                             version=2)
+
+
+def doDetectSignal_v2_2(d_arr, d_norm, rows, cols, wnd, thr, grid, block, sz, d_aux, d_aux_num, d_width, out,
+                        arr_buff=None):
+    """doDetectSignal_v2_2 (detection/kernel.cu:609-640): the energy ratio of
+    doDetectSignal_v2 with a one-block final reduction (its window kernel is
+    declared in a header the reference does not ship)."""
+    return doDetectSignal_v1(d_arr, d_norm, rows, cols, wnd, thr, grid, block, sz, d_aux, d_aux_num, d_width, out,
+                             version=2)
 
 
 def detect_max(d_arr, d_norm, rows, cols, wnd):

@@ -4,8 +4,9 @@
 // Mirrors CDMT/Statistical_preprocessing.cuh:6-17, CDMT/Chunk_gpu.cuh:118-119
 // (fdmt_normalization), Clusterization/Candidate.cuh:4-10 (Cand),
 // Clusterization/Clusterization.cuh:8-68 (clusterization::),
-// Clusterization/select_delegates.cuh:4-26 (select_delegates::) and the
-// detector of detection/kernel.cu:14-20,561-605.  All array pointers are HIP
+// Clusterization/select_delegates.cuh:4-26 (select_delegates::; the
+// thrust::device_vector forms are in sps/thrust_vectors.hpp) and the
+// detectors of detection/kernel.cu:14-20,561-640.  All array pointers are HIP
 // device pointers unless named h_; thresholds and bin metrics are references
 // to device memory like the reference's (Chunk_gpu.cu:373-425).
 #pragma once
@@ -41,6 +42,16 @@ inline void calc_fdmt_inputs_per_channel(float* d_parr_inp, const int NRows, con
     sps_detail::check(
         sps_calc_fdmt_inputs(d_parr_inp, NRows, NCols, d_buff, d_parrNormInput, SPS_NORM_PER_CHANNEL, nullptr));
 }
+// Statistical_preprocessing.cuh:6-7: rows whose variance lies more than
+// 4 sigma from the mean row variance are zeroed in place; d_pbarrNorm (NRows
+// x NCols bools, may be null) receives the keep mask; d_buff (>= NRows + 2
+// floats) the row variances, their mean and standard deviation.
+inline void clean_fdmtInput_and_create_normInput(float* d_arr, const int NRows, const int NCols, bool* d_pbarrNorm,
+                                                 float* d_buff) {
+    static_assert(sizeof(bool) == 1, "the mask is one byte per sample");
+    sps_detail::check(sps_clean_fdmt_input(d_arr, NRows, NCols, reinterpret_cast<uint8_t*>(d_pbarrNorm), d_buff,
+                                           nullptr));
+}
 inline int calcThreadsForMean_and_Disp(unsigned const int nCols) {
     int k = 0;
     while ((1u << k) < nCols && k < 10) ++k;
@@ -54,23 +65,45 @@ inline void fdmt_normalization(fdmt_type_* d_arr, fdmt_type_* d_norm, const int 
 }
 
 namespace select_delegates {
-// select_delegates.cuh:15-25 over plain arrays: delegates (host, sorted by mt)
-// and their group sizes; writes N, t bin, t sec, dedisp, width, SNR, num.
+// select_delegates.cu:16-51 over plain device arrays: group g is positions
+// [d_group_begin[g], d_group_begin[g+1]) of d_indices (the last ends at
+// n_indices); d_delegates[g] = the index there with the largest msnr, the
+// first position on ties (thrust::max_element's rule).
+inline void select_delegates(const Cand* d_cands, const int* d_indices, const int n_indices, const int* d_group_begin,
+                             const int n_groups, int* d_delegates) {
+    sps_detail::check(sps_select_delegates(d_cands, d_indices, n_indices, d_group_begin, n_groups, d_delegates, nullptr));
+    if (hipDeviceSynchronize() != hipSuccess) throw std::runtime_error("select_delegates failed");
+}
+// select_delegates_kernel (select_delegates.cu:62-125), launched by the
+// reference with one block per group; here a host call whose extra last
+// argument is that group count (the reference's gridDim.x).
+inline void select_delegates_kernel(const Cand* d_arrCandHeap, int* d_arrIndices, const int QUantCand,
+                                    int* d_arrGroupBeginIndeces, int* d_arrNumDelegates, const int QUantGroups) {
+    sps_detail::check(sps_select_delegates(d_arrCandHeap, d_arrIndices, QUantCand, d_arrGroupBeginIndeces, QUantGroups,
+                                           d_arrNumDelegates, nullptr));
+}
+// select_delegates.cu:127-177 over plain device arrays: delegate i is
+// d_delegates[i] and its num column the size of group i.
+inline void write_log(const Cand* d_parrCand, const int QUantCandidates, const int* d_group_begin, const int n_groups,
+                      const int n_indices, const int* d_delegates, const std::string& filename, const float delt_bin,
+                      const float t_chunkBegin) {
+    std::vector<int> begin((size_t)n_groups), dele((size_t)n_groups);
+    if (n_groups > 0 &&
+        (hipMemcpy(begin.data(), d_group_begin, sizeof(int) * begin.size(), hipMemcpyDeviceToHost) != hipSuccess ||
+         hipMemcpy(dele.data(), d_delegates, sizeof(int) * dele.size(), hipMemcpyDeviceToHost) != hipSuccess))
+        throw std::runtime_error("copy of the groups failed");
+    std::vector<int64_t> num((size_t)n_groups);
+    for (int g = 0; g < n_groups; ++g) num[(size_t)g] = (g + 1 < n_groups ? begin[(size_t)g + 1] : n_indices) - begin[(size_t)g];
+    sps_detail::check(sps_write_log(d_parrCand, QUantCandidates, dele.data(), num.data(), n_groups, filename.c_str(),
+                                    delt_bin, t_chunkBegin));
+}
+// the same log for delegates and group sizes already on the host
 inline void write_log(const Cand* d_parrCand, const int QUantCandidates, const std::vector<int>& h_delegates,
                       const std::vector<int64_t>& h_group_sizes, const std::string& filename, const float delt_bin,
                       const float t_chunkBegin) {
-    std::vector<Cand> h((size_t)QUantCandidates);
-    if (QUantCandidates > 0 && hipMemcpy(h.data(), d_parrCand, sizeof(Cand) * h.size(), hipMemcpyDeviceToHost) != hipSuccess)
-        throw std::runtime_error("copy of candidates failed");
-    std::ofstream f(filename);
-    f << std::setw(10) << "N" << std::setw(10) << "t, bin" << std::setw(10) << "t, sec" << std::setw(10) << "dedisp"
-      << std::setw(10) << "width" << std::setw(10) << "SNR" << std::setw(10) << "num" << std::endl;
-    for (size_t i = 0; i < h_delegates.size(); ++i) {
-        const Cand& c = h[(size_t)h_delegates[i]];
-        f << std::setw(10) << i + 1 << std::setw(10) << c.mt << std::setw(10) << t_chunkBegin + c.mt * delt_bin
-          << std::setw(10) << c.mdt << std::setw(10) << c.mwidth << std::setw(10) << c.msnr << std::setw(10)
-          << h_group_sizes[i] << std::endl;
-    }
+    if (h_delegates.size() != h_group_sizes.size()) throw std::invalid_argument("one group size per delegate");
+    sps_detail::check(sps_write_log(d_parrCand, QUantCandidates, h_delegates.data(), h_group_sizes.data(),
+                                    (int64_t)h_delegates.size(), filename.c_str(), delt_bin, t_chunkBegin));
 }
 }  // namespace select_delegates
 
@@ -113,12 +146,12 @@ inline bool gather_candidates_in_heap_with_plan(const float* d_fdmt, const int R
     return true;
 }
 
-// Clusterization.cuh:25-32: gap clustering by (mt, mdt, mwidth) with
-// d_pbin_metrics, delegate selection, log.  Returns the delegates (host).
-inline std::vector<int> complete_clusterization(const Cand* d_parrCand, unsigned int quant_candidates,
-                                                const int* d_pbin_metrics, const std::string& filename,
-                                                const float delt_bin, const float t_chunkBegin, bool bwrite) {
+// Three-key gap clustering of the candidates with d_pbin_metrics and
+// delegate selection: the delegates (sorted by mt) and their group sizes.
+inline std::vector<int> cluster_delegates(const Cand* d_parrCand, unsigned int quant_candidates,
+                                          const int* d_pbin_metrics, std::vector<int64_t>* group_sizes = nullptr) {
     std::vector<int> dele;
+    if (group_sizes) group_sizes->clear();
     if (quant_candidates == 0) return dele;
     int32_t* d_del = nullptr;
     int64_t* d_gs = nullptr;
@@ -138,9 +171,18 @@ inline std::vector<int> complete_clusterization(const Cand* d_parrCand, unsigned
     (void)hipFree(d_del);
     (void)hipFree(d_gs);
     sps_detail::check(s);
+    if (group_sizes) *group_sizes = std::move(gs);
+    return dele;
+}
+
+// Clusterization.cuh:25-32: clustering, delegates and (bwrite) the log.
+inline void complete_clusterization(const Cand* d_parrCand, unsigned int quant_candidates, const int* d_pbin_metrics,
+                                    const std::string& filename, const float delt_bin, const float t_chunkBegin,
+                                    bool bwrite) {
+    std::vector<int64_t> gs;
+    std::vector<int> dele = cluster_delegates(d_parrCand, quant_candidates, d_pbin_metrics, &gs);
     if (bwrite)
         select_delegates::write_log(d_parrCand, (int)quant_candidates, dele, gs, filename, delt_bin, t_chunkBegin);
-    return dele;
 }
 
 // Clusterization.cuh:10-20 (returns true on success; the reference's success
@@ -224,9 +266,62 @@ inline void readFreddaCandFile(const std::string& filename, Cand* arrCand, size_
 
 }  // namespace clusterization
 
-// doDetectSignal_v1 (detection/kernel.cu:561-582): the best boxcar S/N of
-// d_arr / sqrtf(1e-7 + d_norm) over widths 1..WndWidth; result in *d_out
-// (device or managed).  Returns true when the best S/N reaches thr.
+// The single-maximum detectors of detection/kernel.cu:561-640 with the
+// reference's signatures: gridSize = (ceil(Cols / blockSize.x), Rows) and
+// blockSize.x is the tile over which the window prefix sums run; the three
+// auxiliary arrays hold one entry per tile (the per-tile best value, its
+// row*Cols + column and its width); pstructOut (device or managed) receives
+// the global best.  sz is the reference's shared-memory size and is not
+// needed.  Returns pstructOut->snr >= valTresh.
+namespace sps_detail {
+inline bool detect(int version, fdmt_type_* d_arr, fdmt_type_* d_norm, const int Rows, const int Cols,
+                   const int WndWidth, const float valTresh, const dim3 gridSize, const dim3 blockSize,
+                   float* d_pAuxArray, int* d_pAuxNumArray, int* d_pWidthArray, structOutDetection* pstructOut) {
+    const int tile = (int)blockSize.x;
+    if (tile < 1 || (int)gridSize.x != (Cols + tile - 1) / tile || (int)gridSize.y != Rows)
+        throw std::invalid_argument("gridSize must be (ceil(Cols / blockSize.x), Rows)");
+    check(sps_detect_signal(version, d_arr, d_norm, Rows, Cols, WndWidth, tile, d_pAuxArray, d_pAuxNumArray,
+                            d_pWidthArray, (int64_t)gridSize.x * gridSize.y, pstructOut, nullptr));
+    structOutDetection h{};
+    if (hipMemcpy(&h, pstructOut, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess)
+        throw std::runtime_error("copy of the detection failed");
+    return h.snr >= valTresh;
+}
+}  // namespace sps_detail
+
+// v1: boxcar S/N of d_arr / sqrtf(1e-7 + d_norm), widths 1..WndWidth, inside a tile
+inline bool doDetectSignal_v1(fdmt_type_* d_arr, fdmt_type_* d_norm, const int Rows, const int Cols,
+                              const int WndWidth, const float valTresh, const dim3 gridSize, const dim3 blockSize,
+                              size_t sz, float* d_pAuxArray, int* d_pAuxNumArray, int* d_pWidthArray,
+                              structOutDetection* pstructOut) {
+    (void)sz;
+    return sps_detail::detect(1, d_arr, d_norm, Rows, Cols, WndWidth, valTresh, gridSize, blockSize, d_pAuxArray,
+                              d_pAuxNumArray, d_pWidthArray, pstructOut);
+}
+// v2: energy ratio sqrtf(sum x^2 / sum n^2), widths 1..WndWidth
+inline bool doDetectSignal_v2(fdmt_type_* d_arr, fdmt_type_* d_norm, const int Rows, const int Cols,
+                              const int WndWidth, const float valTresh, const dim3 gridSize, const dim3 blockSize,
+                              size_t sz, float* d_pAuxArray, int* d_pAuxNumArray, int* d_pWidthArray,
+                              structOutDetection* pstructOut) {
+    (void)sz;
+    return sps_detail::detect(2, d_arr, d_norm, Rows, Cols, WndWidth, valTresh, gridSize, blockSize, d_pAuxArray,
+                              d_pAuxNumArray, d_pWidthArray, pstructOut);
+}
+// v2_2: the energy ratio with a one-block final reduction (its window kernel
+// is declared in a header the reference does not ship); arrBuff is not needed
+inline bool doDetectSignal_v2_2(fdmt_type_* d_arr, fdmt_type_* d_norm, const int Rows, const int Cols,
+                                const int WndWidth, const float valTresh, const dim3 gridSize, const dim3 blockSize,
+                                size_t sz, float* d_pAuxArray, int* d_pAuxNumArray, int* d_pWidthArray,
+                                structOutDetection* pstructOut, char* arrBuff) {
+    (void)sz;
+    (void)arrBuff;
+    return sps_detail::detect(2, d_arr, d_norm, Rows, Cols, WndWidth, valTresh, gridSize, blockSize, d_pAuxArray,
+                              d_pAuxNumArray, d_pWidthArray, pstructOut);
+}
+
+// The intended semantics of the detector: the global maximum over every
+// window of 1..WndWidth samples inside the row (not restricted to tiles);
+// ties to the smaller width, then row, then column.
 inline bool doDetectSignal(fdmt_type_* d_arr, fdmt_type_* d_norm, const int Rows, const int Cols, const int WndWidth,
                            const float thr, structOutDetection* d_out) {
     sps_detail::check(sps_detect_max(d_arr, d_norm, Rows, Cols, WndWidth, d_out, nullptr));

@@ -77,6 +77,8 @@ def lib():
         L.orc_cluster.restype = _i64
         L.orc_cluster.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp, _vp]
         L.orc_detect_max.argtypes = [_vp, _vp, _i64, _i64, C.c_int, _vp, C.POINTER(_f32)]
+        L.orc_select_delegates.argtypes = [_vp, _vp, _i64, _vp, _i64, _vp]
+        L.orc_detect_tiles.argtypes = [C.c_int, _vp, _vp, _i64, _i64, C.c_int, C.c_int, _vp, C.POINTER(_f32)]
         L.orc_num_threads.restype = C.c_int
         L.orc_set_num_threads.argtypes = [C.c_int]
         _lib = L
@@ -291,6 +293,30 @@ def detect_max(a, norm, w):
     o = np.zeros(3, np.int32)
     s = _f32()
     rc = lib().orc_detect_max(_ptr(a), _ptr(na), a.shape[0], a.shape[1], w, _ptr(o), C.byref(s))
+    if rc != 0:
+        return None
+    return int(o[0]), int(o[1]), int(o[2]), s.value
+
+
+def select_delegates(cands, indices, group_begin):
+    """Delegates of caller groups (select_delegates.cu:16-51)."""
+    c = np.ascontiguousarray(cands, CAND_DTYPE)
+    idx = np.ascontiguousarray(indices, np.int32)
+    b = np.ascontiguousarray(group_begin, np.int32)
+    out = np.zeros(max(b.size, 1), np.int32)
+    lib().orc_select_delegates(_ptr(c), _ptr(idx), idx.size, _ptr(b), b.size, _ptr(out))
+    return out[: b.size]
+
+
+def detect_tiles(mode, a, norm, w, tile):
+    """The tiled detectors of doDetectSignal_v1 (mode 1) / _v2 (mode 2):
+    (width, row, col, value) or None."""
+    a = np.ascontiguousarray(a, np.float32)
+    na = None if norm is None else np.ascontiguousarray(norm, np.float32)
+    o = np.zeros(3, np.int32)
+    s = _f32()
+    rc = lib().orc_detect_tiles(int(mode), _ptr(a), _ptr(na), a.shape[0], a.shape[1], int(w), int(tile), _ptr(o),
+                                C.byref(s))
     if rc != 0:
         return None
     return int(o[0]), int(o[1]), int(o[2]), s.value

@@ -29,8 +29,15 @@
  *                                  Candidates2.csv + brute force on tiny inputs.
  *   clustering / delegates       : reference KATs (regrouping plan {4,4},
  *                                  delegates {6,7,10}, final sort {5,6,8}) +
- *                                  brute-force single-linkage on tiny sets.
+ *                                  brute-force single-linkage on tiny sets;
+ *                                  delegates over caller groups: the {6,7,10}
+ *                                  KAT + NaN/tie cases against max_element's
+ *                                  sequential rule.
  *   single-max detector          : brute force over all windows.
+ *   tiled detectors              : the detection/kernel.cu:645-676 input
+ *                                  (closed forms of its maxima), FP64 window
+ *                                  sums within the prefix-difference error
+ *                                  bound on random input.
  * Parity unpinned (no reference test or fixture fixes them): the reference
  * GPU's fast-math bit patterns, CUB/thrust tie order for equal keys and
  * atomicInc emission order; this restatement defines deterministic rules.
@@ -1000,9 +1007,96 @@ ORC_EXPORT int64_t orc_cluster(const orc_cand* c, int64_t n, const int32_t* bins
 /* Single-maximum detector (detection/kernel.cu:43-133,561-582 restated):    */
 /* global max over rows, windows 1..W fully inside, of                       */
 /* sum_{j<w} a[t+j]/sqrtf(1e-7 + n[t+j]) (fp32 left to right) / sqrtf(w).    */
-/* Ties -> smallest width, then row, then column.                            */
+/* Ties -> smallest width, then row, then column; NaN values never win.      */
 /* out4 = {iwidth, irow, icol} ints and *snr.  Returns 0, or -1 if empty.    */
 /* ------------------------------------------------------------------------ */
+/* select_delegates over caller groups (select_delegates.cu:16-51): group g  */
+/* is positions [begin[g], begin[g+1]) of idx (the last ends at nidx); the    */
+/* delegate is the index there whose msnr is largest, scanning in position   */
+/* order and replacing only on a strictly larger value -- thrust::           */
+/* max_element with a less-than comparator (a NaN at the first position      */
+/* stays, later NaNs never win).  -1 for an empty group.                     */
+ORC_EXPORT void orc_select_delegates(const orc_cand* c, const int32_t* idx, int64_t nidx, const int32_t* begin,
+                                     int64_t ngroups, int32_t* out) {
+    for (int64_t g = 0; g < ngroups; ++g) {
+        const int64_t b = begin[g], e = (g + 1 < ngroups) ? begin[g + 1] : nidx;
+        if (b >= e) { out[g] = -1; continue; }
+        int64_t best = b;
+        for (int64_t p = b + 1; p < e; ++p)
+            if (c[idx[best]].msnr < c[idx[p]].msnr) best = p;
+        out[g] = idx[best];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Tiled detectors (detection/kernel.cu:42-226, doDetectSignal_v1/_v2 at     */
+/* :561-605): each row is cut into tiles of `tile` columns; inside a tile    */
+/* fp32 prefix sums run left to right from the tile's first column and a     */
+/* window's value is a difference of prefix sums.                            */
+/*   mode 1: y = x / sqrtf(1e-7 + n) (n NULL: y = x); Y over the tile only;  */
+/*           value = (Y[t+w] - Y[t]) / sqrtf(w); window inside the tile.     */
+/*   mode 2: X2, N2 = prefix sums of x*x and n*n (n NULL: 1) over the tile   */
+/*           and the W samples after it, N2[0] = 1e-6 (the reference's       */
+/*           initial value); value = sqrtf((X2[t+w]-X2[t]) / (N2[t+w]-N2[t])).*/
+/* Windows end inside the row.  Best: largest value (NaN never), then        */
+/* smaller width, row, column.  out3 = {width, row, col}.  Returns -1 when   */
+/* no window exists.                                                         */
+/* ------------------------------------------------------------------------ */
+ORC_EXPORT int orc_detect_tiles(int mode, const float* a, const float* nrm, int64_t R, int64_t N, int W, int tile,
+                                int32_t* out3, float* snr) {
+    const int64_t span_max = (mode == 1) ? tile : (int64_t)tile + W;
+    float* A = (float*)malloc(sizeof(float) * (size_t)(span_max + 1));
+    float* B = (float*)malloc(sizeof(float) * (size_t)(span_max + 1));
+    int found = 0;
+    float best = 0.0f;
+    int32_t bw = -1, br = -1, bc = -1;
+    for (int64_t r = 0; r < R; ++r)
+        for (int64_t c0 = 0; c0 < N; c0 += tile) {
+            const int64_t navail = (N - c0 < span_max) ? N - c0 : span_max;
+            float s = 0.0f, q = 0.0f;
+            A[0] = 0.0f;
+            B[0] = 0.000001f;
+            for (int64_t i = 0; i < navail; ++i) {
+                const float x = a[r * N + c0 + i];
+                if (mode == 1) {
+                    const float y = nrm ? x / sqrtf((float)(1.0E-7 + (double)nrm[r * N + c0 + i])) : x;
+                    s = s + y;
+                    A[i + 1] = s;
+                } else {
+                    const float nv = nrm ? nrm[r * N + c0 + i] : 1.0f;
+                    s = s + x * x;
+                    q = q + nv * nv;
+                    A[i + 1] = s;
+                    B[i + 1] = q;
+                }
+            }
+            const int64_t tmax = (N - c0 < tile) ? N - c0 : tile;
+            for (int64_t t = 0; t < tmax; ++t)
+                for (int w = 1; w <= W; ++w) {
+                    if (t + w > navail) break;
+                    float v;
+                    if (mode == 1)
+                        v = (A[t + w] - A[t]) / sqrtf((float)w);
+                    else
+                        v = sqrtf((A[t + w] - A[t]) / (B[t + w] - B[t]));
+                    if (v != v) continue;
+                    int better = !found || v > best;
+                    if (found && v == best) {
+                        if (w != bw) better = w < bw;
+                        else if (r != br) better = r < br;
+                        else better = (c0 + t) < bc;
+                    }
+                    if (better) { found = 1; best = v; bw = w; br = (int32_t)r; bc = (int32_t)(c0 + t); }
+                }
+        }
+    free(A);
+    free(B);
+    if (!found) return -1;
+    out3[0] = bw; out3[1] = br; out3[2] = bc;
+    *snr = best;
+    return 0;
+}
+
 ORC_EXPORT int orc_detect_max(const float* a, const float* nrm, int64_t R, int64_t N, int W,
                               int32_t* out3, float* snr) {
     int found = 0;
@@ -1018,6 +1112,7 @@ ORC_EXPORT int orc_detect_max(const float* a, const float* nrm, int64_t R, int64
                     s += v;
                 }
                 float val = s / sqrtf((float)w);
+                if (val != val) continue;  /* a NaN window never wins */
                 if (!found || val > best) { found = 1; best = val; bw = w; br = (int32_t)r; bc = (int32_t)t; }
             }
     if (!found) return -1;

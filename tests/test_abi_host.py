@@ -3,6 +3,7 @@ entry point of include/sps.h, and the host plan builders (no GPU needed)
 produce the reference's tables exactly."""
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -11,6 +12,7 @@ from golden_inputs import plan_tables, tables_sha
 from oracle import orc
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def declared_symbols():
@@ -104,10 +106,40 @@ def test_device_count_call_without_gpu():
 
 
 def test_dropin_headers_compile(tmp_path):
-    """include/sps/*.hpp compile with the host compiler against sps.h."""
-    import subprocess
-
-    src = os.path.join(ROOT, "tests", "cpp", "dropin_harness.cpp")
-    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wno-unused-result", "-D__HIP_PLATFORM_AMD__", "-I",
-                        os.path.join(ROOT, "include"), "-I", "/opt/rocm/include", src], capture_output=True, text=True)
+    """include/sps/fdmt.hpp and search.hpp compile with the host compiler
+    against sps.h; the harness, which also uses the thrust::device_vector
+    forms (sps/thrust_vectors.hpp, rocThrust), compiles with hipcc."""
+    src = tmp_path / "h.cpp"
+    src.write_text('#include "sps/fdmt.hpp"\n#include "sps/search.hpp"\nint main() { return 0; }\n')
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I",
+                        os.path.join(ROOT, "include"), "-I", "/opt/rocm/include", str(src)], capture_output=True,
+                       text=True)
     assert r.returncode == 0, r.stderr[-2000:]
+    harness = os.path.join(ROOT, "tests", "cpp", "dropin_harness.cpp")
+    r = subprocess.run(["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", "-std=c++17", "-fsyntax-only", "-x", "hip",
+                        "-I", os.path.join(ROOT, "include"), harness], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+
+
+FREDDA = """# S/N, sampno, secs from file start, boxcar, idt, dm, beamno
+11.5 1000 0.864 3 120 56.2 0
+9.25 2048 1.769 1 7 3.28 2
+13 77 0.066 8 999 468.0 1
+"""
+
+
+def test_fredda_reader_and_csv_roundtrip(tmp_path):
+    """readFreddaCandFile / countCandidatesInFreddaFile (Clusterization.cu:
+    362-415) and writeCandHostArrayToCSV through the C++ header."""
+    src = os.path.join(HERE, "cpp", "fredda_check.cpp")
+    exe = str(tmp_path / "fredda_check")
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    r = subprocess.run(["g++", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I", inc, "-I", "/opt/rocm/include", src,
+                        "-o", exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    (tmp_path / "x.cand").write_text(FREDDA)
+    r = subprocess.run([exe, str(tmp_path / "x.cand"), str(tmp_path / "x.csv")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.split() == ["3"]
+    assert open(tmp_path / "x.csv").read().splitlines() == ["mt,mdt,mwidth,msnr", "1000,120,3,11.5", "2048,7,1,9.25",
+                                                            "77,999,8,13"]

@@ -164,32 +164,3 @@ def test_size_errors(torch_cuda):
         f.execute(d_in, d_in.numel() - 1, d_out, d_out.numel())
     with pytest.raises(InvalidArgument, match="dmt"):
         f.execute(d_in, d_in.numel(), d_out, d_out.numel() + 1)
-
-
-def test_cpp_dropin_harness(torch_cuda, tmp_path):
-    """The C++ drop-in classes (include/sps/*.hpp) used like the reference
-    harnesses; outputs compared bit-exactly with the oracle."""
-    import os
-    import subprocess
-
-    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "dropin_harness")
-    assert os.path.exists(exe), "build() compiles tests/cpp/dropin_harness"
-    r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=120)
-    assert r.returncode == 0, r.stdout + r.stderr
-    x = np.fromfile(tmp_path / "in.bin", np.float32).reshape(256, 2048)
-    ref = orc.Plan(1200.0, 1600.0, 256, 2048, 1.0, 255).execute(x)
-    got = np.fromfile(tmp_path / "fdmt.bin", np.float32).reshape(ref.shape)
-    np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
-    cf = orc.CFdmt(1200.0, 1600.0, 256, 2048, 256)
-    np.testing.assert_array_equal(np.fromfile(tmp_path / "cfdmt.bin", np.float32).reshape(256, 2048), cf.process(x))
-    np.testing.assert_array_equal(np.fromfile(tmp_path / "cfdmt_ones.bin", np.float32).reshape(256, 2048),
-                                  cf.process(None, True))
-    gd = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-    got2 = open(tmp_path / "Candidates2.csv").read().split()
-    exp2 = open(os.path.join(gd, "Candidates2.csv")).read().split()
-    assert got2 == exp2
-    got1 = sorted(open(tmp_path / "Candidates1.csv").read().split()[1:])
-    exp1 = sorted(open(os.path.join(gd, "Candidates1.csv")).read().split()[1:])
-    assert got1 == exp1
-    log = open(tmp_path / "candidates.log").read().splitlines()
-    assert [ln.split()[:2] for ln in log[1:]] == [["1", "4"], ["2", "4"], ["3", "11"]]

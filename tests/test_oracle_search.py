@@ -110,26 +110,44 @@ def test_regrouping_kat():
         assert len(off) - 1 == exp
 
 
+KAT_SNR = [1.0, 2.0, 3.0, 0.5, 2.5, 1.5, 4.0, 3.5, 0.7, 1.3, 5.0, 3.3, 2.7, 4.5, 3.2]
+KAT_PERM = [4, 6, 12, 7, 0, 2, 1, 8, 5, 14, 11, 9, 10, 3, 13]
+
+
 def test_select_delegates_kat():
-    """select_delegates.cu:332-352: groups {0,3,8} over a permutation -> {6,7,10}."""
-    rows = [(1, 2, 3, 1.0), (2, 3, 4, 2.0), (3, 4, 5, 3.0), (4, 5, 6, 0.5), (5, 6, 7, 2.5),
-            (6, 7, 8, 1.5), (7, 8, 9, 4.0), (8, 9, 10, 3.5), (9, 10, 11, 0.7), (10, 11, 12, 1.3),
-            (11, 12, 13, 5.0), (12, 13, 14, 3.3), (13, 14, 15, 2.7), (14, 15, 16, 4.5), (15, 16, 17, 3.2)]
-    c = cands(rows)
-    perm = [4, 6, 12, 7, 0, 2, 1, 8, 5, 14, 11, 9, 10, 3, 13]
-    off = [0, 3, 8, 15]
-    dele = []
-    for g in range(3):
-        members = perm[off[g]:off[g + 1]]
-        dele.append(max(members, key=lambda j: (c[j]["msnr"], -j)))
-    assert dele == [6, 7, 10]
+    """select_delegates/select_delegates.cu:323-347: 15 candidates, groups
+    beginning at {0, 3, 8} of a permuted index array -> delegates {6, 7, 10}."""
+    c = cands([(i + 1, i + 2, i + 3, v) for i, v in enumerate(KAT_SNR)])
+    assert orc.select_delegates(c, KAT_PERM, [0, 3, 8]).tolist() == [6, 7, 10]
+    # the same groups made by the three-key clustering: members of group g
+    # share mt = 100 (g + 1), groups are 100 apart, the other keys are equal
+    c2 = cands([(0, 0, 0, v) for v in KAT_SNR])
+    for g, (b, e) in enumerate(((0, 3), (3, 8), (8, 15))):
+        for j in KAT_PERM[b:e]:
+            c2[j]["mt"] = 100 * (g + 1)
+    _, off, dele, gsz = orc.cluster(c2, [2, 1, 2])
+    assert dele.tolist() == [6, 7, 10] and gsz.tolist() == [3, 5, 7]
+
+
+def test_select_delegates_ties_and_nan():
+    """thrust::max_element's sequential rule: the first maximum in group
+    order wins; a NaN only where it is a group's first element."""
+    c = cands([(0, 0, 0, v) for v in (3.0, 5.0, 5.0, float("nan"), 1.0, float("nan"), 2.0)])
+    d = orc.select_delegates(c, [2, 1, 0, 3, 4, 6, 5], [0, 3, 5])
+    assert d.tolist() == [2, 3, 6]
+    assert orc.select_delegates(c, [0, 1], [0, 2]).tolist() == [1, -1]
 
 
 def test_final_sort_kat():
-    """final_sort/kernel.cu:37-55: delegates {8,6,5} sorted by mt -> {5,6,8}."""
+    """final_sort/kernel.cu:37-55: delegates {8, 6, 5} of candidates with
+    mt = [10, 30, 20, 50, 40, 15, 35, 25, 55, 45] sorted by mt -> {5, 6, 8};
+    here through the clustering, each candidate its own group."""
     mts = [10, 30, 20, 50, 40, 15, 35, 25, 55, 45]
-    d = sorted([8, 6, 5], key=lambda j: mts[j])
-    assert d == [5, 6, 8]
+    picked = [8, 6, 5]
+    c = cands([(mts[j], 3 * k, 0, 1.0) for k, j in enumerate(picked)])
+    _, off, dele, gsz = orc.cluster(c, [0, 0, 0])
+    assert [picked[int(i)] for i in dele] == [5, 6, 8]
+    assert gsz.tolist() == [1, 1, 1]
 
 
 def brute_cluster(c, bins):
@@ -209,6 +227,41 @@ def test_stats_integer_input_exact():
     np.testing.assert_array_equal(st, st2)
 
 
+def variance_rows(var, n=512):
+    """Rows of n samples alternating +a, -a: mean 0, variance exactly a*a in FP64."""
+    a = np.sqrt(np.asarray(var, np.float64)).astype(np.float32)
+    x = np.empty((len(a), n), np.float32)
+    x[:, 0::2] = a[:, None]
+    x[:, 1::2] = -a[:, None]
+    return x, a.astype(np.float64) ** 2
+
+
+def cleaning_ratio(v):
+    v = np.asarray(v, np.float64)
+    return np.abs(v[-1] - v.mean()) / v.std()
+
+
+@pytest.mark.parametrize("target,kept", [(3.99, True), (4.01, False), (3.5, True)])
+def test_stats_cleaning_at_the_4_sigma_boundary(target, kept):
+    """A row whose variance lies just inside 4 sigma of the mean row variance
+    is kept (a 3-sigma rule would drop it) and just outside is dropped
+    (Statistical_preprocessing.cu:521-537, strict >)."""
+    base = [1.0 + 0.05 * (i % 7) for i in range(63)]
+    lo, hi = 1.0, 1e6
+    for _ in range(200):  # the outlier variance giving the target ratio
+        mid = 0.5 * (lo + hi)
+        if cleaning_ratio(base + [mid]) < target:
+            lo = mid
+        else:
+            hi = mid
+    x, v = variance_rows(base + [0.5 * (lo + hi)])
+    r = cleaning_ratio(v)
+    assert abs(r - target) < 1e-3 and (r > 3.0)
+    rm, rv, keep, st = orc.stats(x)
+    np.testing.assert_array_equal(rv, v)
+    assert bool(keep[-1]) == kept and keep[:-1].all()
+
+
 def test_per_channel_mode():
     x = np.random.default_rng(6).standard_normal((8, 512)).astype(np.float32) * np.arange(1, 9, dtype=np.float32)[:, None]
     out, mask, (rm, rv, keep, st) = orc.normalise(x, mode=2)
@@ -240,3 +293,67 @@ def test_detect_max_brute():
                 if best is None or v > best[0]:
                     best = (v, ww, rr, tt)
     assert (w, r, t) == best[1:] and s == best[0]
+
+
+def detector_kat_input():
+    rows, cols = 256, (1 << 20) // 256
+    a = np.ones((rows, cols), np.float32)
+    n = np.ones((rows, cols), np.float32)
+    f = a.reshape(-1)
+    f[3], f[12] = 20, 40
+    f[[8, 9, 11]] = 0
+    f[13:18] = 0
+    n.reshape(-1)[cols + 3:cols + 5] = 0.001
+    return a, n
+
+
+def test_tiled_detectors_on_the_reference_input():
+    """detection/kernel.cu:645-676 with WndWidth 10, blocks of 256: the
+    boxcar S/N maximum is the width-2 window over the two near-zero-norm
+    cells of row 1, (1/sqrt(1e-7 + 0.001)) * 2 / sqrt(2); the energy ratio is
+    maximal on those cells too: 1 / (n*n) with the n*n sums of that tile
+    differing by 4 ulp of 3.0 (2^-20), so sqrt(2^20) = 1024 exactly."""
+    a, n = detector_kat_input()
+    w, r, c, v = orc.detect_tiles(1, a, n, 10, 256)
+    assert (w, r, c) == (2, 1, 3)
+    y = 1.0 / np.sqrt(1e-7 + np.float64(np.float32(0.001)))
+    assert v == pytest.approx(2 * y / np.sqrt(2.0), rel=1e-6)
+    w, r, c, v = orc.detect_tiles(2, a, n, 10, 256)
+    assert (w, r, c, v) == (1, 1, 3, 1024.0)
+    # the whole-row detector finds the same window as the tiled one here
+    assert orc.detect_max(a, n, 10)[:3] == (2, 1, 3)
+
+
+def test_tiled_detectors_vs_fp64_windows():
+    """Brute force: every tile's best window value is within the
+    prefix-difference rounding bound of its FP64 window sum, and windows that
+    cross a tile never win in mode 1."""
+    rng = np.random.default_rng(31)
+    for trial in range(6):
+        R, N, W, tile = 3, int(rng.integers(40, 300)), int(rng.integers(1, 12)), int(rng.choice([16, 32, 64]))
+        a = rng.standard_normal((R, N)).astype(np.float32)
+        a[1, N // 2:N // 2 + 5] += 3.0
+        n = rng.uniform(0.5, 2.0, (R, N)).astype(np.float32)
+        for mode in (1, 2):
+            w, r, c, v = orc.detect_tiles(mode, a, n, W, tile)
+            t0 = (c // tile) * tile
+            assert c + w <= N and (mode == 2 or c + w <= t0 + tile)
+            if mode == 1:
+                y = a[r].astype(np.float64) / np.sqrt(1e-7 + n[r].astype(np.float64))
+                exact = y[c:c + w].sum() / np.sqrt(w)
+                scale = np.abs(y[t0:c + w]).sum() / np.sqrt(w)
+            else:
+                x2 = a[r].astype(np.float64) ** 2
+                n2 = n[r].astype(np.float64) ** 2
+                exact = np.sqrt(x2[c:c + w].sum() / n2[c:c + w].sum())
+                scale = exact * (1 + (x2[t0:c + w].sum() / x2[c:c + w].sum() + n2[t0:c + w].sum() /
+                                      n2[c:c + w].sum()))
+            assert abs(v - exact) <= 2e-6 * (N + W) * scale + 1e-6
+            # no window of the same row scores more than the winner by more than the bound
+            if mode == 1:
+                for t in range(N):
+                    tt = (t // tile) * tile
+                    for ww in range(1, W + 1):
+                        if t + ww <= min(N, tt + tile):
+                            e2 = y[t:t + ww].sum() / np.sqrt(ww)
+                            assert e2 <= exact + 4e-6 * (N + W) * (np.abs(y).sum() / np.sqrt(ww)) + 1e-5
