@@ -65,6 +65,27 @@ def cpu_baseline(x_u8, ndt, sample_cols, threads):
     return ndt * sample_cols / dt, dt, len(cands)
 
 
+def cpu_reference_fdmt(x_u8, ndt, sample_cols, threads):
+    """Times the reference's own FDMTCPU (built from its sources with its
+    OpenMP switch, oracle/_ref) on the same slice: one fresh instance, one
+    execute of the normalised waterfall.  None when that build is absent."""
+    import numpy as np
+
+    from oracle import orc
+
+    if not os.path.exists(os.path.join(orc.REF_DIR, "libref_pravir_omp.so")):
+        return None
+    c = CFG
+    xn, _, _ = orc.normalise(np.ascontiguousarray(x_u8[:, :sample_cols]), 1)
+    orc.ref_lib("pravir_omp").ref_fdmt_set_num_threads(threads)
+    r = orc.RefFDMT("pravir_omp", c["f_min"], c["f_max"], c["nchans"], sample_cols, 1.0, c["dt_max"])
+    t0 = time.perf_counter()
+    r.execute(xn)
+    dt = time.perf_counter() - t0
+    return {"value": ndt * sample_cols / dt, "unit": "DM-trials*samples/s", "cores": threads, "kind": "reference",
+            "seconds": dt, "what": "reference FDMTCPU (FDMT_PRAVIR, USE_OPENMP) alone, one execute, same slice"}
+
+
 def batch_specs(nbeams):
     return workloads.batch_specs(nbeams)
 
@@ -558,7 +579,8 @@ def main():
         out["cpu_baseline"] = {"value": v, "unit": "DM-trials*samples/s", "cores": threads, "kind": "port",
                                "cpu_model": cpu_model(), "nproc": os.cpu_count(),
                                "sample": f"{R}ch x {args.cpu_sample_cols} samples (first slice of the beam), "
-                                         f"same chain, {dt:.2f} s"}
+                                         f"same chain, {dt:.2f} s",
+                               "reference_fdmt": cpu_reference_fdmt(xs, ndt, args.cpu_sample_cols, threads)}
     print(json.dumps(out))
 
 

@@ -104,6 +104,9 @@ SIGNATURES = {
     "sps_search_sort_candidates": [_vp, _vp, _vp, _pi64, _vp],
     "sps_search_gather": [_vp, _vp, _vp, _i64, _vp, _pi64, _vp],
     "sps_search_get_ndt": [_vp, _pi64],
+    "sps_search_cluster": [_vp, _vp, _int, _vp, _pi64, _pi64, _vp],
+    "sps_search_get_delegates": [_vp, _vp, _pi64, _i64, _pi64],
+    "sps_search_write_log": [_vp, C.c_char_p, _f32, _f32],
     "sps_fdmt_set_schedule": [_vp, C.POINTER(C.c_int32), _i64],
     "sps_fdmt_get_schedule": [_vp, C.POINTER(C.c_int32), _i64, _pi64],
     "sps_search_set_schedule": [_vp, C.POINTER(C.c_int32), _i64],

@@ -966,6 +966,75 @@ void sort_cands(CandD* c, int64_t n, int64_t rows, int64_t cols, int64_t wmax, h
     SPS_HIP(hipStreamSynchronize(st));
 }
 
+// Every candidate of the image, sorted by (mdt, mt, mwidth), into `out`
+// (grown to the count); returns the count.  Synchronises st.
+int64_t gather_sorted(const float* img, int64_t rows, int64_t cols, int64_t pitch, const float* d_thr, float h_thr,
+                      int wnd, bool pow2, DevBuf& out, DevBuf& cnt, hipStream_t st) {
+    const int64_t wmax = pow2 ? (int64_t)1 << wnd : wnd;
+    cnt.ensure(sizeof(unsigned long long));
+    int64_t tcap = std::max<int64_t>((int64_t)(out.bytes / sizeof(CandD)), 1024);
+    unsigned long long n = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        out.ensure(sizeof(CandD) * (size_t)tcap);
+        launch_fill_u64(cnt.as<unsigned long long>(), 1, 0ull, st);
+        launch_boxcar(img, rows, cols, pitch, d_thr, h_thr, (int)wmax, pow2, out.as<CandD>(), tcap,
+                      cnt.as<unsigned long long>(), st);
+        SPS_HIP(hipMemcpyAsync(&n, cnt.p, sizeof(n), hipMemcpyDeviceToHost, st));
+        SPS_HIP(hipStreamSynchronize(st));
+        if ((int64_t)n <= tcap) break;
+        tcap = (int64_t)n;
+    }
+    sort_cands(out.as<CandD>(), (int64_t)n, rows, cols, pow2 ? wnd : wmax, st);
+    return (int64_t)n;
+}
+
+// Three-key gap clustering (Clusterization.cu:92-166) of n candidates with the
+// device bins, delegates sorted by mt into d_delegates / d_group_size;
+// returns the group count.  Synchronises st.
+int64_t cluster_candidates(const CandD* c, int64_t n, const int32_t* d_bins, int32_t* d_delegates,
+                           int64_t* d_group_size, hipStream_t st) {
+    if (n == 0) return 0;
+    SPS_REQUIRE(n < 0xffffffffLL, "invalid candidate count");
+    DevBuf k, kt, v, vt, perm, grp, flags, hs, best, start, sc;
+    k.ensure(sizeof(uint64_t) * (size_t)n);
+    kt.ensure(sizeof(uint64_t) * (size_t)n);
+    v.ensure(sizeof(uint32_t) * (size_t)n);
+    vt.ensure(sizeof(uint32_t) * (size_t)n);
+    perm.ensure(sizeof(uint32_t) * (size_t)n);
+    grp.ensure(sizeof(uint32_t) * (size_t)n);
+    flags.ensure(sizeof(uint32_t) * (size_t)n);
+    const int64_t hc = radix_hist_entries(n);
+    hs.ensure(sizeof(uint32_t) * (size_t)hc);
+    sc.ensure(sizeof(uint32_t) * (size_t)scan_scratch_entries(n));
+    launch_iota_u32(perm.as<uint32_t>(), n, st);
+    for (int level = 0; level < 3; ++level) {
+        launch_cluster_keys(c, perm.as<uint32_t>(), level == 0 ? nullptr : grp.as<uint32_t>(), n, level,
+                            k.as<uint64_t>(), v.as<uint32_t>(), st);
+        radix_sort_pairs(k.as<uint64_t>(), v.as<uint32_t>(), kt.as<uint64_t>(), vt.as<uint32_t>(), n,
+                         level == 0 ? 32 : 64, hs.as<uint32_t>(), hc, st);
+        SPS_HIP(hipMemcpyAsync(perm.p, v.p, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice, st));
+        launch_cluster_flags(k.as<uint64_t>(), n, d_bins, level, flags.as<uint32_t>(), st);
+        SPS_HIP(hipMemcpyAsync(grp.p, flags.p, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice, st));
+        scan_inclusive_u32(grp.as<uint32_t>(), n, sc.as<uint32_t>(), st);
+    }
+    uint32_t ng = 0;
+    SPS_HIP(hipMemcpyAsync(&ng, grp.as<uint32_t>() + (n - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    SPS_HIP(hipStreamSynchronize(st));
+    best.ensure(sizeof(unsigned long long) * (size_t)ng);
+    start.ensure(sizeof(uint32_t) * (size_t)ng);
+    launch_fill_u64(best.as<unsigned long long>(), ng, 0ull, st);
+    launch_cluster_delegates(c, perm.as<uint32_t>(), grp.as<uint32_t>(), flags.as<uint32_t>(), n,
+                             best.as<unsigned long long>(), start.as<uint32_t>(), st);
+    launch_delegate_keys(c, best.as<unsigned long long>(), start.as<uint32_t>(), ng, n, k.as<uint64_t>(),
+                         v.as<uint32_t>(), st);
+    radix_sort_pairs(k.as<uint64_t>(), v.as<uint32_t>(), kt.as<uint64_t>(), vt.as<uint32_t>(), ng, 64,
+                     hs.as<uint32_t>(), hc, st);
+    launch_delegate_out(best.as<unsigned long long>(), start.as<uint32_t>(), v.as<uint32_t>(), ng, n, d_delegates,
+                        d_group_size, st);
+    SPS_HIP(hipStreamSynchronize(st));
+    return (int64_t)ng;
+}
+
 }  // namespace
 
 SPS_API sps_status sps_boxcar_candidates(const float* d_img, int64_t rows, int64_t cols, const float* d_thr, int wnd,
@@ -977,29 +1046,15 @@ SPS_API sps_status sps_boxcar_candidates(const float* d_img, int64_t rows, int64
         SPS_REQUIRE(cap == 0 || d_out, "null output");
         const bool pow2 = !all_widths;
         SPS_REQUIRE(wnd >= (pow2 ? 0 : 1) && wnd <= (pow2 ? 13 : 8192), "window parameter out of range");
-        const int64_t wmax = pow2 ? (int64_t)1 << wnd : wnd;
         hipStream_t st = as_stream(stream);
         DevBuf cnt, tmp;
-        cnt.ensure(sizeof(unsigned long long));
-        int64_t tcap = std::max<int64_t>(cap, 1024);
-        unsigned long long n = 0;
-        for (int pass = 0; pass < 2; ++pass) {
-            tmp.ensure(sizeof(CandD) * (size_t)tcap);
-            launch_fill_u64(cnt.as<unsigned long long>(), 1, 0ull, st);
-            launch_boxcar(d_img, rows, cols, cols, d_thr, 0.f, (int)wmax, pow2, tmp.as<CandD>(), tcap,
-                          cnt.as<unsigned long long>(), st);
-            SPS_HIP(hipMemcpyAsync(&n, cnt.p, sizeof(n), hipMemcpyDeviceToHost, st));
-            SPS_HIP(hipStreamSynchronize(st));
-            if ((int64_t)n <= tcap) break;
-            tcap = (int64_t)n;
-        }
-        sort_cands(tmp.as<CandD>(), (int64_t)n, rows, cols, pow2 ? wnd : wmax, st);
-        const int64_t ncopy = std::min<int64_t>((int64_t)n, cap);
+        const int64_t n = gather_sorted(d_img, rows, cols, cols, d_thr, 0.f, wnd, pow2, tmp, cnt, st);
+        const int64_t ncopy = std::min<int64_t>(n, cap);
         if (ncopy > 0)
             SPS_HIP(hipMemcpyAsync(d_out, tmp.p, sizeof(CandD) * (size_t)ncopy, hipMemcpyDeviceToDevice, st));
         SPS_HIP(hipStreamSynchronize(st));
-        *h_count = (int64_t)n;
-        if ((int64_t)n > cap) throw Error(SPS_ERR_CAPACITY, "more candidates than the output capacity");
+        *h_count = n;
+        if (n > cap) throw Error(SPS_ERR_CAPACITY, "more candidates than the output capacity");
     });
 }
 
@@ -1011,46 +1066,8 @@ SPS_API sps_status sps_cluster(const sps_cand* d_cands, int64_t n, const int32_t
         *h_ngroups = 0;
         if (n == 0) return;
         SPS_REQUIRE(d_cands && d_delegates && d_group_size, "null argument");
-        hipStream_t st = as_stream(stream);
-        const CandD* c = reinterpret_cast<const CandD*>(d_cands);
-        DevBuf k, kt, v, vt, perm, grp, flags, hs, best, start, sc;
-        k.ensure(sizeof(uint64_t) * (size_t)n);
-        kt.ensure(sizeof(uint64_t) * (size_t)n);
-        v.ensure(sizeof(uint32_t) * (size_t)n);
-        vt.ensure(sizeof(uint32_t) * (size_t)n);
-        perm.ensure(sizeof(uint32_t) * (size_t)n);
-        grp.ensure(sizeof(uint32_t) * (size_t)n);
-        flags.ensure(sizeof(uint32_t) * (size_t)n);
-        const int64_t hc = radix_hist_entries(n);
-        hs.ensure(sizeof(uint32_t) * (size_t)hc);
-        sc.ensure(sizeof(uint32_t) * (size_t)scan_scratch_entries(n));
-        launch_iota_u32(perm.as<uint32_t>(), n, st);
-        for (int level = 0; level < 3; ++level) {
-            launch_cluster_keys(c, perm.as<uint32_t>(), level == 0 ? nullptr : grp.as<uint32_t>(), n, level,
-                                k.as<uint64_t>(), v.as<uint32_t>(), st);
-            radix_sort_pairs(k.as<uint64_t>(), v.as<uint32_t>(), kt.as<uint64_t>(), vt.as<uint32_t>(), n,
-                             level == 0 ? 32 : 64, hs.as<uint32_t>(), hc, st);
-            SPS_HIP(hipMemcpyAsync(perm.p, v.p, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice, st));
-            launch_cluster_flags(k.as<uint64_t>(), n, d_bins, level, flags.as<uint32_t>(), st);
-            SPS_HIP(hipMemcpyAsync(grp.p, flags.p, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice, st));
-            scan_inclusive_u32(grp.as<uint32_t>(), n, sc.as<uint32_t>(), st);
-        }
-        uint32_t ng = 0;
-        SPS_HIP(hipMemcpyAsync(&ng, grp.as<uint32_t>() + (n - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-        SPS_HIP(hipStreamSynchronize(st));
-        best.ensure(sizeof(unsigned long long) * (size_t)ng);
-        start.ensure(sizeof(uint32_t) * (size_t)ng);
-        launch_fill_u64(best.as<unsigned long long>(), ng, 0ull, st);
-        launch_cluster_delegates(c, perm.as<uint32_t>(), grp.as<uint32_t>(), flags.as<uint32_t>(), n,
-                                 best.as<unsigned long long>(), start.as<uint32_t>(), st);
-        launch_delegate_keys(c, best.as<unsigned long long>(), start.as<uint32_t>(), ng, n, k.as<uint64_t>(),
-                             v.as<uint32_t>(), st);
-        radix_sort_pairs(k.as<uint64_t>(), v.as<uint32_t>(), kt.as<uint64_t>(), vt.as<uint32_t>(), ng, 64,
-                         hs.as<uint32_t>(), hc, st);
-        launch_delegate_out(best.as<unsigned long long>(), start.as<uint32_t>(), v.as<uint32_t>(), ng, n, d_delegates,
-                            d_group_size, st);
-        SPS_HIP(hipStreamSynchronize(st));
-        *h_ngroups = ng;
+        *h_ngroups = cluster_candidates(reinterpret_cast<const CandD*>(d_cands), n, d_bins, d_delegates, d_group_size,
+                                        as_stream(stream));
     });
 }
 
@@ -1103,6 +1120,11 @@ struct sps_search {
     DevBuf norm_img, norm_a, norm_b; // mask transform output and its states
     DevBuf dmt;                      // internal DM-time image when the caller passes none
     bool ready = false;
+    // clustering leg (sps_search_cluster): all-widths candidates, delegates
+    DevBuf heap, heap_cnt, d_del, d_gsz;
+    int64_t heap_n = 0;
+    std::vector<int32_t> h_del;
+    std::vector<int64_t> h_gsz;
     // stage timing: ev[0] start, ev[1] stats, ev[2..] init + merges, then
     // mask transform, S/N normalisation, boxcar
     bool timing = false;
@@ -1455,6 +1477,65 @@ SPS_API sps_status sps_search_gather(sps_search* s, const float* d_dmt, sps_cand
         *h_count = (int64_t)n;
         if ((int64_t)n > cap) throw Error(SPS_ERR_CAPACITY, "more candidates than the gather's capacity");
     });
+}
+
+SPS_API sps_status sps_search_cluster(sps_search* s, const float* d_dmt, int wnd, const int32_t* d_bins,
+                                      int64_t* h_ncand, int64_t* h_ngroups, void* stream) {
+    return guarded([&] {
+        SPS_REQUIRE(s && d_bins && h_ncand && h_ngroups, "null argument");
+        SPS_REQUIRE(wnd >= 1 && wnd <= 8192, "window must be 1..8192");
+        const float* img = d_dmt ? d_dmt : s->dmt.as<float>();
+        SPS_REQUIRE(img, "no DM-time image: run the search first (or pass the image it wrote)");
+        hipStream_t st = as_stream(stream);
+        s->heap_n = 0;
+        s->h_del.clear();
+        s->h_gsz.clear();
+        *h_ncand = 0;
+        *h_ngroups = 0;
+        const int64_t N = s->p.nsamps;
+        const int64_t n = gather_sorted(img, s->ndt, N, N, nullptr, s->p.threshold, wnd, false, s->heap, s->heap_cnt, st);
+        s->heap_n = n;
+        *h_ncand = n;
+        if (n == 0) return;
+        s->d_del.ensure(sizeof(int32_t) * (size_t)n);
+        s->d_gsz.ensure(sizeof(int64_t) * (size_t)n);
+        const int64_t ng = cluster_candidates(s->heap.as<CandD>(), n, d_bins, s->d_del.as<int32_t>(),
+                                              s->d_gsz.as<int64_t>(), st);
+        s->h_del.resize((size_t)ng);
+        s->h_gsz.resize((size_t)ng);
+        if (ng > 0) {
+            SPS_HIP(hipMemcpyAsync(s->h_del.data(), s->d_del.p, sizeof(int32_t) * (size_t)ng, hipMemcpyDeviceToHost, st));
+            SPS_HIP(hipMemcpyAsync(s->h_gsz.data(), s->d_gsz.p, sizeof(int64_t) * (size_t)ng, hipMemcpyDeviceToHost, st));
+        }
+        SPS_HIP(hipStreamSynchronize(st));
+        *h_ngroups = ng;
+    });
+}
+
+SPS_API sps_status sps_search_get_delegates(const sps_search* s, sps_cand* h_delegates, int64_t* h_group_sizes,
+                                            int64_t cap, int64_t* h_n) {
+    return guarded([&] {
+        SPS_REQUIRE(s && h_n && cap >= 0, "null argument");
+        const int64_t ng = (int64_t)s->h_del.size();
+        *h_n = ng;
+        const int64_t m = std::min(ng, cap);
+        for (int64_t i = 0; i < m; ++i) {
+            if (h_delegates)
+                SPS_HIP(hipMemcpy(&h_delegates[i], s->heap.as<CandD>() + s->h_del[(size_t)i], sizeof(CandD),
+                                  hipMemcpyDeviceToHost));
+            if (h_group_sizes) h_group_sizes[i] = s->h_gsz[(size_t)i];
+        }
+    });
+}
+
+SPS_API sps_status sps_search_write_log(const sps_search* s, const char* filename, float delt_bin,
+                                        float t_chunk_begin) {
+    if (!s || !filename) {
+        g_last_error = "null argument";
+        return SPS_ERR_INVALID_ARGUMENT;
+    }
+    return sps_write_log(reinterpret_cast<const sps_cand*>(s->heap.p), s->heap_n, s->h_del.data(), s->h_gsz.data(),
+                         (int64_t)s->h_del.size(), filename, delt_bin, t_chunk_begin);
 }
 
 }  // extern "C"

@@ -328,6 +328,25 @@ class SingleBeamSearch:
              C.byref(n), _stream())
         return n.value
 
+    def cluster(self, d_dmt, wnd, d_bins):
+        """The clustering leg of CChunk_gpu::process (Chunk_gpu.cu:398-425) on
+        the last run's S/N image: all-widths gather (widths 1..wnd), three-key
+        clustering with d_bins, delegates.  Returns (delegate candidates
+        (numpy CAND_DTYPE, sorted by mt), group sizes, gathered count)."""
+        nc, ng = C.c_int64(), C.c_int64()
+        call("sps_search_cluster", self._h, _dptr(d_dmt), int(wnd), _dptr(d_bins), C.byref(nc), C.byref(ng),
+             _stream())
+        dele = np.zeros(max(ng.value, 1), CAND_DTYPE)
+        gsz = np.zeros(max(ng.value, 1), np.int64)
+        n = C.c_int64()
+        call("sps_search_get_delegates", self._h, dele.ctypes.data_as(C.c_void_p),
+             gsz.ctypes.data_as(C.POINTER(C.c_int64)), ng.value, C.byref(n))
+        return dele[: ng.value], gsz[: ng.value], nc.value
+
+    def write_log(self, filename, delt_bin, t_chunk_begin):
+        """select_delegates::write_log of the last cluster() call."""
+        call("sps_search_write_log", self._h, str(filename).encode(), float(delt_bin), float(t_chunk_begin))
+
     def candidates(self, d_dmt, d_cands, d_count):
         """Sorted candidates of the last run: (tensor (n, 4), n).  When the run
         overflowed d_cands, a buffer of exactly the count is allocated and

@@ -335,6 +335,25 @@ SPS_API sps_status sps_search_sort_candidates(sps_search* s, sps_cand* d_cands, 
  * when it exceeds cap. */
 SPS_API sps_status sps_search_gather(sps_search* s, const float* d_dmt, sps_cand* d_cands, int64_t cap,
                                      uint64_t* d_count, int64_t* h_count, void* stream);
+/* The clustering leg of CChunk_gpu::process (CDMT/Chunk_gpu.cu:398-425:
+ * clusterization_main, Clusterization.cu:17-54) on the S/N image of the
+ * last sps_search_run: the all-widths gather (widths 1..wnd, mwidth = width,
+ * the search's threshold, strict >) over d_dmt (the image that run wrote, or
+ * NULL when it kept the image internally), the three-key gap clustering
+ * with d_bins (device, 3 ints for mt, mdt, mwidth) and the delegates (the
+ * largest S/N of each group, sorted by mt).  h_ncand receives the number of
+ * gathered candidates and h_ngroups the number of groups (synchronises). */
+SPS_API sps_status sps_search_cluster(sps_search* s, const float* d_dmt, int wnd, const int32_t* d_bins,
+                                      int64_t* h_ncand, int64_t* h_ngroups, void* stream);
+/* The delegates of the last sps_search_cluster: up to cap candidate records
+ * (sorted by mt) and their group sizes (either pointer may be NULL); h_n
+ * receives the group count. */
+SPS_API sps_status sps_search_get_delegates(const sps_search* s, sps_cand* h_delegates, int64_t* h_group_sizes,
+                                            int64_t cap, int64_t* h_n);
+/* select_delegates::write_log (select_delegates.cu:127-177) of the last
+ * sps_search_cluster's delegates, as sps_write_log writes it. */
+SPS_API sps_status sps_search_write_log(const sps_search* s, const char* filename, float delt_bin,
+                                        float t_chunk_begin);
 SPS_API sps_status sps_search_get_ndt(const sps_search* s, int64_t* h_ndt);
 /* staged passes of the search's data transform (see sps_fdmt_set_staged) */
 SPS_API sps_status sps_search_set_staged(sps_search* s, int staged);

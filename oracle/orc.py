@@ -362,6 +362,7 @@ def ref_lib(kind):
             L.ref_fdmt_df.argtypes = [_vp, _i64, C.POINTER(_f32), C.POINTER(_f32)]
             L.ref_fdmt_execute.argtypes = [_vp, _vp, _i64, _vp, _i64]
             L.ref_fdmt_dm_grid.argtypes = [_vp, _vp]
+            L.ref_fdmt_set_num_threads.argtypes = [C.c_int]
         _ref[kind] = L
     return _ref[kind]
 

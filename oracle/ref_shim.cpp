@@ -28,6 +28,8 @@ void* ref_fdmt_create(float fmin, float fmax, int64_t nchans, int64_t nsamps, fl
     }
 }
 void ref_fdmt_destroy(void* h) { delete static_cast<FDMTCPU*>(h); }
+// FDMTCPU::set_num_threads (fdmt_cpu.hpp:9)
+void ref_fdmt_set_num_threads(int n) { FDMTCPU::set_num_threads(n); }
 
 int64_t ref_fdmt_niters(void* h) { return (int64_t) static_cast<FDMTCPU*>(h)->get_niters(); }
 

@@ -214,9 +214,11 @@ int main(int argc, char** argv) {
         HIP_OK(hipFree(d_c1));
         HIP_OK(hipFree(d_c2));
 
-        // row cleaning without normalisation: row 5 of 16 has 25x the variance
+        // row cleaning without normalisation: row 5 of 64 has 25x the variance
+        // (one outlier among n rows lies at most sqrt(n - 1) sigma from the
+        // mean variance, so 4 sigma needs more than 17 rows)
         {
-            const int R = 16, N = 512;
+            const int R = 64, N = 512;
             std::vector<float> x((size_t)R * N);
             for (int r = 0; r < R; ++r)
                 for (int t = 0; t < N; ++t) {

@@ -79,18 +79,18 @@ def test_cpp_dropin_harness(torch_cuda, tmp_path):
         rows.append([k + 1, j + 1, f"{float(tsec):g}", j + 2, j + 3, f"{snr[j]:g}", num])
     assert open(tmp_path / "delegates.log").read().splitlines() == log_lines(rows)
     # clean_fdmtInput_and_create_normInput: row 5 (25x the variance) zeroed, values otherwise untouched
-    xin = np.fromfile(tmp_path / "clean_in.bin", np.float32).reshape(16, 512)
-    xo = np.fromfile(tmp_path / "clean_out.bin", np.float32).reshape(16, 512)
-    m = np.fromfile(tmp_path / "clean_mask.bin", np.uint8).reshape(16, 512)
+    xin = np.fromfile(tmp_path / "clean_in.bin", np.float32).reshape(64, 512)
+    xo = np.fromfile(tmp_path / "clean_out.bin", np.float32).reshape(64, 512)
+    m = np.fromfile(tmp_path / "clean_mask.bin", np.uint8).reshape(64, 512)
     b = np.fromfile(tmp_path / "clean_buff.bin", np.float32)
     rm, rv, keep, st = orc.stats(xin, 1)
-    assert keep[5] == 0 and keep.sum() == 15
+    assert keep[5] == 0 and keep.sum() == 63
     exp = xin.copy()
     exp[keep == 0] = 0
     np.testing.assert_array_equal(xo, exp)
     np.testing.assert_array_equal(m, np.repeat(keep[:, None], 512, 1))
-    np.testing.assert_array_equal(b[:16], rv.astype(np.float32))
-    assert b[16] == np.float32(st[2]) and b[17] == np.float32(st[3])
+    np.testing.assert_array_equal(b[:64], rv.astype(np.float32))
+    assert b[64] == np.float32(st[2]) and b[65] == np.float32(st[3])
     # the detectors on detection/kernel.cu's input
     arr = np.fromfile(tmp_path / "det_arr.bin", np.float32).reshape(256, 4096)
     nrm = np.fromfile(tmp_path / "det_norm.bin", np.float32).reshape(256, 4096)
