@@ -95,6 +95,7 @@ SIGNATURES = {
     "sps_boxcar_candidates": [_vp, _i64, _i64, _vp, _int, _int, _vp, _i64, _pi64, _vp],
     "sps_cluster": [_vp, _i64, _vp, _vp, _vp, _pi64, _vp],
     "sps_detect_max": [_vp, _vp, _i64, _i64, _int, _vp, _vp],
+    "sps_detect_max_noalloc": [_vp, _vp, _i64, _i64, _int, _vp, _vp],
     "sps_boxcar_snr": [_vp, _i64, _i64, _int, _vp, _vp],
     "sps_digitize": [_vp, _i64, _vp],
     "sps_boxcar_snr_adjoint": [_vp, _i64, _i64, _int, _vp, _vp],

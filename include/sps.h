@@ -280,6 +280,18 @@ SPS_API sps_status sps_detect_signal(int version, const float* d_arr, const floa
                                      int wnd, int tile, float* d_tile_val, int32_t* d_tile_pos, int32_t* d_tile_w,
                                      int64_t ntiles_cap, sps_detection* d_out, void* stream);
 
+/* No counterpart in the reference: sps_detect_max (the same arguments and
+ * the same result, bit for bit) for callers that cannot allocate device
+ * memory -- code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  Its per-block partial results live in a static device
+ * array of 4096 x 16 bytes per GPU instead of a buffer allocated per call,
+ * so it calls no allocator; any image size is accepted (rows and columns
+ * up to 2^31 - 1).  Calls on one GPU share that array: calls that overlap
+ * in time (different streams or threads) race; calls ordered on one stream
+ * do not. */
+SPS_API sps_status sps_detect_max_noalloc(const float* d_arr, const float* d_norm, int64_t rows, int64_t cols, int wnd,
+                                          sps_detection* d_out, void* stream);
+
 /* clusterization::digitize_kernel (Clusterization.cuh:34-35,
  * Clusterization.cu:206-214): d_x[i] = floorf(d_x[i]) in place, i < n (the
  * reference test harness uses it to make integer-valued images). */
