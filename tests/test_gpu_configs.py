@@ -47,8 +47,13 @@ def torch_cuda():
 
 
 def free_cuda(torch):
+    import gc
+
+    gc.collect()
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
+    free, total = torch.cuda.mem_get_info()
+    say(f"[cfg] device memory free {free / 2**30:.1f} of {total / 2**30:.1f} GiB")
 
 
 def oracle_chain(xh, fmin, fmax, nch, ns, dt_max, thr, q, stats=None):
@@ -195,6 +200,35 @@ def test_single_beam_full_size_vs_oracle(torch_cuda):
     np.testing.assert_array_equal(got_c.view(np.int32), ref.view(np.int32))
 
 
+@pytest.mark.timeout(900)
+def test_large_share_ones_closed_form(torch_cuda):
+    from cpp_projects_amd import workloads
+    from cpp_projects_amd.fdmt import FDMTGPU
+
+    torch = torch_cuda
+    c = workloads.LARGE
+    R, N = c["nchans"], c["nsamps"]
+    free_cuda(torch)
+    say(f"[cfg] large share: FDMT of ones {R}x{N}")
+    f = FDMTGPU(F_MIN, F_MAX, R, N, 1.0, c["dt_max"])
+    f.set_staged(3)
+    rows, cols = f.output_shape()
+    out = torch.empty((rows, cols), dtype=torch.float32, device="cuda")
+    f._execute(None, 0, out, out.numel(), ones=True)
+    ext = torch.from_numpy(orc.Plan(F_MIN, F_MAX, R, N, 1.0, c["dt_max"]).track_extent()).cuda()
+    col = torch.arange(cols, device="cuda")
+    bad = 0
+    for r0 in range(0, rows, 512):
+        blk = out[r0:r0 + 512]
+        valid = col[None, :] >= ext[r0:r0 + 512, None]
+        bad += int(((blk != float(R)) & valid).sum().item())
+        # before its extent a row holds fewer than R contributions
+        bad += int(((blk >= float(R)) & ~valid).sum().item())
+    del f, out
+    free_cuda(torch)
+    assert bad == 0
+
+
 @pytest.mark.timeout(1200)
 def test_large_share_leading_slice_vs_oracle(torch_cuda):
     from cpp_projects_amd import workloads
@@ -234,34 +268,6 @@ def test_large_share_leading_slice_vs_oracle(torch_cuda):
     assert same_bits(got_img, snr), "S/N image slice differs from the oracle"
     assert len(got_c) == len(ref)
     np.testing.assert_array_equal(got_c.view(np.int32), ref.view(np.int32))
-
-
-@pytest.mark.timeout(900)
-def test_large_share_ones_closed_form(torch_cuda):
-    from cpp_projects_amd import workloads
-    from cpp_projects_amd.fdmt import FDMTGPU
-
-    torch = torch_cuda
-    c = workloads.LARGE
-    R, N = c["nchans"], c["nsamps"]
-    say(f"[cfg] large share: FDMT of ones {R}x{N}")
-    f = FDMTGPU(F_MIN, F_MAX, R, N, 1.0, c["dt_max"])
-    f.set_staged(3)
-    rows, cols = f.output_shape()
-    out = torch.empty((rows, cols), dtype=torch.float32, device="cuda")
-    f._execute(None, 0, out, out.numel(), ones=True)
-    ext = torch.from_numpy(orc.Plan(F_MIN, F_MAX, R, N, 1.0, c["dt_max"]).track_extent()).cuda()
-    col = torch.arange(cols, device="cuda")
-    bad = 0
-    for r0 in range(0, rows, 512):
-        blk = out[r0:r0 + 512]
-        valid = col[None, :] >= ext[r0:r0 + 512, None]
-        bad += int(((blk != float(R)) & valid).sum().item())
-        # before its extent a row holds fewer than R contributions
-        bad += int(((blk >= float(R)) & ~valid).sum().item())
-    del f, out
-    free_cuda(torch)
-    assert bad == 0
 
 
 @pytest.mark.timeout(1200)
