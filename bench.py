@@ -473,15 +473,20 @@ def main():
         unfused.append(sum(4.0 * N * (rows[i - 1] + rows[i]) for i in range(i0, i0 + L)))
         i0 += L
 
+    fused0 = staged in (2, 4) and sbytes[2] == 0.0  # the first pass built state 0 itself
+
     def kname(k):
         L, last = schedule[k], k == npass - 1
+        if k == 0 and fused0:
+            return f"k_merge_group_init<{L}, 1, true, *>"  # u8 input, normalised
         if staged and not (last and staged in (1, 2) and L <= 3):
             # a staged pass is two launches: <L, SNR, true> over the whole
             # tiles past every row's shifts, <L, SNR, false> over the rest
             return f"k_merge_group<{L}, {'true' if last else 'false'}, *>"
         return f"k_merge_fused<{L}, true, {'true' if last else 'false'}>"
 
-    stage_names = (["stats", "mask_transform", "init"] + [f"pass{k}(L={schedule[k]})" for k in range(npass - 1)] +
+    stage_names = (["stats", "mask_transform", "init"] +
+                   [f"pass{k}(L={schedule[k]})" + ("+init" if k == 0 and fused0 else "") for k in range(npass - 1)] +
                    [f"pass{npass - 1}(L={schedule[-1]})+snr"] + ["snr_norm_separate", "boxcar"])
     stage_kernels = (["k_row_stats<true> + k_stats_finalize", "k_init_consec<2, true> + k_merge_fused<L, true, false>",
                       "k_init_consec<1, true>"] + [kname(k) for k in range(npass)] +

@@ -72,11 +72,12 @@ struct sps_fdmt {
     // staged (LDS-window) passes
     bool staged = false;
     bool fuse_init = false;    // first staged pass builds state 0 itself (mean init)
+    bool ran_fused0 = false;   // the last execution did (stage accounting)
     bool staged_last = false;  // the final pass is staged too
     int group_floats = 8192;   // LDS window per tile buffer (floats; two buffers per block)
     int group_rows = 16;       // output rows per group at most
     int group_tiles = 8;       // consecutive 512-sample tiles per block
-    static constexpr int kInitGroupFloats = 16384;  // first pass with the init fused
+    static constexpr int kInitGroupFloats = 8192;  // first pass with the init fused
     struct StagedPass {
         DevBuf groups, leaves, rows, unit_leaf;
         int64_t ngroups = 0;
@@ -234,9 +235,8 @@ struct sps_fdmt {
                 shift[(size_t)r * NL + j] = sh[(size_t)(NL + j)];
             }
         }
-        auto units = [](int64_t smin, int64_t smax) -> int64_t {
-            return (kGroupTile + round_up(smax, 4) - smin + 3) / 4;
-        };
+        const int64_t tile = init_leaves ? kInitGroupTile : kGroupTile;
+        auto units = [tile](int64_t smin, int64_t smax) -> int64_t { return (tile + round_up(smax, 4) - smin + 3) / 4; };
         constexpr size_t kMaxLeaves = 64;  // leaf indices of the unit table are bytes
         std::vector<uint8_t> unit_leaf;
         std::vector<GroupDescD> groups;
@@ -281,8 +281,13 @@ struct sps_fdmt {
                                           std::max(it->second.second, kv.second.second)};
                     }
                     ok = C.size() <= kMaxLeaves;
-                    for (const auto& kv : C)
-                        floats += 4 * units(kv.second.first, kv.second.second) * chan_rows(kv.first);
+                    int64_t chunks = 0;
+                    for (const auto& kv : C) {
+                        const int64_t u = units(kv.second.first, kv.second.second);
+                        floats += 4 * (u * chan_rows(kv.first) + 1);  // row 0 has a history chunk in front
+                        chunks += u + 1;
+                    }
+                    ok = ok && chunks <= kInitGroupChunks;
                 } else {
                     for (const auto& kv : S2) floats += 4 * units(kv.second.first, kv.second.second);
                 }
@@ -315,6 +320,8 @@ struct sps_fdmt {
                 }
                 g.leaf0 = (int64_t)chans.size();
                 g.nleaves = (int32_t)C.size();
+                g.unit0 = (int64_t)unit_leaf.size();
+                int32_t xc = 0, ci = 0;
                 for (const auto& kv : C) {
                     const int64_t c = kv.first;
                     ChanWinD cw{};
@@ -324,11 +331,18 @@ struct sps_fdmt {
                     cw.s4 = (int32_t)round_up(kv.second.second, 4);
                     cw.units = (int32_t)units(kv.second.first, kv.second.second);
                     cw.row_off = off;
-                    for (int64_t k = 0; k < cw.nrows; ++k)
-                        base[eng.chan_row_off[(size_t)c] + k] = off + (int32_t)(4 * cw.units * k) + cw.s4;
-                    off += 4 * cw.units * cw.nrows;
+                    cw.xc = xc;
+                    // row 0 after its history chunk; rows k >= 1 after row 0
+                    base[eng.chan_row_off[(size_t)c]] = off + 4 + cw.s4;
+                    for (int64_t k = 1; k < cw.nrows; ++k)
+                        base[eng.chan_row_off[(size_t)c] + k] =
+                            off + 4 * (cw.units + 1) + (int32_t)(4 * cw.units * (k - 1)) + cw.s4;
+                    off += 4 * (cw.units * cw.nrows + 1);
+                    xc += cw.units + 1;
+                    unit_leaf.insert(unit_leaf.end(), (size_t)cw.units + 1, (uint8_t)ci++);
                     chans.push_back(cw);
                 }
+                g.nunits = xc;
             } else {
                 g.leaf0 = (int64_t)leaves.size();
                 g.nleaves = (int32_t)S.size();
@@ -454,7 +468,8 @@ struct sps_fdmt {
             i0 += L;
         }
         if (staged) {
-            if (fuse_init && eng.init_mode == kInitMean && np >= 2 && pass_staged(0)) {
+            if (fuse_init && eng.init_mode == kInitMean && grid_consec && eng.nsamps % 4 == 0 && np >= 2 &&
+                pass_staged(0)) {
                 // a first pass whose trees need more init rows than one LDS
                 // window holds keeps the separate init
                 std::unique_ptr<StagedPass> sp(new StagedPass());
@@ -507,6 +522,7 @@ struct sps_fdmt {
              const MergeEpilogue* snr = nullptr, bool aux = false) {
         SPS_REQUIRE(!snr || eng.niters > 0, "the S/N epilogue needs at least one merge");
         const ChanNorm* nm = norm;
+        if (!aux) ran_fused0 = false;
         if (eng.niters == 0) {
             launch_fdmt_init(eng.init_mode, in, in_u8, ones, in_pitch, nm, eng.nchans, nsamps, d_grid.as<int32_t>(),
                              d_grid_off.as<int64_t>(), d_chan_row_off.as<int64_t>(), out, out_pitch, st, dmax,
@@ -516,7 +532,11 @@ struct sps_fdmt {
         }
         ensure_trees(st);
         // the first staged pass builds state 0 itself when its tables exist
-        const bool fused0 = !aux && !ones && in != nullptr && staged && d_stage0_init != nullptr;
+        // (whole, aligned 4-sample input chunks: the kernel loads them as one word or one float4)
+        const uintptr_t ia = reinterpret_cast<uintptr_t>(in);
+        const bool fused0 = !aux && !ones && in != nullptr && staged && d_stage0_init != nullptr &&
+                            in_pitch % 4 == 0 && (in_u8 ? ia % 4 == 0 : ia % 16 == 0);
+        if (!aux) ran_fused0 = fused0;
         if (!fused0)
             launch_fdmt_init(eng.init_mode, in, in_u8, ones, in_pitch, nm, eng.nchans, nsamps, d_grid.as<int32_t>(),
                              d_grid_off.as<int64_t>(), d_chan_row_off.as<int64_t>(), sa, p, st, dmax,
@@ -559,8 +579,9 @@ struct sps_fdmt {
             float* dst = last ? out : ((k % 2 == 0) ? sb : sa);
             if (k == 0 && fused0) {
                 const StagedPass& sp = *d_stage0_init;
-                launch_fdmt_group_init(L, in, in_u8, in_pitch, nm, d_grid.as<int32_t>(), dst, p, sp.groups.p,
-                                       sp.ngroups, sp.leaves.p, sp.rows.p, sp.lds_floats, nsamps, st);
+                launch_fdmt_group_init(L, in, in_u8, in_pitch, nm, dst, p, sp.groups.p, sp.ngroups, sp.leaves.p,
+                                       sp.rows.p, sp.unit_leaf.as<uint8_t>(), sp.lds_floats,
+                                       group_tiles * (kGroupTile / kInitGroupTile), nsamps, sp.max_shift, st);
             } else if (pass_staged(k)) {
                 const StagedPass& sp = *d_staged[k];
                 launch_fdmt_group(L, src, p, dst, last ? out_pitch : p, sp.groups.p, sp.ngroups, sp.leaves.p,
@@ -1428,10 +1449,13 @@ SPS_API sps_status sps_search_get_stage_bytes(const sps_search* s, double* h_b, 
         // mask transform: its output image (ndt rows over its extent), written
         // once and read once by the final pass; its states stay in the caches
         b.push_back(s->p.snr_norm ? 2.0 * 4.0 * (double)s->norm_len * (double)s->ndt : 0.0);
-        b.push_back(in_bytes + N4 * (double)f.live_rows(0));             // init: input read + live state-0 rows
+        // init: input read + live state-0 rows; with the init fused into the
+        // first pass, state 0 never reaches HBM and that pass reads the input
+        b.push_back(f.ran_fused0 ? 0.0 : in_bytes + N4 * (double)f.live_rows(0));
         int i0 = 1;
         for (int L : f.schedule) {  // a pass reads the live rows of state i0-1 and writes those of state i0+L-1
-            b.push_back(N4 * (double)(f.live_rows(i0 - 1) + f.live_rows(i0 + L - 1)));
+            const double rd = (i0 == 1 && f.ran_fused0) ? in_bytes : N4 * (double)f.live_rows(i0 - 1);
+            b.push_back(rd + N4 * (double)f.live_rows(i0 + L - 1));
             i0 += L;
         }
         b.push_back(0.0);                                                // S/N normalisation (fused into the last pass)

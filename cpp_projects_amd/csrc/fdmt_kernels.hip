@@ -279,15 +279,15 @@ struct LdsTreeFast {
     }
 };
 
-// evaluates the trees of every row of group g for times [T, T + kGroupTile):
-// wave w takes rows w, w + 4, ... over the whole tile (8 samples per lane), so
+// evaluates the trees of every row of group g for times [T, T + TILE): wave w
+// takes rows w, w + 4, ... over the whole tile (TILE / 64 samples per lane), so
 // the per-row scalar work (descriptor, output address, path test) is done
 // once per row rather than once per wave
 // FAST: the caller guarantees that the tile is whole and past every row's
 // shifts (T >= maxshift) and, with SNR, past the mask transform's extent, so
 // only the branch-free path is compiled (the general one's registers made the
 // hot loop spill).
-template <int L, bool SNR, bool FAST = false>
+template <int L, bool SNR, bool FAST = false, int TILE = kGroupTile>
 __device__ __forceinline__ void group_trees(const float* __restrict__ lds, const GroupDescD& g,
                                             const GRowDesc<L>* __restrict__ rows, float* __restrict__ out,
                                             int64_t out_pitch, int64_t T, int64_t nsamps, int64_t out_rows_limit,
@@ -305,7 +305,7 @@ __device__ __forceinline__ void group_trees(const float* __restrict__ lds, const
         if (d.out_row >= out_rows_limit) continue;
         float* __restrict__ o = out + d.out_row * out_pitch;
         const float* nrow = SNR ? epi.nrm + d.out_row * epi.nrm_len : nullptr;
-        if (FAST || (T >= d.maxshift && T + kGroupTile <= nsamps)) {
+        if (FAST || (T >= d.maxshift && T + TILE <= nsamps)) {
             // the whole tile is past every shift: no per-sample tests.  The
             // outputs are stored non-temporally: a state is far larger than
             // the caches and is read back only by the next pass (measured:
@@ -315,14 +315,14 @@ __device__ __forceinline__ void group_trees(const float* __restrict__ lds, const
                 if (!FAST && SNR && T < epi.nrm_len) {
                     // within the mask transform's extent: per-sample divisor
                     // there, all of the row-tile's divisors loaded first
-                    float nv[kGroupTile / 64];
+                    float nv[TILE / 64];
 #pragma unroll
-                    for (int j = 0; j < kGroupTile / 64; ++j) {
+                    for (int j = 0; j < TILE / 64; ++j) {
                         const int64_t t = T + j * 64 + lane;
                         nv[j] = (t < epi.nrm_len) ? nrow[t] : 0.0f;
                     }
 #pragma unroll 4
-                    for (int j = 0; j < kGroupTile / 64; ++j) {
+                    for (int j = 0; j < TILE / 64; ++j) {
                         const int u = j * 64 + lane;
                         const int64_t t = T + u;
                         const float v = LdsTreeFast<L, 1, FULL>::eval(lds, d, u);
@@ -333,7 +333,7 @@ __device__ __forceinline__ void group_trees(const float* __restrict__ lds, const
                     // 1e-8) lies in [1e-4, 256], so only the dividends need the
                     // range test, and the IEEE division runs in one rare branch
 #pragma unroll
-                    for (int j0 = 0; j0 < kGroupTile / 64; j0 += 4) {
+                    for (int j0 = 0; j0 < TILE / 64; j0 += 4) {
                         float v[4], qv[4];
                         bool slow = false;
 #pragma unroll
@@ -358,7 +358,7 @@ __device__ __forceinline__ void group_trees(const float* __restrict__ lds, const
                     }
                 } else {
 #pragma unroll 4
-                    for (int j = 0; j < kGroupTile / 64; ++j) {
+                    for (int j = 0; j < TILE / 64; ++j) {
                         const int u = j * 64 + lane;
                         __builtin_nontemporal_store(LdsTreeFast<L, 1, FULL>::eval(lds, d, u), &o[T + u]);
                     }
@@ -371,16 +371,16 @@ __device__ __forceinline__ void group_trees(const float* __restrict__ lds, const
             continue;
         }
         if constexpr (!FAST) {
-            float nv[kGroupTile / 64];
+            float nv[TILE / 64];
             if (SNR) {
 #pragma unroll
-                for (int j = 0; j < kGroupTile / 64; ++j) {
+                for (int j = 0; j < TILE / 64; ++j) {
                     const int64_t t = T + j * 64 + lane;
                     nv[j] = (t < epi.nrm_len && t < nsamps) ? nrow[t] : 0.0f;
                 }
             }
 #pragma unroll 4
-            for (int j = 0; j < kGroupTile / 64; ++j) {
+            for (int j = 0; j < TILE / 64; ++j) {
                 const int64_t t = T + j * 64 + lane;
                 if (t < nsamps) {
                     float v = LdsTree<L, 1>::eval(lds, d, t, T);
@@ -809,68 +809,211 @@ __global__ void __launch_bounds__(kThreads) k_init_consec(const void* __restrict
     }
 }
 
-// First staged pass with the mean init fused in: every init row a group's
-// trees read is built in LDS straight from the waterfall, with the operations
-// of k_init in the same order (normalised samples (x - mu) / sigma, running
-// means), so state 0 never reaches HBM.
-template <int L, int IN_KIND, bool NORM>
+// First staged pass with the mean init fused in (plans whose per-channel init
+// dts are 0, 1, ..., K-1 with K <= kInitConsecMax; nsamps % 4 == 0): state 0
+// never reaches HBM.  A block takes one group over tiles_per_block
+// consecutive 512-sample tiles.  For each channel of the group the LDS holds
+// its K init rows over the group's window [T - s4, T - s4 + 4 units), row 0
+// with one more 4-sample chunk of history in front; row 0 is 0 + x^ (the
+// normalised sample) and row k >= 1 is (row_{k-1} * k + row0(t - k)) / (k + 1)
+// (0 for t < k) -- the operations of k_init_consec in the same order (a
+// division by 2 or 4 is the exact multiplication by 0.5 or 0.25).  Per tile
+// every thread takes chunks of 4 samples (ChanWinD::xc numbers a channel's
+// units + 1 chunks, history first): it turns a chunk's raw input, prefetched
+// into registers during the previous tile's trees, into the four row-0
+// values; after a barrier it builds the chunk's rows k >= 1 from row 0 in LDS
+// (this chunk and the one before it); after another the trees of the group's
+// rows are evaluated from LDS as in k_merge_group.
+constexpr int kInitSlots = kInitGroupChunks / kThreads;  // chunks per thread and tile
+constexpr int kInitMaxChans = 64;
+
+struct InitChan {
+    int64_t xb;       // input offset of the channel's row
+    float mu, sg, rsg;
+    int32_t keep;
+    int32_t t_off;    // time of the channel's chunk q at T = 0: 4 q + t_off (= 4 q - s4 - 4)
+    int32_t row_off;  // LDS float offset of row 0's history chunk
+    int32_t rk_off;   // LDS float offset of row 1's first chunk (q = 1)
+    int32_t rstride;  // 4 units: row stride of rows k >= 1
+    int32_t nrows;
+    int32_t xc;
+};
+
+template <int IN_KIND>
+__device__ __forceinline__ float4a unpack_chunk(uint32_t w8, const float4a& wf) {
+    if constexpr (IN_KIND == 1) {
+        float4a v;
+        v.x = (float)(w8 & 0xffu);
+        v.y = (float)((w8 >> 8) & 0xffu);
+        v.z = (float)((w8 >> 16) & 0xffu);
+        v.w = (float)(w8 >> 24);
+        return v;
+    } else {
+        return wf;
+    }
+}
+
+// (a - mu) / sigma correctly rounded for four samples, as div_rn computes it,
+// with one branch for the rare operands outside its fast range
+__device__ __forceinline__ float4a norm4(float4a v, float mu, float sg, float rsg) {
+    const float a[4] = {v.x - mu, v.y - mu, v.z - mu, v.w - mu};
+    float q[4];
+    bool slow = !(sg >= 0x1p-60f && sg <= 0x1p60f);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const float ax = fabsf(a[e]);
+        slow = slow || !(ax >= 0x1p-60f && ax <= 0x1p60f);
+        const float q0 = a[e] * rsg;
+        const float r = fmaf(-q0, sg, a[e]);
+        q[e] = fmaf(r, rsg, q0);
+    }
+    if (slow) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) q[e] = div_rn(a[e], sg, rsg);
+    }
+    return float4a{q[0], q[1], q[2], q[3]};
+}
+
+template <int L, int IN_KIND, bool NORM, bool FAST>
 __global__ void __launch_bounds__(kThreads) k_merge_group_init(const void* __restrict__ in, int64_t in_pitch,
-                                                               ChanNorm norm, const int32_t* __restrict__ grid,
-                                                               float* __restrict__ out, int64_t out_pitch,
+                                                               ChanNorm norm, float* __restrict__ out,
+                                                               int64_t out_pitch,
                                                                const GroupDescD* __restrict__ groups, int64_t ngroups,
                                                                const ChanWinD* __restrict__ chans,
-                                                               const GRowDesc<L>* __restrict__ rows, int64_t nsamps) {
+                                                               const GRowDesc<L>* __restrict__ rows,
+                                                               const uint8_t* __restrict__ unit_chan,
+                                                               int tiles_per_block, int64_t nsamps, int64_t tile_lo,
+                                                               int64_t tile_hi) {
+    static_assert(IN_KIND == 0 || IN_KIND == 1, "f32 or u8 input");
     extern __shared__ __attribute__((aligned(16))) float lds[];
+    __shared__ InitChan ch[kInitMaxChans];
     const uint32_t logical = xcd_remap(blockIdx.x, gridDim.x);
     const int64_t gi = (int64_t)(logical % (uint64_t)ngroups);
-    const int64_t T = (int64_t)(logical / (uint64_t)ngroups) * kGroupTile;
+    const int64_t chunk = (int64_t)(logical / (uint64_t)ngroups);
     const GroupDescD g = groups[gi];
-    for (int ci = 0; ci < g.nleaves; ++ci) {
-        const ChanWinD cw = chans[g.leaf0 + ci];
-        const int32_t* __restrict__ gr = grid + cw.grid0;
-        const int64_t xb = cw.chan * in_pitch;
-        float mu = 0.f, sg = 1.f, rsg = 1.f;
-        bool keep = true;
+    const int64_t k0 = tile_lo + chunk * tiles_per_block;
+    const int64_t k1 = min(tile_hi, k0 + tiles_per_block);
+    if (k0 >= k1) return;  // block-uniform
+    if ((int)threadIdx.x < g.nleaves) {
+        const ChanWinD cw = chans[g.leaf0 + threadIdx.x];
+        InitChan c;
+        c.xb = cw.chan * in_pitch;
+        c.mu = 0.0f;
+        c.sg = 1.0f;
+        c.rsg = 1.0f;
+        c.keep = 1;
         if (NORM) {
-            keep = norm.keep[cw.chan] != 0;
-            mu = norm.mu[cw.chan];
-            sg = norm.sigma[cw.chan];
-            rsg = 1.0f / sg;
+            c.keep = norm.keep[cw.chan] != 0;
+            c.mu = norm.mu[cw.chan];
+            c.sg = norm.sigma[cw.chan];
+            c.rsg = 1.0f / c.sg;
         }
-        auto X = [&](int64_t i) -> float {
-            float v = load_in<IN_KIND>(in, xb + i);
-            if (NORM) v = keep ? div_rn(v - mu, sg, rsg) : 0.0f;
-            return v;
-        };
-        const int n = 4 * cw.units;
-        float* __restrict__ rb = lds + cw.row_off;
-        const int64_t d0 = gr[0];
-        const float y0 = (float)(d0 + 1), r0 = 1.0f / y0;
-        for (int u = threadIdx.x; u < n; u += kThreads) {
-            const int64_t t = T - cw.s4 + u;
-            float prev = 0.0f;
-            if (t >= d0 && t < nsamps) {
-                float sum = 0.0f;
-                for (int64_t i = t - d0; i <= t; ++i) sum += X(i);
-                prev = div_rn(sum, y0, r0);
-            }
-            rb[u] = prev;
-            for (int k = 1; k < cw.nrows; ++k) {
-                const int64_t dc = gr[k], dp = gr[k - 1];
-                float v = 0.0f;
-                if (t >= dc && t < nsamps) {
-                    float sum = 0.0f;
-                    for (int64_t i = t - dc; i < t - dp; ++i) sum += X(i);
-                    const float yk = (float)dc + 1.0f;
-                    v = div_rn(prev * ((float)dp + 1.0f) + sum, yk, 1.0f / yk);
-                }
-                rb[k * n + u] = v;
-                prev = v;
-            }
-        }
+        c.t_off = -cw.s4 - 4;
+        c.row_off = cw.row_off;
+        c.rk_off = cw.row_off + 4 * (cw.units + 1);
+        c.rstride = 4 * cw.units;
+        c.nrows = cw.nrows;
+        c.xc = cw.xc;
+        ch[threadIdx.x] = c;
+    }
+    // per slot: the chunk's channel (-1: unused) and number within it
+    int32_t sc[kInitSlots], sq[kInitSlots];
+#pragma unroll
+    for (int j = 0; j < kInitSlots; ++j) {
+        const int f = (int)threadIdx.x + j * kThreads;
+        sc[j] = -1;
+        sq[j] = 0;
+        if (f < g.nunits) sc[j] = unit_chan[g.unit0 + f];
     }
     __syncthreads();
-    group_trees<L, false>(lds, g, rows, out, out_pitch, T, nsamps, INT64_MAX, SnrEpi{nullptr, 0, nullptr});
+#pragma unroll
+    for (int j = 0; j < kInitSlots; ++j)
+        if (sc[j] >= 0) sq[j] = (int)threadIdx.x + j * kThreads - ch[sc[j]].xc;
+    uint32_t raw8[kInitSlots];
+    float4a rawf[kInitSlots];
+#pragma unroll
+    for (int j = 0; j < kInitSlots; ++j) {
+        raw8[j] = 0u;
+        rawf[j] = float4a{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+    auto issue = [&](int64_t T) {
+#pragma unroll
+        for (int j = 0; j < kInitSlots; ++j) {
+            if (sc[j] < 0) continue;
+            const InitChan& c = ch[sc[j]];
+            const int64_t t0 = T + 4 * sq[j] + c.t_off;
+            if (t0 >= 0 && t0 + 4 <= nsamps) {
+                if constexpr (IN_KIND == 1)
+                    raw8[j] = *reinterpret_cast<const uint32_t*>(static_cast<const uint8_t*>(in) + c.xb + t0);
+                else
+                    rawf[j] = *reinterpret_cast<const float4a*>(static_cast<const float*>(in) + c.xb + t0);
+            } else {
+                raw8[j] = 0u;
+                rawf[j] = float4a{0.0f, 0.0f, 0.0f, 0.0f};
+            }
+        }
+    };
+    // row 0 (0 + x^; 0 outside the row) of every chunk
+    auto build0 = [&](int64_t T) {
+#pragma unroll
+        for (int j = 0; j < kInitSlots; ++j) {
+            if (sc[j] < 0) continue;
+            const InitChan& c = ch[sc[j]];
+            const int64_t t0 = T + 4 * sq[j] + c.t_off;
+            float4a v = unpack_chunk<IN_KIND>(raw8[j], rawf[j]);
+            if (NORM) {
+                v = norm4(v, c.mu, c.sg, c.rsg);
+                if (!c.keep) v = float4a{0.0f, 0.0f, 0.0f, 0.0f};
+            }
+            const bool inside = t0 >= 0 && t0 + 4 <= nsamps;
+            float4a r0;
+            r0.x = inside ? 0.0f + v.x : 0.0f;
+            r0.y = inside ? 0.0f + v.y : 0.0f;
+            r0.z = inside ? 0.0f + v.z : 0.0f;
+            r0.w = inside ? 0.0f + v.w : 0.0f;
+            *reinterpret_cast<float4a*>(lds + c.row_off + 4 * sq[j]) = r0;
+        }
+    };
+    // rows k >= 1 of every chunk but the history ones
+    auto buildk = [&](int64_t T) {
+#pragma unroll
+        for (int j = 0; j < kInitSlots; ++j) {
+            if (sc[j] < 0 || sq[j] == 0) continue;
+            const InitChan& c = ch[sc[j]];
+            if (c.nrows < 2) continue;
+            const int64_t t0 = T + 4 * sq[j] + c.t_off;
+            const float4a h = *reinterpret_cast<const float4a*>(lds + c.row_off + 4 * (sq[j] - 1));
+            const float4a o = *reinterpret_cast<const float4a*>(lds + c.row_off + 4 * sq[j]);
+            const float w[8] = {h.x, h.y, h.z, h.w, o.x, o.y, o.z, o.w};  // w[4 + e] = row 0 at t0 + e
+            float prev[4] = {o.x, o.y, o.z, o.w};
+            float* dst = lds + c.rk_off + 4 * (sq[j] - 1);
+#pragma unroll
+            for (int kk = 1; kk < kInitConsecMax; ++kk) {
+                if (kk >= c.nrows) break;
+                const float yk = (float)kk + 1.0f, rk = 1.0f / yk;
+                const float mk = (float)(kk - 1) + 1.0f;
+                const bool pow2 = kk == 1 || kk == 3;  // x / 2 and x / 4 are exact multiplications
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const float num = prev[e] * mk + w[4 + e - kk];
+                    const float val = pow2 ? num * rk : div_rn(num, yk, rk);
+                    prev[e] = t0 + e >= kk ? val : 0.0f;
+                }
+                *reinterpret_cast<float4a*>(dst + (kk - 1) * c.rstride) = float4a{prev[0], prev[1], prev[2], prev[3]};
+            }
+        }
+    };
+    issue(k0 * kInitGroupTile);
+    for (int64_t k = k0; k < k1; ++k) {
+        const int64_t T = k * kInitGroupTile;
+        build0(T);
+        __syncthreads();
+        if (k + 1 < k1) issue(T + kInitGroupTile);
+        buildk(T);
+        __syncthreads();
+        group_trees<L, false, FAST, kInitGroupTile>(lds, g, rows, out, out_pitch, T, nsamps, INT64_MAX, SnrEpi{nullptr, 0, nullptr});
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1107,60 +1250,97 @@ void launch_fdmt_group(int L, const float* in, int64_t in_pitch, float* out, int
     SPS_HIP(hipGetLastError());
 }
 
-template <int L, int IN_KIND>
-static void group_init_dispatch(bool nrm, uint32_t nb, size_t lds, const void* in, int64_t in_pitch, ChanNorm n,
-                                const int32_t* grid, float* out, int64_t out_pitch, const void* g, int64_t ng,
-                                const void* cw, const void* rw, int64_t nsamps, hipStream_t st) {
-    const GroupDescD* gp = static_cast<const GroupDescD*>(g);
-    const ChanWinD* cp = static_cast<const ChanWinD*>(cw);
-    const GRowDesc<L>* rp = static_cast<const GRowDesc<L>*>(rw);
-    if (nrm)
-        hipLaunchKernelGGL((k_merge_group_init<L, IN_KIND, true>), dim3(nb), dim3(kThreads), lds, st, in, in_pitch,
-                           n, grid, out, out_pitch, gp, ng, cp, rp, nsamps);
-    else
-        hipLaunchKernelGGL((k_merge_group_init<L, IN_KIND, false>), dim3(nb), dim3(kThreads), lds, st, in, in_pitch,
-                           n, grid, out, out_pitch, gp, ng, cp, rp, nsamps);
+struct GroupInitLaunch {
+    size_t lds;
+    const void* in;
+    int64_t in_pitch;
+    ChanNorm norm;
+    float* out;
+    int64_t out_pitch;
+    const void* groups;
+    int64_t ngroups;
+    const void* chans;
+    const void* rows;
+    const uint8_t* unit_chan;
+    int tiles_per_block;
+    int64_t nsamps;
+};
+
+template <int L, int IN_KIND, bool NORM, bool FAST>
+static void group_init_launch(const GroupInitLaunch& a, int64_t lo, int64_t hi, int tpb, hipStream_t st) {
+    if (hi <= lo) return;
+    const int64_t nb = a.ngroups * ceil_div(hi - lo, (int64_t)tpb);
+    if (nb > 0x7fffffffLL) throw Error(SPS_ERR_INVALID_ARGUMENT, "merge grid too large");
+    hipLaunchKernelGGL((k_merge_group_init<L, IN_KIND, NORM, FAST>), dim3((uint32_t)nb), dim3(kThreads), a.lds, st,
+                       a.in, a.in_pitch, a.norm, a.out, a.out_pitch, static_cast<const GroupDescD*>(a.groups),
+                       a.ngroups, static_cast<const ChanWinD*>(a.chans), static_cast<const GRowDesc<L>*>(a.rows),
+                       a.unit_chan, tpb, a.nsamps, lo, hi);
+}
+
+// the tile split of group_dispatch
+template <int L, int IN_KIND, bool NORM>
+static void group_init_dispatch(const GroupInitLaunch& a, int64_t fast_lo, hipStream_t st) {
+    const int64_t ntiles = ceil_div(a.nsamps, kInitGroupTile);
+    const int64_t full = a.nsamps / kInitGroupTile;
+    if (fast_lo < full) {
+        group_init_launch<L, IN_KIND, NORM, false>(a, 0, fast_lo, 1, st);
+        group_init_launch<L, IN_KIND, NORM, true>(a, fast_lo, full, a.tiles_per_block, st);
+        group_init_launch<L, IN_KIND, NORM, false>(a, full, ntiles, 1, st);
+    } else {
+        group_init_launch<L, IN_KIND, NORM, false>(a, 0, ntiles, a.tiles_per_block, st);
+    }
 }
 
 template <int L>
-static void group_init_kind(bool u8, bool nrm, uint32_t nb, size_t lds, const void* in, int64_t in_pitch,
-                            ChanNorm n, const int32_t* grid, float* out, int64_t out_pitch, const void* g, int64_t ng,
-                            const void* cw, const void* rw, int64_t nsamps, hipStream_t st) {
+static void group_init_kind(bool u8, bool nrm, const GroupInitLaunch& a, int64_t fast_lo, hipStream_t st) {
     if (u8)
-        group_init_dispatch<L, 1>(nrm, nb, lds, in, in_pitch, n, grid, out, out_pitch, g, ng, cw, rw, nsamps, st);
+        nrm ? group_init_dispatch<L, 1, true>(a, fast_lo, st) : group_init_dispatch<L, 1, false>(a, fast_lo, st);
     else
-        group_init_dispatch<L, 0>(nrm, nb, lds, in, in_pitch, n, grid, out, out_pitch, g, ng, cw, rw, nsamps, st);
+        nrm ? group_init_dispatch<L, 0, true>(a, fast_lo, st) : group_init_dispatch<L, 0, false>(a, fast_lo, st);
 }
 
-void launch_fdmt_group_init(int L, const void* in, int in_u8, int64_t in_pitch, const ChanNorm* norm,
-                            const int32_t* d_grid, float* out, int64_t out_pitch, const void* d_groups,
-                            int64_t ngroups, const void* d_chans, const void* d_rows, int lds_floats_max,
-                            int64_t nsamps, hipStream_t st) {
+void launch_fdmt_group_init(int L, const void* in, int in_u8, int64_t in_pitch, const ChanNorm* norm, float* out,
+                            int64_t out_pitch, const void* d_groups, int64_t ngroups, const void* d_chans,
+                            const void* d_rows, const uint8_t* d_unit_chan, int lds_floats_max, int tiles_per_block,
+                            int64_t nsamps, int64_t max_shift, hipStream_t st) {
     if (ngroups <= 0 || nsamps <= 0) return;
-    if (lds_floats_max <= 0 || lds_floats_max > 40 * 1024)
+    if (nsamps % 4 != 0 || in_pitch % 4 != 0)
+        throw Error(SPS_ERR_INVALID_ARGUMENT, "the fused init reads whole 4-sample chunks");
+    const uintptr_t ia = reinterpret_cast<uintptr_t>(in);
+    if (in_u8 ? ia % 4 != 0 : ia % 16 != 0)
+        throw Error(SPS_ERR_INVALID_ARGUMENT, "the fused init needs an aligned waterfall");
+    if (lds_floats_max <= 0 || lds_floats_max > 16384 + 64)
         throw Error(SPS_ERR_INVALID_ARGUMENT, "staged pass LDS window out of range");
-    const int64_t nb = ngroups * ceil_div(nsamps, kGroupTile);
-    if (nb > 0x7fffffffLL) throw Error(SPS_ERR_INVALID_ARGUMENT, "merge grid too large");
-    ChanNorm n{};
-    if (norm) n = *norm;
-    const size_t lds = sizeof(float) * (size_t)lds_floats_max;
+    if (tiles_per_block < 1) throw Error(SPS_ERR_INVALID_ARGUMENT, "tiles per block must be positive");
+    if (max_shift < 0) throw Error(SPS_ERR_INVALID_ARGUMENT, "negative row shift");
+    GroupInitLaunch a{};
+    a.lds = sizeof(float) * (size_t)lds_floats_max;
+    a.in = in;
+    a.in_pitch = in_pitch;
+    if (norm) a.norm = *norm;
+    a.out = out;
+    a.out_pitch = out_pitch;
+    a.groups = d_groups;
+    a.ngroups = ngroups;
+    a.chans = d_chans;
+    a.rows = d_rows;
+    a.unit_chan = d_unit_chan;
+    a.tiles_per_block = tiles_per_block;
+    a.nsamps = nsamps;
+    const int64_t fast_lo = ceil_div(max_shift, (int64_t)kInitGroupTile);
     const bool u8 = in_u8 != 0, nrm = norm != nullptr;
     switch (L) {
         case 1:
-            group_init_kind<1>(u8, nrm, (uint32_t)nb, lds, in, in_pitch, n, d_grid, out, out_pitch, d_groups, ngroups,
-                               d_chans, d_rows, nsamps, st);
+            group_init_kind<1>(u8, nrm, a, fast_lo, st);
             break;
         case 2:
-            group_init_kind<2>(u8, nrm, (uint32_t)nb, lds, in, in_pitch, n, d_grid, out, out_pitch, d_groups, ngroups,
-                               d_chans, d_rows, nsamps, st);
+            group_init_kind<2>(u8, nrm, a, fast_lo, st);
             break;
         case 3:
-            group_init_kind<3>(u8, nrm, (uint32_t)nb, lds, in, in_pitch, n, d_grid, out, out_pitch, d_groups, ngroups,
-                               d_chans, d_rows, nsamps, st);
+            group_init_kind<3>(u8, nrm, a, fast_lo, st);
             break;
         case 4:
-            group_init_kind<4>(u8, nrm, (uint32_t)nb, lds, in, in_pitch, n, d_grid, out, out_pitch, d_groups, ngroups,
-                               d_chans, d_rows, nsamps, st);
+            group_init_kind<4>(u8, nrm, a, fast_lo, st);
             break;
         default:
             throw Error(SPS_ERR_INVALID_ARGUMENT, "staged fusion depth must be 1..4");

@@ -89,18 +89,30 @@ struct GRowDesc {
     int32_t off[1 << L];
 };
 
-// First staged pass that computes state 0 itself (mean init only): for every
-// channel whose init rows are leaves of the group, rows 0..nrows-1 are built
-// from the input over [T - s4, T - s4 + 4*units) at LDS row_off + 4*units*k.
-// GroupDescD.leaf0 / nleaves then index ChanWinD entries.
+// First staged pass that computes state 0 itself (mean init, init dts 0..K-1
+// per channel): for every channel whose init rows are leaves of the group,
+// the K rows over [T - s4, T - s4 + 4*units) are built in LDS from the
+// waterfall; row 0 (with one 4-sample history chunk in front) starts at
+// row_off, row k >= 1 at row_off + 4*(units + 1) + 4*units*(k - 1).  The
+// channel's units + 1 chunks are numbered from xc in the group's chunk table.
+// GroupDescD.leaf0 / nleaves then index ChanWinD entries, unit0 / nunits the
+// chunk -> channel table.
 struct ChanWinD {
     int64_t chan;
     int64_t grid0;    // first init dt of the channel in the grid table
     int32_t nrows;    // init rows of the channel
     int32_t s4;       // largest leaf shift, rounded up to 4
     int32_t units;    // row window length / 4
-    int32_t row_off;  // LDS float offset of row 0
+    int32_t row_off;  // LDS float offset of row 0's history chunk
+    int32_t xc;       // first chunk of the channel in the group's chunk numbering
+    int32_t pad;
 };
+// The first staged pass with the init fused works on tiles of kInitGroupTile
+// samples (its LDS holds every init row of the group's channels, so a shorter
+// tile keeps more blocks resident); a group's channel windows hold at most
+// kInitGroupChunks 4-sample chunks (3 per thread).
+constexpr int kInitGroupTile = 256;
+constexpr int kInitGroupChunks = 3 * 256;
 
 // Optional epilogue of the last merge: divide by sqrtf(n + 1e-8) with n from
 // nrm (rows x nrm_len) for t < nrm_len and stats[4] (kept channels) beyond.
@@ -134,11 +146,12 @@ void launch_fdmt_group(int L, const float* in, int64_t in_pitch, float* out, int
                        int64_t max_shift, hipStream_t st, const MergeEpilogue* snr = nullptr);
 
 // First staged pass with the (mean) init fused in: reads the waterfall (f32 or
-// u8, optional per-channel normalisation) instead of state 0.
-void launch_fdmt_group_init(int L, const void* in, int in_u8, int64_t in_pitch, const ChanNorm* norm,
-                            const int32_t* d_grid, float* out, int64_t out_pitch, const void* d_groups,
-                            int64_t ngroups, const void* d_chans, const void* d_rows, int lds_floats_max,
-                            int64_t nsamps, hipStream_t st);
+// u8, optional per-channel normalisation) instead of state 0; nsamps and
+// in_pitch multiples of 4.  max_shift: as for launch_fdmt_group.
+void launch_fdmt_group_init(int L, const void* in, int in_u8, int64_t in_pitch, const ChanNorm* norm, float* out,
+                            int64_t out_pitch, const void* d_groups, int64_t ngroups, const void* d_chans,
+                            const void* d_rows, const uint8_t* d_unit_chan, int lds_floats_max, int tiles_per_block,
+                            int64_t nsamps, int64_t max_shift, hipStream_t st);
 
 // Adjoint of one merge: g_prev[p][t] = sum over (o, s) in csr[p] of g_out[o][t+s] (t+s < nsamps).
 void launch_fdmt_merge_adjoint(const float* g_out, int64_t out_pitch, float* g_prev, int64_t prev_pitch,

@@ -29,13 +29,14 @@ def candidate_schedules(n, max_depth=3):
     return sorted(s for s in c if sum(s) == n and all(1 <= d <= max_depth for d in s) and s.count(1) <= 2)
 
 
-def tune_search(search, run_once, reps=2, log=None, modes=(0, 1, 3), shapes=DEFAULT_SHAPES):
+def tune_search(search, run_once, reps=2, log=None, modes=(0, 1, 3, 4), shapes=DEFAULT_SHAPES):
     """Times every (staging mode, group shape, schedule) candidate on
     `search` (a SingleBeamSearch) with run_once() issuing one search step;
     keeps the fastest on the handle and returns
     ({"staged": mode, "shape": shape or None, "schedule": list}, results).
-    Modes: 0 tree passes, 1 staged passes with a tree final pass, 2 as 1
-    with the init fused into the first pass, 3 all passes staged."""
+    Modes: 0 tree passes, 1 staged passes with a tree final pass, 3 all
+    passes staged; 2 and 4 are 1 and 3 with the mean init fused into the
+    first pass."""
     n = sum(search.get_schedule())
     results = []
     search.enable_timing(True)

@@ -313,6 +313,85 @@ def test_search_fused_init_vs_oracle_chain(torch_cuda):
     np.testing.assert_array_equal(cands_to_numpy(c, k).view(np.int32), ref.view(np.int32))
 
 
+@pytest.mark.parametrize("nch,ns,dtm", [(512, 8192, 511), (1000, 6000, 999), (64, 4096, 63)])
+def test_fused_init_streaming_f32(torch_cuda, nch, ns, dtm):
+    """The streaming fused init (modes 2 and 4, init dts 0..K-1 with K = 2 or 3,
+    nsamps % 4 == 0) over several tile counts per block and first-pass depths,
+    bit-exact against the oracle; a waterfall that is not 16-byte aligned
+    takes the separate init."""
+    from cpp_projects_amd.fdmt import FDMTGPU
+
+    torch = torch_cuda
+    rng = np.random.default_rng(nch + ns)
+    x = rng.standard_normal((nch, ns)).astype(np.float32)
+    x[:, 100:103] += 4.0
+    ref = orc.Plan(1200.0, 1600.0, nch, ns, 1.0, dtm).execute(x)
+    f = FDMTGPU(1200.0, 1600.0, nch, ns, 1.0, dtm)
+    n = f.get_niters()
+    for mode in (2, 4):
+        f.set_staged(mode)
+        for tiles in (1, 5, 32):
+            f.set_group_shape(6144, 16, tiles)
+            for first in (1, 2, 3):
+                rest = n - first
+                sched = [first] + [3] * (rest // 3) + ([rest % 3] if rest % 3 else [])
+                f.set_schedule(sched)
+                got = run(torch, f, x)
+                np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32),
+                                              err_msg=f"mode {mode} tiles {tiles} schedule {sched}")
+    # a view one float into a buffer: 4-byte aligned only
+    buf = torch.zeros(nch * ns + 4, dtype=torch.float32, device="cuda")
+    buf[1:1 + nch * ns] = torch.from_numpy(x.reshape(-1)).cuda()
+    view = buf[1:1 + nch * ns]
+    out = torch.empty((dtm + 1, ns), dtype=torch.float32, device="cuda")
+    f.execute(view, view.numel(), out, out.numel())
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(out.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("in_dtype", ["u8", "f32"])
+def test_search_streaming_fused_init_vs_oracle(torch_cuda, in_dtype):
+    """The search with the init fused into its first staged pass (mode 4):
+    normalisation applied while the waterfall is read, bit-exact S/N image and
+    candidates; the init stage then takes no time of its own."""
+    from cpp_projects_amd.search import SingleBeamSearch, cands_to_numpy
+
+    torch = torch_cuda
+    rng = np.random.default_rng(31)
+    nch, ns, dtm = 1024, 16384, 1023
+    if in_dtype == "u8":
+        x = np.clip(np.round(rng.normal(128, 16, (nch, ns))), 0, 255).astype(np.uint8)
+        x[77] = np.clip(np.round(rng.normal(128, 90, ns)), 0, 255).astype(np.uint8)  # a row the cleaning drops
+        x[:, 5000:5003] = 200
+    else:
+        x = rng.standard_normal((nch, ns)).astype(np.float32)
+        x[77] *= 40.0
+        x[:, 5000:5003] += 3.0
+    s = SingleBeamSearch(1200.0, 1600.0, nch, ns, dtm, in_dtype=in_dtype, box_q=5, threshold=5.0,
+                         cand_capacity=1 << 18)
+    s.set_staged(4)
+    s.set_group_shape(6144, 16, 8)
+    s.set_schedule([3, 3, 3, 1])
+    d_in = torch.from_numpy(x).cuda()
+    dmt = torch.empty((s.ndt, ns), dtype=torch.float32, device="cuda")
+    c = torch.empty((1 << 18, 4), dtype=torch.int32, device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    s.enable_timing(True)
+    s.run(d_in, dmt, c, cnt)
+    st = s.stage_times_ms()
+    s.enable_timing(False)
+    assert st[2] < 0.02, f"init stage {st[2]} ms: the fused first pass did not run"
+    ct, k = s.candidates(dmt, c, cnt)
+    xn, mask, (_, _, keep, _) = orc.normalise(x, 1)
+    assert keep[77] == 0
+    p = orc.Plan(1200.0, 1600.0, nch, ns, 1.0, dtm)
+    snr = orc.fdmt_normalization(p.execute(xn), p.execute(mask))
+    np.testing.assert_array_equal(dmt.cpu().numpy().view(np.uint32), snr.view(np.uint32))
+    ref = orc.boxcar_pow2(snr, 5.0, 5)
+    assert k == len(ref) and k > 0
+    np.testing.assert_array_equal(cands_to_numpy(ct, k).view(np.int32), ref.view(np.int32))
+
+
 def test_staged_group_shapes(torch_cuda, golden):
     """Group windows from 2048 to 9216 floats, 1 to 64 rows per group and 1 to
     64 tiles per block give the same bits (plain staged and fused init)."""
