@@ -839,6 +839,22 @@ struct InitChan {
     int32_t xc;
 };
 
+// One whole 16-byte LDS read (p 16-byte aligned).  The empty asm keeps every
+// component live, so the read is not narrowed to the components used: a
+// narrowed read of each lane's 16 bytes is a strided pattern that conflicts
+// in the LDS banks.
+__device__ __forceinline__ float4a lds_load4(const float* p) {
+    float4a v = *static_cast<const float4a*>(__builtin_assume_aligned(p, 16));
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+// One whole 16-byte LDS write (p 16-byte aligned; said so, or the compiler
+// splits it into 4-byte pairs with the same strided pattern).
+__device__ __forceinline__ void lds_store4(float* p, float4a v) {
+    *static_cast<float4a*>(__builtin_assume_aligned(p, 16)) = v;
+}
+
 template <int IN_KIND>
 __device__ __forceinline__ float4a unpack_chunk(uint32_t w8, const float4a& wf) {
     if constexpr (IN_KIND == 1) {
@@ -971,7 +987,7 @@ __global__ void __launch_bounds__(kThreads) k_merge_group_init(const void* __res
             r0.y = inside ? 0.0f + v.y : 0.0f;
             r0.z = inside ? 0.0f + v.z : 0.0f;
             r0.w = inside ? 0.0f + v.w : 0.0f;
-            *reinterpret_cast<float4a*>(lds + c.row_off + 4 * sq[j]) = r0;
+            lds_store4(lds + c.row_off + 4 * sq[j], r0);
         }
     };
     // rows k >= 1 of every chunk but the history ones
@@ -982,8 +998,8 @@ __global__ void __launch_bounds__(kThreads) k_merge_group_init(const void* __res
             const InitChan& c = ch[sc[j]];
             if (c.nrows < 2) continue;
             const int64_t t0 = T + 4 * sq[j] + c.t_off;
-            const float4a h = *reinterpret_cast<const float4a*>(lds + c.row_off + 4 * (sq[j] - 1));
-            const float4a o = *reinterpret_cast<const float4a*>(lds + c.row_off + 4 * sq[j]);
+            const float4a h = lds_load4(lds + c.row_off + 4 * (sq[j] - 1));
+            const float4a o = lds_load4(lds + c.row_off + 4 * sq[j]);
             const float w[8] = {h.x, h.y, h.z, h.w, o.x, o.y, o.z, o.w};  // w[4 + e] = row 0 at t0 + e
             float prev[4] = {o.x, o.y, o.z, o.w};
             float* dst = lds + c.rk_off + 4 * (sq[j] - 1);
@@ -999,7 +1015,7 @@ __global__ void __launch_bounds__(kThreads) k_merge_group_init(const void* __res
                     const float val = pow2 ? num * rk : div_rn(num, yk, rk);
                     prev[e] = t0 + e >= kk ? val : 0.0f;
                 }
-                *reinterpret_cast<float4a*>(dst + (kk - 1) * c.rstride) = float4a{prev[0], prev[1], prev[2], prev[3]};
+                lds_store4(dst + (kk - 1) * c.rstride, float4a{prev[0], prev[1], prev[2], prev[3]});
             }
         }
     };

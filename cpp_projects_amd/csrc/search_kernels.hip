@@ -566,9 +566,37 @@ __device__ __forceinline__ uint32_t chunk_flag(float a, float b, float c, float 
     return !(fmaxf(fmaxf(a, b), fmaxf(c, d)) <= l) ? 1u : 0u;
 }
 
-// The flag phase of an interior window (as pow2_flag_chunks: span = 2048 +
-// W - 1, finite samples, W * M < 2^125) with thread t owning segment t =
-// cells 8t .. 8t+7, i.e. chunks 2t (cf[0]) and 2t + 1 (cf[1]).
+// Filter limit of width 2^k from the block's coefficients lc[k] = {A_k, E_k}
+// (A_k <= thr*sqrtf(w)*(1 - 2^-22) and E_k >= e_k, both rounded on the safe
+// side; see level_limit): t = RN(A_k - E_k*M) is within half an ulp of the
+// exact value, so t - |t|*2^-22 - 2^-100 (each step rounded) lies below
+// A_k - E_k*M <= the limit of level_limit.  A lower limit only flags more
+// cells; the recompute decides each exactly.
+__device__ __forceinline__ float seg_limit(float2 lc, float M) {
+    const float t = fmaf(-lc.y, M, lc.x);
+    return fmaf(-fabsf(t), 0x1p-22f, t) - 0x1p-100f;
+}
+
+// The block's limit coefficients for widths 2^0 .. 2^q (thread k computes
+// level k's; the caller makes them visible with a barrier).
+__device__ __forceinline__ float2 seg_coef(float thr, int k, double e_k) {
+    if (!(thr > 0.0f)) return make_float2(-INFINITY, 0.0f);
+    const double a = (double)thr * (double)sqrtf((float)(1 << k)) * (1.0 - 0x1p-22);
+    return make_float2(__double2float_rd(a), __double2float_ru(e_k));
+}
+
+// |v| as an ordered integer: finite magnitudes < 0x7f800000 (inf) < NaNs
+__device__ __forceinline__ uint32_t abs_bits(float v) { return __float_as_uint(v) & 0x7fffffffu; }
+
+// The flag phase of an interior window (span = 2048 + W - 1, every level
+// valid at every own cell) with thread t owning segment t = cells 8t ..
+// 8t+7, i.e. chunks 2t (cf[0]) and 2t + 1 (cf[1]).  It also takes the
+// window's max |x| M from the cells it loads (the cells of the DMA'd buffer
+// [0, 2048 + W) for W >= 16, of [0, 2064) below; cells past the window only
+// raise M) with one barrier, then the limits from M.  Returns false, flags
+// unset, when the window needs the careful per-cell path: NaN or infinite
+// samples, or W * M >= 2^125 (a level sum could overflow); M is returned
+// either way.
 //  * Widths 1 .. 8: the per-cell pairwise level sums of pow2_flag_rounds
 //    (P_2w(i) = P_w(i) + P_w(i + w)), from the thread's own cells and the
 //    next 7; a chunk is flagged when its largest sum exceeds the limit.
@@ -583,9 +611,11 @@ __device__ __forceinline__ uint32_t chunk_flag(float a, float b, float c, float 
 //    prefixes of <= 7, two more operations).  D doubles over segment sums
 //    (D_2n(g) = D_n(g) + D_n(g + n)), one barrier per width.
 // Unlike the level rounds it keeps the samples; LDS traffic per cell is a
-// few floats instead of about two per level.  aux: 5 (256 + W/8) floats.
+// few floats instead of about two per level.  aux: 5 (256 + W/8) floats;
+// wbits: one word per wave.
 template <int Q>
-__device__ __forceinline__ void pow2_flag_segments(const float* x, float* aux, const float* lim, uint32_t (&cf)[2]) {
+__device__ __forceinline__ bool pow2_flag_segments(const float* x, float* aux, const float2* lc, uint32_t* wbits,
+                                                   uint32_t (&cf)[2], float& M) {
     static_assert(Q <= kPowPfMaxQ, "segment filter window");  // the halo segments fit one wave
     const unsigned tid = opaque_tid();
     const float4* x4 = reinterpret_cast<const float4*>(x);
@@ -600,36 +630,47 @@ __device__ __forceinline__ void pow2_flag_segments(const float* x, float* aux, c
             c[15] = v3.w;
         }
     }
-    auto flag8 = [&](const float* v, float l) {
-        cf[0] = cf[0] + cf[0] + chunk_flag(v[0], v[1], v[2], v[3], l);
-        cf[1] = cf[1] + cf[1] + chunk_flag(v[4], v[5], v[6], v[7], l);
+    // M over exactly the window's cells: the own segments (cells 0 .. 2047),
+    // then cells 2048 .. 2046 + W from the last thread's next 8 (W <= 8) or
+    // from the halo segments
+    uint32_t mb = 0u;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) mb = max(mb, abs_bits(c[i]));
+    if constexpr (Q >= 1 && Q < 4) {
+        if (tid == kThreads - 1) {
+#pragma unroll
+            for (int i = 0; i < (1 << Q) - 1; ++i) mb = max(mb, abs_bits(c[8 + i]));
+        }
+    }
+    // the chunk maxima of each level, compared once the limits are known
+    float lv[4][2];
+    auto max8 = [&](const float* v, int k) {
+        lv[k][0] = fmaxf(fmaxf(v[0], v[1]), fmaxf(v[2], v[3]));
+        lv[k][1] = fmaxf(fmaxf(v[4], v[5]), fmaxf(v[6], v[7]));
     };
-    cf[0] = 0u;
-    cf[1] = 0u;
-    flag8(c, lim[0]);
+    max8(c, 0);
     if constexpr (Q >= 1) {
         float p1[14], p2[12], p3[8];
 #pragma unroll
         for (int i = 0; i < 14; ++i) p1[i] = c[i] + c[i + 1];
-        flag8(p1, lim[1]);
+        max8(p1, 1);
         if constexpr (Q >= 2) {
 #pragma unroll
             for (int i = 0; i < 12; ++i) p2[i] = p1[i] + p1[i + 2];
-            flag8(p2, lim[2]);
+            max8(p2, 2);
         }
         if constexpr (Q >= 3) {
 #pragma unroll
             for (int i = 0; i < 8; ++i) p3[i] = p2[i] + p2[i + 4];
-            flag8(p3, lim[3]);
+            max8(p3, 3);
         }
     }
+    constexpr int NQ = Q >= 4 ? (1 << Q) / 8 : 1;  // segments of the widest window
+    constexpr int G = kPowCells / 8 + NQ;
+    float* S = aux;
+    float* MN = aux + G;
+    float* MX = aux + 2 * G;
     if constexpr (Q >= 4) {
-        constexpr int NQ = (1 << Q) / 8;  // segments of the widest window
-        constexpr int G = kPowCells / 8 + NQ;
-        float* S = aux;
-        float* MN = aux + G;
-        float* MX = aux + 2 * G;
-        float* DB[2] = {aux + 3 * G, aux + 4 * G};
         auto stats = [&](const float* v, int g) {
             float pre = v[0];
             float mn = fminf(0.0f, pre), mx = fmaxf(0.0f, pre);
@@ -649,8 +690,33 @@ __device__ __forceinline__ void pow2_flag_segments(const float* x, float* aux, c
             const float4 v0 = x4[2 * g], v1 = x4[2 * g + 1];
             const float h[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
             stats(h, g);
+            const int hn = ((int)tid == NQ - 1) ? 7 : 8;  // cell 2047 + W is past the window
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                if (i < hn) mb = max(mb, abs_bits(h[i]));
         }
-        __syncthreads();
+    }
+    mb = (uint32_t)__ockl_wfred_max_i32((int)mb);  // every value is < 2^31
+    if ((tid & 63) == 0) wbits[tid >> 6] = mb;
+    __syncthreads();
+    uint32_t Mb = wbits[0];
+#pragma unroll
+    for (int i = 1; i < kThreads / 64; ++i) Mb = max(Mb, wbits[i]);
+    M = __uint_as_float(min(Mb, 0x7f800000u));  // a NaN counts as +inf
+    if (!(M * (float)(1 << Q) < 0x1p125f)) return false;  // block-uniform
+    // from Q = 5 on, thread k computes limit k once for the block; the D
+    // levels' first barrier publishes them and every comparison comes after it
+    __shared__ float limv[kPowPfMaxQ + 1];
+    if constexpr (Q >= 5) {
+        if ((int)tid <= Q) limv[tid] = seg_limit(lc[tid], M);
+    }
+    auto limit = [&](int k) -> float {
+        if constexpr (Q >= 5) return limv[k];
+        return seg_limit(lc[k], M);
+    };
+    float u[Q >= 4 ? Q - 3 : 1];
+    if constexpr (Q >= 4) {
+        float* DB[2] = {aux + 3 * G, aux + 4 * G};
         const float mn_own = MN[tid];
         // D_n(g) is kept for g = t and, while needed by wider levels, for
         // g = t + 256 (g <= 255 + NQ - n)
@@ -664,10 +730,7 @@ __device__ __forceinline__ void pow2_flag_segments(const float* x, float* aux, c
                 d0 = d0 + prev[tid + n / 2];
                 if ((int)tid <= NQ - n - 1) d1 = d1 + prev[tid + 256 + n / 2];
             }
-            const float u = (d0 + MX[tid + n]) - mn_own;
-            const uint32_t f = !(u <= lim[k]) ? 1u : 0u;
-            cf[0] = cf[0] + cf[0] + f;
-            cf[1] = cf[1] + cf[1] + f;
+            u[k - 4] = (d0 + MX[tid + n]) - mn_own;
             if (k < Q) {
                 float* nxt = DB[(k + 1) & 1];
                 nxt[tid] = d0;
@@ -676,51 +739,86 @@ __device__ __forceinline__ void pow2_flag_segments(const float* x, float* aux, c
             }
         }
     }
+    cf[0] = 0u;
+    cf[1] = 0u;
+#pragma unroll
+    for (int k = 0; k <= (Q < 3 ? Q : 3); ++k) {
+        const float l = limit(k);
+        cf[0] = cf[0] + cf[0] + (!(lv[k][0] <= l) ? 1u : 0u);
+        cf[1] = cf[1] + cf[1] + (!(lv[k][1] <= l) ? 1u : 0u);
+    }
+#pragma unroll
+    for (int k = 4; k <= Q; ++k) {
+        const uint32_t f = !(u[k - 4] <= limit(k)) ? 1u : 0u;
+        cf[0] = cf[0] + cf[0] + f;
+        cf[1] = cf[1] + cf[1] + f;
+    }
+    return true;
 }
 
 // Runs pow2_flag_segments<q> for a runtime q in [Q, QM].
 template <int Q, int QM>
-__device__ __forceinline__ void pow2_flag_segments_at(int q, const float* x, float* aux, const float* lim,
-                                                      uint32_t (&cf)[2]) {
-    if (q == Q) {
-        pow2_flag_segments<Q>(x, aux, lim, cf);
-    } else {
-        if constexpr (Q < QM) pow2_flag_segments_at<Q + 1, QM>(q, x, aux, lim, cf);
-    }
+__device__ __forceinline__ bool pow2_flag_segments_at(int q, const float* x, float* aux, const float2* lc,
+                                                      uint32_t* wbits, uint32_t (&cf)[2], float& M) {
+    if (q == Q) return pow2_flag_segments<Q>(x, aux, lc, wbits, cf, M);
+    if constexpr (Q < QM) return pow2_flag_segments_at<Q + 1, QM>(q, x, aux, lc, wbits, cf, M);
+    return false;
 }
 
-// One tile: the window [T, T + span) of row r is in x (la: a second buffer of
-// the same size, stage: kPowStage entries per wave), m is this thread's
-// max |x| over its share of the window, err_e the filter's error terms
-// (BoxErr::e, in LDS).  Finds the tile's candidates and
-// appends them to out.  Every thread of the block calls it; it ends with no
-// barrier after its last LDS access, so a caller reusing x, la or stage
-// synchronises first.
+// Recompute layout after the segment filter: one lane per flagged cell
+// (false), or one lane per flagged 4-cell chunk running its four sums side by
+// side (true: a quarter of the LDS reads, but a tile with few flagged chunks
+// waits four times as long for its longest sums).
+constexpr bool kChunkRecompute = false;
+
+// One tile: the window [T, T + span) of row r is in x, visible to the block
+// (la: a second buffer of the same size, stage: kPowStage entries per wave),
+// err_e the filter's error terms (BoxErr::e, in LDS).  SEG: an interior
+// window (span = 2048 + W - 1) goes through the segment filter, which takes
+// the window's max |x| itself, with the limit coefficients lc (seg_coef, in
+// LDS); every other window uses m, this thread's max |x| over its share of
+// the window (+inf after a NaN).  Finds the tile's candidates and appends
+// them to out.  Every thread of the block calls it; it ends with no barrier
+// after its last LDS access, so a caller reusing x, la or stage synchronises
+// first.
 template <int QMAX, bool SEG>
 __device__ __forceinline__ void pow2_tile(float* x, float* la, uint2* stage, float m, const float* row, int64_t T,
-                                          int span, bool vec, int q, float thr, const double* err_e, int64_t r,
-                                          CandD* __restrict__ out, int64_t cap,
+                                          int span, bool vec, int q, float thr, const double* err_e,
+                                          const float2* lc, int64_t r, CandD* __restrict__ out, int64_t cap,
                                           unsigned long long* __restrict__ count) {
     const unsigned tid = opaque_tid();
-    __shared__ float wmaxabs[kThreads / 64];
-    const int W = 1 << q;
-    m = __ockl_wfred_max_f32(m);  // wave max (m is never NaN: a NaN sample made it +inf)
-    if ((tid & 63) == 0) wmaxabs[tid >> 6] = m;
-    __syncthreads();
-    float M = wmaxabs[0];
-    for (int i = 1; i < kThreads / 64; ++i) M = fmaxf(M, wmaxabs[i]);
-    // the filter limits are block-uniform: thread k computes level k's once
-    // (each is double-precision work that every wave would otherwise repeat)
+    __shared__ uint32_t wbits[kThreads / 64];
     __shared__ float lim[kMaxPowQ + 1];
-    if ((int)tid <= q) lim[tid] = level_limit(thr, (int)tid, err_e[tid], M);
-    __syncthreads();
+    const int W = 1 << q;
+    const bool edge = span < kPowCells + W - 1;
+    float M = 0.0f;
+    uint32_t cf[2] = {0u, 0u};
+    bool seg = false;
+    if constexpr (SEG) {
+        if (!edge) seg = pow2_flag_segments_at<0, QMAX>(q, x, la, lc, wbits, cf, M);  // block-uniform
+    }
+    if (!seg) {
+        if (!SEG || edge) {
+            m = __ockl_wfred_max_f32(m);  // wave max (m is never NaN: a NaN sample made it +inf)
+            if ((tid & 63) == 0) wbits[tid >> 6] = __float_as_uint(m);
+            __syncthreads();
+            M = __uint_as_float(wbits[0]);
+            for (int i = 1; i < kThreads / 64; ++i) M = fmaxf(M, __uint_as_float(wbits[i]));
+        }
+        // the per-cell paths' limits are block-uniform: thread k computes
+        // level k's once (double-precision work that every wave would
+        // otherwise repeat)
+        if ((int)tid <= q) lim[tid] = level_limit(thr, (int)tid, err_e[tid], M);
+        __syncthreads();
+    }
 
     // Flag phase.  Flags are shifted in one bit per level, so level k ends at
     // bit q - k; reversing the word puts it at bit k.  The flagged cells are
     // then compacted into an LDS queue (cell << 14 | width flags; the level
     // buffers are free by then) in cell order, ranked by (slot, wave, lane)
     // with one ballot per slot; the exact recompute runs one chain per lane
-    // over the queue.
+    // over the queue (four, one per cell, for the segment filter's chunk
+    // entries).
     uint32_t* queue = reinterpret_cast<uint32_t*>(la);
     __shared__ uint32_t slot_cnt[kPowPer][kThreads / 64];
     const int lane = tid & 63;
@@ -730,13 +828,12 @@ __device__ __forceinline__ void pow2_tile(float* x, float* la, uint2* stage, flo
     // Per-cell flags where exactness needs them: the last tile of a row (its
     // window is cut by the row end, so level spans are checked) and windows
     // with NaN/inf samples or magnitudes where a level sum could overflow.
-    const bool careful = span < kPowCells + W - 1 || !(M * (float)W < 0x1p125f);
-    if (SEG && !careful) {
+    const bool careful = edge || !(M * (float)W < 0x1p125f);
+    if (seg) {
         // segment layout: thread t holds chunks 2t and 2t + 1 (cells 8t ..
-        // 8t + 7); a flagged chunk queues its four cells, ranked by (wave,
+        // 8t + 7); a flagged chunk queues its four cells (or, with
+        // kChunkRecompute, one entry for its first cell), ranked by (wave,
         // lane, chunk)
-        uint32_t cf[2];
-        if constexpr (SEG) pow2_flag_segments_at<0, QMAX>(q, x, la, lim, cf);
         cf[0] = __brev(cf[0]) >> (31 - q);
         cf[1] = __brev(cf[1]) >> (31 - q);
         const uint64_t b0 = __ballot(cf[0] != 0), b1 = __ballot(cf[1] != 0);
@@ -751,17 +848,20 @@ __device__ __forceinline__ void pow2_tile(float* x, float* la, uint2* stage, flo
         for (int s = 0; s < 2; ++s) {
             if (cf[s]) {
                 const uint32_t c0 = 8u * tid + 4u * (uint32_t)s;
-                reinterpret_cast<uint4*>(queue)[pos] =
-                    make_uint4((c0 << 14) | cf[s], ((c0 + 1) << 14) | cf[s], ((c0 + 2) << 14) | cf[s],
-                               ((c0 + 3) << 14) | cf[s]);
+                if constexpr (kChunkRecompute) {
+                    queue[pos] = (c0 << 14) | cf[s];
+                } else {
+                    reinterpret_cast<uint4*>(queue)[pos] =
+                        make_uint4((c0 << 14) | cf[s], ((c0 + 1) << 14) | cf[s], ((c0 + 2) << 14) | cf[s],
+                                   ((c0 + 3) << 14) | cf[s]);
+                }
                 ++pos;
             }
         }
-        total *= 4;
-    } else if (!careful) {
+        if constexpr (!kChunkRecompute) total *= 4;
+    } else if (!SEG && !careful) {
         // chunk layout: slot s of thread t is chunk s*256 + t (4 cells); a
         // flagged chunk queues its four cells
-        uint32_t cf[2];
         if constexpr (!SEG) pow2_flag_chunks_at<0, QMAX>(q, x, la, lim, cf);
         uint64_t mk[2];
 #pragma unroll
@@ -818,12 +918,13 @@ __device__ __forceinline__ void pow2_tile(float* x, float* la, uint2* stage, flo
     // from q = 4 on the level rounds' second round stored its level over the
     // samples: reload them (the block's window was just read, so this mostly
     // hits L2); the segment filter leaves them in place
-    if (q >= 4 && (!SEG || careful)) pow2_load_window(x, row + T, span, vec);
+    if (q >= 4 && !seg) pow2_load_window(x, row + T, span, vec);
     __syncthreads();
 
     // Exact recompute: lane i of a round takes queue entry i and runs the
-    // left-to-right running sum of its cell up to its largest flagged width.
-    // The queue is in cell order, so the lanes of a wave read neighbouring
+    // left-to-right running sum of its cell (of each of its chunk's four
+    // cells after the segment filter) up to its largest flagged width.  The
+    // queue is in cell order, so the lanes of a wave read neighbouring
     // samples (no LDS bank conflicts).  The loop bound is wave-uniform.
     // Candidates go to a per-wave LDS stage (cell << 4 | width index, S/N
     // bits).  A wave whose stage would overflow flushes it with one global
@@ -850,84 +951,163 @@ __device__ __forceinline__ void pow2_tile(float* x, float* la, uint2* stage, flo
         __builtin_amdgcn_wave_barrier();
         wn = 0;
     };
-    for (uint32_t q0 = 0; q0 < total; q0 += kThreads) {  // block-uniform
-        const uint32_t qi = q0 + tid;
-        const uint32_t e = qi < total ? queue[qi] : 0u;
-        const int lt = (int)(e >> 14);
-        const uint32_t fl = e & 0x3fffu;
-        int top = fl ? 31 - __clz(fl) : -1;
-        top = __builtin_amdgcn_readfirstlane(__ockl_wfred_max_i32(top));  // wave max: a scalar loop bound
-        if (top < 0) continue;  // wave-uniform; no barrier in this loop
-        // The running sum s over samples lt, lt+1, ... is taken to the
-        // wave's largest flagged width 2^top by every lane (uniform control
-        // flow), and width 2^p is decided when s covers 2^p samples.  A lane
-        // reads at most to lt + 2^top - 1 <= 2047 + W - 1 < span_max; samples
-        // past its own flagged windows only enter widths it does not decide.
-        float s = 0.0f;
-        const float* xs = x + lt;
-        auto decide = [&](int p) {
-            float snr = 0.0f;
-            bool pred = false;
-            if ((fl >> p) & 1u) {
-                snr = s / sqrtf((float)(1 << p));
-                pred = snr > thr;
-            }
-            const uint64_t m = __ballot(pred);
-            if (m) {
-                const uint32_t n = (uint32_t)__popcll(m);
-                if (wn + n > wcap) flush();
-                if (pred)
-                    wst[wn + (uint32_t)__popcll(m & lt_mask)] =
-                        make_uint2(((uint32_t)lt << 4) | (uint32_t)p, __float_as_uint(snr));
-                wn += n;
-            }
-        };
-        // widths 1 .. 16 (p <= 4): samples 0 .. 15
-        {
-            float v[16];
-#pragma unroll
-            for (int u = 0; u < 16; ++u) v[u] = (u < (1 << top)) ? xs[u] : 0.0f;
-            s += v[0];
-            decide(0);
+    // width 2^p of the cell at lt whose running sum covers 2^p samples
+    // (wave-uniform call: the stage append is a wave operation)
+    auto decide = [&](float sum, int lt, uint32_t fl, int p) {
+        float snr = 0.0f;
+        bool pred = false;
+        if ((fl >> p) & 1u) {
+            snr = sum / sqrtf((float)(1 << p));
+            pred = snr > thr;
+        }
+        const uint64_t m = __ballot(pred);
+        if (m) {
+            const uint32_t n = (uint32_t)__popcll(m);
+            if (wn + n > wcap) flush();
+            if (pred)
+                wst[wn + (uint32_t)__popcll(m & lt_mask)] =
+                    make_uint2(((uint32_t)lt << 4) | (uint32_t)p, __float_as_uint(snr));
+            wn += n;
+        }
+    };
+    if (kChunkRecompute && seg) {
+        // chunk entries (first cell c0, a multiple of 4): the four running
+        // sums advance together, four samples per step from two aligned
+        // float4s A = x[c0+4m .. c0+4m+3] and B = the next four; sum j adds
+        // x[c0+j+4m .. c0+j+4m+3] in order.  Reads stay below 2304 floats:
+        // c0 <= 2044 and the last step's B ends at c0 + 2^top + 3.
+        for (uint32_t q0 = 0; q0 < total; q0 += kThreads) {  // block-uniform
+            const uint32_t qi = q0 + tid;
+            const uint32_t e = qi < total ? queue[qi] : 0u;
+            const int c0 = (int)(e >> 14);
+            const uint32_t fl = e & 0x3fffu;
+            int top = fl ? 31 - __clz(fl) : -1;
+            top = __builtin_amdgcn_readfirstlane(__ockl_wfred_max_i32(top));  // wave max: a scalar loop bound
+            if (top < 0) continue;  // wave-uniform; no barrier in this loop
+            const float4* x4 = reinterpret_cast<const float4*>(x) + (c0 >> 2);
+            float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+            auto decide4 = [&](int p) {
+                decide(s0, c0, fl, p);
+                decide(s1, c0 + 1, fl, p);
+                decide(s2, c0 + 2, fl, p);
+                decide(s3, c0 + 3, fl, p);
+            };
+            float4 A = x4[0], B = x4[1];
+            s0 += A.x;
+            s1 += A.y;
+            s2 += A.z;
+            s3 += A.w;
+            decide4(0);
             if (top >= 1) {
-                s += v[1];
-                decide(1);
+                s0 += A.y;
+                s1 += A.z;
+                s2 += A.w;
+                s3 += B.x;
+                decide4(1);
             }
             if (top >= 2) {
-                s += v[2];
-                s += v[3];
-                decide(2);
+                s0 += A.z;
+                s1 += A.w;
+                s2 += B.x;
+                s3 += B.y;
+                s0 += A.w;
+                s1 += B.x;
+                s2 += B.y;
+                s3 += B.z;
+                decide4(2);
             }
             if (top >= 3) {
-#pragma unroll
-                for (int u = 4; u < 8; ++u) s += v[u];
-                decide(3);
-            }
-            if (top >= 4) {
-#pragma unroll
-                for (int u = 8; u < 16; ++u) s += v[u];
-                decide(4);
+                const int steps = 1 << (top - 2);
+                float4 Bn = x4[2];
+                for (int m = 1; m < steps; ++m) {  // wave-uniform
+                    A = B;
+                    B = Bn;
+                    if (m + 2 <= steps) Bn = x4[m + 2];
+                    s0 += A.x;
+                    s1 += A.y;
+                    s2 += A.z;
+                    s3 += A.w;
+                    s0 += A.y;
+                    s1 += A.z;
+                    s2 += A.w;
+                    s3 += B.x;
+                    s0 += A.z;
+                    s1 += A.w;
+                    s2 += B.x;
+                    s3 += B.y;
+                    s0 += A.w;
+                    s1 += B.x;
+                    s2 += B.y;
+                    s3 += B.z;
+                    const int w = 4 * (m + 1);
+                    if ((w & (w - 1)) == 0) decide4(31 - __clz(w));
+                }
             }
         }
-        // widths 32 .. 2^top: blocks of 16 samples, the next block's reads
-        // issued before the current block is added
-        if (top >= 5) {
-            const int end = 1 << top;
-            float nx[16];
+    } else {
+        for (uint32_t q0 = 0; q0 < total; q0 += kThreads) {  // block-uniform
+            const uint32_t qi = q0 + tid;
+            const uint32_t e = qi < total ? queue[qi] : 0u;
+            const int lt = (int)(e >> 14);
+            const uint32_t fl = e & 0x3fffu;
+            int top = fl ? 31 - __clz(fl) : -1;
+            top = __builtin_amdgcn_readfirstlane(__ockl_wfred_max_i32(top));  // wave max: a scalar loop bound
+            if (top < 0) continue;  // wave-uniform; no barrier in this loop
+            // The running sum s over samples lt, lt+1, ... is taken to the
+            // wave's largest flagged width 2^top by every lane (uniform
+            // control flow), and width 2^p is decided when s covers 2^p
+            // samples.  A lane reads at most to lt + 2^top - 1 <= 2047 + W -
+            // 1 < span_max; samples past its own flagged windows only enter
+            // widths it does not decide.
+            float s = 0.0f;
+            const float* xs = x + lt;
+            // widths 1 .. 16 (p <= 4): samples 0 .. 15
+            {
+                float v[16];
 #pragma unroll
-            for (int u = 0; u < 16; ++u) nx[u] = xs[16 + u];
-            for (int b = 16; b < end; b += 16) {  // wave-uniform
-                float cur[16];
-#pragma unroll
-                for (int u = 0; u < 16; ++u) cur[u] = nx[u];
-                if (b + 16 < end) {
-#pragma unroll
-                    for (int u = 0; u < 16; ++u) nx[u] = xs[b + 16 + u];
+                for (int u = 0; u < 16; ++u) v[u] = (u < (1 << top)) ? xs[u] : 0.0f;
+                s += v[0];
+                decide(s, lt, fl, 0);
+                if (top >= 1) {
+                    s += v[1];
+                    decide(s, lt, fl, 1);
                 }
+                if (top >= 2) {
+                    s += v[2];
+                    s += v[3];
+                    decide(s, lt, fl, 2);
+                }
+                if (top >= 3) {
 #pragma unroll
-                for (int u = 0; u < 16; ++u) s += cur[u];
-                const int w = b + 16;
-                if ((w & (w - 1)) == 0) decide(31 - __clz(w));
+                    for (int u = 4; u < 8; ++u) s += v[u];
+                    decide(s, lt, fl, 3);
+                }
+                if (top >= 4) {
+#pragma unroll
+                    for (int u = 8; u < 16; ++u) s += v[u];
+                    decide(s, lt, fl, 4);
+                }
+            }
+            // widths 32 .. 2^top: blocks of 16 samples, the next block's
+            // reads issued before the current block is added
+            if (top >= 5) {
+                const int end = 1 << top;
+                float nx[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) nx[u] = xs[16 + u];
+                for (int b = 16; b < end; b += 16) {  // wave-uniform
+                    float cur[16];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) cur[u] = nx[u];
+                    if (b + 16 < end) {
+#pragma unroll
+                        for (int u = 0; u < 16; ++u) nx[u] = xs[b + 16 + u];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) s += cur[u];
+                    const int w = b + 16;
+                    if ((w & (w - 1)) == 0) decide(s, lt, fl, 31 - __clz(w));
+                }
             }
         }
     }
@@ -973,7 +1153,7 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
     __shared__ double err_e[kMaxPowQ + 1];
     if ((int)threadIdx.x <= q) err_e[threadIdx.x] = err.e[threadIdx.x];  // read after pow2_tile's first barrier
     const float m = pow2_load_window(x, row + T, span, vec);
-    pow2_tile<kMaxPowQ, false>(x, la, stage, m, row, T, span, vec, q, d_thr ? *d_thr : h_thr, err_e, r, out, cap,
+    pow2_tile<kMaxPowQ, false>(x, la, stage, m, row, T, span, vec, q, d_thr ? *d_thr : h_thr, err_e, nullptr, r, out, cap,
                                count);
 }
 
@@ -995,41 +1175,30 @@ constexpr int kPowPfTiles = 4;  // consecutive tiles of a row per block
 __device__ __forceinline__ void pow2_issue_window(float* dst, const float* src, int span4) {
     const unsigned tid = opaque_tid();
     const int n4 = span4 >> 2;
-    const int wbase = (int)(tid & ~63u);
+    // the wave's first float4 and its LDS address are wave-uniform (SGPRs);
+    // a lane adds only its 32-bit byte offset (the saddr form of the load)
+    const int wbase = (int)__builtin_amdgcn_readfirstlane(tid & ~63u);
+    const uint32_t voff = 16u * (tid & 63u);
     for (int j0 = 0; j0 < n4; j0 += kThreads) {  // block-uniform
         const int i = j0 + (int)tid;
         const uint32_t m0 = __builtin_amdgcn_readfirstlane(
             (uint32_t)(uintptr_t)(__attribute__((address_space(3))) float*)(dst + 4 * (j0 + wbase)));
-        const float* g = src + 4 * i;
+        const float* g = src + 4 * (j0 + wbase);
         // M0 is saved and restored around the load; one wait state between
         // the SALU write of M0 and the load that reads it
         uint32_t saved;
         if (i < n4)
-            asm volatile("s_mov_b32 %0, m0\n\ts_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %2, off\n\t"
+            asm volatile("s_mov_b32 %0, m0\n\ts_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %2, %3\n\t"
                          "s_mov_b32 m0, %0"
                          : "=&s"(saved)
-                         : "s"(m0), "v"(g));
+                         : "s"(m0), "v"(voff), "s"(g));
     }
-}
-
-// This thread's max |x| over float4 units tid, tid + 256, ... of x[0, span4)
-// (+inf once it saw a NaN), as pow2_load_window reports it.
-__device__ __forceinline__ float pow2_window_max(const float* x, int span4) {
-    const unsigned tid = opaque_tid();
-    float m = 0.0f;
-    for (int i = tid; i < (span4 >> 2); i += kThreads) {
-        const float4 v = reinterpret_cast<const float4*>(x)[i];
-        m = fmaxf(m, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
-        if (v.x != v.x || v.y != v.y || v.z != v.z || v.w != v.w) m = __int_as_float(0x7f800000);
-    }
-    return m;
 }
 
 // One block = kPowPfTiles consecutive tiles of one row (q <= QM).  A
 // tile whose whole rounded window lies inside a 16-byte aligned row is
 // fetched by global_load_lds, issued when the previous tile starts; the row's
-// last tiles are loaded directly.  M may then include the one sample after
-// the window (a larger bound: the filter only gets more conservative).
+// last tiles are loaded directly.
 template <int QM>
 __global__ void __launch_bounds__(kThreads) k_boxcar_pow2_pf(const float* __restrict__ img, int64_t cols,
                                                              int64_t pitch, const float* __restrict__ d_thr,
@@ -1043,7 +1212,12 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2_pf(const float* __rest
     __shared__ __attribute__((aligned(16))) float la[kSpan];
     __shared__ __attribute__((aligned(16))) uint2 stage[kPowStage * (kThreads / 64)];
     __shared__ double err_e[QM + 1];
-    if ((int)threadIdx.x <= q) err_e[threadIdx.x] = err.e[threadIdx.x];  // read after the first tile's barriers
+    __shared__ float2 lc[QM + 1];
+    const float thr = d_thr ? *d_thr : h_thr;
+    if ((int)threadIdx.x <= q) {  // read after the first tile's barriers
+        err_e[threadIdx.x] = err.e[threadIdx.x];
+        lc[threadIdx.x] = seg_coef(thr, (int)threadIdx.x, err.e[threadIdx.x]);
+    }
     const int full = kPowCells + (1 << q) - 1;
     const int full4 = round_up4(full);
     const int64_t r = blockIdx.y;
@@ -1052,22 +1226,21 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2_pf(const float* __rest
     const int64_t k1 = min(ntiles, k0 + (int64_t)kPowPfTiles);
     const float* row = img + r * pitch;
     const bool vec = (pitch & 3) == 0 && (reinterpret_cast<uintptr_t>(img) & 15) == 0;
-    const float thr = d_thr ? *d_thr : h_thr;
     auto dma = [&](int64_t k) { return vec && k * kPowCells + full4 <= cols; };  // monotone in k
     auto step = [&](float* cur, float* nxt, int64_t k) {
         const int64_t T = k * kPowCells;
         const int span = (int)min((int64_t)full, cols - T);
-        float m;
+        float m = 0.0f;  // a DMA'd window is interior: the segment filter takes its max itself
         if (dma(k)) {
             asm volatile("s_waitcnt vmcnt(0)" : : : "memory");  // this wave's part of the window has landed
             __syncthreads();
             if (k + 1 < k1 && dma(k + 1)) pow2_issue_window(nxt, row + T + kPowCells, full4);
-            m = pow2_window_max(cur, full4);
         } else {
             __syncthreads();
             m = pow2_load_window(cur, row + T, span, vec);
+            __syncthreads();
         }
-        pow2_tile<QM, true>(cur, la, stage, m, row, T, span, vec, q, thr, err_e, r, out, cap, count);
+        pow2_tile<QM, true>(cur, la, stage, m, row, T, span, vec, q, thr, err_e, lc, r, out, cap, count);
     };
     if (k0 < k1 && dma(k0)) pow2_issue_window(xa, row + k0 * kPowCells, full4);
     for (int64_t k = k0; k < k1; ++k) {  // block-uniform
