@@ -400,9 +400,11 @@ __device__ __forceinline__ void group_trees(const float* __restrict__ lds, const
 // thread works out its unit slots once; per tile it issues the loads of the
 // next tile into registers, evaluates the current tile from one LDS buffer,
 // then commits the registers to the other buffer (one barrier per tile).
+// ONEBUF: one LDS buffer, committed after a second barrier once the trees
+// have read it (half the LDS, so wide windows keep more blocks resident).
 constexpr int kStageSlots = kGroupMaxUnits / kThreads;
 
-template <int L, bool SNR, bool FAST>
+template <int L, bool SNR, bool FAST, bool ONEBUF>
 __global__ void __launch_bounds__(kThreads) k_merge_group(const float* __restrict__ in, int64_t in_pitch,
                                                           float* __restrict__ out, int64_t out_pitch,
                                                           const GroupDescD* __restrict__ groups, int64_t ngroups,
@@ -453,11 +455,12 @@ __global__ void __launch_bounds__(kThreads) k_merge_group(const float* __restric
     commit(lds);
     __syncthreads();
     for (int64_t k = k0; k < k1; ++k) {
-        float* cur = lds + ((k - k0) & 1) * lds_stride;
-        float* nxt = lds + ((k - k0 + 1) & 1) * lds_stride;
+        float* cur = ONEBUF ? lds : lds + ((k - k0) & 1) * lds_stride;
+        float* nxt = ONEBUF ? lds : lds + ((k - k0 + 1) & 1) * lds_stride;
         const bool more = k + 1 < k1;
         if (more) issue((k + 1) * kGroupTile);
         group_trees<L, SNR, FAST>(cur, g, rows, out, out_pitch, k * kGroupTile, nsamps, out_rows_limit, epi);
+        if (ONEBUF && more) __syncthreads();  // block-uniform: every read of cur is done
         if (more) commit(nxt);
         __syncthreads();
     }
@@ -1171,6 +1174,7 @@ void launch_fdmt_fused(int L, const float* in, int64_t in_pitch, float* out, int
 
 struct GroupLaunch {
     size_t lds;
+    bool onebuf;
     const float* in;
     int64_t in_pitch;
     float* out;
@@ -1193,10 +1197,16 @@ static void group_launch(const GroupLaunch& a, int64_t lo, int64_t hi, int tpb, 
     if (hi <= lo) return;
     const int64_t nb = a.ngroups * ceil_div(hi - lo, (int64_t)tpb);
     if (nb > 0x7fffffffLL) throw Error(SPS_ERR_INVALID_ARGUMENT, "merge grid too large");
-    hipLaunchKernelGGL((k_merge_group<L, SNR, FAST>), dim3((uint32_t)nb), dim3(kThreads), a.lds, st, a.in, a.in_pitch,
-                       a.out, a.out_pitch, static_cast<const GroupDescD*>(a.groups), a.ngroups,
-                       static_cast<const LeafDescD*>(a.leaves), static_cast<const GRowDesc<L>*>(a.rows), a.unit_leaf,
-                       tpb, a.lds_stride, a.nsamps, a.limit, a.epi, lo, hi);
+    if (a.onebuf)
+        hipLaunchKernelGGL((k_merge_group<L, SNR, FAST, true>), dim3((uint32_t)nb), dim3(kThreads), a.lds, st, a.in,
+                           a.in_pitch, a.out, a.out_pitch, static_cast<const GroupDescD*>(a.groups), a.ngroups,
+                           static_cast<const LeafDescD*>(a.leaves), static_cast<const GRowDesc<L>*>(a.rows),
+                           a.unit_leaf, tpb, a.lds_stride, a.nsamps, a.limit, a.epi, lo, hi);
+    else
+        hipLaunchKernelGGL((k_merge_group<L, SNR, FAST, false>), dim3((uint32_t)nb), dim3(kThreads), a.lds, st, a.in,
+                           a.in_pitch, a.out, a.out_pitch, static_cast<const GroupDescD*>(a.groups), a.ngroups,
+                           static_cast<const LeafDescD*>(a.leaves), static_cast<const GRowDesc<L>*>(a.rows),
+                           a.unit_leaf, tpb, a.lds_stride, a.nsamps, a.limit, a.epi, lo, hi);
 }
 
 // Tiles [fast_lo, full) take the fast-only kernel (fast_lo: the first tile at
@@ -1229,7 +1239,10 @@ void launch_fdmt_group(int L, const float* in, int64_t in_pitch, float* out, int
     if (tiles_per_block < 1) throw Error(SPS_ERR_INVALID_ARGUMENT, "tiles per block must be positive");
     if (max_shift < 0) throw Error(SPS_ERR_INVALID_ARGUMENT, "negative row shift");
     GroupLaunch a{};
-    a.lds = sizeof(float) * 2 * (size_t)lds_floats_max;  // two tile buffers
+    // two tile buffers while they fit 48 KiB (three blocks per CU); wider
+    // windows take one buffer and a second barrier per tile
+    a.onebuf = lds_floats_max > kGroupTwoBufFloats;
+    a.lds = sizeof(float) * (a.onebuf ? 1 : 2) * (size_t)lds_floats_max;
     a.in = in;
     a.in_pitch = in_pitch;
     a.out = out;

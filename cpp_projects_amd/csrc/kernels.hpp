@@ -69,6 +69,7 @@ struct GroupDescD {
     int32_t lds_floats;
     int32_t nunits;    // float4 units of all its leaf windows (lds_floats / 4)
 };
+constexpr int kGroupTwoBufFloats = 6144;  // widest window that staged passes double-buffer
 constexpr int kGroupMaxUnits = 2304;  // per group: 9 float4 per thread (a lone depth-4 tree needs <= 2080)
 struct LeafDescD {
     int64_t row;       // input state row

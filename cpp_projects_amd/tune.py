@@ -5,7 +5,7 @@ FDMT iterations each pass fuses.  Candidates are timed with the stage events
 of the search handle on the real input; the fastest is kept on the handle."""
 
 DEFAULT_SHAPES = ((6144, 16, 16), (5120, 16, 16), (7168, 16, 16), (6144, 16, 32), (6144, 16, 64), (7168, 16, 32),
-                  (5120, 16, 32), (6144, 32, 32))
+                  (5120, 16, 32), (6144, 32, 32), (8192, 32, 16), (9216, 32, 16), (8192, 64, 16))
 
 
 def candidate_schedules(n, max_depth=3):
