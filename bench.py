@@ -44,6 +44,31 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+class Marker:
+    """A named roctx range around a region (rocprofv3 --marker-trace shows it
+    beside the kernel trace); does nothing unless enabled."""
+
+    lib = None
+
+    @classmethod
+    def enable(cls):
+        import ctypes
+
+        cls.lib = ctypes.CDLL("librocprofiler-sdk-roctx.so")
+        cls.lib.roctxRangePushA.argtypes = [ctypes.c_char_p]
+
+    def __init__(self, name):
+        self.name = name
+
+    def __enter__(self):
+        if Marker.lib is not None:
+            Marker.lib.roctxRangePushA(self.name.encode())
+
+    def __exit__(self, *exc):
+        if Marker.lib is not None:
+            Marker.lib.roctxRangePop()
+
+
 def cpu_baseline(x_u8, ndt, sample_cols, threads):
     """Times the CPU oracle chain (normalise -> FDMT -> mask FDMT -> S/N ->
     boxcar) on the first sample_cols samples of every channel."""
@@ -147,6 +172,9 @@ def bench_batch(args):
     tmp_c = torch.empty((1 << 16, 4), dtype=torch.int32, device="cuda")
     tmp_n = torch.zeros(1, dtype=torch.int64, device="cuda")
     for nch in nchs:
+        if args.batch_no_tune:  # the autotuner's earlier choice (profiling runs)
+            sched[nch] = dict(workloads.BATCH_TUNED[nch])
+            continue
         i = max((k for k, s in enumerate(specs) if s["nch"] == nch), key=lambda k: specs[k]["ns"])
         s = SingleBeamSearch(CFG["f_min"], CFG["f_max"], nch, specs[i]["ns"], nch - 1, cand_capacity=1 << 16, **mk)
         s.run(ins[i], None, tmp_c, tmp_n)
@@ -175,10 +203,11 @@ def bench_batch(args):
         cb = torch.empty((cap, 4), dtype=torch.int32, device="cuda")
         sb.run(ins[i], None, cb, cnt)
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        sb.run(ins[i], None, cb, cnt)
-        counts_found.append(int(cnt.item()))
-        plain_ms.append((time.perf_counter() - t0) * 1e3)
+        with Marker(f"plain_{i}_{s['nch']}x{s['ns']}"):
+            t0 = time.perf_counter()
+            sb.run(ins[i], None, cb, cnt)
+            counts_found.append(int(cnt.item()))
+            plain_ms.append((time.perf_counter() - t0) * 1e3)
         comp_bytes.append(sum(sb.stage_bytes()))
         cands.append(cb)
         counts.append(cnt)
@@ -200,18 +229,20 @@ def bench_batch(args):
             b.set_schedule(k, sc["schedule"])
         return b
 
-    def time_batch(b, steps):
+    def time_batch(b, steps, mark="batch"):
         b.launch()  # capture + first replay
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            b.launch()
-        torch.cuda.synchronize()
-        return (time.perf_counter() - t0) / steps
+        with Marker(mark):
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                b.launch()
+            torch.cuda.synchronize()
+            t = (time.perf_counter() - t0) / steps
+        return t
 
     all_idx = list(range(len(specs)))
     b = make_batch(all_idx)
-    batch_s = time_batch(b, args.batch_steps)
+    batch_s = time_batch(b, args.batch_steps, "batch_all_beams")
     batch_counts = [int(c.item()) for c in counts]
     sorted_ok = True
     for k in all_idx:
@@ -230,15 +261,28 @@ def bench_batch(args):
     for nch in nchs:
         idx = [i for i in all_idx if specs[i]["nch"] == nch]
         bb = make_batch(idx)
-        t = time_batch(bb, args.batch_steps)
+        t = time_batch(bb, args.batch_steps, f"batch_nch_{nch}")
         del bb
         p = sum(plain_ms[i] for i in idx) / 1e3
         by_nch[str(nch)] = {"beams": len(idx), "samples": sum(specs[i]["ns"] for i in idx),
                             "batched_ms": t * 1e3, "plain_ms": p * 1e3, "speedup": p / t,
                             "compulsory_frac": sum(comp_bytes[i] for i in idx) / t / 1e9 / HBM_PEAK_GBS}
-    # per length: plain time per beam against its compulsory bytes
+    # per length: plain time per beam against its compulsory bytes, and the
+    # short and the long beams batched alone
     short = [i for i in all_idx if specs[i]["ns"] < (1 << 16)]
     long_ = [i for i in all_idx if specs[i]["ns"] >= (1 << 16)]
+    by_len = {}
+    for name, idx in (("short_lt_2e16", short), ("long_ge_2e16", long_)):
+        if not idx:
+            continue
+        bb = make_batch(idx)
+        t = time_batch(bb, args.batch_steps, f"batch_{name}")
+        del bb
+        p = sum(plain_ms[i] for i in idx) / 1e3
+        by_len[name] = {"beams": len(idx), "samples": sum(specs[i]["ns"] for i in idx), "batched_ms": t * 1e3,
+                        "plain_ms": p * 1e3, "speedup": p / t,
+                        "compulsory_frac": sum(comp_bytes[i] for i in idx) / t / 1e9 / HBM_PEAK_GBS,
+                        "plain_compulsory_frac": sum(comp_bytes[i] for i in idx) / p / 1e9 / HBM_PEAK_GBS}
 
     def plain_rate(idx):
         if not idx:
@@ -263,6 +307,7 @@ def bench_batch(args):
         "compulsory_frac": tot_comp / batch_s / 1e9 / HBM_PEAK_GBS,
         "plain_compulsory_frac": tot_comp / plain_s / 1e9 / HBM_PEAK_GBS,
         "by_nchans": by_nch,
+        "by_length": by_len,
         "plain_GBs_short_beams_lt_2e16": plain_rate(short),
         "plain_GBs_long_beams": plain_rate(long_),
         "workers": args.batch_workers,
@@ -375,6 +420,10 @@ def main():
     ap.add_argument("--batch-beams", type=int, default=BATCH["nbeams"])
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--batch-workers", type=int, default=4)
+    ap.add_argument("--batch-only", action="store_true", help="run the batched multi-beam measurement alone")
+    ap.add_argument("--roctx", action="store_true", help="mark the timed batched replays as roctx ranges")
+    ap.add_argument("--batch-no-tune", action="store_true",
+                    help="batched measurement: take the stored per-channel-count schedules instead of tuning")
     ap.add_argument("--no-large", action="store_true", help="skip the 8192 x 2^20 large-share measurement")
     ap.add_argument("--large-steps", type=int, default=3)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -388,6 +437,11 @@ def main():
 
     from cpp_projects_amd.search import SingleBeamSearch
 
+    if args.roctx:
+        Marker.enable()
+    if args.batch_only:
+        print(json.dumps({"metric": METRIC, "n_gpus": 1, "batched": bench_batch(args)}))
+        return
     c = CFG
     N, R = c["nsamps"], c["nchans"]
     log(f"[bench] generating {R}x{N} u8 filterbank")

@@ -468,8 +468,7 @@ struct sps_fdmt {
             i0 += L;
         }
         if (staged) {
-            if (fuse_init && eng.init_mode == kInitMean && grid_consec && eng.nsamps % 4 == 0 && np >= 2 &&
-                pass_staged(0)) {
+            if (fuse_init && eng.init_mode == kInitMean && grid_consec && np >= 2 && pass_staged(0)) {
                 // a first pass whose trees need more init rows than one LDS
                 // window holds keeps the separate init
                 std::unique_ptr<StagedPass> sp(new StagedPass());
@@ -532,10 +531,7 @@ struct sps_fdmt {
         }
         ensure_trees(st);
         // the first staged pass builds state 0 itself when its tables exist
-        // (whole, aligned 4-sample input chunks: the kernel loads them as one word or one float4)
-        const uintptr_t ia = reinterpret_cast<uintptr_t>(in);
-        const bool fused0 = !aux && !ones && in != nullptr && staged && d_stage0_init != nullptr &&
-                            in_pitch % 4 == 0 && (in_u8 ? ia % 4 == 0 : ia % 16 == 0);
+        const bool fused0 = !aux && !ones && in != nullptr && staged && d_stage0_init != nullptr;
         if (!aux) ran_fused0 = fused0;
         if (!fused0)
             launch_fdmt_init(eng.init_mode, in, in_u8, ones, in_pitch, nm, eng.nchans, nsamps, d_grid.as<int32_t>(),

@@ -813,7 +813,7 @@ __global__ void __launch_bounds__(kThreads) k_init_consec(const void* __restrict
 }
 
 // First staged pass with the mean init fused in (plans whose per-channel init
-// dts are 0, 1, ..., K-1 with K <= kInitConsecMax; nsamps % 4 == 0): state 0
+// dts are 0, 1, ..., K-1 with K <= kInitConsecMax): state 0
 // never reaches HBM.  A block takes one group over tiles_per_block
 // consecutive 512-sample tiles.  For each channel of the group the LDS holds
 // its K init rows over the group's window [T - s4, T - s4 + 4 units), row 0
@@ -893,7 +893,7 @@ __device__ __forceinline__ float4a norm4(float4a v, float mu, float sg, float rs
     return float4a{q[0], q[1], q[2], q[3]};
 }
 
-template <int L, int IN_KIND, bool NORM, bool FAST>
+template <int L, int IN_KIND, bool NORM, bool FAST, bool VEC>
 __global__ void __launch_bounds__(kThreads) k_merge_group_init(const void* __restrict__ in, int64_t in_pitch,
                                                                ChanNorm norm, float* __restrict__ out,
                                                                int64_t out_pitch,
@@ -955,20 +955,41 @@ __global__ void __launch_bounds__(kThreads) k_merge_group_init(const void* __res
         raw8[j] = 0u;
         rawf[j] = float4a{0.0f, 0.0f, 0.0f, 0.0f};
     }
+    // VEC: a chunk is one aligned word (u8) or float4 (f32) load (nsamps,
+    // the row pitch and the row starts multiples of 4 samples, so a chunk,
+    // which starts at a multiple of 4, is whole inside the row or wholly
+    // outside); otherwise four sample loads, each 0 outside [0, nsamps)
     auto issue = [&](int64_t T) {
 #pragma unroll
         for (int j = 0; j < kInitSlots; ++j) {
             if (sc[j] < 0) continue;
             const InitChan& c = ch[sc[j]];
             const int64_t t0 = T + 4 * sq[j] + c.t_off;
-            if (t0 >= 0 && t0 + 4 <= nsamps) {
-                if constexpr (IN_KIND == 1)
-                    raw8[j] = *reinterpret_cast<const uint32_t*>(static_cast<const uint8_t*>(in) + c.xb + t0);
-                else
-                    rawf[j] = *reinterpret_cast<const float4a*>(static_cast<const float*>(in) + c.xb + t0);
+            if constexpr (VEC) {
+                if (t0 >= 0 && t0 + 4 <= nsamps) {
+                    if constexpr (IN_KIND == 1)
+                        raw8[j] = *reinterpret_cast<const uint32_t*>(static_cast<const uint8_t*>(in) + c.xb + t0);
+                    else
+                        rawf[j] = *reinterpret_cast<const float4a*>(static_cast<const float*>(in) + c.xb + t0);
+                } else {
+                    raw8[j] = 0u;
+                    rawf[j] = float4a{0.0f, 0.0f, 0.0f, 0.0f};
+                }
             } else {
-                raw8[j] = 0u;
-                rawf[j] = float4a{0.0f, 0.0f, 0.0f, 0.0f};
+                uint32_t w8 = 0u;
+                float4a wf = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int64_t t = t0 + e;
+                    if (t >= 0 && t < nsamps) {
+                        if constexpr (IN_KIND == 1)
+                            w8 |= (uint32_t)static_cast<const uint8_t*>(in)[c.xb + t] << (8 * e);
+                        else
+                            wf[e] = static_cast<const float*>(in)[c.xb + t];
+                    }
+                }
+                raw8[j] = w8;
+                rawf[j] = wf;
             }
         }
     };
@@ -984,12 +1005,19 @@ __global__ void __launch_bounds__(kThreads) k_merge_group_init(const void* __res
                 v = norm4(v, c.mu, c.sg, c.rsg);
                 if (!c.keep) v = float4a{0.0f, 0.0f, 0.0f, 0.0f};
             }
-            const bool inside = t0 >= 0 && t0 + 4 <= nsamps;
             float4a r0;
-            r0.x = inside ? 0.0f + v.x : 0.0f;
-            r0.y = inside ? 0.0f + v.y : 0.0f;
-            r0.z = inside ? 0.0f + v.z : 0.0f;
-            r0.w = inside ? 0.0f + v.w : 0.0f;
+            if constexpr (VEC) {
+                const bool inside = t0 >= 0 && t0 + 4 <= nsamps;
+                r0.x = inside ? 0.0f + v.x : 0.0f;
+                r0.y = inside ? 0.0f + v.y : 0.0f;
+                r0.z = inside ? 0.0f + v.z : 0.0f;
+                r0.w = inside ? 0.0f + v.w : 0.0f;
+            } else {
+                r0.x = (t0 + 0 >= 0 && t0 + 0 < nsamps) ? 0.0f + v.x : 0.0f;
+                r0.y = (t0 + 1 >= 0 && t0 + 1 < nsamps) ? 0.0f + v.y : 0.0f;
+                r0.z = (t0 + 2 >= 0 && t0 + 2 < nsamps) ? 0.0f + v.z : 0.0f;
+                r0.w = (t0 + 3 >= 0 && t0 + 3 < nsamps) ? 0.0f + v.w : 0.0f;
+            }
             lds_store4(lds + c.row_off + 4 * sq[j], r0);
         }
     };
@@ -1293,6 +1321,7 @@ struct GroupInitLaunch {
     const uint8_t* unit_chan;
     int tiles_per_block;
     int64_t nsamps;
+    bool vec;
 };
 
 template <int L, int IN_KIND, bool NORM, bool FAST>
@@ -1300,10 +1329,16 @@ static void group_init_launch(const GroupInitLaunch& a, int64_t lo, int64_t hi, 
     if (hi <= lo) return;
     const int64_t nb = a.ngroups * ceil_div(hi - lo, (int64_t)tpb);
     if (nb > 0x7fffffffLL) throw Error(SPS_ERR_INVALID_ARGUMENT, "merge grid too large");
-    hipLaunchKernelGGL((k_merge_group_init<L, IN_KIND, NORM, FAST>), dim3((uint32_t)nb), dim3(kThreads), a.lds, st,
-                       a.in, a.in_pitch, a.norm, a.out, a.out_pitch, static_cast<const GroupDescD*>(a.groups),
-                       a.ngroups, static_cast<const ChanWinD*>(a.chans), static_cast<const GRowDesc<L>*>(a.rows),
-                       a.unit_chan, tpb, a.nsamps, lo, hi);
+    if (a.vec)
+        hipLaunchKernelGGL((k_merge_group_init<L, IN_KIND, NORM, FAST, true>), dim3((uint32_t)nb), dim3(kThreads),
+                           a.lds, st, a.in, a.in_pitch, a.norm, a.out, a.out_pitch,
+                           static_cast<const GroupDescD*>(a.groups), a.ngroups, static_cast<const ChanWinD*>(a.chans),
+                           static_cast<const GRowDesc<L>*>(a.rows), a.unit_chan, tpb, a.nsamps, lo, hi);
+    else
+        hipLaunchKernelGGL((k_merge_group_init<L, IN_KIND, NORM, FAST, false>), dim3((uint32_t)nb), dim3(kThreads),
+                           a.lds, st, a.in, a.in_pitch, a.norm, a.out, a.out_pitch,
+                           static_cast<const GroupDescD*>(a.groups), a.ngroups, static_cast<const ChanWinD*>(a.chans),
+                           static_cast<const GRowDesc<L>*>(a.rows), a.unit_chan, tpb, a.nsamps, lo, hi);
 }
 
 // the tile split of group_dispatch
@@ -1333,11 +1368,10 @@ void launch_fdmt_group_init(int L, const void* in, int in_u8, int64_t in_pitch, 
                             const void* d_rows, const uint8_t* d_unit_chan, int lds_floats_max, int tiles_per_block,
                             int64_t nsamps, int64_t max_shift, hipStream_t st) {
     if (ngroups <= 0 || nsamps <= 0) return;
-    if (nsamps % 4 != 0 || in_pitch % 4 != 0)
-        throw Error(SPS_ERR_INVALID_ARGUMENT, "the fused init reads whole 4-sample chunks");
+    if (in_pitch < nsamps) throw Error(SPS_ERR_INVALID_ARGUMENT, "waterfall pitch below its length");
     const uintptr_t ia = reinterpret_cast<uintptr_t>(in);
-    if (in_u8 ? ia % 4 != 0 : ia % 16 != 0)
-        throw Error(SPS_ERR_INVALID_ARGUMENT, "the fused init needs an aligned waterfall");
+    // whole-chunk loads need 4-sample multiples and an aligned waterfall
+    const bool vec = nsamps % 4 == 0 && in_pitch % 4 == 0 && (in_u8 ? ia % 4 == 0 : ia % 16 == 0);
     if (lds_floats_max <= 0 || lds_floats_max > 16384 + 64)
         throw Error(SPS_ERR_INVALID_ARGUMENT, "staged pass LDS window out of range");
     if (tiles_per_block < 1) throw Error(SPS_ERR_INVALID_ARGUMENT, "tiles per block must be positive");
@@ -1356,6 +1390,7 @@ void launch_fdmt_group_init(int L, const void* in, int in_u8, int64_t in_pitch, 
     a.unit_chan = d_unit_chan;
     a.tiles_per_block = tiles_per_block;
     a.nsamps = nsamps;
+    a.vec = vec;
     const int64_t fast_lo = ceil_div(max_shift, (int64_t)kInitGroupTile);
     const bool u8 = in_u8 != 0, nrm = norm != nullptr;
     switch (L) {

@@ -147,8 +147,9 @@ void launch_fdmt_group(int L, const float* in, int64_t in_pitch, float* out, int
                        int64_t max_shift, hipStream_t st, const MergeEpilogue* snr = nullptr);
 
 // First staged pass with the (mean) init fused in: reads the waterfall (f32 or
-// u8, optional per-channel normalisation) instead of state 0; nsamps and
-// in_pitch multiples of 4.  max_shift: as for launch_fdmt_group.
+// u8, optional per-channel normalisation) instead of state 0; whole 4-sample
+// chunk loads when nsamps, in_pitch and the waterfall's alignment allow,
+// sample loads otherwise.  max_shift: as for launch_fdmt_group.
 void launch_fdmt_group_init(int L, const void* in, int in_u8, int64_t in_pitch, const ChanNorm* norm, float* out,
                             int64_t out_pitch, const void* d_groups, int64_t ngroups, const void* d_chans,
                             const void* d_rows, const uint8_t* d_unit_chan, int lds_floats_max, int tiles_per_block,
