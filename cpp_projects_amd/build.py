@@ -27,7 +27,8 @@ HIP_FLAGS = [
 ]
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-I", os.path.join(ROOT, "include")]
 
-HIP_SOURCES = ["fdmt_kernels.hip", "search_kernels.hip", "sort_kernels.hip", "detect_noalloc.hip"]
+HIP_SOURCES = ["fdmt_kernels.hip", "search_kernels.hip", "sort_kernels.hip", "detect_noalloc.hip",
+               "unpack_kernels.hip"]
 # per-source additions: the boxcar's multi-tile kernel loops over tiles, and
 # machine LICM hoists constants of the tile body out of that loop into
 # registers (112 VGPRs instead of 90: four waves per SIMD instead of five)

@@ -1,8 +1,9 @@
 // dropin.cpp -- C ABI entry points behind the reference's free functions
 // that sit beside the search pipeline: delegate selection over caller groups
 // and its log (select_delegates::), row cleaning without normalisation
-// (statistical_preprocessing::clean_fdmtInput_and_create_normInput) and the
-// tiled single-maximum detectors (doDetectSignal_v1 / _v2 / _v2_2).
+// (statistical_preprocessing::clean_fdmtInput_and_create_normInput), the
+// tiled single-maximum detectors (doDetectSignal_v1 / _v2 / _v2_2) and the
+// 8-bit GUPPI / LOFAR raw-voltage unpack of the sessions' unpack_chunk.
 #include <fstream>
 #include <iomanip>
 #include <vector>
@@ -85,6 +86,22 @@ SPS_API sps_status sps_detect_signal(int version, const float* d_arr, const floa
         SPS_REQUIRE(ceil_div(cols, tile) * rows <= ntiles_cap, "per-tile arrays hold fewer than rows * ceil(cols/tile)");
         launch_detect_tiles(version, d_arr, d_norm, rows, cols, tile, wnd, d_tile_val, d_tile_pos, d_tile_w, d_out,
                             as_stream(stream));
+    });
+}
+
+SPS_API sps_status sps_unpack_guppi(const int8_t* d_in, int64_t nsamp, int noverlap, int nbin, int nfft, int npol,
+                                    int nchan, float* d_out, void* stream) {
+    return guarded([&] {
+        SPS_REQUIRE(d_in && d_out, "null argument");
+        launch_unpack_guppi(d_in, nsamp, noverlap, nbin, nfft, npol, nchan, d_out, as_stream(stream));
+    });
+}
+
+SPS_API sps_status sps_unpack_lofar(const int8_t* d_in, int64_t nsamp, int noverlap, int nbin, int nfft, int npol,
+                                    int nchan, float* d_out, void* stream) {
+    return guarded([&] {
+        SPS_REQUIRE(d_in && d_out, "null argument");
+        launch_unpack_lofar(d_in, nsamp, noverlap, nbin, nfft, npol, nchan, d_out, as_stream(stream));
     });
 }
 

@@ -236,6 +236,12 @@ void launch_delegate_out(const unsigned long long* best, const uint32_t* start, 
 void launch_select_delegates(const CandD* c, const int32_t* idx, int64_t nidx, const int32_t* begin, int64_t ngroups,
                              int32_t* out, hipStream_t st);
 void launch_iota_u32(uint32_t* v, int64_t n, hipStream_t st);
+// 8-bit raw voltages -> overlapping complex FFT blocks out[pol][fft][chan][bin]
+// (re/im float pairs; GUPPI input [chan][samp][npol], LOFAR [npol][samp][chan])
+void launch_unpack_guppi(const int8_t* in, int64_t nsamp, int noverlap, int nbin, int nfft, int npol, int nchan,
+                         float* out, hipStream_t st);
+void launch_unpack_lofar(const int8_t* in, int64_t nsamp, int noverlap, int nbin, int nfft, int npol, int nchan,
+                         float* out, hipStream_t st);
 void launch_fill_u64(unsigned long long* v, int64_t n, unsigned long long val, hipStream_t st);
 
 }  // namespace sps

@@ -247,6 +247,25 @@ SPS_API sps_status sps_write_log(const sps_cand* d_cands, int64_t ncand, const i
                                  const int64_t* h_group_sizes, int64_t ndel, const char* filename, float delt_bin,
                                  float t_chunk_begin);
 
+/* 8-bit raw-voltage unpack into overlapping complex FFT blocks, the input
+ * side of the coherent front-end: d_out[pol][fft][chan][bin] (pol < npol/2,
+ * re/im float pairs, nfft * nchan * nbin * npol complex-halves in all) takes
+ * sample isamp = bin + (nbin - 2*noverlap)*fft - noverlap of the chunk, 0 + 0i
+ * outside [0, nsamp); signed bytes promote exactly to float.  npol is 2 or 4
+ * (re/im per polarisation), nbin > 2*noverlap.
+ *   sps_unpack_guppi: d_in[chan][nsamp][npol] -- CSession_guppi_gpu::
+ *     unpack_chunk (CDMT/Session_guppi_gpu.cu:53-60) and its kernel
+ *     unpack_chunk_guppi_gpu (:116-140; unpack_guppi/kernel.cu:161-187);
+ *   sps_unpack_lofar: d_in[npol][nsamp][nchan] (component 2p is pol p's
+ *     real part, 2p + 1 its imaginary part) -- CSession_lofar_gpu::
+ *     unpack_chunk (CDMT/Session_lofar_gpu.cu:54-67) and unpackInput_lofar
+ *     (:144-175; unpack_guppi/kernel.cu:81-122).
+ * Asynchronous on the stream. */
+SPS_API sps_status sps_unpack_guppi(const int8_t* d_in, int64_t nsamp, int noverlap, int nbin, int nfft, int npol,
+                                    int nchan, float* d_out, void* stream);
+SPS_API sps_status sps_unpack_lofar(const int8_t* d_in, int64_t nsamp, int noverlap, int nbin, int nfft, int npol,
+                                    int nchan, float* d_out, void* stream);
+
 /* statistical_preprocessing::clean_fdmtInput_and_create_normInput
  * (CDMT/Statistical_preprocessing.cuh:6-7, .cu:102-120,142-268): rows whose
  * variance lies more than 4 standard deviations from the mean row variance

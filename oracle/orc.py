@@ -79,6 +79,8 @@ def lib():
         L.orc_detect_max.argtypes = [_vp, _vp, _i64, _i64, C.c_int, _vp, C.POINTER(_f32)]
         L.orc_select_delegates.argtypes = [_vp, _vp, _i64, _vp, _i64, _vp]
         L.orc_detect_tiles.argtypes = [C.c_int, _vp, _vp, _i64, _i64, C.c_int, C.c_int, _vp, C.POINTER(_f32)]
+        L.orc_unpack_guppi.argtypes = [_vp, _i64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp]
+        L.orc_unpack_lofar.argtypes = [_vp, _i64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp]
         L.orc_num_threads.restype = C.c_int
         L.orc_set_num_threads.argtypes = [C.c_int]
         _lib = L
@@ -306,6 +308,18 @@ def select_delegates(cands, indices, group_begin):
     out = np.zeros(max(b.size, 1), np.int32)
     lib().orc_select_delegates(_ptr(c), _ptr(idx), idx.size, _ptr(b), b.size, _ptr(out))
     return out[: b.size]
+
+
+def unpack(layout, raw, nsamp, noverlap, nbin, nfft, npol, nchan):
+    """8-bit raw voltages -> complex64 blocks [pol][fft][chan][bin]
+    (unpack_guppi/kernel.cu:81-122 LOFAR, :161-187 GUPPI)."""
+    x = np.ascontiguousarray(raw, np.int8).reshape(-1)
+    if x.size < nsamp * nchan * npol:
+        raise ValueError("raw chunk too short")
+    out = np.zeros((npol // 2, nfft, nchan, nbin), np.complex64)
+    f = lib().orc_unpack_guppi if layout == "guppi" else lib().orc_unpack_lofar
+    f(_ptr(x), nsamp, noverlap, nbin, nfft, npol, nchan, _ptr(out))
+    return out
 
 
 def detect_tiles(mode, a, norm, w, tile):

@@ -38,6 +38,12 @@
  *                                  (closed forms of its maxima), FP64 window
  *                                  sums within the prefix-difference error
  *                                  bound on random input.
+ *   GUPPI / LOFAR unpack         : the reference's own example of
+ *                                  unpack_guppi/kernel.cu:192-216 (input
+ *                                  1 + i) against the closed form of the
+ *                                  index maps; every in-chunk sample appears
+ *                                  in each block that covers it, all else is
+ *                                  zero; signed-byte promotion.
  * Parity unpinned (no reference test or fixture fixes them): the reference
  * GPU's fast-math bit patterns, CUB/thrust tie order for equal keys and
  * atomicInc emission order; this restatement defines deterministic rules.
@@ -1119,4 +1125,52 @@ ORC_EXPORT int orc_detect_max(const float* a, const float* nrm, int64_t R, int64
     out3[0] = bw; out3[1] = br; out3[2] = bc;
     *snr = best;
     return 0;
+}
+
+/* 8-bit raw-voltage unpack into overlapping complex FFT blocks
+ * out[pol][fft][chan][bin] (re, im interleaved), sample isamp = bin +
+ * (nbin - 2*noverlap)*fft - noverlap, zero outside [0, nsamp): the GPU
+ * kernels' semantics (zero fill; the host loops of unpack_guppi/kernel.cu
+ * fill the same values but leave LOFAR's out-of-chunk entries unwritten).
+ * GUPPI input in[chan][isamp][npol] (unpack_chunk_guppi_gpu,
+ * unpack_guppi/kernel.cu:161-187 and CDMT/Session_guppi_gpu.cu:116-140;
+ * host form unpack_chunk_guppi :124-156). */
+ORC_EXPORT void orc_unpack_guppi(const int8_t* in, int64_t nsamp, int noverlap, int nbin, int nfft, int npol,
+                                 int nchan, float* out) {
+    for (int ichan = 0; ichan < nchan; ++ichan)
+        for (int ifft = 0; ifft < nfft; ++ifft)
+            for (int ibin = 0; ibin < nbin; ++ibin) {
+                const int64_t isamp = ibin + (int64_t)(nbin - 2 * noverlap) * ifft - noverlap;
+                for (int ipol = 0; ipol < npol / 2; ++ipol) {
+                    const int64_t o = 2 * ((((int64_t)ipol * nfft + ifft) * nchan + ichan) * nbin + ibin);
+                    out[o] = 0.0f;
+                    out[o + 1] = 0.0f;
+                    if (isamp >= 0 && isamp < nsamp) {
+                        const int64_t k = ((int64_t)ichan * nsamp + isamp) * npol + 2 * ipol;
+                        out[o] = (float)in[k];
+                        out[o + 1] = (float)in[k + 1];
+                    }
+                }
+            }
+}
+
+/* LOFAR input in[component][isamp][chan], pol p's (re, im) in components
+ * 2p, 2p + 1 (unpackInput_lofar, unpack_guppi/kernel.cu:81-122 and
+ * CDMT/Session_lofar_gpu.cu:144-175; host form unpack_chunk_lofar :42-75). */
+ORC_EXPORT void orc_unpack_lofar(const int8_t* in, int64_t nsamp, int noverlap, int nbin, int nfft, int npol,
+                                 int nchan, float* out) {
+    for (int ipol = 0; ipol < npol / 2; ++ipol)
+        for (int ifft = 0; ifft < nfft; ++ifft)
+            for (int isub = 0; isub < nchan; ++isub)
+                for (int ibin = 0; ibin < nbin; ++ibin) {
+                    const int64_t isamp = ibin + (int64_t)(nbin - 2 * noverlap) * ifft - noverlap;
+                    const int64_t o = 2 * ((((int64_t)ipol * nfft + ifft) * nchan + isub) * nbin + ibin);
+                    out[o] = 0.0f;
+                    out[o + 1] = 0.0f;
+                    if (isamp >= 0 && isamp < nsamp) {
+                        const int64_t k = isamp * nchan + isub;
+                        out[o] = (float)in[(2 * (int64_t)ipol) * nsamp * nchan + k];
+                        out[o + 1] = (float)in[(2 * (int64_t)ipol + 1) * nsamp * nchan + k];
+                    }
+                }
 }
