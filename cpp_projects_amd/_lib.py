@@ -88,6 +88,8 @@ SIGNATURES = {
     "sps_select_delegates": [_vp, _vp, _i64, _vp, _i64, _vp, _vp],
     "sps_write_log": [_vp, _i64, _vp, _vp, _i64, C.c_char_p, _f32, _f32],
     "sps_clean_fdmt_input": [_vp, _i64, _i64, _vp, _vp, _vp],
+    "sps_scan_inclusive_i32": [_vp, _vp, _i64, _vp],
+    "sps_scan_inclusive_f32": [_vp, _vp, _i64, _vp],
     "sps_unpack_guppi": [_vp, _i64, _int, _int, _int, _int, _int, _vp, _vp],
     "sps_unpack_lofar": [_vp, _i64, _int, _int, _int, _int, _int, _vp, _vp],
     "sps_detect_signal": [_int, _vp, _vp, _i64, _i64, _int, _int, _vp, _vp, _vp, _i64, _vp, _vp],

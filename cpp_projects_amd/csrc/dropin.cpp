@@ -2,10 +2,12 @@
 // that sit beside the search pipeline: delegate selection over caller groups
 // and its log (select_delegates::), row cleaning without normalisation
 // (statistical_preprocessing::clean_fdmtInput_and_create_normInput), the
-// tiled single-maximum detectors (doDetectSignal_v1 / _v2 / _v2_2) and the
-// 8-bit GUPPI / LOFAR raw-voltage unpack of the sessions' unpack_chunk.
+// tiled single-maximum detectors (doDetectSignal_v1 / _v2 / _v2_2), the
+// prefix sums of cumsum_customized and the 8-bit GUPPI / LOFAR raw-voltage
+// unpack of the sessions' unpack_chunk.
 #include <fstream>
 #include <iomanip>
+#include <type_traits>
 #include <vector>
 
 #include "abi.hpp"
@@ -13,6 +15,20 @@
 #include "kernels.hpp"
 
 using namespace sps;
+
+// inclusive prefix sums: d_out (may be d_in) over n values; the scratch is
+// the library's and released before returning (the call synchronises)
+template <typename T, typename S>
+static void scan_entry(const T* d_in, T* d_out, int64_t n, hipStream_t st, S scan) {
+    SPS_REQUIRE(n >= 0, "negative length");
+    if (n == 0) return;
+    SPS_REQUIRE(d_in && d_out, "null argument");
+    if (d_out != d_in) SPS_HIP(hipMemcpyAsync(d_out, d_in, sizeof(T) * (size_t)n, hipMemcpyDeviceToDevice, st));
+    DevBuf scratch;
+    scratch.ensure(sizeof(T) * (size_t)scan_scratch_entries(n));
+    scan(d_out, n, scratch.as<typename std::remove_const<T>::type>(), st);
+    SPS_HIP(hipStreamSynchronize(st));
+}
 
 extern "C" {
 
@@ -87,6 +103,18 @@ SPS_API sps_status sps_detect_signal(int version, const float* d_arr, const floa
         launch_detect_tiles(version, d_arr, d_norm, rows, cols, tile, wnd, d_tile_val, d_tile_pos, d_tile_w, d_out,
                             as_stream(stream));
     });
+}
+
+SPS_API sps_status sps_scan_inclusive_i32(const int32_t* d_in, int32_t* d_out, int64_t n, void* stream) {
+    return guarded([&] {
+        // two's-complement sums are the uint32 sums of the same bits
+        scan_entry<uint32_t>(reinterpret_cast<const uint32_t*>(d_in), reinterpret_cast<uint32_t*>(d_out), n,
+                             as_stream(stream), scan_inclusive_u32);
+    });
+}
+
+SPS_API sps_status sps_scan_inclusive_f32(const float* d_in, float* d_out, int64_t n, void* stream) {
+    return guarded([&] { scan_entry<float>(d_in, d_out, n, as_stream(stream), scan_inclusive_f32); });
 }
 
 SPS_API sps_status sps_unpack_guppi(const int8_t* d_in, int64_t nsamp, int noverlap, int nbin, int nfft, int npol,

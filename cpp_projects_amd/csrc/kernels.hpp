@@ -216,6 +216,9 @@ void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_tmp, uint32
 int64_t radix_hist_entries(int64_t n);
 // inclusive scan of n uint32 values in place; scratch of scan_scratch_entries(n)
 void scan_inclusive_u32(uint32_t* data, int64_t n, uint32_t* scratch, hipStream_t st);
+// the same for floats: per-thread sums of 4 in order, a block tree over them,
+// block prefixes added last (deterministic, not the sequential order)
+void scan_inclusive_f32(float* data, int64_t n, float* scratch, hipStream_t st);
 int64_t scan_scratch_entries(int64_t n);
 
 void launch_cand_keys(const CandD* c, int64_t n, int bits_t, int bits_w, uint64_t* keys, uint32_t* vals,

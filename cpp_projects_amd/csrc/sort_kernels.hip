@@ -95,33 +95,37 @@ __global__ void __launch_bounds__(kThreads) k_radix_scatter(const uint64_t* __re
 constexpr int kScanItems = 4;
 constexpr int kScanBlock = kThreads * kScanItems;
 
-__device__ uint32_t block_excl_scan(uint32_t v, uint32_t* sh, uint32_t* total) {
+// exclusive prefix of v over the block (Hillis-Steele over the thread
+// values; the exclusive value is the previous thread's inclusive one, so no
+// subtraction enters a float scan)
+template <typename T>
+__device__ T block_excl_scan(T v, T* sh, T* total) {
     sh[threadIdx.x] = v;
     __syncthreads();
     for (int off = 1; off < kThreads; off <<= 1) {
-        const uint32_t add = (int)threadIdx.x >= off ? sh[threadIdx.x - off] : 0u;
+        const T add = (int)threadIdx.x >= off ? sh[threadIdx.x - off] : T(0);
         __syncthreads();
         sh[threadIdx.x] += add;
         __syncthreads();
     }
-    const uint32_t incl = sh[threadIdx.x];
+    const T excl = threadIdx.x > 0 ? sh[threadIdx.x - 1] : T(0);
     *total = sh[kThreads - 1];
     __syncthreads();
-    return incl - v;
+    return excl;
 }
 
-__global__ void __launch_bounds__(kThreads) k_scan_local(uint32_t* __restrict__ data, int64_t n,
-                                                         uint32_t* __restrict__ block_sums) {
-    __shared__ uint32_t sh[kThreads];
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_scan_local(T* __restrict__ data, int64_t n, T* __restrict__ block_sums) {
+    __shared__ T sh[kThreads];
     const int64_t base = (int64_t)blockIdx.x * kScanBlock + (int64_t)threadIdx.x * kScanItems;
-    uint32_t x[kScanItems];
-    uint32_t s = 0;
+    T x[kScanItems];
+    T s = T(0);
     for (int e = 0; e < kScanItems; ++e) {
-        x[e] = (base + e < n) ? data[base + e] : 0u;
+        x[e] = (base + e < n) ? data[base + e] : T(0);
         s += x[e];
     }
-    uint32_t total;
-    uint32_t pre = block_excl_scan(s, sh, &total);
+    T total;
+    T pre = block_excl_scan<T>(s, sh, &total);
     for (int e = 0; e < kScanItems; ++e) {
         pre += x[e];
         if (base + e < n) data[base + e] = pre;
@@ -129,10 +133,11 @@ __global__ void __launch_bounds__(kThreads) k_scan_local(uint32_t* __restrict__ 
     if (threadIdx.x == 0 && block_sums) block_sums[blockIdx.x] = total;
 }
 
-__global__ void __launch_bounds__(kThreads) k_scan_add(uint32_t* __restrict__ data, int64_t n,
-                                                       const uint32_t* __restrict__ block_prefix) {
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_scan_add(T* __restrict__ data, int64_t n,
+                                                       const T* __restrict__ block_prefix) {
     if (blockIdx.x == 0) return;
-    const uint32_t add = block_prefix[blockIdx.x - 1];
+    const T add = block_prefix[blockIdx.x - 1];
     const int64_t base = (int64_t)blockIdx.x * kScanBlock;
     for (int e = threadIdx.x; e < kScanBlock; e += kThreads)
         if (base + e < n) data[base + e] += add;
@@ -302,19 +307,29 @@ int64_t scan_scratch_entries(int64_t n) {
     return tot + 1;
 }
 
-void scan_inclusive_u32(uint32_t* data, int64_t n, uint32_t* scratch, hipStream_t st) {
+template <typename T>
+static void scan_inclusive(T* data, int64_t n, T* scratch, hipStream_t st) {
     if (n <= 0) return;
     const int64_t nb = ceil_div(n, kScanBlock);
+    if (nb > 0x7fffffffLL) throw Error(SPS_ERR_INVALID_ARGUMENT, "scan too long");
     if (nb == 1) {
-        hipLaunchKernelGGL(k_scan_local, dim3(1), dim3(kThreads), 0, st, data, n, (uint32_t*)nullptr);
+        hipLaunchKernelGGL(k_scan_local<T>, dim3(1), dim3(kThreads), 0, st, data, n, (T*)nullptr);
         SPS_HIP(hipGetLastError());
         return;
     }
-    uint32_t* sums = scratch;
-    hipLaunchKernelGGL(k_scan_local, dim3((uint32_t)nb), dim3(kThreads), 0, st, data, n, sums);
-    scan_inclusive_u32(sums, nb, scratch + nb, st);
-    hipLaunchKernelGGL(k_scan_add, dim3((uint32_t)nb), dim3(kThreads), 0, st, data, n, (const uint32_t*)sums);
+    T* sums = scratch;
+    hipLaunchKernelGGL(k_scan_local<T>, dim3((uint32_t)nb), dim3(kThreads), 0, st, data, n, sums);
+    scan_inclusive<T>(sums, nb, scratch + nb, st);
+    hipLaunchKernelGGL(k_scan_add<T>, dim3((uint32_t)nb), dim3(kThreads), 0, st, data, n, (const T*)sums);
     SPS_HIP(hipGetLastError());
+}
+
+void scan_inclusive_u32(uint32_t* data, int64_t n, uint32_t* scratch, hipStream_t st) {
+    scan_inclusive<uint32_t>(data, n, scratch, st);
+}
+
+void scan_inclusive_f32(float* data, int64_t n, float* scratch, hipStream_t st) {
+    scan_inclusive<float>(data, n, scratch, st);
 }
 
 int64_t radix_hist_entries(int64_t n) {

@@ -58,6 +58,19 @@ class statistical_preprocessing:
         return 1 << k
 
 
+def fnc_cumsum_customized_gpu(d_in, d_buff, length, d_out):
+    """Inclusive int32 prefix sums (cumsum_customized.cu:158-179); d_buff is
+    not used (the scratch is the library's)."""
+    call("sps_scan_inclusive_i32", _dptr(d_in), _dptr(d_out), int(length), _stream())
+
+
+def inclusiveScan_CUB(d_arr, length):
+    """In-place inclusive prefix sums of an int32 or float32 tensor
+    (cumsum_customized.cu:259-305)."""
+    name = "sps_scan_inclusive_f32" if d_arr.dtype.is_floating_point else "sps_scan_inclusive_i32"
+    call(name, _dptr(d_arr), _dptr(d_arr), int(length), _stream())
+
+
 def fdmt_normalization(d_arr, d_norm, length, d_out):
     """fdmt_normalization (CDMT/Chunk_gpu.cu:841-852)."""
     call("sps_fdmt_normalization", _dptr(d_arr), _dptr(d_norm), int(length), _dptr(d_out), _stream())

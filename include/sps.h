@@ -247,6 +247,17 @@ SPS_API sps_status sps_write_log(const sps_cand* d_cands, int64_t ncand, const i
                                  const int64_t* h_group_sizes, int64_t ndel, const char* filename, float delt_bin,
                                  float t_chunk_begin);
 
+/* Inclusive prefix sums over n values into d_out (may equal d_in):
+ * fnc_cumsum_customized_gpu / calc_cumsum_custom_gpu and inclusiveScan_CUB
+ * (Clusterization/cumsum_customized.cu:147-179,259-305).  int32 sums wrap
+ * like two's-complement addition (bit-exact with any order); float sums are
+ * formed in a fixed blocked order (4 values per thread, a tree over the
+ * block, block prefixes added last), deterministic but not the reference's
+ * CUB order: equal to the sequential sums within the error of a sum of n
+ * terms.  The scratch is allocated and released inside; synchronises. */
+SPS_API sps_status sps_scan_inclusive_i32(const int32_t* d_in, int32_t* d_out, int64_t n, void* stream);
+SPS_API sps_status sps_scan_inclusive_f32(const float* d_in, float* d_out, int64_t n, void* stream);
+
 /* 8-bit raw-voltage unpack into overlapping complex FFT blocks, the input
  * side of the coherent front-end: d_out[pol][fft][chan][bin] (pol < npol/2,
  * re/im float pairs, nfft * nchan * nbin * npol complex-halves in all) takes

@@ -5,8 +5,9 @@
 // (fdmt_normalization), Clusterization/Candidate.cuh:4-10 (Cand),
 // Clusterization/Clusterization.cuh:8-68 (clusterization::),
 // Clusterization/select_delegates.cuh:4-26 (select_delegates::; the
-// thrust::device_vector forms are in sps/thrust_vectors.hpp) and the
-// detectors of detection/kernel.cu:14-20,561-640.  All array pointers are HIP
+// thrust::device_vector forms are in sps/thrust_vectors.hpp), the prefix
+// sums of Clusterization/cumsum_customized.cuh:11-32 and the detectors of
+// detection/kernel.cu:14-20,561-640.  All array pointers are HIP
 // device pointers unless named h_; thresholds and bin metrics are references
 // to device memory like the reference's (Chunk_gpu.cu:373-425).
 #pragma once
@@ -130,6 +131,50 @@ inline void gather_candidates_in_dynamicalArray(const float* d_arr_fdmt, const i
     if (n > 0)
         sps_detail::check(sps_boxcar_candidates(d_arr_fdmt, Rows, Cols, &val_thresh, QUantPower2_Wnd, 0, *d_arrCand, n,
                                                 &n, nullptr));
+}
+
+// Clusterization.cuh:38-40 (Clusterization.cu:218-263), a host call here (the
+// reference launches the kernel with Rows = gridDim.y): widths 1,2,..,
+// 2^QUantPower2_Wnd, mwidth = log2(w), into the caller's array of
+// *d_pimax_candidates_per_chunk entries (device memory); d_quantCand (device)
+// receives the number found.  The reference's atomicInc wraps past the
+// capacity and overwrites earlier entries; here the array holds the first
+// capacity candidates in (mdt, mt, mwidth) order and the count may exceed the
+// capacity.  Synchronises.
+inline void gather_candidates_in_fixedArray_kernel_v0(const float* d_arr_fdmt, const int Rows, const int Cols,
+                                                      const float& val_thresh, const int QUantPower2_Wnd,
+                                                      unsigned int* d_pimax_candidates_per_chunk, Cand* d_arrCand,
+                                                      unsigned int& d_quantCand) {
+    unsigned int cap = 0;
+    if (hipMemcpy(&cap, d_pimax_candidates_per_chunk, sizeof(cap), hipMemcpyDeviceToHost) != hipSuccess)
+        throw std::runtime_error("hipMemcpy failed");
+    int64_t n = 0;
+    sps_status s = sps_boxcar_candidates(d_arr_fdmt, Rows, Cols, &val_thresh, QUantPower2_Wnd, 0, d_arrCand, cap, &n,
+                                         nullptr);
+    if (s != SPS_OK && s != SPS_ERR_CAPACITY) sps_detail::check(s);
+    const unsigned int n32 = (unsigned int)n;
+    if (hipMemcpy(&d_quantCand, &n32, sizeof(n32), hipMemcpyHostToDevice) != hipSuccess)
+        throw std::runtime_error("hipMemcpy failed");
+}
+
+// Clusterization/cumsum_customized.cu:147-179,259-305: inclusive prefix sums
+// (the buffer and block-size arguments of the reference's two-pass forms are
+// not needed; the scratch is the library's).  Synchronise.
+inline void fnc_cumsum_customized_gpu(int* d_arrInput, int* d_arr_buff, const int LEnarr, int* d_arrOutput) {
+    (void)d_arr_buff;
+    sps_detail::check(sps_scan_inclusive_i32(d_arrInput, d_arrOutput, LEnarr, nullptr));
+}
+inline void calc_cumsum_custom_gpu(int* d_arrInput, int* d_arr_buff, const int LEnarr, const int treads_per_block,
+                                   int* d_arrOutput) {
+    (void)d_arr_buff;
+    (void)treads_per_block;
+    sps_detail::check(sps_scan_inclusive_i32(d_arrInput, d_arrOutput, LEnarr, nullptr));
+}
+inline void inclusiveScan_CUB(float* d_arr, const int LEn) {
+    sps_detail::check(sps_scan_inclusive_f32(d_arr, d_arr, LEn, nullptr));
+}
+inline void inclusiveScan_CUB(int* d_arr, const int LEn) {
+    sps_detail::check(sps_scan_inclusive_i32(d_arr, d_arr, LEn, nullptr));
 }
 
 // Clusterization.cuh:56-63: widths 1..WndWidth, mwidth = w; false when none.

@@ -178,6 +178,32 @@ int main(int argc, char** argv) {
         clusterization::gather_candidates_in_heap_with_plan(d_img, 3, 16, *d_thr, 2, &d_c2, &n2);
         clusterization::writeCandDeviceArrayToCSV(d_c2, (int)n2, out + "/Candidates2.csv");
         printf("candidates %u %u\n", n1, n2);
+        // the fixed-array gather into the caller's array: room for all, then for 2
+        {
+            unsigned int* d_cap = nullptr;
+            unsigned int* d_cnt = nullptr;
+            Cand* d_fix = nullptr;
+            HIP_OK(hipMalloc(&d_cap, 4));
+            HIP_OK(hipMalloc(&d_cnt, 4));
+            HIP_OK(hipMalloc(&d_fix, sizeof(Cand) * (n1 + 1)));
+            unsigned int caps[2] = {n1 + 1, 2};
+            std::ofstream f(out + "/fixed.txt");
+            for (unsigned int cap : caps) {
+                HIP_OK(hipMemcpy(d_cap, &cap, 4, hipMemcpyHostToDevice));
+                clusterization::gather_candidates_in_fixedArray_kernel_v0(d_img, 3, 16, *d_thr, 1, d_cap, d_fix,
+                                                                          *d_cnt);
+                unsigned int cnt = 0;
+                HIP_OK(hipMemcpy(&cnt, d_cnt, 4, hipMemcpyDeviceToHost));
+                std::vector<Cand> h(std::min(cnt, cap));
+                HIP_OK(hipMemcpy(h.data(), d_fix, sizeof(Cand) * h.size(), hipMemcpyDeviceToHost));
+                f << cnt;
+                for (const Cand& c : h) f << " " << c.mt << "," << c.mdt << "," << c.mwidth << "," << c.msnr;
+                f << "\n";
+            }
+            HIP_OK(hipFree(d_cap));
+            HIP_OK(hipFree(d_cnt));
+            HIP_OK(hipFree(d_fix));
+        }
 
         // select_delegates over caller groups (thrust::device_vector forms):
         // the 15-candidate example of select_delegates/select_delegates.cu:

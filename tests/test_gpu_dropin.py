@@ -65,6 +65,21 @@ def test_cpp_dropin_harness(torch_cuda, tmp_path):
     assert open(tmp_path / "Candidates2.csv").read().split() == open(os.path.join(gd, "Candidates2.csv")).read().split()
     got1 = sorted(open(tmp_path / "Candidates1.csv").read().split()[1:])
     assert got1 == sorted(open(os.path.join(gd, "Candidates1.csv")).read().split()[1:])
+    # gather_candidates_in_fixedArray_kernel_v0: the same candidates with room for all; with room
+    # for 2 the total count and the first 2 in (mdt, mt, mwidth) order
+    img = np.full((3, 16), 2.0, np.float32)  # clusterization_test.cu:63-81, as the harness builds it
+    img[0, 4], img[0, 11], img[2, 4] = 140.0, 120.0, 100.0
+    ref1 = orc.boxcar_pow2(img, 4.0, 1)
+    fx = open(tmp_path / "fixed.txt").read().splitlines()
+    for line, cap in zip(fx, (len(ref1) + 1, 2)):
+        parts = line.split()
+        assert int(parts[0]) == len(ref1)
+        got = [p.split(",") for p in parts[1:]]
+        exp = ref1[:cap]
+        assert len(got) == len(exp), cap
+        for g, e in zip(got, exp):  # S/N printed with 6 significant digits
+            assert [int(v) for v in g[:3]] == [int(e["mt"]), int(e["mdt"]), int(e["mwidth"])], cap
+            assert float(g[3]) == pytest.approx(float(e["msnr"]), rel=1e-5), cap
     # candidates.log, every column: delegates (4,0,1,140) (4,2,1,100) (11,0,1,120), 3 members each
     log = open(tmp_path / "candidates.log").read().splitlines()
     assert log == log_lines([[1, 4, "4e-05", 0, 1, 140, 3], [2, 4, "4e-05", 2, 1, 100, 3],
@@ -203,3 +218,36 @@ def test_tiled_detectors_vs_oracle(torch_cuda, mode):
         exp = orc.detect_tiles(mode, a, n, W, tile)
         assert got[:3] == exp[:3] and np.float32(got[3]) == np.float32(exp[3])
         assert hit == (got[3] >= 3.0)
+
+
+@pytest.mark.parametrize("n", [1, 1023, 1024, 1025, 5_000_003])
+def test_prefix_sums(torch_cuda, n):
+    """cumsum_customized: int32 sums bit-exact with a sequential cumsum (wrap
+    included); float sums within the error bound of an n-term sum, and exact
+    where every partial sum is an integer below 2^24."""
+    from cpp_projects_amd.search import fnc_cumsum_customized_gpu, inclusiveScan_CUB
+
+    torch = torch_cuda
+    rng = np.random.default_rng(n)
+    xi = rng.integers(-(1 << 30), 1 << 30, n, dtype=np.int32)
+    d = torch.from_numpy(xi).cuda()
+    out = torch.empty_like(d)
+    fnc_cumsum_customized_gpu(d, None, n, out)
+    inclusiveScan_CUB(d, n)  # in place
+    torch.cuda.synchronize()
+    ref = np.cumsum(xi, dtype=np.int32)
+    np.testing.assert_array_equal(out.cpu().numpy(), ref)
+    np.testing.assert_array_equal(d.cpu().numpy(), ref)
+    xf = rng.standard_normal(n).astype(np.float32)
+    df = torch.from_numpy(xf).cuda()
+    inclusiveScan_CUB(df, n)
+    torch.cuda.synchronize()
+    exact = np.cumsum(xf.astype(np.float64))
+    bound = n * np.finfo(np.float32).eps * np.cumsum(np.abs(xf.astype(np.float64)))
+    assert (np.abs(df.cpu().numpy() - exact) <= bound + 1e-30).all()
+    small = rng.integers(0, 3, n).astype(np.float32) if n < (1 << 23) else np.ones(n, np.float32)
+    ds = torch.from_numpy(small).cuda()
+    inclusiveScan_CUB(ds, n)
+    torch.cuda.synchronize()
+    if n < (1 << 23):
+        np.testing.assert_array_equal(ds.cpu().numpy(), np.cumsum(small.astype(np.int64)).astype(np.float32))
