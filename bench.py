@@ -337,7 +337,10 @@ def bench_large(args):
     s = SingleBeamSearch(CFG["f_min"], CFG["f_max"], R, N, L["dt_max"], cand_capacity=1 << 16, **mk)
     s.run(x, None, probe, count)
     torch.cuda.synchronize()
-    best, _ = tune_search(s, lambda: s.run(x, None, probe, count), reps=1)
+    if args.large_no_tune:  # the autotuner's earlier choice (profiling runs)
+        best = dict(workloads.LARGE_TUNED)
+    else:
+        best, _ = tune_search(s, lambda: s.run(x, None, probe, count), reps=1)
     log(f"[large] schedule {best}")
     cap = sized_capacity(int(count.item()))
     del s, probe
@@ -349,11 +352,12 @@ def bench_large(args):
     cands = torch.empty((cap, 4), dtype=torch.int32, device="cuda")
     s.run(x, None, cands, count)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.large_steps):
-        s.run(x, None, cands, count)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / args.large_steps
+    with Marker("large_steps"):
+        t0 = time.perf_counter()
+        for _ in range(args.large_steps):
+            s.run(x, None, cands, count)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.large_steps
     ncand = int(count.item())
     n_sorted = s.sort_candidates(cands, count)  # raises on overflow
     ndt = s.ndt
@@ -421,6 +425,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--batch-workers", type=int, default=4)
     ap.add_argument("--batch-only", action="store_true", help="run the batched multi-beam measurement alone")
+    ap.add_argument("--large-only", action="store_true", help="run the large-share measurement alone")
+    ap.add_argument("--large-no-tune", action="store_true",
+                    help="large share: take the stored schedule instead of tuning")
     ap.add_argument("--roctx", action="store_true", help="mark the timed batched replays as roctx ranges")
     ap.add_argument("--batch-no-tune", action="store_true",
                     help="batched measurement: take the stored per-channel-count schedules instead of tuning")
@@ -441,6 +448,9 @@ def main():
         Marker.enable()
     if args.batch_only:
         print(json.dumps({"metric": METRIC, "n_gpus": 1, "batched": bench_batch(args)}))
+        return
+    if args.large_only:
+        print(json.dumps({"metric": METRIC, "n_gpus": 1, "large": bench_large(args)}))
         return
     c = CFG
     N, R = c["nsamps"], c["nchans"]

@@ -37,6 +37,34 @@ __global__ void __launch_bounds__(kLanes) k_row_stats(const void* __restrict__ i
     double a = 0.0, b = 0.0;
     const int64_t nchunks = ceil_div(cols, kChunk);
     int64_t ch0 = threadIdx.x;
+    if (!U8) {
+        // four 64-byte chunks (16 floats each, four float4 loads) in flight
+        // per lane, the lane's elements still added in increasing t
+        const float* x = static_cast<const float*>(in) + r * cols;
+        if ((reinterpret_cast<uintptr_t>(x) & 15) == 0) {
+            const int64_t nfull = cols / kChunk;
+            for (; ch0 + 3 * kLanes < nfull; ch0 += 4 * kLanes) {
+                float4 v[4][kChunk / 4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int q = 0; q < kChunk / 4; ++q)
+                        v[u][q] = *reinterpret_cast<const float4*>(x + (ch0 + u * kLanes) * kChunk + 4 * q);
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int q = 0; q < kChunk / 4; ++q) {
+                        const float e4[4] = {v[u][q].x, v[u][q].y, v[u][q].z, v[u][q].w};
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            const double d = (double)e4[e];
+                            a += d;
+                            b += d * d;
+                        }
+                    }
+            }
+        }
+    }
     if (U8) {
         // four 16-byte loads in flight per lane: one per iteration left the
         // kernel waiting on load latency.  u8 sums and sums of squares are

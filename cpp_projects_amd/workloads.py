@@ -32,10 +32,12 @@ LARGE = dict(nchans=8192, nsamps=1 << 20, dt_max=8191, box_q=9, threshold=6.0, n
 
 # the staged-pass configuration the autotuner picks for SINGLE_BEAM on MI355X
 SINGLE_BEAM_TUNED = dict(staged=4, shape=(9216, 64, 16), schedule=[3, 3, 3, 3])
+# and for the large share
+LARGE_TUNED = dict(staged=4, shape=(9216, 32, 16), schedule=[2, 2, 3, 3, 3])
 # and for the batched beams of each channel count (tuned on the longest beam)
-BATCH_TUNED = {1024: dict(staged=4, shape=(5120, 16, 16), schedule=[3, 3, 2, 2]),
-               2048: dict(staged=4, shape=(6144, 16, 16), schedule=[3, 3, 3, 2]),
-               4096: dict(staged=4, shape=(9216, 64, 16), schedule=[3, 3, 3, 3])}
+BATCH_TUNED = {1024: dict(staged=4, shape=(8192, 32, 16), schedule=[2, 2, 3, 3]),
+               2048: dict(staged=4, shape=(8192, 64, 16), schedule=[2, 3, 3, 3]),
+               4096: dict(staged=4, shape=(8192, 32, 16), schedule=[3, 3, 3, 3])}
 
 
 def plain_cpu_input(device="cuda"):
