@@ -468,7 +468,8 @@ struct sps_fdmt {
             i0 += L;
         }
         if (staged) {
-            if (fuse_init && eng.init_mode == kInitMean && grid_consec && np >= 2 && pass_staged(0)) {
+            if (fuse_init && eng.init_mode == kInitMean && grid_consec && np >= 2 && pass_staged(0) &&
+                eng.nsamps <= kInitGroupMaxSamps) {
                 // a first pass whose trees need more init rows than one LDS
                 // window holds keeps the separate init
                 std::unique_ptr<StagedPass> sp(new StagedPass());

@@ -815,9 +815,9 @@ __global__ void __launch_bounds__(kThreads) k_init_consec(const void* __restrict
 // First staged pass with the mean init fused in (plans whose per-channel init
 // dts are 0, 1, ..., K-1 with K <= kInitConsecMax): state 0
 // never reaches HBM.  A block takes one group over tiles_per_block
-// consecutive 512-sample tiles.  For each channel of the group the LDS holds
-// its K init rows over the group's window [T - s4, T - s4 + 4 units), row 0
-// with one more 4-sample chunk of history in front; row 0 is 0 + x^ (the
+// consecutive kInitGroupTile-sample tiles.  For each channel of the group the
+// LDS holds its K init rows over the group's window [T - s4, T - s4 + 4 units),
+// row 0 with one more 4-sample chunk of history in front; row 0 is 0 + x^ (the
 // normalised sample) and row k >= 1 is (row_{k-1} * k + row0(t - k)) / (k + 1)
 // (0 for t < k) -- the operations of k_init_consec in the same order (a
 // division by 2 or 4 is the exact multiplication by 0.5 or 0.25).  Per tile
@@ -826,7 +826,8 @@ __global__ void __launch_bounds__(kThreads) k_init_consec(const void* __restrict
 // into registers during the previous tile's trees, into the four row-0
 // values; after a barrier it builds the chunk's rows k >= 1 from row 0 in LDS
 // (this chunk and the one before it); after another the trees of the group's
-// rows are evaluated from LDS as in k_merge_group.
+// rows are evaluated from LDS as in k_merge_group.  Times within a row are
+// 32-bit (the host keeps nsamps below kInitGroupMaxSamps).
 constexpr int kInitSlots = kInitGroupChunks / kThreads;  // chunks per thread and tile
 constexpr int kInitMaxChans = 64;
 
@@ -893,6 +894,39 @@ __device__ __forceinline__ float4a norm4(float4a v, float mu, float sg, float rs
     return float4a{q[0], q[1], q[2], q[3]};
 }
 
+// 8-bit channels whose every normalised sample lies in a range where the
+// init needs no operand checks.  For v = 0..255, a(v) = fl(v - mu) and
+// x(v) = fl(a(v) / sigma) are non-decreasing in v and vanish only at v = mu,
+// so |a| and |x| are smallest at the integers next to mu and largest at 0 or
+// 255; checking those five samples checks all 256.  If sigma and every
+// nonzero |a| lie in [2^-60, 2^60], the quotient's fast form q0 = a * rsg,
+// q0 + rsg * (a - q0 * sigma) is the division (norm4's fast branch); +0 for
+// a = +0, never -0.  If every nonzero |x| lies in [2^-60, 2^60] too, every
+// init row value is +0 or between 2^-113 and 2^64 in magnitude: sums and
+// integer multiples of values on the 2^-83 grid stay on it, so the dividend
+// of the division by 3 (row 2) is +0 or at least 2^-83, row 2 at least
+// 2^-85 (a float on the 2^-108 grid), the dividend of the division by 5
+// (row 4) +0 or at least 2^-110.  The fast quotient by the constants 3 and 5
+// is then exact without checks: it scales exactly with its operand's binade
+// while every step stays normal, and tests/cpp/div_check.c tries every
+// mantissa of one binade against both divisors.
+__device__ __forceinline__ bool u8_init_plain(float mu, float sg) {
+    if (!(sg >= 0x1p-60f && sg <= 0x1p60f) || !(fabsf(mu) <= 0x1p30f)) return false;
+    const int f = (int)floorf(mu);
+    const int cand[5] = {0, 255, f - 1, f, f + 1};
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        const int v = cand[i];
+        if (v < 0 || v > 255) continue;
+        const float a = (float)v - mu;
+        if (a == 0.0f) continue;
+        const float aa = fabsf(a), ax = fabsf(a / sg);
+        ok = ok && aa >= 0x1p-60f && aa <= 0x1p60f && ax >= 0x1p-60f && ax <= 0x1p60f;
+    }
+    return ok;
+}
+
 template <int L, int IN_KIND, bool NORM, bool FAST, bool VEC>
 __global__ void __launch_bounds__(kThreads) k_merge_group_init(const void* __restrict__ in, int64_t in_pitch,
                                                                ChanNorm norm, float* __restrict__ out,
@@ -906,6 +940,7 @@ __global__ void __launch_bounds__(kThreads) k_merge_group_init(const void* __res
     static_assert(IN_KIND == 0 || IN_KIND == 1, "f32 or u8 input");
     extern __shared__ __attribute__((aligned(16))) float lds[];
     __shared__ InitChan ch[kInitMaxChans];
+    __shared__ int plain_all;
     const uint32_t logical = xcd_remap(blockIdx.x, gridDim.x);
     const int64_t gi = (int64_t)(logical % (uint64_t)ngroups);
     const int64_t chunk = (int64_t)(logical / (uint64_t)ngroups);
@@ -913,27 +948,38 @@ __global__ void __launch_bounds__(kThreads) k_merge_group_init(const void* __res
     const int64_t k0 = tile_lo + chunk * tiles_per_block;
     const int64_t k1 = min(tile_hi, k0 + tiles_per_block);
     if (k0 >= k1) return;  // block-uniform
-    if ((int)threadIdx.x < g.nleaves) {
-        const ChanWinD cw = chans[g.leaf0 + threadIdx.x];
-        InitChan c;
-        c.xb = cw.chan * in_pitch;
-        c.mu = 0.0f;
-        c.sg = 1.0f;
-        c.rsg = 1.0f;
-        c.keep = 1;
-        if (NORM) {
-            c.keep = norm.keep[cw.chan] != 0;
-            c.mu = norm.mu[cw.chan];
-            c.sg = norm.sigma[cw.chan];
-            c.rsg = 1.0f / c.sg;
+    if (threadIdx.x < kInitMaxChans) {  // the first wave: one lane per channel
+        bool plain = true;
+        if ((int)threadIdx.x < g.nleaves) {
+            const ChanWinD cw = chans[g.leaf0 + threadIdx.x];
+            InitChan c;
+            c.xb = cw.chan * in_pitch;
+            c.mu = 0.0f;
+            c.sg = 1.0f;
+            c.rsg = 1.0f;
+            c.keep = 1;
+            if (NORM) {
+                c.keep = norm.keep[cw.chan] != 0;
+                if (c.keep) {
+                    c.mu = norm.mu[cw.chan];
+                    c.sg = norm.sigma[cw.chan];
+                    c.rsg = 1.0f / c.sg;
+                } else {
+                    c.rsg = 0.0f;  // the fast quotient of v - 0 then is +0
+                }
+                if (IN_KIND == 1 && c.keep) plain = u8_init_plain(c.mu, c.sg);
+            }
+            if (IN_KIND != 1) plain = false;
+            c.t_off = -cw.s4 - 4;
+            c.row_off = cw.row_off;
+            c.rk_off = cw.row_off + 4 * (cw.units + 1);
+            c.rstride = 4 * cw.units;
+            c.nrows = cw.nrows;
+            c.xc = cw.xc;
+            ch[threadIdx.x] = c;
         }
-        c.t_off = -cw.s4 - 4;
-        c.row_off = cw.row_off;
-        c.rk_off = cw.row_off + 4 * (cw.units + 1);
-        c.rstride = 4 * cw.units;
-        c.nrows = cw.nrows;
-        c.xc = cw.xc;
-        ch[threadIdx.x] = c;
+        const uint64_t odd = __ballot(!plain);
+        if (threadIdx.x == 0) plain_all = odd == 0;
     }
     // per slot: the chunk's channel (-1: unused) and number within it
     int32_t sc[kInitSlots], sq[kInitSlots];
@@ -945,9 +991,27 @@ __global__ void __launch_bounds__(kThreads) k_merge_group_init(const void* __res
         if (f < g.nunits) sc[j] = unit_chan[g.unit0 + f];
     }
     __syncthreads();
+    // per slot: the chunk's time at T = 0, its input at T = 0, its row-0 LDS
+    // offset
+    int32_t qo[kInitSlots], l0[kInitSlots];
+    const uint8_t* src8[kInitSlots];
+    const float* srcf[kInitSlots];
 #pragma unroll
-    for (int j = 0; j < kInitSlots; ++j)
-        if (sc[j] >= 0) sq[j] = (int)threadIdx.x + j * kThreads - ch[sc[j]].xc;
+    for (int j = 0; j < kInitSlots; ++j) {
+        qo[j] = 0;
+        l0[j] = 0;
+        src8[j] = nullptr;
+        srcf[j] = nullptr;
+        if (sc[j] >= 0) {
+            const InitChan& c = ch[sc[j]];
+            sq[j] = (int)threadIdx.x + j * kThreads - c.xc;
+            qo[j] = 4 * sq[j] + c.t_off;
+            l0[j] = c.row_off + 4 * sq[j];
+            if constexpr (IN_KIND == 1) src8[j] = static_cast<const uint8_t*>(in) + c.xb + qo[j];
+            else srcf[j] = static_cast<const float*>(in) + c.xb + qo[j];
+        }
+    }
+    const int ns = (int)nsamps;
     uint32_t raw8[kInitSlots];
     float4a rawf[kInitSlots];
 #pragma unroll
@@ -959,18 +1023,17 @@ __global__ void __launch_bounds__(kThreads) k_merge_group_init(const void* __res
     // the row pitch and the row starts multiples of 4 samples, so a chunk,
     // which starts at a multiple of 4, is whole inside the row or wholly
     // outside); otherwise four sample loads, each 0 outside [0, nsamps)
-    auto issue = [&](int64_t T) {
+    auto issue = [&](int T) {
 #pragma unroll
         for (int j = 0; j < kInitSlots; ++j) {
             if (sc[j] < 0) continue;
-            const InitChan& c = ch[sc[j]];
-            const int64_t t0 = T + 4 * sq[j] + c.t_off;
+            const int t0 = T + qo[j];
             if constexpr (VEC) {
-                if (t0 >= 0 && t0 + 4 <= nsamps) {
+                if (t0 >= 0 && t0 <= ns - 4) {
                     if constexpr (IN_KIND == 1)
-                        raw8[j] = *reinterpret_cast<const uint32_t*>(static_cast<const uint8_t*>(in) + c.xb + t0);
+                        raw8[j] = *reinterpret_cast<const uint32_t*>(src8[j] + T);
                     else
-                        rawf[j] = *reinterpret_cast<const float4a*>(static_cast<const float*>(in) + c.xb + t0);
+                        rawf[j] = *reinterpret_cast<const float4a*>(srcf[j] + T);
                 } else {
                     raw8[j] = 0u;
                     rawf[j] = float4a{0.0f, 0.0f, 0.0f, 0.0f};
@@ -980,12 +1043,11 @@ __global__ void __launch_bounds__(kThreads) k_merge_group_init(const void* __res
                 float4a wf = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const int64_t t = t0 + e;
-                    if (t >= 0 && t < nsamps) {
+                    if (t0 + e >= 0 && t0 + e < ns) {
                         if constexpr (IN_KIND == 1)
-                            w8 |= (uint32_t)static_cast<const uint8_t*>(in)[c.xb + t] << (8 * e);
+                            w8 |= (uint32_t)src8[j][T + e] << (8 * e);
                         else
-                            wf[e] = static_cast<const float*>(in)[c.xb + t];
+                            wf[e] = srcf[j][T + e];
                     }
                 }
                 raw8[j] = w8;
@@ -993,73 +1055,112 @@ __global__ void __launch_bounds__(kThreads) k_merge_group_init(const void* __res
             }
         }
     };
-    // row 0 (0 + x^; 0 outside the row) of every chunk
-    auto build0 = [&](int64_t T) {
+    // PLAIN: every channel of the group passed u8_init_plain (block-uniform),
+    // so no operand checks, no keep select and no -0 to clear
+    auto tiles = [&](auto plain_tag) {
+        constexpr bool PLAIN = decltype(plain_tag)::value;
+        // row 0 (0 + x^; 0 outside the row) of every chunk
+        auto build0 = [&](int T) {
 #pragma unroll
-        for (int j = 0; j < kInitSlots; ++j) {
-            if (sc[j] < 0) continue;
-            const InitChan& c = ch[sc[j]];
-            const int64_t t0 = T + 4 * sq[j] + c.t_off;
-            float4a v = unpack_chunk<IN_KIND>(raw8[j], rawf[j]);
-            if (NORM) {
-                v = norm4(v, c.mu, c.sg, c.rsg);
-                if (!c.keep) v = float4a{0.0f, 0.0f, 0.0f, 0.0f};
-            }
-            float4a r0;
-            if constexpr (VEC) {
-                const bool inside = t0 >= 0 && t0 + 4 <= nsamps;
-                r0.x = inside ? 0.0f + v.x : 0.0f;
-                r0.y = inside ? 0.0f + v.y : 0.0f;
-                r0.z = inside ? 0.0f + v.z : 0.0f;
-                r0.w = inside ? 0.0f + v.w : 0.0f;
-            } else {
-                r0.x = (t0 + 0 >= 0 && t0 + 0 < nsamps) ? 0.0f + v.x : 0.0f;
-                r0.y = (t0 + 1 >= 0 && t0 + 1 < nsamps) ? 0.0f + v.y : 0.0f;
-                r0.z = (t0 + 2 >= 0 && t0 + 2 < nsamps) ? 0.0f + v.z : 0.0f;
-                r0.w = (t0 + 3 >= 0 && t0 + 3 < nsamps) ? 0.0f + v.w : 0.0f;
-            }
-            lds_store4(lds + c.row_off + 4 * sq[j], r0);
-        }
-    };
-    // rows k >= 1 of every chunk but the history ones
-    auto buildk = [&](int64_t T) {
+            for (int j = 0; j < kInitSlots; ++j) {
+                if (sc[j] < 0) continue;
+                const int t0 = T + qo[j];
+                float4a v = unpack_chunk<IN_KIND>(raw8[j], rawf[j]);
+                if constexpr (PLAIN) {
+                    if (NORM) {
+                        const InitChan& c = ch[sc[j]];
+                        const float mu = c.mu, sg = c.sg, rsg = c.rsg;
 #pragma unroll
-        for (int j = 0; j < kInitSlots; ++j) {
-            if (sc[j] < 0 || sq[j] == 0) continue;
-            const InitChan& c = ch[sc[j]];
-            if (c.nrows < 2) continue;
-            const int64_t t0 = T + 4 * sq[j] + c.t_off;
-            const float4a h = lds_load4(lds + c.row_off + 4 * (sq[j] - 1));
-            const float4a o = lds_load4(lds + c.row_off + 4 * sq[j]);
-            const float w[8] = {h.x, h.y, h.z, h.w, o.x, o.y, o.z, o.w};  // w[4 + e] = row 0 at t0 + e
-            float prev[4] = {o.x, o.y, o.z, o.w};
-            float* dst = lds + c.rk_off + 4 * (sq[j] - 1);
+                        for (int e = 0; e < 4; ++e) {
+                            const float a = v[e] - mu;
+                            const float q0 = a * rsg;
+                            v[e] = fmaf(fmaf(-q0, sg, a), rsg, q0);
+                        }
+                    }
+                } else {
+                    if (NORM) {
+                        const InitChan& c = ch[sc[j]];
+                        v = norm4(v, c.mu, c.sg, c.rsg);
+                        if (!c.keep) v = float4a{0.0f, 0.0f, 0.0f, 0.0f};
+                    }
 #pragma unroll
-            for (int kk = 1; kk < kInitConsecMax; ++kk) {
-                if (kk >= c.nrows) break;
-                const float yk = (float)kk + 1.0f, rk = 1.0f / yk;
-                const float mk = (float)(kk - 1) + 1.0f;
-                const bool pow2 = kk == 1 || kk == 3;  // x / 2 and x / 4 are exact multiplications
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const float num = prev[e] * mk + w[4 + e - kk];
-                    const float val = pow2 ? num * rk : div_rn(num, yk, rk);
-                    prev[e] = t0 + e >= kk ? val : 0.0f;
+                    for (int e = 0; e < 4; ++e) v[e] = 0.0f + v[e];
                 }
-                lds_store4(dst + (kk - 1) * c.rstride, float4a{prev[0], prev[1], prev[2], prev[3]});
+                float4a r0;
+                if constexpr (VEC) {
+                    const bool inside = t0 >= 0 && t0 <= ns - 4;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) r0[e] = inside ? v[e] : 0.0f;
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) r0[e] = (t0 + e >= 0 && t0 + e < ns) ? v[e] : 0.0f;
+                }
+                lds_store4(lds + l0[j], r0);
             }
+        };
+        // rows k >= 1 of every chunk but the history ones
+        auto buildk = [&](int T) {
+#pragma unroll
+            for (int j = 0; j < kInitSlots; ++j) {
+                if (sc[j] < 0 || sq[j] == 0) continue;
+                const InitChan& c = ch[sc[j]];
+                const int nrows = c.nrows;
+                if (nrows < 2) continue;
+                const int t0 = T + qo[j];
+                const float4a h = lds_load4(lds + l0[j] - 4);
+                const float4a o = lds_load4(lds + l0[j]);
+                const float w[8] = {h.x, h.y, h.z, h.w, o.x, o.y, o.z, o.w};  // w[4 + e] = row 0 at t0 + e
+                float* dst = lds + c.rk_off + 4 * (sq[j] - 1);
+                const int rstride = c.rstride;
+                // rows past the first kInitConsecMax - 1 samples need no t < k zeros
+                auto rowsk = [&](auto clip_tag) {
+                    constexpr bool CLIP = decltype(clip_tag)::value;
+                    float prev[4] = {o.x, o.y, o.z, o.w};
+#pragma unroll
+                    for (int kk = 1; kk < kInitConsecMax; ++kk) {
+                        if (kk >= nrows) break;
+                        const float yk = (float)kk + 1.0f, rk = 1.0f / yk;
+                        const float mk = (float)(kk - 1) + 1.0f;
+                        const bool pow2 = kk == 1 || kk == 3;  // x / 2 and x / 4 are exact multiplications
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            const float num = prev[e] * mk + w[4 + e - kk];
+                            float val;
+                            if (pow2) {
+                                val = num * rk;
+                            } else if (PLAIN) {
+                                const float q0 = num * rk;
+                                val = fmaf(fmaf(-q0, yk, num), rk, q0);
+                            } else {
+                                val = div_rn(num, yk, rk);
+                            }
+                            prev[e] = (!CLIP || t0 + e >= kk) ? val : 0.0f;
+                        }
+                        lds_store4(dst + (kk - 1) * rstride, float4a{prev[0], prev[1], prev[2], prev[3]});
+                    }
+                };
+                if (t0 >= kInitConsecMax - 1) rowsk(std::false_type{});
+                else rowsk(std::true_type{});
+            }
+        };
+        issue((int)(k0 * kInitGroupTile));
+        for (int64_t k = k0; k < k1; ++k) {
+            const int T = (int)(k * kInitGroupTile);
+            build0(T);
+            __syncthreads();
+            if (k + 1 < k1) issue(T + kInitGroupTile);
+            buildk(T);
+            __syncthreads();
+            group_trees<L, false, FAST, kInitGroupTile>(lds, g, rows, out, out_pitch, T, nsamps, INT64_MAX,
+                                                        SnrEpi{nullptr, 0, nullptr});
+            __syncthreads();
         }
     };
-    issue(k0 * kInitGroupTile);
-    for (int64_t k = k0; k < k1; ++k) {
-        const int64_t T = k * kInitGroupTile;
-        build0(T);
-        __syncthreads();
-        if (k + 1 < k1) issue(T + kInitGroupTile);
-        buildk(T);
-        __syncthreads();
-        group_trees<L, false, FAST, kInitGroupTile>(lds, g, rows, out, out_pitch, T, nsamps, INT64_MAX, SnrEpi{nullptr, 0, nullptr});
-        __syncthreads();
+    if constexpr (IN_KIND == 1) {
+        if (plain_all) tiles(std::true_type{});
+        else tiles(std::false_type{});
+    } else {
+        tiles(std::false_type{});
     }
 }
 
@@ -1369,6 +1470,7 @@ void launch_fdmt_group_init(int L, const void* in, int in_u8, int64_t in_pitch, 
                             int64_t nsamps, int64_t max_shift, hipStream_t st) {
     if (ngroups <= 0 || nsamps <= 0) return;
     if (in_pitch < nsamps) throw Error(SPS_ERR_INVALID_ARGUMENT, "waterfall pitch below its length");
+    if (nsamps > kInitGroupMaxSamps) throw Error(SPS_ERR_INVALID_ARGUMENT, "fused init rows too long");
     const uintptr_t ia = reinterpret_cast<uintptr_t>(in);
     // whole-chunk loads need 4-sample multiples and an aligned waterfall
     const bool vec = nsamps % 4 == 0 && in_pitch % 4 == 0 && (in_u8 ? ia % 4 == 0 : ia % 16 == 0);

@@ -114,6 +114,8 @@ struct ChanWinD {
 // kInitGroupChunks 4-sample chunks (3 per thread).
 constexpr int kInitGroupTile = 256;
 constexpr int kInitGroupChunks = 3 * 256;
+// and times within a row are 32-bit there: rows up to 2^30 samples
+constexpr int64_t kInitGroupMaxSamps = int64_t(1) << 30;
 
 // Optional epilogue of the last merge: divide by sqrtf(n + 1e-8) with n from
 // nrm (rows x nrm_len) for t < nrm_len and stats[4] (kept channels) beyond.

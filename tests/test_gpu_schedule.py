@@ -395,6 +395,40 @@ def test_search_streaming_fused_init_vs_oracle(torch_cuda, in_dtype, ns):
     np.testing.assert_array_equal(cands_to_numpy(ct, k).view(np.int32), ref.view(np.int32))
 
 
+def test_search_fused_init_flat_u8(torch_cuda):
+    """A flat 8-bit waterfall: the global sigma is 0, so no channel passes the
+    fused init's range test and the pass takes its checked branch (every
+    sample through the IEEE division); the S/N image holds the oracle's NaNs
+    and zeros, and there are no candidates."""
+    from cpp_projects_amd.search import SingleBeamSearch
+
+    torch = torch_cuda
+    nch, ns, dtm = 256, 8192, 255
+    x = np.full((nch, ns), 100, np.uint8)
+    s = SingleBeamSearch(1200.0, 1600.0, nch, ns, dtm, in_dtype="u8", box_q=4, threshold=5.0, cand_capacity=1 << 10)
+    s.set_staged(4)
+    s.set_schedule([3, 3, 2])
+    d_in = torch.from_numpy(x).cuda()
+    dmt = torch.empty((s.ndt, ns), dtype=torch.float32, device="cuda")
+    c = torch.empty((1 << 10, 4), dtype=torch.int32, device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    s.enable_timing(True)
+    s.run(d_in, dmt, c, cnt)
+    st = s.stage_times_ms()
+    s.enable_timing(False)
+    assert st[2] < 0.02, f"init stage {st[2]} ms: the fused first pass did not run"
+    k = s.sort_candidates(c, cnt)
+    xn, mask, (_, _, keep, stv) = orc.normalise(x, 1)
+    assert stv[1] == 0.0 and keep.all()
+    p = orc.Plan(1200.0, 1600.0, nch, ns, 1.0, dtm)
+    snr = orc.fdmt_normalization(p.execute(xn), p.execute(mask))
+    got = dmt.cpu().numpy()
+    assert np.isnan(snr).any()
+    np.testing.assert_array_equal(np.isnan(got), np.isnan(snr))
+    np.testing.assert_array_equal(got[~np.isnan(got)], snr[~np.isnan(snr)])
+    assert k == 0 and len(orc.boxcar_pow2(snr, 5.0, 4)) == 0
+
+
 def test_staged_group_shapes(torch_cuda, golden):
     """Group windows from 2048 to 9216 floats (one LDS buffer above 6144), 1 to
     64 rows per group and 1 to 64 tiles per block give the same bits (plain
