@@ -74,6 +74,10 @@ struct sps_fdmt {
     bool fuse_init = false;    // first staged pass builds state 0 itself (mean init)
     bool ran_fused0 = false;   // the last execution did (stage accounting)
     bool staged_last = false;  // the final pass is staged too
+    // wide-window fast tiles on the producer/consumer kernel; off where other
+    // streams' kernels share the GPU (its two 74 KiB blocks per CU crowd them
+    // out: a 4-stream batch of 64 beams took 236 instead of 225 ms)
+    bool pc_fast = true;
     int group_floats = 8192;   // LDS window per tile buffer (floats; two buffers per block)
     int group_rows = 16;       // output rows per group at most
     int group_tiles = 8;       // consecutive 512-sample tiles per block
@@ -439,6 +443,7 @@ struct sps_fdmt {
         tree_rows.clear();
         d_staged.clear();
         d_stage0_init.reset();
+        if (staged) prepare_fdmt_group_kernels();
         int i0 = 1;
         const size_t np = schedule.size();
         d_trees.resize(np);
@@ -583,7 +588,7 @@ struct sps_fdmt {
                 const StagedPass& sp = *d_staged[k];
                 launch_fdmt_group(L, src, p, dst, last ? out_pitch : p, sp.groups.p, sp.ngroups, sp.leaves.p,
                                   sp.rows.p, sp.unit_leaf.as<uint8_t>(), sp.lds_floats, group_tiles, nsamps,
-                                  last ? eng.out_rows : INT64_MAX, sp.max_shift, st, last ? snr : nullptr);
+                                  last ? eng.out_rows : INT64_MAX, sp.max_shift, st, last ? snr : nullptr, pc_fast);
             } else {
                 launch_fdmt_fused(L, src, p, dst, last ? out_pitch : p, d_trees[k]->p, tree_rows[k], nsamps,
                                   last ? eng.out_rows : INT64_MAX, st, last ? snr : nullptr);
@@ -1729,6 +1734,7 @@ SPS_API sps_status sps_batch_create(const sps_batch_params* p, const sps_batch_b
             sps_batch::Beam& bm = b->beams[(size_t)i];
             bm.io = io;
             bm.s = make_search(&sp);
+            bm.s->f->pc_fast = p->nworkers <= 1;
             // byte cost of the beam: input read twice, every state written
             // once and read once, the image read by the boxcar
             const EnginePlan& e = bm.s->f->eng;

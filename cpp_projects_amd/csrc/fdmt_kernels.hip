@@ -287,14 +287,15 @@ struct LdsTreeFast {
 // shifts (T >= maxshift) and, with SNR, past the mask transform's extent, so
 // only the branch-free path is compiled (the general one's registers made the
 // hot loop spill).
-template <int L, bool SNR, bool FAST = false, int TILE = kGroupTile>
+// W0, NW: the rows go to waves W0 .. W0 + NW - 1 of the block.
+template <int L, bool SNR, bool FAST = false, int TILE = kGroupTile, int W0 = 0, int NW = kThreads / 64>
 __device__ __forceinline__ void group_trees(const float* __restrict__ lds, const GroupDescD& g,
                                             const GRowDesc<L>* __restrict__ rows, float* __restrict__ out,
                                             int64_t out_pitch, int64_t T, int64_t nsamps, int64_t out_rows_limit,
                                             const SnrEpi& epi) {
-    constexpr int kWaves = kThreads / 64;
+    constexpr int kWaves = NW;
     const int lane = (int)threadIdx.x & 63;
-    const int wid = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int wid = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6) - W0;
     float dk = 1.0f, rdk = 1.0f;
     if (SNR) {
         dk = sqrtf((float)((double)(float)epi.stats[4] + 1.0E-8));
@@ -463,6 +464,96 @@ __global__ void __launch_bounds__(kThreads) k_merge_group(const float* __restric
         if (ONEBUF && more) __syncthreads();  // block-uniform: every read of cur is done
         if (more) commit(nxt);
         __syncthreads();
+    }
+}
+
+// Producer/consumer form of the fast launch (tiles whose every unit lies in
+// [0, in_pitch): T is a multiple of 4 at or past every leaf's rounded shift,
+// and the window ends at most 3 samples past T + 512 <= nsamps, inside the
+// 4-sample padded row).  Wave 0 only loads: per tile it copies the group's
+// units of the next tile straight into the other LDS buffer with
+// global_load_lds_dwordx4 and waits for its own loads alone; waves 1 .. 7
+// only evaluate trees and store.  In k_merge_group every wave waits for
+// the whole vector-memory counter before committing the next tile, and on
+// this GPU that counter also holds the tile's output stores, so each tile
+// waits for its stores to drain; here the consumers never wait on memory.
+constexpr int kPCThreads = 512;
+constexpr int kPCUnits = kGroupMaxUnits / 64;  // units per producer lane
+
+template <int L, bool SNR>
+__global__ void __launch_bounds__(kPCThreads) k_merge_group_pc(const float* __restrict__ in, int64_t in_pitch,
+                                                               float* __restrict__ out, int64_t out_pitch,
+                                                               const GroupDescD* __restrict__ groups,
+                                                               int64_t ngroups, const LeafDescD* __restrict__ leaves,
+                                                               const GRowDesc<L>* __restrict__ rows,
+                                                               const uint8_t* __restrict__ unit_leaf,
+                                                               int tiles_per_block, int lds_stride, int64_t nsamps,
+                                                               int64_t out_rows_limit, SnrEpi epi, int64_t tile_lo,
+                                                               int64_t tile_hi) {
+    extern __shared__ __attribute__((aligned(16))) float lds[];
+    const uint32_t logical = xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t gi = (int64_t)(logical % (uint64_t)ngroups);
+    const int64_t chunk = (int64_t)(logical / (uint64_t)ngroups);
+    const GroupDescD g = groups[gi];
+    const int64_t k0 = tile_lo + chunk * tiles_per_block;
+    const int64_t k1 = min(tile_hi, k0 + tiles_per_block);
+    if (k0 >= k1) return;  // block-uniform
+    // the group's leaves (at most 64) in LDS for the producer's set-up
+    __shared__ LeafDescD lfs[64];
+    if ((int)threadIdx.x < g.nleaves) lfs[threadIdx.x] = leaves[g.leaf0 + threadIdx.x];
+    __syncthreads();
+    const int wid = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    if (wid == 0) {
+        const int lane = (int)threadIdx.x & 63;
+        // lane l, slot i: unit 64 i + l, its element offset at T = 0 (the
+        // table reads are unconditional, clamped to the group's units, so
+        // they are all in flight together)
+        uint32_t li[kPCUnits];
+#pragma unroll
+        for (int i = 0; i < kPCUnits; ++i) li[i] = unit_leaf[g.unit0 + min(lane + 64 * i, g.nunits - 1)];
+        int64_t src[kPCUnits];
+#pragma unroll
+        for (int i = 0; i < kPCUnits; ++i) {
+            const LeafDescD& lf = lfs[li[i]];
+            src[i] = lf.row * in_pitch + (4 * (lane + 64 * i) - lf.lds_off) - lf.smax4;
+        }
+        // slot i of the wave fills LDS bytes [1024 i, 1024 (i + 1)) of the
+        // buffer: lane l's 16 bytes at M0 + 16 l.  Issued as inline asm, so
+        // the compiler counts none of it; the wave waits itself.
+        auto issue = [&](int64_t T, float* dst) {
+#pragma unroll
+            for (int i = 0; i < kPCUnits; ++i) {
+                if (64 * i >= g.nunits) break;  // wave-uniform
+                const uint32_t m0 = __builtin_amdgcn_readfirstlane(
+                    (uint32_t)(uintptr_t)(__attribute__((address_space(3))) float*)(dst + 256 * i));
+                const float* gp = in + src[i] + T;
+                uint32_t saved;
+                if (lane + 64 * i < g.nunits)
+                    asm volatile("s_mov_b32 %0, m0\n\ts_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %2, off\n\t"
+                                 "s_mov_b32 m0, %0"
+                                 : "=&s"(saved)
+                                 : "s"(m0), "v"(gp)
+                                 : "memory");
+            }
+        };
+        issue(k0 * kGroupTile, lds);
+        asm volatile("s_waitcnt vmcnt(0)" : : : "memory");
+        __syncthreads();
+        for (int64_t k = k0; k < k1; ++k) {
+            if (k + 1 < k1) {
+                issue((k + 1) * kGroupTile, lds + ((k - k0 + 1) & 1) * lds_stride);
+                asm volatile("s_waitcnt vmcnt(0)" : : : "memory");
+            }
+            __syncthreads();  // the next tile has landed; the consumers are done with this one
+        }
+    } else {
+        __syncthreads();
+        for (int64_t k = k0; k < k1; ++k) {
+            const float* cur = lds + ((k - k0) & 1) * lds_stride;
+            group_trees<L, SNR, true, kGroupTile, 1, kPCThreads / 64 - 1>(cur, g, rows, out, out_pitch,
+                                                                          k * kGroupTile, nsamps, out_rows_limit, epi);
+            __syncthreads();
+        }
     }
 }
 
@@ -1304,6 +1395,7 @@ void launch_fdmt_fused(int L, const float* in, int64_t in_pitch, float* out, int
 struct GroupLaunch {
     size_t lds;
     bool onebuf;
+    bool pc;  // fast tiles on the producer/consumer kernel
     const float* in;
     int64_t in_pitch;
     float* out;
@@ -1338,9 +1430,44 @@ static void group_launch(const GroupLaunch& a, int64_t lo, int64_t hi, int tpb, 
                            a.unit_leaf, tpb, a.lds_stride, a.nsamps, a.limit, a.epi, lo, hi);
 }
 
+// the producer/consumer kernel over tiles [lo, hi) (fast tiles only)
+template <int L, bool SNR>
+static void group_launch_pc(const GroupLaunch& a, int64_t lo, int64_t hi, int tpb, hipStream_t st) {
+    if (hi <= lo) return;
+    const int64_t nb = a.ngroups * ceil_div(hi - lo, (int64_t)tpb);
+    if (nb > 0x7fffffffLL) throw Error(SPS_ERR_INVALID_ARGUMENT, "merge grid too large");
+    const size_t bytes = 2 * sizeof(float) * (size_t)a.lds_stride;
+    hipLaunchKernelGGL((k_merge_group_pc<L, SNR>), dim3((uint32_t)nb), dim3(kPCThreads), bytes, st, a.in, a.in_pitch,
+                       a.out, a.out_pitch, static_cast<const GroupDescD*>(a.groups), a.ngroups,
+                       static_cast<const LeafDescD*>(a.leaves), static_cast<const GRowDesc<L>*>(a.rows), a.unit_leaf,
+                       tpb, a.lds_stride, a.nsamps, a.limit, a.epi, lo, hi);
+}
+
+template <int L, bool SNR>
+static void pc_attributes() {
+    SPS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_merge_group_pc<L, SNR>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(2 * sizeof(float) * 4 * (size_t)kGroupMaxUnits)));
+}
+
+void prepare_fdmt_group_kernels() {
+    pc_attributes<1, false>();
+    pc_attributes<1, true>();
+    pc_attributes<2, false>();
+    pc_attributes<2, true>();
+    pc_attributes<3, false>();
+    pc_attributes<3, true>();
+    pc_attributes<4, false>();
+    pc_attributes<4, true>();
+}
+
 // Tiles [fast_lo, full) take the fast-only kernel (fast_lo: the first tile at
 // or past every row's largest shift and, with the S/N epilogue, the mask
 // transform's extent; full: the whole tiles); the others the general one.
+// Windows above kGroupTwoBufFloats run their fast tiles on the
+// producer/consumer kernel when a.pc (two buffers, two 512-thread blocks per
+// CU: measured 1.72 against 1.77 ms for a middle pass of the 4096 x 2^18
+// search at 9216 floats, and no slower at 7168 or 8192).
 template <int L, bool SNR>
 static void group_dispatch(const GroupLaunch& a, int64_t fast_lo, hipStream_t st) {
     const int64_t ntiles = ceil_div(a.nsamps, kGroupTile);
@@ -1349,7 +1476,10 @@ static void group_dispatch(const GroupLaunch& a, int64_t fast_lo, hipStream_t st
         // the few first and last tiles: one per block, so that they spread
         // over the GPU instead of queueing in one block per group
         group_launch<L, SNR, false>(a, 0, fast_lo, 1, st);
-        group_launch<L, SNR, true>(a, fast_lo, full, a.tiles_per_block, st);
+        if (a.pc)
+            group_launch_pc<L, SNR>(a, fast_lo, full, a.tiles_per_block, st);
+        else
+            group_launch<L, SNR, true>(a, fast_lo, full, a.tiles_per_block, st);
         group_launch<L, SNR, false>(a, full, ntiles, 1, st);
     } else {
         group_launch<L, SNR, false>(a, 0, ntiles, a.tiles_per_block, st);
@@ -1359,7 +1489,7 @@ static void group_dispatch(const GroupLaunch& a, int64_t fast_lo, hipStream_t st
 void launch_fdmt_group(int L, const float* in, int64_t in_pitch, float* out, int64_t out_pitch, const void* d_groups,
                        int64_t ngroups, const void* d_leaves, const void* d_rows, const uint8_t* d_unit_leaf,
                        int lds_floats_max, int tiles_per_block, int64_t nsamps, int64_t out_rows_limit,
-                       int64_t max_shift, hipStream_t st, const MergeEpilogue* snr) {
+                       int64_t max_shift, hipStream_t st, const MergeEpilogue* snr, bool pc) {
     if (ngroups <= 0 || nsamps <= 0) return;
     if (in_pitch % 4 != 0 || reinterpret_cast<uintptr_t>(in) % 16 != 0)
         throw Error(SPS_ERR_INVALID_ARGUMENT, "staged pass needs a 16-byte aligned input state");
@@ -1369,8 +1499,10 @@ void launch_fdmt_group(int L, const float* in, int64_t in_pitch, float* out, int
     if (max_shift < 0) throw Error(SPS_ERR_INVALID_ARGUMENT, "negative row shift");
     GroupLaunch a{};
     // two tile buffers while they fit 48 KiB (three blocks per CU); wider
-    // windows take one buffer and a second barrier per tile
+    // windows take one buffer and a second barrier per tile (fast tiles:
+    // the producer/consumer kernel)
     a.onebuf = lds_floats_max > kGroupTwoBufFloats;
+    a.pc = pc && a.onebuf;
     a.lds = sizeof(float) * (a.onebuf ? 1 : 2) * (size_t)lds_floats_max;
     a.in = in;
     a.in_pitch = in_pitch;

@@ -136,9 +136,12 @@ void launch_fdmt_fused(int L, const float* in, int64_t in_pitch, float* out, int
                        int64_t nrows, int64_t nsamps, int64_t out_rows_limit, hipStream_t st,
                        const MergeEpilogue* snr = nullptr);
 
+// Sets what the staged passes' producer/consumer kernels need (their LDS
+// above 64 KiB); called on the current device before their first launch.
+void prepare_fdmt_group_kernels();
 // A staged pass (1 <= L <= kMaxGroupFuse) over ngroups GroupDescD; in must be
 // 16-byte aligned with in_pitch % 4 == 0 (the internal states are).  Each
-// block streams tiles_per_block consecutive tiles of one group through two
+// block streams tiles_per_block consecutive tiles of one group through its
 // LDS windows; d_unit_leaf maps every float4 unit of a group's windows to
 // its leaf (uint8, GroupDescD.unit0 based).  max_shift: the largest
 // GRowDesc::maxshift of the pass; whole tiles from there on (and past the
@@ -146,7 +149,7 @@ void launch_fdmt_fused(int L, const float* in, int64_t in_pitch, float* out, int
 void launch_fdmt_group(int L, const float* in, int64_t in_pitch, float* out, int64_t out_pitch, const void* d_groups,
                        int64_t ngroups, const void* d_leaves, const void* d_rows, const uint8_t* d_unit_leaf,
                        int lds_floats_max, int tiles_per_block, int64_t nsamps, int64_t out_rows_limit,
-                       int64_t max_shift, hipStream_t st, const MergeEpilogue* snr = nullptr);
+                       int64_t max_shift, hipStream_t st, const MergeEpilogue* snr = nullptr, bool pc = true);
 
 // First staged pass with the (mean) init fused in: reads the waterfall (f32 or
 // u8, optional per-channel normalisation) instead of state 0; whole 4-sample
