@@ -81,7 +81,7 @@ struct sps_fdmt {
     int group_floats = 8192;   // LDS window per tile buffer (floats; two buffers per block)
     int group_rows = 16;       // output rows per group at most
     int group_tiles = 8;       // consecutive 512-sample tiles per block
-    static constexpr int kInitGroupFloats = 8192;  // first pass with the init fused
+    static constexpr int kInitGroupFloats = 4096;  // first pass with the init fused (16 KiB per block)
     struct StagedPass {
         DevBuf groups, leaves, rows, unit_leaf;
         int64_t ngroups = 0;

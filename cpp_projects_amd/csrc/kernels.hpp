@@ -110,10 +110,11 @@ struct ChanWinD {
 };
 // The first staged pass with the init fused works on tiles of kInitGroupTile
 // samples (its LDS holds every init row of the group's channels, so a shorter
-// tile keeps more blocks resident); a group's channel windows hold at most
-// kInitGroupChunks 4-sample chunks (3 per thread).
-constexpr int kInitGroupTile = 256;
-constexpr int kInitGroupChunks = 3 * 256;
+// tile keeps more blocks resident: at 4096 x 2^18, 128 samples took 1.89 ms,
+// 256 took 1.99 and 64 took 2.03); a group's channel windows hold at most
+// kInitGroupChunks 4-sample chunks (2 per thread).
+constexpr int kInitGroupTile = 128;
+constexpr int kInitGroupChunks = 2 * 256;
 // and times within a row are 32-bit there: rows up to 2^30 samples
 constexpr int64_t kInitGroupMaxSamps = int64_t(1) << 30;
 
