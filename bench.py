@@ -544,9 +544,13 @@ def main():
         if k == 0 and fused0:
             return f"k_merge_group_init<{L}, 1, true, *>"  # u8 input, normalised
         if staged and not (last and staged in (1, 2) and L <= 3):
-            # a staged pass is two launches: <L, SNR, true> over the whole
-            # tiles past every row's shifts, <L, SNR, false> over the rest
-            return f"k_merge_group<{L}, {'true' if last else 'false'}, *>"
+            # a staged pass is two launches: the fast kernel over the whole
+            # tiles past every row's shifts (the producer/consumer form for
+            # windows above 6144 floats), <L, SNR, false> over the rest
+            snr = 'true' if last else 'false'
+            if shape[0] > 6144:
+                return f"k_merge_group_pc<{L}, {snr}> + k_merge_group<{L}, {snr}, false, *>"
+            return f"k_merge_group<{L}, {snr}, *>"
         return f"k_merge_fused<{L}, true, {'true' if last else 'false'}>"
 
     stage_names = (["stats", "mask_transform", "init"] +
