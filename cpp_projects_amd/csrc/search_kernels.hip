@@ -69,27 +69,33 @@ __global__ void __launch_bounds__(kLanes) k_row_stats(const void* __restrict__ i
         // four 16-byte loads in flight per lane: one per iteration left the
         // kernel waiting on load latency.  u8 sums and sums of squares are
         // integers below 2^53 for rows under 1.3e11 samples, exact in FP64,
-        // so the grouping does not change them
+        // so neither the grouping nor integer arithmetic changes them: the
+        // 64 samples of an iteration go through 4-byte integer dot products
+        // (sums <= 16320 and sums of squares <= 4161600 fit 32 bits) into
+        // 64-bit lane totals, converted once
         const uint8_t* x = static_cast<const uint8_t*>(in) + r * cols;
         if ((reinterpret_cast<uintptr_t>(x) & 15) == 0) {
             const int64_t nfull = cols / kChunk;
+            uint64_t ai = 0, bi = 0;
             for (; ch0 + 3 * kLanes < nfull; ch0 += 4 * kLanes) {
                 uint4 v[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) v[u] = *reinterpret_cast<const uint4*>(x + (ch0 + u * kLanes) * kChunk);
+                uint32_t s = 0u, q = 0u;
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const uint32_t w[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
 #pragma unroll
-                    for (int q = 0; q < 4; ++q)
-#pragma unroll
-                        for (int e = 0; e < 4; ++e) {
-                            const double d = (double)((w[q] >> (8 * e)) & 0xffu);
-                            a += d;
-                            b += d * d;
-                        }
+                    for (int e = 0; e < 4; ++e) {
+                        s = __builtin_amdgcn_udot4(w[e], 0x01010101u, s, false);
+                        q = __builtin_amdgcn_udot4(w[e], w[e], q, false);
+                    }
                 }
+                ai += s;
+                bi += q;
             }
+            a = (double)ai;
+            b = (double)bi;
         }
     }
     for (int64_t ch = ch0; ch < nchunks; ch += kLanes) {

@@ -49,6 +49,32 @@ def test_unpack_vs_oracle(torch_cuda, layout, nbin, nfft, noverlap, npol, nchan,
     np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
 
 
+@pytest.mark.parametrize("layout", ["guppi", "lofar"])
+def test_unpack_unaligned_input(torch_cuda, layout):
+    """An input view one byte into its buffer takes the byte loads instead of
+    the 4-byte ones (GUPPI's four-polarisation word, LOFAR's four-channel
+    word): same bits."""
+    from cpp_projects_amd import unpack
+
+    torch = torch_cuda
+    nbin, nfft, noverlap, npol, nchan, nsamp = 512, 3, 32, 4, 68, 1500
+    rng = np.random.default_rng(5)
+    shape = (nchan, nsamp, npol) if layout == "guppi" else (npol, nsamp, nchan)
+    raw = rng.integers(-128, 128, shape, dtype=np.int8)
+    ref = orc.unpack(layout, raw, nsamp, noverlap, nbin, nfft, npol, nchan)
+    print("case unpack unaligned", layout, flush=True)
+    buf = torch.zeros(raw.size + 4, dtype=torch.int8, device="cuda")
+    buf[1:1 + raw.size] = torch.from_numpy(raw.reshape(-1)).cuda()
+    d_in = buf[1:1 + raw.size]
+    d_out = torch.full((unpack.out_size(nbin, nfft, npol, nchan),), complex(7.0, 7.0), dtype=torch.complex64,
+                       device="cuda")
+    f = unpack.unpack_chunk_guppi if layout == "guppi" else unpack.unpack_chunk_lofar
+    f(d_in, nsamp, noverlap, nbin, nfft, npol, nchan, d_out)
+    torch.cuda.synchronize()
+    got = d_out.cpu().numpy().reshape(ref.shape)
+    np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
 def test_unpack_size_checks(torch_cuda):
     from cpp_projects_amd import _lib, unpack
 
