@@ -619,6 +619,13 @@ __device__ __forceinline__ float2 seg_coef(float thr, int k, double e_k) {
     return make_float2(__double2float_rd(a), __double2float_ru(e_k));
 }
 
+// The divisor of width 2^p, sqrtf(2^p) (IEEE), and its reciprocal rounded to
+// nearest (exactly 2^(-p/2) for even p)
+__device__ __forceinline__ float2 box_divisor(int p) {
+    const float d = sqrtf((float)(1 << p));
+    return make_float2(d, 1.0f / d);
+}
+
 // |v| as an ordered integer: finite magnitudes < 0x7f800000 (inf) < NaNs
 __device__ __forceinline__ uint32_t abs_bits(float v) { return __float_as_uint(v) & 0x7fffffffu; }
 
@@ -818,8 +825,8 @@ constexpr bool kChunkRecompute = false;
 template <int QMAX, bool SEG>
 __device__ __forceinline__ void pow2_tile(float* x, float* la, uint2* stage, float m, const float* row, int64_t T,
                                           int span, bool vec, int q, float thr, const double* err_e,
-                                          const float2* lc, int64_t r, CandD* __restrict__ out, int64_t cap,
-                                          unsigned long long* __restrict__ count) {
+                                          const float2* lc, const float2* dvt, int64_t r, CandD* __restrict__ out,
+                                          int64_t cap, unsigned long long* __restrict__ count) {
     const unsigned tid = opaque_tid();
     __shared__ uint32_t wbits[kThreads / 64];
     __shared__ float lim[kMaxPowQ + 1];
@@ -991,7 +998,11 @@ __device__ __forceinline__ void pow2_tile(float* x, float* la, uint2* stage, flo
         float snr = 0.0f;
         bool pred = false;
         if ((fl >> p) & 1u) {
-            snr = sum / sqrtf((float)(1 << p));
+            // dvt[p] = {sqrtf(2^p), 1 / it}: for even p the divisor is
+            // 2^(p/2), so the IEEE quotient is the rounding of the exact
+            // scaling sum * 2^(-p/2); for odd p the shared-reciprocal quotient
+            const float2 dv = dvt[p];
+            snr = (p & 1) ? div_rn(sum, dv.x, dv.y) : sum * dv.y;
             pred = snr > thr;
         }
         const uint64_t m = __ballot(pred);
@@ -1125,22 +1136,28 @@ __device__ __forceinline__ void pow2_tile(float* x, float* la, uint2* stage, flo
             // widths 32 .. 2^top: blocks of 16 samples, the next block's
             // reads issued before the current block is added
             if (top >= 5) {
+                // two register blocks in turn (no copies): the block at b is
+                // added while the one at b + 16 loads
                 const int end = 1 << top;
-                float nx[16];
+                float A[16], B[16];
 #pragma unroll
-                for (int u = 0; u < 16; ++u) nx[u] = xs[16 + u];
-                for (int b = 16; b < end; b += 16) {  // wave-uniform
-                    float cur[16];
-#pragma unroll
-                    for (int u = 0; u < 16; ++u) cur[u] = nx[u];
+                for (int u = 0; u < 16; ++u) A[u] = xs[16 + u];
+                for (int b = 16; b < end; b += 32) {  // wave-uniform
                     if (b + 16 < end) {
 #pragma unroll
-                        for (int u = 0; u < 16; ++u) nx[u] = xs[b + 16 + u];
+                        for (int u = 0; u < 16; ++u) B[u] = xs[b + 16 + u];
                     }
 #pragma unroll
-                    for (int u = 0; u < 16; ++u) s += cur[u];
-                    const int w = b + 16;
-                    if ((w & (w - 1)) == 0) decide(s, lt, fl, 31 - __clz(w));
+                    for (int u = 0; u < 16; ++u) s += A[u];
+                    if (((b + 16) & (b + 15)) == 0) decide(s, lt, fl, 31 - __clz(b + 16));
+                    if (b + 16 >= end) break;
+                    if (b + 32 < end) {
+#pragma unroll
+                        for (int u = 0; u < 16; ++u) A[u] = xs[b + 32 + u];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) s += B[u];
+                    if (((b + 32) & (b + 31)) == 0) decide(s, lt, fl, 31 - __clz(b + 32));
                 }
             }
         }
@@ -1185,10 +1202,14 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2(const float* __restric
     const float* row = img + r * pitch;
     const bool vec = (pitch & 3) == 0 && (reinterpret_cast<uintptr_t>(img) & 15) == 0;
     __shared__ double err_e[kMaxPowQ + 1];
-    if ((int)threadIdx.x <= q) err_e[threadIdx.x] = err.e[threadIdx.x];  // read after pow2_tile's first barrier
+    __shared__ float2 dvt[kMaxPowQ + 1];
+    if ((int)threadIdx.x <= q) {  // read after pow2_tile's first barrier
+        err_e[threadIdx.x] = err.e[threadIdx.x];
+        dvt[threadIdx.x] = box_divisor((int)threadIdx.x);
+    }
     const float m = pow2_load_window(x, row + T, span, vec);
-    pow2_tile<kMaxPowQ, false>(x, la, stage, m, row, T, span, vec, q, d_thr ? *d_thr : h_thr, err_e, nullptr, r, out, cap,
-                               count);
+    pow2_tile<kMaxPowQ, false>(x, la, stage, m, row, T, span, vec, q, d_thr ? *d_thr : h_thr, err_e, nullptr, dvt, r,
+                               out, cap, count);
 }
 
 // Windows up to q = QM in static LDS (QM = 8, or 9 for q = 9): two sample
@@ -1247,10 +1268,12 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2_pf(const float* __rest
     __shared__ __attribute__((aligned(16))) uint2 stage[kPowStage * (kThreads / 64)];
     __shared__ double err_e[QM + 1];
     __shared__ float2 lc[QM + 1];
+    __shared__ float2 dvt[QM + 1];
     const float thr = d_thr ? *d_thr : h_thr;
     if ((int)threadIdx.x <= q) {  // read after the first tile's barriers
         err_e[threadIdx.x] = err.e[threadIdx.x];
         lc[threadIdx.x] = seg_coef(thr, (int)threadIdx.x, err.e[threadIdx.x]);
+        dvt[threadIdx.x] = box_divisor((int)threadIdx.x);
     }
     const int full = kPowCells + (1 << q) - 1;
     const int full4 = round_up4(full);
@@ -1274,7 +1297,7 @@ __global__ void __launch_bounds__(kThreads) k_boxcar_pow2_pf(const float* __rest
             m = pow2_load_window(cur, row + T, span, vec);
             __syncthreads();
         }
-        pow2_tile<QM, true>(cur, la, stage, m, row, T, span, vec, q, thr, err_e, lc, r, out, cap, count);
+        pow2_tile<QM, true>(cur, la, stage, m, row, T, span, vec, q, thr, err_e, lc, dvt, r, out, cap, count);
     };
     if (k0 < k1 && dma(k0)) pow2_issue_window(xa, row + k0 * kPowCells, full4);
     for (int64_t k = k0; k < k1; ++k) {  // block-uniform

@@ -3,7 +3,8 @@
  * q0 = x*r, e = fma(-q0, y, x), q1 = fma(e, r, q0) must equal RN(x/y).
  * Every x mantissa in [1, 2) is tried (other binades scale exactly) against
  * every odd integer divisor up to argv[1] (powers of two scale exactly) and
- * argv[2] pseudo-random float divisors over 2^-20..2^20.  Prints the number
+ * argv[2] pseudo-random float divisors over 2^-20..2^20, and sqrtf(2) (the
+ * boxcar's odd-width divisors are it times powers of two).  Prints the number
  * of mismatches. */
 #include <math.h>
 #include <stdint.h>
@@ -50,6 +51,8 @@ int main(int argc, char** argv) {
         }
         bad += check(fy);
     }
+    /* the boxcar's odd-width divisor sqrtf(2^p) = sqrtf(2) * 2^((p-1)/2) */
+    bad += check(sqrtf(2.0f));
     printf("%ld\n", bad);
     return 0;
 }

@@ -2,9 +2,10 @@
 dt + 1, a channel's sigma, the S/N divisor beyond the mask extent) with a
 rounded reciprocal and one FMA correction instead of a full division.  This
 checks, on the CPU and bit for bit, that the sequence reproduces IEEE x / y
-for every x mantissa, every odd integer divisor up to 1023 and 64 random
-float divisors (the exhaustive run over all odd divisors below 32768 gives
-no mismatch either)."""
+for every x mantissa, every odd integer divisor up to 1023, 64 random float
+divisors and sqrtf(2) (the boxcar's S/N divisor for odd widths, up to a power
+of two; the exhaustive run over all odd divisors below 32768 gives no
+mismatch either)."""
 import os
 import shutil
 import subprocess
