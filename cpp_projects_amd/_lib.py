@@ -92,6 +92,14 @@ SIGNATURES = {
     "sps_scan_inclusive_f32": [_vp, _vp, _i64, _vp],
     "sps_unpack_guppi": [_vp, _i64, _int, _int, _int, _int, _int, _vp, _vp],
     "sps_unpack_lofar": [_vp, _i64, _int, _int, _int, _int, _int, _vp, _vp],
+    "sps_coherent_create": [_f32, _f32, _int, _int, _int, _int, _int, _int, _f32, C.POINTER(_vp)],
+    "sps_coherent_destroy": [_vp],
+    "sps_coherent_get_shape": [_vp, _pi64, _pi64, _pi64],
+    "sps_coherent_forward": [_vp, _vp, _vp],
+    "sps_coherent_waterfall": [_vp, _vp, C.c_double, _vp, _vp],
+    "sps_coherent_chirp": [_vp, _vp, C.c_double, _vp, _vp],
+    "sps_coherent_inverse": [_vp, _vp, _vp],
+    "sps_coherent_detect": [_vp, _vp, C.c_double, _vp, _vp],
     "sps_detect_signal": [_int, _vp, _vp, _i64, _i64, _int, _int, _vp, _vp, _vp, _i64, _vp, _vp],
     "sps_fdmt_adjoint": [_vp, _vp, C.c_size_t, _vp, C.c_size_t, _vp],
     "sps_calc_fdmt_inputs": [_vp, _i64, _i64, _vp, _vp, _int, _vp],

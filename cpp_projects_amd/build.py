@@ -28,7 +28,7 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-I", os.path.join(ROOT, "include")]
 
 HIP_SOURCES = ["fdmt_kernels.hip", "search_kernels.hip", "sort_kernels.hip", "detect_noalloc.hip",
-               "unpack_kernels.hip"]
+               "unpack_kernels.hip", "coherent_kernels.hip"]
 # per-source additions: the boxcar's multi-tile kernel loops over tiles, and
 # machine LICM hoists constants of the tile body out of that loop into
 # registers (112 VGPRs instead of 90: four waves per SIMD instead of five)
@@ -65,7 +65,8 @@ def build(verbose=False):
                 print(out)
     objs = [j[-1] for j in jobs]
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    # rocFFT through hipFFT for the coherent front-end's transforms
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-L/opt/rocm/lib", "-lhipfft"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -277,6 +277,41 @@ SPS_API sps_status sps_unpack_guppi(const int8_t* d_in, int64_t nsamp, int nover
 SPS_API sps_status sps_unpack_lofar(const int8_t* d_in, int64_t nsamp, int noverlap, int nbin, int nfft, int npol,
                                     int nchan, float* d_out, void* stream);
 
+/* Coherent front-end between the unpack and the search: CChunk_gpu::process
+ * with CChunk_fly_gpu's on-the-fly chirp (CDMT/Chunk_gpu.cu:197-209 plans,
+ * :254-322 steps 2-8; Chunk_fly_gpu.cu:94-164 chirp).  A handle holds the
+ * two rocFFT plans of CChunk_gpu::create_fft_plans (forward: nbin points,
+ * nfft * nchan * npol/2 blocks; inverse: mbin = nbin / len_sft points, len_sft
+ * times as many) and one scratch chunk.  Spectra are complex float
+ * [pol][fft][chan][bin] (the unpack's layout, npol/2 polarisations); the
+ * waterfall is float [nchan * len_sft][msamp], msamp = nfft * (mbin - 2 *
+ * (noverlap / len_sft)), rows in the reference's order (both frequency axes
+ * reversed), each row shifted by its incoherent dispersion delay.  All calls
+ * are asynchronous on the stream. */
+typedef struct sps_coherent sps_coherent;
+SPS_API sps_status sps_coherent_create(float f_min, float f_max, int npol, int nchan, int len_sft, int nbin,
+                                       int nfft, int noverlap, float tsamp, sps_coherent** out);
+SPS_API sps_status sps_coherent_destroy(sps_coherent* h);
+/* waterfall rows and columns, and the complex values of one chunk */
+SPS_API sps_status sps_coherent_get_shape(const sps_coherent* h, int64_t* h_rows, int64_t* h_cols,
+                                          int64_t* h_spectra);
+/* step 2: cufftExecC2C(m_fftPlanForward) in place (Chunk_gpu.cu:267) */
+SPS_API sps_status sps_coherent_forward(sps_coherent* h, float* d_blocks, void* stream);
+/* steps 4-8 for one coherent DM: chirp into the handle's scratch, inverse
+ * FFT there, then the intensity waterfall into d_waterfall */
+SPS_API sps_status sps_coherent_waterfall(sps_coherent* h, const float* d_spectra, double dm, float* d_waterfall,
+                                          void* stream);
+/* the steps one at a time: the chirp (kernel_el_wise_mult_onthe_fly,
+ * Chunk_fly_gpu.cu:119-164) from d_spectra into d_out; the inverse FFT in
+ * place (Chunk_gpu.cu:288); the intensity waterfall from inverse-FFT output
+ * (roll_rows_normalize_sum_kernel, transpose_unpadd_intensity and
+ * dedisperse, Chunk_gpu.cu:476-515,613-632, as one pass) */
+SPS_API sps_status sps_coherent_chirp(sps_coherent* h, const float* d_spectra, double dm, float* d_out,
+                                      void* stream);
+SPS_API sps_status sps_coherent_inverse(sps_coherent* h, float* d_buf, void* stream);
+SPS_API sps_status sps_coherent_detect(sps_coherent* h, const float* d_buf, double dm, float* d_waterfall,
+                                       void* stream);
+
 /* statistical_preprocessing::clean_fdmtInput_and_create_normInput
  * (CDMT/Statistical_preprocessing.cuh:6-7, .cu:102-120,142-268): rows whose
  * variance lies more than 4 standard deviations from the mean row variance

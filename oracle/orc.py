@@ -322,6 +322,94 @@ def unpack(layout, raw, nsamp, noverlap, nbin, nfft, npol, nchan):
     return out
 
 
+def coherent_chirp(spec, dm, f_min, f_max, nchan, len_sft, nbin, nfft, npol):
+    """kernel_el_wise_mult_onthe_fly (CDMT/Chunk_fly_gpu.cu:94-164) on
+    spectra [pol][fft][chan][bin]: bin col of channel c (col = (isft * mbin +
+    ibin + nbin/2) mod nbin) times the factor of sub-band isft, bin ibin.
+    Restated in numpy: the sub-band centre rounded to float, the rest in
+    double, the complex product in float32 without contraction."""
+    mbin = nbin // len_sft
+    bw = float(np.float32(f_max) - np.float32(f_min))  # a float difference
+    bw_sub = bw / nchan
+    bw_chan = bw_sub / len_sft
+    ichan = np.arange(nchan, dtype=np.float64)[:, None, None]
+    isft = np.arange(len_sft, dtype=np.float64)[None, :, None]
+    ibin = np.arange(mbin, dtype=np.float64)[None, None, :]
+    f0 = (float(np.float32(f_min)) + bw_sub * (0.5 + ichan) + bw_chan * (isft - len_sft / 2.0 + 0.5))
+    f0 = f0.astype(np.float32).astype(np.float64)
+    bf = -0.5 * bw_chan + (ibin + 0.5) * bw_chan / mbin
+    taper = 1.0 / np.sqrt(1.0 + np.power(bf / (0.47 * bw_chan), 80.0))
+    r = bf / f0
+    delay = dm * r * r / (f0 + bf) * 4.148808e9
+    t = -np.fmod(delay, 1.0) * 2.0
+    c, sn = np.cos(np.pi * t), np.sin(np.pi * t)
+    dx = (c * taper / float(nbin)).astype(np.float32)
+    dy = (sn * taper / float(nbin)).astype(np.float32)
+    col = ((np.arange(len_sft)[:, None] * mbin + np.arange(mbin)[None, :] + nbin // 2) % nbin)
+    fx = np.zeros((nchan, nbin), np.float32)
+    fy = np.zeros((nchan, nbin), np.float32)
+    fx[:, col] = dx.reshape(nchan, len_sft, mbin)
+    fy[:, col] = dy.reshape(nchan, len_sft, mbin)
+    a = np.asarray(spec, np.complex64).reshape(npol // 2, nfft, nchan, nbin)
+    ax, ay = a.real.astype(np.float32), a.imag.astype(np.float32)
+    ox = ax * fx - ay * fy
+    oy = ax * fy + ay * fx
+    return (ox + 1j * oy).astype(np.complex64)
+
+
+def coherent_detect(buf, dm, f_min, f_max, nchan, len_sft, nbin, nfft, npol, noverlap, tsamp):
+    """The three reference passes after the inverse FFT, literally
+    (CDMT/Chunk_gpu.cu:290-322): roll_rows_normalize_sum_kernel (:476-493),
+    transpose_unpadd_intensity (:497-515), dedisperse (:613-632).  Returns
+    the float32 waterfall [nchan * len_sft][msamp]."""
+    mbin = nbin // len_sft
+    nov = noverlap // len_sft
+    madj = mbin - 2 * nov
+    msamp = nfft * madj
+    a = np.asarray(buf, np.complex64).reshape(npol // 2, nfft * nchan, nbin)
+    # roll, normalise, sum: rolled[row][(i + nbin/2) % nbin] += float((x^2 + y^2)/nbin/nbin)
+    rolled = np.zeros((nfft * nchan, nbin), np.float32)
+    dst = (np.arange(nbin) + nbin // 2) % nbin
+    for p in range(npol // 2):
+        x = a[p].real.astype(np.float64)
+        y = a[p].imag.astype(np.float64)
+        rolled[:, dst] = rolled[:, dst] + ((x * x + y * y) / nbin / nbin).astype(np.float32)
+    # transpose and unpad (nsub = nchan coarse channels, len_sft sub-channels)
+    r4 = rolled.reshape(nfft, nchan, len_sft, mbin)
+    inten = np.zeros((nchan * len_sft, msamp), np.float32)
+    for isub in range(nchan):
+        for ich in range(len_sft):
+            row = isub * len_sft + len_sft - ich - 1
+            seg = r4[:, nchan - isub - 1, ich, nov:nov + madj]  # [fft][madj]
+            inten[row] = seg.reshape(-1)
+    # dedisperse
+    foff = (float(np.float32(f_max)) - float(np.float32(f_min))) / (nchan * len_sft)
+    fmin = float(np.float32(f_min))
+    tsw = float(np.float32(np.float32(len_sft) * np.float32(tsamp)))
+    out = np.zeros_like(inten)
+    nrows = nchan * len_sft
+    for row in range(nrows):
+        fr = (nrows - 1 - row) * foff + fmin
+        d = 4.148808e3 * dm * (1.0 / (fmin * fmin) - 1.0 / (fr * fr))
+        q = d / tsw
+        ish = int(np.floor(q + 0.5)) if q >= 0 else -int(np.floor(-q + 0.5))  # C round(): halves away from 0
+        out[row, (np.arange(msamp) + ish) % msamp] = inten[row]
+    return out
+
+
+def coherent_chain(spec_time, dm, f_min, f_max, nchan, len_sft, nbin, nfft, npol, noverlap, tsamp):
+    """The whole front-end in double-precision FFTs (numpy): forward FFT of
+    the blocks, the chirp, the unnormalised inverse FFT of every sub-band
+    and the three passes above; a tolerance reference for the rocFFT path."""
+    mbin = nbin // len_sft
+    a = np.asarray(spec_time, np.complex64).reshape(npol // 2, nfft, nchan, nbin)
+    f = np.fft.fft(a.astype(np.complex128), axis=-1).astype(np.complex64)
+    ch = coherent_chirp(f, dm, f_min, f_max, nchan, len_sft, nbin, nfft, npol)
+    sub = ch.reshape(npol // 2, nfft, nchan, len_sft, mbin).astype(np.complex128)
+    inv = (np.fft.ifft(sub, axis=-1) * mbin).astype(np.complex64).reshape(npol // 2, nfft, nchan, nbin)
+    return coherent_detect(inv, dm, f_min, f_max, nchan, len_sft, nbin, nfft, npol, noverlap, tsamp)
+
+
 def detect_tiles(mode, a, norm, w, tile):
     """The tiled detectors of doDetectSignal_v1 (mode 1) / _v2 (mode 2):
     (width, row, col, value) or None."""
