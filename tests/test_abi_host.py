@@ -106,13 +106,13 @@ def test_device_count_call_without_gpu():
 
 
 def test_dropin_headers_compile(tmp_path):
-    """include/sps/fdmt.hpp, search.hpp and unpack.hpp compile with the host
-    compiler against sps.h; the harness, which also uses the
+    """include/sps/fdmt.hpp, search.hpp, unpack.hpp and coherent.hpp compile
+    with the host compiler against sps.h; the harness, which also uses the
     thrust::device_vector forms (sps/thrust_vectors.hpp, rocThrust), compiles
     with hipcc."""
     src = tmp_path / "h.cpp"
     src.write_text('#include "sps/fdmt.hpp"\n#include "sps/search.hpp"\n#include "sps/unpack.hpp"\n'
-                   'int main() { return 0; }\n')
+                   '#include "sps/coherent.hpp"\nint main() { return 0; }\n')
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I",
                         os.path.join(ROOT, "include"), "-I", "/opt/rocm/include", str(src)], capture_output=True,
                        text=True)
