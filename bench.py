@@ -318,6 +318,74 @@ def bench_batch(args):
     }
 
 
+def bench_coherent(args):
+    """The coherent front-end on one GUPPI-like chunk (cpp_projects_amd/
+    workloads.py COHERENT): 8-bit unpack, forward FFT, then per coherent DM
+    the chirp, the sub-band inverse FFTs and the intensity waterfall that the
+    search takes.  Compulsory bytes: unpack reads the raw chunk and writes the
+    blocks; each FFT reads and writes the chunk once; per DM the chirp reads
+    and writes it, the inverse FFT reads and writes it, the intensity pass
+    reads the kept bins and writes the waterfall."""
+    import torch
+
+    from cpp_projects_amd import unpack
+    from cpp_projects_amd.coherent import CoherentChunk
+
+    C = workloads.COHERENT
+    npol, nchan, nbin, nfft, nov = C["npol"], C["nchan"], C["nbin"], C["nfft"], C["noverlap"]
+    nsamp = (nbin - 2 * nov) * nfft
+    g = torch.Generator(device="cuda")
+    g.manual_seed(C["seed"])
+    raw = torch.randint(-30, 31, (nchan * nsamp * npol,), dtype=torch.int8, device="cuda", generator=g)
+    cc = CoherentChunk(C["f_min"], C["f_max"], npol, nchan, C["len_sft"], nbin, nfft, nov, C["tsamp"])
+    blocks = torch.empty(unpack.out_size(nbin, nfft, npol, nchan), dtype=torch.complex64, device="cuda")
+    wf = torch.empty((cc.rows, cc.cols), dtype=torch.float32, device="cuda")
+    dms = C["dms"]
+
+    def run():
+        unpack.unpack_chunk_guppi(raw, nsamp, nov, nbin, nfft, npol, nchan, blocks)
+        cc.forward(blocks)
+        for dm in dms:
+            cc.waterfall(blocks, dm, wf)
+
+    run()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    steps = 3
+    t_unpack = t_fwd = t_dm = 0.0
+    for _ in range(steps):
+        ev[0].record()
+        unpack.unpack_chunk_guppi(raw, nsamp, nov, nbin, nfft, npol, nchan, blocks)
+        ev[1].record()
+        cc.forward(blocks)
+        ev[2].record()
+        for dm in dms:
+            cc.waterfall(blocks, dm, wf)
+        ev[3].record()
+        torch.cuda.synchronize()
+        t_unpack += ev[0].elapsed_time(ev[1]) / steps
+        t_fwd += ev[1].elapsed_time(ev[2]) / steps
+        t_dm += ev[2].elapsed_time(ev[3]) / steps / len(dms)
+    ok = bool(torch.isfinite(wf).all().item()) and float(wf.max().item()) > 0.0
+    chunk = 8.0 * cc.spectra
+    kept = chunk * (nbin // C["len_sft"] - 2 * (nov // C["len_sft"])) / (nbin // C["len_sft"])
+    b_unpack = raw.numel() + chunk
+    b_fwd = 2 * chunk
+    b_dm = 2 * chunk + 2 * chunk + kept + 4.0 * cc.rows * cc.cols
+    gbs = lambda b, ms: b / (ms * 1e-3) / 1e9
+    return {
+        "workload": f"{nchan} coarse ch x {C['len_sft']} sub-ch, {npol // 2} pol, {nfft} blocks of {nbin} bins "
+                    f"({nov} overlap), 8-bit GUPPI, {len(dms)} coherent DMs -> {cc.rows} x {cc.cols} waterfalls",
+        "ms_per_chunk": t_unpack + t_fwd + t_dm * len(dms), "steps": steps,
+        "unpack_ms": t_unpack, "forward_fft_ms": t_fwd, "ms_per_dm": t_dm,
+        "waterfall_samples_per_s": cc.rows * cc.cols * len(dms) / ((t_unpack + t_fwd + t_dm * len(dms)) * 1e-3),
+        "compulsory_GBs": {"unpack": gbs(b_unpack, t_unpack), "forward_fft": gbs(b_fwd, t_fwd),
+                           "per_dm": gbs(b_dm, t_dm)},
+        "frac_of_peak_per_dm": gbs(b_dm, t_dm) / 8000.0,
+        "output_ok": ok,
+    }
+
+
 def bench_large(args):
     """The large single-GPU share: 8192 ch x 2^20 float32 samples (32 GiB),
     10% of the channels replaced by N(0, 25) interference, 50 pulses (sweep up
@@ -432,6 +500,8 @@ def main():
     ap.add_argument("--batch-no-tune", action="store_true",
                     help="batched measurement: take the stored per-channel-count schedules instead of tuning")
     ap.add_argument("--no-large", action="store_true", help="skip the 8192 x 2^20 large-share measurement")
+    ap.add_argument("--no-coherent", action="store_true", help="skip the coherent front-end measurement")
+    ap.add_argument("--coherent-only", action="store_true", help="run the coherent front-end measurement alone")
     ap.add_argument("--large-steps", type=int, default=3)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one plain device copy of this size first (counter calibration under rocprofv3 --pmc)")
@@ -451,6 +521,9 @@ def main():
         return
     if args.large_only:
         print(json.dumps({"metric": METRIC, "n_gpus": 1, "large": bench_large(args)}))
+        return
+    if args.coherent_only:
+        print(json.dumps({"metric": METRIC, "n_gpus": 1, "coherent": bench_coherent(args)}))
         return
     c = CFG
     N, R = c["nsamps"], c["nchans"]
@@ -644,6 +717,9 @@ def main():
         torch.cuda.empty_cache()
     if not args.no_large:
         out["large"] = bench_large(args)
+        torch.cuda.empty_cache()
+    if not args.no_coherent:
+        out["coherent"] = bench_coherent(args)
         torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))

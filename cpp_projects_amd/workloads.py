@@ -27,6 +27,11 @@ PLAIN_CPU = dict(nchans=512, nsamps=1 << 15, dt_max=511, box_q=5, threshold=6.0,
                  min_sweep=50, max_sweep=450, width_range=(1.0, 16.0), bins=(2, 1, 2))
 SINGLE_BEAM = dict(nchans=4096, nsamps=1 << 18, dt_max=4095, box_q=8, threshold=6.0, npulses=20, seed=2)
 BATCH = dict(nbeams=64, nch_choices=(1024, 2048, 4096), n_lo=1 << 14, n_hi=1 << 18, seed=3, box_q=8, threshold=6.0)
+# the coherent front-end on a GUPPI-like chunk: 64 coarse channels of 6.25 MHz
+# (1200-1600 MHz), two polarisations, 32 sub-channels each, 2^16-bin FFT
+# blocks with 2^13 bins of overlap, 16 blocks per chunk, 8 coherent DMs
+COHERENT = dict(f_min=1200.0, f_max=1600.0, npol=4, nchan=64, len_sft=32, nbin=1 << 16, nfft=16, noverlap=1 << 13,
+                tsamp=1.6e-7, dms=(0.0, 25.0, 50.0, 75.0, 100.0, 125.0, 150.0, 175.0), seed=9)
 LARGE = dict(nchans=8192, nsamps=1 << 20, dt_max=8191, box_q=9, threshold=6.0, npulses=50, seed=5, rfi_frac=0.1,
              rfi_sigma=5.0)
 
