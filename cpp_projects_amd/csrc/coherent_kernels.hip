@@ -71,7 +71,17 @@ __global__ void __launch_bounds__(kThreads) k_coherent_chirp(const float2* __res
     const float2 d = make_float2((float)(c * taper / (double)nbin), (float)(s * taper / (double)nbin));
     int64_t idx = (int64_t)ichan * nbin + col;
     const int64_t stride = (int64_t)nbin * nchan;
-    for (int b = 0; b < nblocks; ++b, idx += stride) {
+    // four blocks' loads in flight before their products are stored
+    int b = 0;
+    for (; b + 4 <= nblocks; b += 4, idx += 4 * stride) {
+        float2 a[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) a[u] = in[idx + u * stride];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            out[idx + u * stride] = make_float2(a[u].x * d.x - a[u].y * d.y, a[u].x * d.y + a[u].y * d.x);
+    }
+    for (; b < nblocks; ++b, idx += stride) {
         const float2 a = in[idx];
         out[idx] = make_float2(a.x * d.x - a.y * d.y, a.x * d.y + a.y * d.x);
     }
@@ -84,25 +94,31 @@ __global__ void __launch_bounds__(kThreads) k_coherent_chirp(const float2* __res
 // rounded to float and added in float (roll_rows_normalize_sum_kernel,
 // Chunk_gpu.cu:476-493; transpose_unpadd_intensity, :497-515; dedisperse,
 // :613-632).
+// inv_nbin2 > 0: nbin is a power of two and inv_nbin2 = nbin^-2 exactly, so
+// the two divisions are this one exact scaling (each rounds the same real
+// value once); otherwise the divisions themselves.
 __global__ void __launch_bounds__(kThreads) k_coherent_detect(const float2* __restrict__ buf, float* __restrict__ wf,
                                                               int npol, int nchan, int len_sft, int mbin,
-                                                              int nov_sub, int mbin_adj, int nfft, int64_t msamp,
-                                                              double dm, double fmin, double foff, double tsamp_wf) {
-    const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+                                                              int nov_sub, int mbin_adj, int nfft, int msamp,
+                                                              double dm, double fmin, double foff, double tsamp_wf,
+                                                              double inv_nbin2) {
+    const int t = (int)(blockIdx.x * kThreads + threadIdx.x);
     if (t >= msamp) return;
     const int row = (int)blockIdx.y;
     const int nrows = nchan * len_sft;
     // dedisperse: wf[row][(isamp + ishift) % msamp] = intensity[row][isamp]
+    // (the row's shift is block-uniform; reduced modulo msamp once)
     const double fr = (double)(nrows - 1 - row) * foff + fmin;
     const double delay = 4.148808e3 * dm * (1.0 / (fmin * fmin) - 1.0 / (fr * fr));
     const int64_t ishift = (int64_t)round(delay / tsamp_wf);
-    int64_t isamp = (t - ishift) % msamp;
+    const int sh = (int)(ishift % msamp);  // dm >= 0, so ishift >= 0
+    int isamp = t - sh;
     if (isamp < 0) isamp += msamp;
     // transpose_unpadd_intensity: row = isub * len_sft + len_sft - 1 - ich
     const int isub = row / len_sft;
     const int ich = len_sft - 1 - row % len_sft;
-    const int ifft = (int)(isamp / mbin_adj);
-    const int ibin = (int)(isamp % mbin_adj) + nov_sub;
+    const int ifft = isamp / mbin_adj;
+    const int ibin = isamp - ifft * mbin_adj + nov_sub;
     const int c = nchan - 1 - isub;
     // roll_rows_normalize_sum_kernel: rolled[j] = f(block[(j - nbin/2) mod nbin])
     const int nbin = mbin * len_sft;
@@ -114,7 +130,8 @@ __global__ void __launch_bounds__(kThreads) k_coherent_detect(const float2* __re
     for (int p = 0; p < npol / 2; ++p, idx += plane) {
         const float2 a = buf[idx];
         const double x = a.x, y = a.y;
-        v += (float)((x * x + y * y) / nbin / nbin);
+        const double e = x * x + y * y;
+        v += (float)(inv_nbin2 > 0.0 ? e * inv_nbin2 : e / nbin / nbin);
     }
     wf[(int64_t)row * msamp + t] = v;
 }
@@ -161,9 +178,11 @@ void coherent_detect(const sps_coherent& h, const float* d_buf, double dm, float
     // (double)(m_len_sft * m_tsamp): the product is taken in float
     const double tsamp_wf = (double)((float)(unsigned)s.len_sft * s.tsamp);
     const dim3 grid((unsigned)ceil_div(s.msamp(), (int64_t)kThreads), (unsigned)s.rows());
+    const bool pow2 = (s.nbin & (s.nbin - 1)) == 0;
+    const double inv_nbin2 = pow2 ? std::ldexp(1.0, -2 * (int)std::log2((double)s.nbin)) : 0.0;
     hipLaunchKernelGGL(k_coherent_detect, grid, dim3(kThreads), 0, st, reinterpret_cast<const float2*>(d_buf), d_wf,
-                       s.npol, s.nchan, s.len_sft, s.mbin(), s.nov_sub(), s.mbin_adj(), s.nfft, s.msamp(), dm, s.fmin,
-                       foff, tsamp_wf);
+                       s.npol, s.nchan, s.len_sft, s.mbin(), s.nov_sub(), s.mbin_adj(), s.nfft, (int)s.msamp(), dm,
+                       s.fmin, foff, tsamp_wf, inv_nbin2);
     SPS_HIP(hipGetLastError());
 }
 
@@ -191,6 +210,7 @@ SPS_API sps_status sps_coherent_create(float f_min, float f_max, int npol, int n
         SPS_REQUIRE(h->sh.mbin_adj() >= 1, "the overlap leaves no samples per block");
         SPS_REQUIRE(h->sh.spectra() < ((int64_t)1 << 31), "chunk too large");
         SPS_REQUIRE(h->sh.rows() <= 65535, "nchan * len_sft must be below 65536");
+        SPS_REQUIRE(h->sh.msamp() < ((int64_t)1 << 31), "waterfall rows too long");
         // create_fft_plans (Chunk_gpu.cu:197-209)
         int n1 = nbin, n2 = h->sh.mbin();
         check_hipfft(hipfftPlanMany(&h->fwd, 1, &n1, nullptr, 1, nbin, nullptr, 1, nbin, HIPFFT_C2C,

@@ -153,3 +153,7 @@ def test_coherent_size_checks(torch_cuda):
     small = torch.zeros(16, dtype=torch.complex64, device="cuda")
     with pytest.raises(ValueError):
         cc.forward(small)
+    spec = torch.zeros(cc.spectra, dtype=torch.complex64, device="cuda")
+    wf = torch.empty((cc.rows, cc.cols), dtype=torch.float32, device="cuda")
+    with pytest.raises(_lib.InvalidArgument):  # the DM is a non-negative dispersion measure
+        cc.detect(spec, -30.0, wf)
