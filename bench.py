@@ -323,9 +323,11 @@ def bench_coherent(args):
     workloads.py COHERENT): 8-bit unpack, forward FFT, then per coherent DM
     the chirp, the sub-band inverse FFTs and the intensity waterfall that the
     search takes.  Compulsory bytes: unpack reads the raw chunk and writes the
-    blocks; each FFT reads and writes the chunk once; per DM the chirp reads
-    and writes it, the inverse FFT reads and writes it, the intensity pass
-    reads the kept bins and writes the waterfall."""
+    blocks; the forward FFT reads and writes the chunk once; per DM the fused
+    pass reads the spectra once and writes the waterfall (its 32 MB factor
+    table is not counted).  Where the chunk does not take the fused pass, per
+    DM the chirp reads and writes the chunk, the inverse FFT reads and writes
+    it, the intensity pass reads the kept bins and writes the waterfall."""
     import torch
 
     from cpp_projects_amd import unpack
@@ -371,12 +373,15 @@ def bench_coherent(args):
     kept = chunk * (nbin // C["len_sft"] - 2 * (nov // C["len_sft"])) / (nbin // C["len_sft"])
     b_unpack = raw.numel() + chunk
     b_fwd = 2 * chunk
-    b_dm = 2 * chunk + 2 * chunk + kept + 4.0 * cc.rows * cc.cols
+    if cc.fused:
+        b_dm = chunk + 4.0 * cc.rows * cc.cols
+    else:
+        b_dm = 2 * chunk + 2 * chunk + kept + 4.0 * cc.rows * cc.cols
     gbs = lambda b, ms: b / (ms * 1e-3) / 1e9
     return {
         "workload": f"{nchan} coarse ch x {C['len_sft']} sub-ch, {npol // 2} pol, {nfft} blocks of {nbin} bins "
                     f"({nov} overlap), 8-bit GUPPI, {len(dms)} coherent DMs -> {cc.rows} x {cc.cols} waterfalls",
-        "ms_per_chunk": t_unpack + t_fwd + t_dm * len(dms), "steps": steps,
+        "fused": cc.fused, "ms_per_chunk": t_unpack + t_fwd + t_dm * len(dms), "steps": steps,
         "unpack_ms": t_unpack, "forward_fft_ms": t_fwd, "ms_per_dm": t_dm,
         "waterfall_samples_per_s": cc.rows * cc.cols * len(dms) / ((t_unpack + t_fwd + t_dm * len(dms)) * 1e-3),
         "compulsory_GBs": {"unpack": gbs(b_unpack, t_unpack), "forward_fft": gbs(b_fwd, t_fwd),

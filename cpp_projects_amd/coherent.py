@@ -24,6 +24,12 @@ class CoherentChunk:
         r, c, n = C.c_int64(), C.c_int64(), C.c_int64()
         call("sps_coherent_get_shape", h, C.byref(r), C.byref(c), C.byref(n))
         self.rows, self.cols, self.spectra = r.value, c.value, n.value
+        f = C.c_int()
+        call("sps_coherent_is_fused", h, C.byref(f))
+        # waterfall() runs one fused pass per DM (power-of-two sub-band length
+        # 128..4096, even sub-band count) rather than chirp, inverse FFT and
+        # intensity pass
+        self.fused = bool(f.value)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -41,7 +47,8 @@ class CoherentChunk:
         call("sps_coherent_forward", self._h, _dptr(d_blocks), _stream())
 
     def waterfall(self, d_spectra, dm, d_waterfall):
-        """Chirp, inverse FFT and intensity waterfall for one coherent DM."""
+        """Chirp, inverse FFT and intensity waterfall for one coherent DM
+        (Chunk_gpu.cu:276-322), fused into one pass where `fused`."""
         self._check(d_spectra, 2 * self.spectra, "spectra")
         self._check(d_waterfall, self.rows * self.cols, "waterfall")
         call("sps_coherent_waterfall", self._h, _dptr(d_spectra), float(dm), _dptr(d_waterfall), _stream())

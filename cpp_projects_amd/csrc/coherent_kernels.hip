@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
 
+#include <algorithm>
 #include <cmath>
 #include <memory>
 
@@ -46,17 +47,12 @@ struct CoherentShape {
     int64_t spectra() const { return (int64_t)nfft * nchan * (npol / 2) * nbin; }  // complex values
 };
 
-// kernel_el_wise_mult_onthe_fly (Chunk_fly_gpu.cu:119-164): one thread per
-// (chan, sub-band, bin); the factor for every block of that channel.  The
-// sub-band centre is taken to float as in the reference, the rest in double.
-__global__ void __launch_bounds__(kThreads) k_coherent_chirp(const float2* __restrict__ in, float2* __restrict__ out,
-                                                             double dm, int nchan, int len_sft, int mbin,
-                                                             int nblocks, double fmin, double bw_sub,
-                                                             double bw_chan) {
-    const int ibin = (int)(blockIdx.x * kThreads + threadIdx.x);
-    if (ibin >= mbin) return;
-    const int isft = (int)blockIdx.y;
-    const int ichan = (int)blockIdx.z;
+// The chirp factor of (chan, sub-band, bin) (kernel_el_wise_mult_onthe_fly,
+// Chunk_fly_gpu.cu:119-164): the sub-band centre is taken to float as in the
+// reference, the rest in double; the factor includes the taper and 1/nbin.
+// *col receives its column in the half-rotated spectrum row.
+__device__ inline float2 chirp_factor(int ichan, int isft, int ibin, int len_sft, int mbin, double dm, double fmin,
+                                      double bw_sub, double bw_chan, int* col) {
     const float f0 = (float)(fmin + bw_sub * (0.5 + ichan) + bw_chan * (isft - len_sft / 2.0 + 0.5));
     const double bf = -0.5 * bw_chan + (ibin + 0.5) * bw_chan / mbin;
     const double taper = 1.0 / sqrt(1.0 + pow(bf / (0.47 * bw_chan), 80.0));
@@ -67,8 +63,28 @@ __global__ void __launch_bounds__(kThreads) k_coherent_chirp(const float2* __res
     double s, c;
     sincospi(t, &s, &c);
     const int nbin = mbin * len_sft;
-    const int col = (isft * mbin + ibin + nbin / 2) % nbin;
-    const float2 d = make_float2((float)(c * taper / (double)nbin), (float)(s * taper / (double)nbin));
+    *col = (isft * mbin + ibin + nbin / 2) % nbin;
+    return make_float2((float)(c * taper / (double)nbin), (float)(s * taper / (double)nbin));
+}
+
+// a * d, each product and sum rounded (no contraction)
+__device__ inline float2 chirp_mul(float2 a, float2 d) {
+    return make_float2(a.x * d.x - a.y * d.y, a.x * d.y + a.y * d.x);
+}
+
+// One thread per (chan, sub-band, bin); the factor for every block of that
+// channel.
+__global__ void __launch_bounds__(kThreads) k_coherent_chirp(const float2* __restrict__ in, float2* __restrict__ out,
+                                                             double dm, int nchan, int len_sft, int mbin,
+                                                             int nblocks, double fmin, double bw_sub,
+                                                             double bw_chan) {
+    const int ibin = (int)(blockIdx.x * kThreads + threadIdx.x);
+    if (ibin >= mbin) return;
+    const int isft = (int)blockIdx.y;
+    const int ichan = (int)blockIdx.z;
+    const int nbin = mbin * len_sft;
+    int col;
+    const float2 d = chirp_factor(ichan, isft, ibin, len_sft, mbin, dm, fmin, bw_sub, bw_chan, &col);
     int64_t idx = (int64_t)ichan * nbin + col;
     const int64_t stride = (int64_t)nbin * nchan;
     // four blocks' loads in flight before their products are stored
@@ -78,13 +94,42 @@ __global__ void __launch_bounds__(kThreads) k_coherent_chirp(const float2* __res
 #pragma unroll
         for (int u = 0; u < 4; ++u) a[u] = in[idx + u * stride];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-            out[idx + u * stride] = make_float2(a[u].x * d.x - a[u].y * d.y, a[u].x * d.y + a[u].y * d.x);
+        for (int u = 0; u < 4; ++u) out[idx + u * stride] = chirp_mul(a[u], d);
     }
-    for (; b < nblocks; ++b, idx += stride) {
-        const float2 a = in[idx];
-        out[idx] = make_float2(a.x * d.x - a.y * d.y, a.x * d.y + a.y * d.x);
-    }
+    for (; b < nblocks; ++b, idx += stride) out[idx] = chirp_mul(in[idx], d);
+}
+
+// The factors alone, fac[chan][col] (the spectrum rows' layout), for the
+// fused pass.
+__global__ void __launch_bounds__(kThreads) k_coherent_factors(float2* __restrict__ fac, double dm, int len_sft,
+                                                               int mbin, double fmin, double bw_sub, double bw_chan) {
+    const int ibin = (int)(blockIdx.x * kThreads + threadIdx.x);
+    if (ibin >= mbin) return;
+    const int isft = (int)blockIdx.y;
+    const int ichan = (int)blockIdx.z;
+    int col;
+    const float2 d = chirp_factor(ichan, isft, ibin, len_sft, mbin, dm, fmin, bw_sub, bw_chan, &col);
+    fac[(int64_t)ichan * mbin * len_sft + col] = d;
+}
+
+// dedisperse (Chunk_gpu.cu:613-632): the shift of waterfall row `row`,
+// round(4.148808e3 * dm * (fmin^-2 - f_row^-2) / tsamp_wf), reduced modulo
+// msamp (dm >= 0, so the shift is >= 0).
+__device__ inline int dedisp_shift(int row, int nrows, int msamp, double dm, double fmin, double foff,
+                                   double tsamp_wf) {
+    const double fr = (double)(nrows - 1 - row) * foff + fmin;
+    const double delay = 4.148808e3 * dm * (1.0 / (fmin * fmin) - 1.0 / (fr * fr));
+    const int64_t ishift = (int64_t)round(delay / tsamp_wf);
+    return (int)(ishift % msamp);
+}
+
+// ((x*x + y*y) / nbin) / nbin in double, rounded to float; inv_nbin2 > 0
+// means nbin is a power of two and inv_nbin2 = nbin^-2 exactly, and the one
+// exact scaling rounds the same real value once as the two divisions do.
+__device__ inline float pol_intensity(float2 a, int nbin, double inv_nbin2) {
+    const double x = a.x, y = a.y;
+    const double e = x * x + y * y;
+    return (float)(inv_nbin2 > 0.0 ? e * inv_nbin2 : e / nbin / nbin);
 }
 
 // The intensity waterfall of one DM, wf[row][t] (see the file comment),
@@ -94,9 +139,6 @@ __global__ void __launch_bounds__(kThreads) k_coherent_chirp(const float2* __res
 // rounded to float and added in float (roll_rows_normalize_sum_kernel,
 // Chunk_gpu.cu:476-493; transpose_unpadd_intensity, :497-515; dedisperse,
 // :613-632).
-// inv_nbin2 > 0: nbin is a power of two and inv_nbin2 = nbin^-2 exactly, so
-// the two divisions are this one exact scaling (each rounds the same real
-// value once); otherwise the divisions themselves.
 __global__ void __launch_bounds__(kThreads) k_coherent_detect(const float2* __restrict__ buf, float* __restrict__ wf,
                                                               int npol, int nchan, int len_sft, int mbin,
                                                               int nov_sub, int mbin_adj, int nfft, int msamp,
@@ -107,12 +149,7 @@ __global__ void __launch_bounds__(kThreads) k_coherent_detect(const float2* __re
     const int row = (int)blockIdx.y;
     const int nrows = nchan * len_sft;
     // dedisperse: wf[row][(isamp + ishift) % msamp] = intensity[row][isamp]
-    // (the row's shift is block-uniform; reduced modulo msamp once)
-    const double fr = (double)(nrows - 1 - row) * foff + fmin;
-    const double delay = 4.148808e3 * dm * (1.0 / (fmin * fmin) - 1.0 / (fr * fr));
-    const int64_t ishift = (int64_t)round(delay / tsamp_wf);
-    const int sh = (int)(ishift % msamp);  // dm >= 0, so ishift >= 0
-    int isamp = t - sh;
+    int isamp = t - dedisp_shift(row, nrows, msamp, dm, fmin, foff, tsamp_wf);
     if (isamp < 0) isamp += msamp;
     // transpose_unpadd_intensity: row = isub * len_sft + len_sft - 1 - ich
     const int isub = row / len_sft;
@@ -127,13 +164,213 @@ __global__ void __launch_bounds__(kThreads) k_coherent_detect(const float2* __re
     int64_t idx = ((int64_t)ifft * nchan + c) * nbin + src;
     const int64_t plane = (int64_t)nfft * nchan * nbin;
     float v = 0.0f;
-    for (int p = 0; p < npol / 2; ++p, idx += plane) {
-        const float2 a = buf[idx];
-        const double x = a.x, y = a.y;
-        const double e = x * x + y * y;
-        v += (float)(inv_nbin2 > 0.0 ? e * inv_nbin2 : e / nbin / nbin);
-    }
+    for (int p = 0; p < npol / 2; ++p, idx += plane) v += pol_intensity(buf[idx], nbin, inv_nbin2);
     wf[(int64_t)row * msamp + t] = v;
+}
+
+// ---- the fused per-DM pass: chirp, inverse FFT and detection in one kernel
+//
+// One block per (channel c, sub-band segment k, run of blocks): for each block
+// and polarisation it loads the segment's mbin = N spectrum values, multiplies
+// them by their chirp factors, takes the unnormalised inverse DFT of length N
+// (exp(+2 pi i n q / N), as the inverse FFT of step 3), and adds each output's
+// intensity over the polarisations in registers; the kept samples go straight
+// to their dedispersed waterfall positions.  The segment's output lands in
+// one waterfall row: the half-row rotation moves segment k to sub-channel
+// ich = (k + len_sft/2) mod len_sft (len_sft even, so nbin/2 is a whole number
+// of segments).
+//
+// The DFT is a Stockham autosort FFT over N = 8^a * {1, 2, 4} points with
+// T = N/8 lanes: lane t always holds elements t + T*m (m = 0..7) of the
+// current sequence; a pass of radix R does butterflies j = t + T*u
+// (u < 8/R) on x[j + r*N/R] = z[u + r*8/R], twiddles them by
+// exp(2 pi i r (j mod Ns) / (Ns R)) and writes output q of butterfly j to
+// (j / Ns) * Ns * R + (j mod Ns) + q * Ns, where Ns is the product of the
+// earlier radices.  After the last pass (Ns R = N) that position is
+// t + T*(u + q*8/R), so the lane keeps its outputs in registers.  Between
+// passes the sequence goes through LDS with a pass-dependent padding (one
+// complex every 32, then every 8, then none) that keeps the scattered writes
+// and linear reads near conflict-free.
+
+template <int R>
+__device__ inline void dft_inv(float2 (&v)[R]);
+
+__device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ inline float2 mul_i(float2 a) { return make_float2(-a.y, a.x); }  // a * i
+// a * w with fused multiply-adds (the FFT's own rounding, not the reference's)
+__device__ inline float2 cmul_fma(float2 a, float2 w) {
+    return make_float2(fmaf(a.x, w.x, -a.y * w.y), fmaf(a.x, w.y, a.y * w.x));
+}
+
+template <>
+__device__ inline void dft_inv<2>(float2 (&v)[2]) {
+    const float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+
+template <>
+__device__ inline void dft_inv<4>(float2 (&v)[4]) {
+    const float2 s0 = cadd(v[0], v[2]), s1 = csub(v[0], v[2]);
+    const float2 s2 = cadd(v[1], v[3]), s3 = mul_i(csub(v[1], v[3]));
+    v[0] = cadd(s0, s2);
+    v[2] = csub(s0, s2);
+    v[1] = cadd(s1, s3);
+    v[3] = csub(s1, s3);
+}
+
+template <>
+__device__ inline void dft_inv<8>(float2 (&v)[8]) {
+    float2 e[4] = {v[0], v[2], v[4], v[6]};
+    float2 o[4] = {v[1], v[3], v[5], v[7]};
+    dft_inv<4>(e);
+    dft_inv<4>(o);
+    constexpr float h = 0.70710678118654752440f;
+    // o[q] * exp(2 pi i q / 8)
+    const float2 o1 = make_float2((o[1].x - o[1].y) * h, (o[1].x + o[1].y) * h);
+    const float2 o2 = mul_i(o[2]);
+    const float2 o3 = make_float2(-(o[3].x + o[3].y) * h, (o[3].x - o[3].y) * h);
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);
+    v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);
+    v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);
+    v[7] = csub(e[3], o3);
+}
+
+template <int SH>
+__device__ inline int fft_pad(int i) {
+    return SH >= 31 ? i : i + (i >> SH);
+}
+
+template <int P>
+struct FftPad {
+    static constexpr int sh = P == 0 ? 5 : (P == 1 ? 3 : 31);
+};
+
+template <int N>
+struct FftLds {
+    static constexpr int size = N + N / 8;  // the widest padding
+};
+
+// the twiddle bases exp(2 pi i (j mod Ns) / (Ns R)) of every pass after the
+// first, per butterfly slot u: tw[TB + u]
+template <int N, int NS, int TB>
+__device__ inline void fft_twiddles(float2 (&tw)[8], int t) {
+    constexpr int T = N / 8;
+    constexpr int R = (N / NS) >= 8 ? 8 : N / NS;
+    constexpr int U = 8 / R;
+    if constexpr (NS > 1) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int kk = (t + T * u) & (NS - 1);
+            float s, c;
+            sincospif(2.0f * (float)kk / (float)(NS * R), &s, &c);
+            tw[TB + u] = make_float2(c, s);
+        }
+    }
+    if constexpr (NS * R < N) fft_twiddles<N, NS * R, TB + (NS > 1 ? U : 0)>(tw, t);
+}
+
+template <int N, int NS, int P, int TB>
+__device__ inline void fft_passes(float2 (&z)[8], float2* lds, int t, const float2 (&tw)[8]) {
+    constexpr int T = N / 8;
+    constexpr int R = (N / NS) >= 8 ? 8 : N / NS;
+    constexpr int U = 8 / R;
+    constexpr bool last = NS * R == N;
+    float2 o[8];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        float2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = z[u + r * U];
+        if constexpr (NS > 1) {
+            const float2 w1 = tw[TB + u];
+            float2 w = w1;
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                v[r] = cmul_fma(v[r], w);
+                if (r + 1 < R) w = cmul_fma(w, w1);
+            }
+        }
+        dft_inv<R>(v);
+#pragma unroll
+        for (int q = 0; q < R; ++q) o[u + q * U] = v[q];
+    }
+    if constexpr (last) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) z[m] = o[m];
+    } else {
+        constexpr int sh = FftPad<P>::sh;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = t + T * u;
+            const int kk = j & (NS - 1);
+            const int base = (j - kk) * R + kk;
+#pragma unroll
+            for (int q = 0; q < R; ++q) lds[fft_pad<sh>(base + q * NS)] = o[u + q * U];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < 8; ++m) z[m] = lds[fft_pad<sh>(t + T * m)];
+        __syncthreads();
+        fft_passes<N, NS * R, P + 1, TB + (NS > 1 ? U : 0)>(z, lds, t, tw);
+    }
+}
+
+template <int N>
+__global__ void __launch_bounds__(N / 8) k_coherent_fused(const float2* __restrict__ spec,
+                                                          const float2* __restrict__ fac, float* __restrict__ wf,
+                                                          int npairs, int nchan, int len_sft, int nfft, int bpw,
+                                                          int nov_sub, int mbin_adj, int msamp, double dm,
+                                                          double fmin, double foff, double tsamp_wf,
+                                                          double inv_nbin2) {
+    constexpr int T = N / 8;
+    __shared__ float2 lds[FftLds<N>::size];
+    const int t = (int)threadIdx.x;
+    const int seg = (int)blockIdx.x;
+    const int c = seg / len_sft;
+    const int k = seg - c * len_sft;
+    const int ich = (k + len_sft / 2) % len_sft;
+    const int nrows = nchan * len_sft;
+    const int row = (nchan - 1 - c) * len_sft + len_sft - 1 - ich;
+    const int sh = dedisp_shift(row, nrows, msamp, dm, fmin, foff, tsamp_wf);
+    const int nbin = N * len_sft;
+    const int64_t seg_off = (int64_t)c * nbin + (int64_t)k * N + t;
+    float2 d[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) d[m] = fac[seg_off + T * m];
+    float2 tw[8];
+    fft_twiddles<N, 1, 0>(tw, t);
+    const int64_t plane = (int64_t)nfft * nchan * nbin;
+    const int b0 = (int)blockIdx.y * bpw;
+    const int b1 = min(nfft, b0 + bpw);
+    float* out = wf + (int64_t)row * msamp;
+    for (int b = b0; b < b1; ++b) {
+        float v[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) v[m] = 0.0f;
+        const float2* src = spec + (int64_t)b * nchan * nbin + seg_off;
+        for (int p = 0; p < npairs; ++p, src += plane) {
+            float2 z[8];
+#pragma unroll
+            for (int m = 0; m < 8; ++m) z[m] = chirp_mul(src[T * m], d[m]);
+            fft_passes<N, 1, 0, 0>(z, lds, t, tw);
+#pragma unroll
+            for (int m = 0; m < 8; ++m) v[m] += pol_intensity(z[m], nbin, inv_nbin2);
+        }
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const int n = t + T * m;
+            if (n >= nov_sub && n < N - nov_sub) {
+                const int s0 = b * mbin_adj + (n - nov_sub);  // + sh, modulo msamp
+                out[s0 >= msamp - sh ? s0 - (msamp - sh) : s0 + sh] = v[m];
+            }
+        }
+    }
 }
 
 void check_hipfft(hipfftResult r, const char* what) {
@@ -149,7 +386,8 @@ struct sps_coherent {
     CoherentShape sh{};
     hipfftHandle fwd = 0, inv = 0;
     bool have_fwd = false, have_inv = false;
-    DevBuf chirped;  // the chirped spectra, then their sub-band time series
+    DevBuf chirped;  // the chirped spectra, then their sub-band time series (unfused path)
+    DevBuf factors;  // one DM's chirp factors [chan][nbin] (fused path)
     ~sps_coherent() {
         if (have_fwd) (void)hipfftDestroy(fwd);
         if (have_inv) (void)hipfftDestroy(inv);
@@ -172,6 +410,14 @@ void coherent_chirp(const sps_coherent& h, const float* d_in, double dm, float* 
     SPS_HIP(hipGetLastError());
 }
 
+// the fused pass applies: a power-of-two sub-band length the kernel is built
+// for, and an even sub-band count (the half-row rotation moves whole
+// segments)
+bool coherent_fusable(const CoherentShape& s) {
+    const int n = s.mbin();
+    return s.len_sft % 2 == 0 && n >= 128 && n <= 4096 && (n & (n - 1)) == 0;
+}
+
 void coherent_detect(const sps_coherent& h, const float* d_buf, double dm, float* d_wf, hipStream_t st) {
     const CoherentShape& s = h.sh;
     const double foff = (s.fmax - s.fmin) / ((double)s.nchan * s.len_sft);
@@ -186,10 +432,52 @@ void coherent_detect(const sps_coherent& h, const float* d_buf, double dm, float
     SPS_HIP(hipGetLastError());
 }
 
+template <int N>
+void launch_fused(const CoherentShape& s, const float2* spec, const float2* fac, float* wf, double dm, int bpw,
+                  double foff, double tsamp_wf, double inv_nbin2, hipStream_t st) {
+    const dim3 grid((unsigned)s.rows(), (unsigned)ceil_div((int64_t)s.nfft, (int64_t)bpw));
+    hipLaunchKernelGGL(k_coherent_fused<N>, grid, dim3(N / 8), 0, st, spec, fac, wf, s.npol / 2, s.nchan, s.len_sft,
+                       s.nfft, bpw, s.nov_sub(), s.mbin_adj(), (int)s.msamp(), dm, s.fmin, foff, tsamp_wf, inv_nbin2);
+}
+
+void coherent_fused(sps_coherent& h, const float* d_spectra, double dm, float* d_wf, hipStream_t st);
+
 void coherent_fft(sps_coherent& h, hipfftHandle plan, float* d, int dir, hipStream_t st) {
     check_hipfft(hipfftSetStream(plan, st), "set stream");
     check_hipfft(hipfftExecC2C(plan, reinterpret_cast<hipfftComplex*>(d), reinterpret_cast<hipfftComplex*>(d), dir),
                  "C2C");
+}
+
+void coherent_fused(sps_coherent& h, const float* d_spectra, double dm, float* d_wf, hipStream_t st) {
+    const CoherentShape& s = h.sh;
+    const double bw = (double)((float)s.fmax - (float)s.fmin);
+    const double bw_sub = bw / s.nchan;
+    const double bw_chan = bw_sub / s.len_sft;
+    float2* fac = h.factors.as<float2>();
+    const dim3 fgrid((unsigned)ceil_div((int64_t)s.mbin(), (int64_t)kThreads), (unsigned)s.len_sft,
+                     (unsigned)s.nchan);
+    hipLaunchKernelGGL(k_coherent_factors, fgrid, dim3(kThreads), 0, st, fac, dm, s.len_sft, s.mbin(), s.fmin, bw_sub,
+                       bw_chan);
+    SPS_HIP(hipGetLastError());
+    const double foff = (s.fmax - s.fmin) / ((double)s.nchan * s.len_sft);
+    const double tsamp_wf = (double)((float)(unsigned)s.len_sft * s.tsamp);
+    const bool pow2 = (s.nbin & (s.nbin - 1)) == 0;
+    const double inv_nbin2 = pow2 ? std::ldexp(1.0, -2 * (int)std::log2((double)s.nbin)) : 0.0;
+    // blocks per launch block: enough launch blocks to fill the chip several
+    // times over, each reusing its segment's factors for the rest
+    const int64_t want = 8192;
+    const int bpw = (int)std::max<int64_t>(1, std::min<int64_t>(s.nfft, ceil_div(s.rows() * s.nfft, want)));
+    const float2* spec = reinterpret_cast<const float2*>(d_spectra);
+    switch (s.mbin()) {
+        case 128: launch_fused<128>(s, spec, fac, d_wf, dm, bpw, foff, tsamp_wf, inv_nbin2, st); break;
+        case 256: launch_fused<256>(s, spec, fac, d_wf, dm, bpw, foff, tsamp_wf, inv_nbin2, st); break;
+        case 512: launch_fused<512>(s, spec, fac, d_wf, dm, bpw, foff, tsamp_wf, inv_nbin2, st); break;
+        case 1024: launch_fused<1024>(s, spec, fac, d_wf, dm, bpw, foff, tsamp_wf, inv_nbin2, st); break;
+        case 2048: launch_fused<2048>(s, spec, fac, d_wf, dm, bpw, foff, tsamp_wf, inv_nbin2, st); break;
+        case 4096: launch_fused<4096>(s, spec, fac, d_wf, dm, bpw, foff, tsamp_wf, inv_nbin2, st); break;
+        default: throw Error(SPS_ERR_RUNTIME, "fused coherent pass: unsupported sub-band length");
+    }
+    SPS_HIP(hipGetLastError());
 }
 
 }  // namespace
@@ -221,7 +509,7 @@ SPS_API sps_status sps_coherent_create(float f_min, float f_max, int npol, int n
                                     nfft * nchan * len_sft * (npol / 2)),
                      "inverse plan");
         h->have_inv = true;
-        h->chirped.ensure(sizeof(float) * 2 * (size_t)h->sh.spectra());
+        if (coherent_fusable(h->sh)) h->factors.ensure(sizeof(float) * 2 * (size_t)nchan * nbin);
         *out = h.release();
     });
 }
@@ -237,6 +525,13 @@ SPS_API sps_status sps_coherent_get_shape(const sps_coherent* h, int64_t* h_rows
         if (h_rows) *h_rows = h->sh.rows();
         if (h_cols) *h_cols = h->sh.msamp();
         if (h_spectra) *h_spectra = h->sh.spectra();
+    });
+}
+
+SPS_API sps_status sps_coherent_is_fused(const sps_coherent* h, int* h_fused) {
+    return guarded([&] {
+        SPS_REQUIRE(h && h_fused, "null argument");
+        *h_fused = coherent_fusable(h->sh) ? 1 : 0;
     });
 }
 
@@ -278,6 +573,11 @@ SPS_API sps_status sps_coherent_waterfall(sps_coherent* h, const float* d_spectr
         SPS_REQUIRE(h && d_spectra && d_waterfall, "null argument");
         SPS_REQUIRE(dm >= 0.0 && std::isfinite(dm), "the coherent DM must be finite and non-negative");
         hipStream_t st = as_stream(stream);
+        if (coherent_fusable(h->sh)) {
+            coherent_fused(*h, d_spectra, dm, d_waterfall, st);
+            return;
+        }
+        h->chirped.ensure(sizeof(float) * 2 * (size_t)h->sh.spectra());
         float* buf = h->chirped.as<float>();
         coherent_chirp(*h, d_spectra, dm, buf, st);
         coherent_fft(*h, h->inv, buf, HIPFFT_BACKWARD, st);

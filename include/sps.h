@@ -282,7 +282,8 @@ SPS_API sps_status sps_unpack_lofar(const int8_t* d_in, int64_t nsamp, int nover
  * :254-322 steps 2-8; Chunk_fly_gpu.cu:94-164 chirp).  A handle holds the
  * two rocFFT plans of CChunk_gpu::create_fft_plans (forward: nbin points,
  * nfft * nchan * npol/2 blocks; inverse: mbin = nbin / len_sft points, len_sft
- * times as many) and one scratch chunk.  Spectra are complex float
+ * times as many), a one-DM chirp factor table (nchan * nbin complex) and,
+ * on the unfused path only, one scratch chunk.  Spectra are complex float
  * [pol][fft][chan][bin] (the unpack's layout, npol/2 polarisations); the
  * waterfall is float [nchan * len_sft][msamp], msamp = nfft * (mbin - 2 *
  * (noverlap / len_sft)), rows in the reference's order (both frequency axes
@@ -295,10 +296,17 @@ SPS_API sps_status sps_coherent_destroy(sps_coherent* h);
 /* waterfall rows and columns, and the complex values of one chunk */
 SPS_API sps_status sps_coherent_get_shape(const sps_coherent* h, int64_t* h_rows, int64_t* h_cols,
                                           int64_t* h_spectra);
+/* *h_fused = 1 when sps_coherent_waterfall takes the fused pass (no
+ * reference counterpart: a property of this implementation) */
+SPS_API sps_status sps_coherent_is_fused(const sps_coherent* h, int* h_fused);
 /* step 2: cufftExecC2C(m_fftPlanForward) in place (Chunk_gpu.cu:267) */
 SPS_API sps_status sps_coherent_forward(sps_coherent* h, float* d_blocks, void* stream);
-/* steps 4-8 for one coherent DM: chirp into the handle's scratch, inverse
- * FFT there, then the intensity waterfall into d_waterfall */
+/* steps 4-8 for one coherent DM into d_waterfall.  When the sub-band length
+ * mbin is a power of two in 128..4096 and len_sft is even, one fused pass per
+ * DM does the chirp, a length-mbin inverse FFT in LDS and the intensity
+ * waterfall, reading the spectra once (results within float FFT rounding of
+ * the step-by-step path); otherwise chirp into the handle's scratch chunk,
+ * inverse FFT there (rocFFT) and the intensity pass. */
 SPS_API sps_status sps_coherent_waterfall(sps_coherent* h, const float* d_spectra, double dm, float* d_waterfall,
                                           void* stream);
 /* the steps one at a time: the chirp (kernel_el_wise_mult_onthe_fly,

@@ -24,6 +24,9 @@ public:
                   float tsamp) {
         sps_detail::check(sps_coherent_create(Fmin, Fmax, npol, nchan, len_sft, nbin, nfft, noverlap, tsamp, &m_h));
         sps_detail::check(sps_coherent_get_shape(m_h, &m_rows, &m_cols, &m_spectra));
+        int f = 0;
+        sps_detail::check(sps_coherent_is_fused(m_h, &f));
+        m_fused = f != 0;
     }
     ~CoherentChunk() {
         if (m_h) (void)sps_coherent_destroy(m_h);
@@ -36,6 +39,8 @@ public:
     int64_t rows() const { return m_rows; }
     int64_t msamp() const { return m_cols; }
     int64_t spectra() const { return m_spectra; }
+    // waterfall() runs one fused pass per DM
+    bool fused() const { return m_fused; }
 
     // Chunk_gpu.cu:267
     void forward(float2* pcmparrRawSignalCur, hipStream_t stream = nullptr) {
@@ -63,4 +68,5 @@ public:
 private:
     sps_coherent* m_h = nullptr;
     int64_t m_rows = 0, m_cols = 0, m_spectra = 0;
+    bool m_fused = false;
 };

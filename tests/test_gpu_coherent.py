@@ -4,7 +4,11 @@ the C ABI: the intensity pass bit-exact against the oracle's literal
 restatement of the reference's three passes, the chirp within float rounding
 of the oracle's double-precision factors, the whole chain (rocFFT) within
 float-FFT tolerance of double-precision numpy FFTs, and a pure tone landing
-in the waterfall row the reference's channel order puts it in."""
+in the waterfall row the reference's channel order puts it in.  The chain runs
+the fused per-DM pass (chirp, LDS inverse FFT and detection in one kernel) for
+power-of-two sub-band lengths 128..4096 with an even sub-band count, and the
+three-step path (chirp, rocFFT inverse, intensity pass) otherwise; both are
+checked, and against each other."""
 import numpy as np
 import pytest
 
@@ -78,7 +82,19 @@ def test_chirp_vs_oracle(torch_cuda, geom):
         assert exact > 0.99, exact
 
 
-@pytest.mark.parametrize("geom", GEOM)
+# the chain's geometries: every fused sub-band length (nbin / len_sft = 128 ..
+# 4096), then two the fused pass does not take (96 bins; an odd sub-band count)
+CHAIN_GEOM = GEOM + [
+    (1200.0, 1600.0, 4, 2, 4, 2048, 2, 256, 1.0e-6),     # mbin 512
+    (1300.0, 1500.0, 2, 2, 8, 8192, 2, 512, 2.0e-7),     # mbin 1024
+    (1200.0, 1600.0, 4, 2, 4, 8192, 2, 1024, 1.6e-7),    # mbin 2048, the bench's sub-band length
+    (1200.0, 1600.0, 2, 1, 4, 16384, 3, 2048, 1.0e-7),   # mbin 4096
+    (1200.0, 1600.0, 4, 2, 8, 768, 2, 64, 1.0e-6),       # mbin 96: three-step path
+    (1200.0, 1600.0, 2, 2, 3, 384, 2, 48, 1.0e-6),       # len_sft 3: three-step path
+]
+
+
+@pytest.mark.parametrize("geom", CHAIN_GEOM)
 def test_chain_vs_double_fft(torch_cuda, geom):
     """Unpacked 8-bit GUPPI voltages through forward FFT, chirp, inverse FFT
     and the intensity pass, against numpy's double-precision FFTs."""
@@ -88,6 +104,8 @@ def test_chain_vs_double_fft(torch_cuda, geom):
     torch = torch_cuda
     f_min, f_max, npol, nchan, len_sft, nbin, nfft, nov, tsamp = geom
     cc = CoherentChunk(f_min, f_max, npol, nchan, len_sft, nbin, nfft, nov, tsamp)
+    mbin = nbin // len_sft
+    assert cc.fused == (len_sft % 2 == 0 and mbin in (128, 256, 512, 1024, 2048, 4096))
     rng = np.random.default_rng(nbin + 2)
     nsamp = (nbin - 2 * nov) * nfft
     raw = rng.integers(-40, 41, (nchan, nsamp, npol), dtype=np.int8)
@@ -105,6 +123,33 @@ def test_chain_vs_double_fft(torch_cuda, geom):
         got = d_wf.cpu().numpy()
         assert np.all(np.isfinite(got))
         np.testing.assert_allclose(got, ref, rtol=2e-4, atol=2e-5 * float(ref.max()))
+
+
+@pytest.mark.parametrize("geom", [GEOM[0], CHAIN_GEOM[5]])
+def test_fused_vs_steps(torch_cuda, geom):
+    """The fused pass against the step entry points chained by hand (chirp,
+    rocFFT inverse, intensity pass): the same chirp products and the same
+    intensity arithmetic around two float FFTs."""
+    from cpp_projects_amd.coherent import CoherentChunk
+
+    torch = torch_cuda
+    f_min, f_max, npol, nchan, len_sft, nbin, nfft, nov, tsamp = geom
+    cc = CoherentChunk(f_min, f_max, npol, nchan, len_sft, nbin, nfft, nov, tsamp)
+    rng = np.random.default_rng(nbin + 3)
+    spec = torch.from_numpy(rand_spectra(rng, npol, nfft, nchan, nbin).reshape(-1)).cuda()
+    buf = torch.empty_like(spec)
+    w_steps = torch.empty((cc.rows, cc.cols), dtype=torch.float32, device="cuda")
+    w_fused = torch.full((cc.rows, cc.cols), float("nan"), dtype=torch.float32, device="cuda")
+    for dm in (0.0, 90.0):
+        print("case coherent fused vs steps", geom, dm, flush=True)
+        cc.chirp(spec, dm, buf)
+        cc.inverse(buf)
+        cc.detect(buf, dm, w_steps)
+        cc.waterfall(spec, dm, w_fused)
+        torch.cuda.synchronize()
+        a, b = w_steps.cpu().numpy(), w_fused.cpu().numpy()
+        assert np.all(np.isfinite(b))
+        np.testing.assert_allclose(b, a, rtol=1e-4, atol=1e-5 * float(a.max()))
 
 
 def test_tone_lands_in_its_row(torch_cuda):
