@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <cmath>
 #include <memory>
+#include <vector>
 
 #include "abi.hpp"
 #include "common.hpp"
@@ -47,15 +48,28 @@ struct CoherentShape {
     int64_t spectra() const { return (int64_t)nfft * nchan * (npol / 2) * nbin; }  // complex values
 };
 
+// The sub-band offset of bin ibin and its taper 1/sqrt(1 + (bf/(0.47
+// bw_chan))^80), in double (Chunk_fly_gpu.cu:139-146).  The taper depends on
+// the bin alone, so the host tabulates it once per geometry.
+__host__ __device__ inline double chirp_bf(int ibin, int mbin, double bw_chan) {
+    return -0.5 * bw_chan + (ibin + 0.5) * bw_chan / mbin;
+}
+inline double chirp_taper(int ibin, int mbin, double bw_chan) {
+    const double bf = chirp_bf(ibin, mbin, bw_chan);
+    return 1.0 / std::sqrt(1.0 + std::pow(bf / (0.47 * bw_chan), 80.0));
+}
+
 // The chirp factor of (chan, sub-band, bin) (kernel_el_wise_mult_onthe_fly,
 // Chunk_fly_gpu.cu:119-164): the sub-band centre is taken to float as in the
-// reference, the rest in double; the factor includes the taper and 1/nbin.
-// *col receives its column in the half-rotated spectrum row.
+// reference, the rest in double; the factor includes the taper and 1/nbin
+// (inv_nbin > 0: nbin is a power of two and the division is this exact
+// scaling).  *col receives its column in the half-rotated spectrum row.
 __device__ inline float2 chirp_factor(int ichan, int isft, int ibin, int len_sft, int mbin, double dm, double fmin,
-                                      double bw_sub, double bw_chan, int* col) {
+                                      double bw_sub, double bw_chan, const double* __restrict__ taper_tab,
+                                      double inv_nbin, int* col) {
     const float f0 = (float)(fmin + bw_sub * (0.5 + ichan) + bw_chan * (isft - len_sft / 2.0 + 0.5));
-    const double bf = -0.5 * bw_chan + (ibin + 0.5) * bw_chan / mbin;
-    const double taper = 1.0 / sqrt(1.0 + pow(bf / (0.47 * bw_chan), 80.0));
+    const double bf = chirp_bf(ibin, mbin, bw_chan);
+    const double taper = taper_tab[ibin];
     const double r = bf / (double)f0;
     const double delay = dm * r * r / ((double)f0 + bf) * 4.148808e9;
     double whole;
@@ -64,6 +78,7 @@ __device__ inline float2 chirp_factor(int ichan, int isft, int ibin, int len_sft
     sincospi(t, &s, &c);
     const int nbin = mbin * len_sft;
     *col = (isft * mbin + ibin + nbin / 2) % nbin;
+    if (inv_nbin > 0.0) return make_float2((float)(c * taper * inv_nbin), (float)(s * taper * inv_nbin));
     return make_float2((float)(c * taper / (double)nbin), (float)(s * taper / (double)nbin));
 }
 
@@ -77,14 +92,16 @@ __device__ inline float2 chirp_mul(float2 a, float2 d) {
 __global__ void __launch_bounds__(kThreads) k_coherent_chirp(const float2* __restrict__ in, float2* __restrict__ out,
                                                              double dm, int nchan, int len_sft, int mbin,
                                                              int nblocks, double fmin, double bw_sub,
-                                                             double bw_chan) {
+                                                             double bw_chan, const double* __restrict__ taper_tab,
+                                                             double inv_nbin) {
     const int ibin = (int)(blockIdx.x * kThreads + threadIdx.x);
     if (ibin >= mbin) return;
     const int isft = (int)blockIdx.y;
     const int ichan = (int)blockIdx.z;
     const int nbin = mbin * len_sft;
     int col;
-    const float2 d = chirp_factor(ichan, isft, ibin, len_sft, mbin, dm, fmin, bw_sub, bw_chan, &col);
+    const float2 d = chirp_factor(ichan, isft, ibin, len_sft, mbin, dm, fmin, bw_sub, bw_chan, taper_tab, inv_nbin,
+                                  &col);
     int64_t idx = (int64_t)ichan * nbin + col;
     const int64_t stride = (int64_t)nbin * nchan;
     // four blocks' loads in flight before their products are stored
@@ -102,13 +119,16 @@ __global__ void __launch_bounds__(kThreads) k_coherent_chirp(const float2* __res
 // The factors alone, fac[chan][col] (the spectrum rows' layout), for the
 // fused pass.
 __global__ void __launch_bounds__(kThreads) k_coherent_factors(float2* __restrict__ fac, double dm, int len_sft,
-                                                               int mbin, double fmin, double bw_sub, double bw_chan) {
+                                                               int mbin, double fmin, double bw_sub, double bw_chan,
+                                                               const double* __restrict__ taper_tab,
+                                                               double inv_nbin) {
     const int ibin = (int)(blockIdx.x * kThreads + threadIdx.x);
     if (ibin >= mbin) return;
     const int isft = (int)blockIdx.y;
     const int ichan = (int)blockIdx.z;
     int col;
-    const float2 d = chirp_factor(ichan, isft, ibin, len_sft, mbin, dm, fmin, bw_sub, bw_chan, &col);
+    const float2 d = chirp_factor(ichan, isft, ibin, len_sft, mbin, dm, fmin, bw_sub, bw_chan, taper_tab, inv_nbin,
+                                  &col);
     fac[(int64_t)ichan * mbin * len_sft + col] = d;
 }
 
@@ -251,36 +271,55 @@ struct FftPad {
     static constexpr int sh = P == 0 ? 5 : (P == 1 ? 3 : 31);
 };
 
+// The pass structure of an N-point transform: radix 8 while at least 8
+// points per butterfly remain, then 2 or 4; a pass's stride Ns is the
+// product of the earlier radices.  Passes after the first read their
+// twiddles exp(2 pi i r kk / (Ns R)) (r = 1..R-1, kk < Ns) from a table
+// tw[off(Ns) + (r - 1) * Ns + kk] that each launch block fills once.
 template <int N>
-struct FftLds {
-    static constexpr int size = N + N / 8;  // the widest padding
+struct FftPlan {
+    static constexpr int T = N / 8;
+    static constexpr int radix(int ns) { return (N / ns) >= 8 ? 8 : N / ns; }
+    static constexpr int tw_off(int ns) {
+        int o = 0;
+        for (int n = 8; n < ns; n *= radix(n)) o += (radix(n) - 1) * n;
+        return o;
+    }
+    static constexpr int tw_size = tw_off(N);
+    static constexpr int data_size = N + N / 8;  // the widest padding
 };
 
-// the twiddle bases exp(2 pi i (j mod Ns) / (Ns R)) of every pass after the
-// first, per butterfly slot u: tw[TB + u]
-template <int N, int NS, int TB>
-__device__ inline void fft_twiddles(float2 (&tw)[8], int t) {
-    constexpr int T = N / 8;
-    constexpr int R = (N / NS) >= 8 ? 8 : N / NS;
-    constexpr int U = 8 / R;
+template <int N, int NS>
+__device__ inline void fft_fill_twiddles(float2* tw, int t) {
+    using F = FftPlan<N>;
+    constexpr int R = F::radix(NS);
     if constexpr (NS > 1) {
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int kk = (t + T * u) & (NS - 1);
-            float s, c;
-            sincospif(2.0f * (float)kk / (float)(NS * R), &s, &c);
-            tw[TB + u] = make_float2(c, s);
+        for (int e = t; e < (R - 1) * NS; e += F::T) {
+            const int r = e / NS + 1;
+            const int kk = e & (NS - 1);
+            float sn, cs;
+            sincospif(2.0f * (float)(r * kk) / (float)(NS * R), &sn, &cs);  // exact argument
+            tw[F::tw_off(NS) + e] = make_float2(cs, sn);
         }
     }
-    if constexpr (NS * R < N) fft_twiddles<N, NS * R, TB + (NS > 1 ? U : 0)>(tw, t);
+    if constexpr (NS * R < N) fft_fill_twiddles<N, NS * R>(tw, t);
 }
 
-template <int N, int NS, int P, int TB>
-__device__ inline void fft_passes(float2 (&z)[8], float2* lds, int t, const float2 (&tw)[8]) {
-    constexpr int T = N / 8;
-    constexpr int R = (N / NS) >= 8 ? 8 : N / NS;
+// Pass P of stride NS on the lane's elements z[m] = x[t + T*m].  With the
+// paddings used (pass 0: one complex in 32, its butterflies writing 8
+// consecutive slots from a multiple of 8; pass 1: one in 8 at stride 8;
+// later: none) a padded index base + c never carries across a padding
+// boundary, so pad(base + c) = pad(base) + c + (c >> sh): every LDS access is
+// a per-lane base plus a compile-time offset.
+template <int N, int NS, int P>
+__device__ inline void fft_passes(float2 (&z)[8], float2* lds, const float2* tw, int t) {
+    using F = FftPlan<N>;
+    constexpr int T = F::T;
+    constexpr int R = F::radix(NS);
     constexpr int U = 8 / R;
     constexpr bool last = NS * R == N;
+    static_assert(P != 0 || (NS == 1 && R == 8), "pass 0 is radix 8");
+    static_assert(P != 1 || NS == 8, "pass 1 has stride 8");
     float2 o[8];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -288,13 +327,9 @@ __device__ inline void fft_passes(float2 (&z)[8], float2* lds, int t, const floa
 #pragma unroll
         for (int r = 0; r < R; ++r) v[r] = z[u + r * U];
         if constexpr (NS > 1) {
-            const float2 w1 = tw[TB + u];
-            float2 w = w1;
+            const float2* w = tw + F::tw_off(NS) + ((t + T * u) & (NS - 1));
 #pragma unroll
-            for (int r = 1; r < R; ++r) {
-                v[r] = cmul_fma(v[r], w);
-                if (r + 1 < R) w = cmul_fma(w, w1);
-            }
+            for (int r = 1; r < R; ++r) v[r] = cmul_fma(v[r], w[(r - 1) * NS]);
         }
         dft_inv<R>(v);
 #pragma unroll
@@ -309,15 +344,16 @@ __device__ inline void fft_passes(float2 (&z)[8], float2* lds, int t, const floa
         for (int u = 0; u < U; ++u) {
             const int j = t + T * u;
             const int kk = j & (NS - 1);
-            const int base = (j - kk) * R + kk;
+            float2* wp = lds + fft_pad<sh>((j - kk) * R + kk);
 #pragma unroll
-            for (int q = 0; q < R; ++q) lds[fft_pad<sh>(base + q * NS)] = o[u + q * U];
+            for (int q = 0; q < R; ++q) wp[q * NS + ((q * NS) >> sh)] = o[u + q * U];
         }
         __syncthreads();
+        const float2* rp = lds + fft_pad<sh>(t);
 #pragma unroll
-        for (int m = 0; m < 8; ++m) z[m] = lds[fft_pad<sh>(t + T * m)];
+        for (int m = 0; m < 8; ++m) z[m] = rp[T * m + ((T * m) >> sh)];
         __syncthreads();
-        fft_passes<N, NS * R, P + 1, TB + (NS > 1 ? U : 0)>(z, lds, t, tw);
+        fft_passes<N, NS * R, P + 1>(z, lds, tw, t);
     }
 }
 
@@ -326,11 +362,13 @@ __global__ void __launch_bounds__(N / 8) k_coherent_fused(const float2* __restri
                                                           const float2* __restrict__ fac, float* __restrict__ wf,
                                                           int npairs, int nchan, int len_sft, int nfft, int bpw,
                                                           int nov_sub, int mbin_adj, int msamp, double dm,
-                                                          double fmin, double foff, double tsamp_wf,
-                                                          double inv_nbin2) {
-    constexpr int T = N / 8;
-    __shared__ float2 lds[FftLds<N>::size];
+                                                          double fmin, double foff, double tsamp_wf, float scale) {
+    using F = FftPlan<N>;
+    constexpr int T = F::T;
+    __shared__ float2 lds[F::data_size];
+    __shared__ float2 tw[F::tw_size];
     const int t = (int)threadIdx.x;
+    fft_fill_twiddles<N, 1>(tw, t);
     const int seg = (int)blockIdx.x;
     const int c = seg / len_sft;
     const int k = seg - c * len_sft;
@@ -343,12 +381,11 @@ __global__ void __launch_bounds__(N / 8) k_coherent_fused(const float2* __restri
     float2 d[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) d[m] = fac[seg_off + T * m];
-    float2 tw[8];
-    fft_twiddles<N, 1, 0>(tw, t);
     const int64_t plane = (int64_t)nfft * nchan * nbin;
     const int b0 = (int)blockIdx.y * bpw;
     const int b1 = min(nfft, b0 + bpw);
     float* out = wf + (int64_t)row * msamp;
+    __syncthreads();  // the twiddle table
     for (int b = b0; b < b1; ++b) {
         float v[8];
 #pragma unroll
@@ -358,9 +395,10 @@ __global__ void __launch_bounds__(N / 8) k_coherent_fused(const float2* __restri
             float2 z[8];
 #pragma unroll
             for (int m = 0; m < 8; ++m) z[m] = chirp_mul(src[T * m], d[m]);
-            fft_passes<N, 1, 0, 0>(z, lds, t, tw);
+            fft_passes<N, 1, 0>(z, lds, tw, t);
+            // |x|^2 / nbin^2 added over the polarisations, in float
 #pragma unroll
-            for (int m = 0; m < 8; ++m) v[m] += pol_intensity(z[m], nbin, inv_nbin2);
+            for (int m = 0; m < 8; ++m) v[m] = fmaf(fmaf(z[m].x, z[m].x, z[m].y * z[m].y), scale, v[m]);
         }
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
@@ -388,6 +426,8 @@ struct sps_coherent {
     bool have_fwd = false, have_inv = false;
     DevBuf chirped;  // the chirped spectra, then their sub-band time series (unfused path)
     DevBuf factors;  // one DM's chirp factors [chan][nbin] (fused path)
+    DevBuf taper;    // the sub-band taper per bin, mbin doubles
+    double bw_sub = 0, bw_chan = 0, inv_nbin = 0;
     ~sps_coherent() {
         if (have_fwd) (void)hipfftDestroy(fwd);
         if (have_inv) (void)hipfftDestroy(inv);
@@ -398,15 +438,11 @@ namespace {
 
 void coherent_chirp(const sps_coherent& h, const float* d_in, double dm, float* d_out, hipStream_t st) {
     const CoherentShape& s = h.sh;
-    // double bw = m_Fmax - m_Fmin: a float difference (Chunk_fly_gpu.cu:96)
-    const double bw = (double)((float)s.fmax - (float)s.fmin);
-    const double bw_sub = bw / s.nchan;
-    const double bw_chan = bw_sub / s.len_sft;
     const dim3 grid((unsigned)ceil_div((int64_t)s.mbin(), (int64_t)kThreads), (unsigned)s.len_sft,
                     (unsigned)s.nchan);
     hipLaunchKernelGGL(k_coherent_chirp, grid, dim3(kThreads), 0, st, reinterpret_cast<const float2*>(d_in),
                        reinterpret_cast<float2*>(d_out), dm, s.nchan, s.len_sft, s.mbin(), s.nfft * (s.npol / 2),
-                       s.fmin, bw_sub, bw_chan);
+                       s.fmin, h.bw_sub, h.bw_chan, h.taper.as<double>(), h.inv_nbin);
     SPS_HIP(hipGetLastError());
 }
 
@@ -434,10 +470,10 @@ void coherent_detect(const sps_coherent& h, const float* d_buf, double dm, float
 
 template <int N>
 void launch_fused(const CoherentShape& s, const float2* spec, const float2* fac, float* wf, double dm, int bpw,
-                  double foff, double tsamp_wf, double inv_nbin2, hipStream_t st) {
+                  double foff, double tsamp_wf, float scale, hipStream_t st) {
     const dim3 grid((unsigned)s.rows(), (unsigned)ceil_div((int64_t)s.nfft, (int64_t)bpw));
     hipLaunchKernelGGL(k_coherent_fused<N>, grid, dim3(N / 8), 0, st, spec, fac, wf, s.npol / 2, s.nchan, s.len_sft,
-                       s.nfft, bpw, s.nov_sub(), s.mbin_adj(), (int)s.msamp(), dm, s.fmin, foff, tsamp_wf, inv_nbin2);
+                       s.nfft, bpw, s.nov_sub(), s.mbin_adj(), (int)s.msamp(), dm, s.fmin, foff, tsamp_wf, scale);
 }
 
 void coherent_fused(sps_coherent& h, const float* d_spectra, double dm, float* d_wf, hipStream_t st);
@@ -450,31 +486,27 @@ void coherent_fft(sps_coherent& h, hipfftHandle plan, float* d, int dir, hipStre
 
 void coherent_fused(sps_coherent& h, const float* d_spectra, double dm, float* d_wf, hipStream_t st) {
     const CoherentShape& s = h.sh;
-    const double bw = (double)((float)s.fmax - (float)s.fmin);
-    const double bw_sub = bw / s.nchan;
-    const double bw_chan = bw_sub / s.len_sft;
     float2* fac = h.factors.as<float2>();
     const dim3 fgrid((unsigned)ceil_div((int64_t)s.mbin(), (int64_t)kThreads), (unsigned)s.len_sft,
                      (unsigned)s.nchan);
-    hipLaunchKernelGGL(k_coherent_factors, fgrid, dim3(kThreads), 0, st, fac, dm, s.len_sft, s.mbin(), s.fmin, bw_sub,
-                       bw_chan);
+    hipLaunchKernelGGL(k_coherent_factors, fgrid, dim3(kThreads), 0, st, fac, dm, s.len_sft, s.mbin(), s.fmin,
+                       h.bw_sub, h.bw_chan, h.taper.as<double>(), h.inv_nbin);
     SPS_HIP(hipGetLastError());
     const double foff = (s.fmax - s.fmin) / ((double)s.nchan * s.len_sft);
     const double tsamp_wf = (double)((float)(unsigned)s.len_sft * s.tsamp);
-    const bool pow2 = (s.nbin & (s.nbin - 1)) == 0;
-    const double inv_nbin2 = pow2 ? std::ldexp(1.0, -2 * (int)std::log2((double)s.nbin)) : 0.0;
+    const float scale = (float)(1.0 / ((double)s.nbin * (double)s.nbin));
     // blocks per launch block: enough launch blocks to fill the chip several
     // times over, each reusing its segment's factors for the rest
     const int64_t want = 8192;
     const int bpw = (int)std::max<int64_t>(1, std::min<int64_t>(s.nfft, ceil_div(s.rows() * s.nfft, want)));
     const float2* spec = reinterpret_cast<const float2*>(d_spectra);
     switch (s.mbin()) {
-        case 128: launch_fused<128>(s, spec, fac, d_wf, dm, bpw, foff, tsamp_wf, inv_nbin2, st); break;
-        case 256: launch_fused<256>(s, spec, fac, d_wf, dm, bpw, foff, tsamp_wf, inv_nbin2, st); break;
-        case 512: launch_fused<512>(s, spec, fac, d_wf, dm, bpw, foff, tsamp_wf, inv_nbin2, st); break;
-        case 1024: launch_fused<1024>(s, spec, fac, d_wf, dm, bpw, foff, tsamp_wf, inv_nbin2, st); break;
-        case 2048: launch_fused<2048>(s, spec, fac, d_wf, dm, bpw, foff, tsamp_wf, inv_nbin2, st); break;
-        case 4096: launch_fused<4096>(s, spec, fac, d_wf, dm, bpw, foff, tsamp_wf, inv_nbin2, st); break;
+        case 128: launch_fused<128>(s, spec, fac, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
+        case 256: launch_fused<256>(s, spec, fac, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
+        case 512: launch_fused<512>(s, spec, fac, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
+        case 1024: launch_fused<1024>(s, spec, fac, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
+        case 2048: launch_fused<2048>(s, spec, fac, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
+        case 4096: launch_fused<4096>(s, spec, fac, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
         default: throw Error(SPS_ERR_RUNTIME, "fused coherent pass: unsupported sub-band length");
     }
     SPS_HIP(hipGetLastError());
@@ -510,6 +542,16 @@ SPS_API sps_status sps_coherent_create(float f_min, float f_max, int npol, int n
                      "inverse plan");
         h->have_inv = true;
         if (coherent_fusable(h->sh)) h->factors.ensure(sizeof(float) * 2 * (size_t)nchan * nbin);
+        // double bw = m_Fmax - m_Fmin: a float difference (Chunk_fly_gpu.cu:96)
+        const double bw = (double)(f_max - f_min);
+        h->bw_sub = bw / nchan;
+        h->bw_chan = h->bw_sub / len_sft;
+        h->inv_nbin = (nbin & (nbin - 1)) == 0 ? 1.0 / nbin : 0.0;
+        const int mbin = h->sh.mbin();
+        std::vector<double> taper((size_t)mbin);
+        for (int i = 0; i < mbin; ++i) taper[(size_t)i] = chirp_taper(i, mbin, h->bw_chan);
+        h->taper.ensure(sizeof(double) * (size_t)mbin);
+        SPS_HIP(hipMemcpy(h->taper.p, taper.data(), sizeof(double) * (size_t)mbin, hipMemcpyHostToDevice));
         *out = h.release();
     });
 }
