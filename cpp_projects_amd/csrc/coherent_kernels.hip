@@ -247,18 +247,19 @@ __device__ inline void dft_inv<8>(float2 (&v)[8]) {
     dft_inv<4>(e);
     dft_inv<4>(o);
     constexpr float h = 0.70710678118654752440f;
-    // o[q] * exp(2 pi i q / 8)
-    const float2 o1 = make_float2((o[1].x - o[1].y) * h, (o[1].x + o[1].y) * h);
+    // o[q] * exp(2 pi i q / 8): o1 = a * h, o3 = b * h, the scaling fused into
+    // the outputs' additions
+    const float2 a = make_float2(o[1].x - o[1].y, o[1].x + o[1].y);
+    const float2 b = make_float2(-(o[3].x + o[3].y), o[3].x - o[3].y);
     const float2 o2 = mul_i(o[2]);
-    const float2 o3 = make_float2(-(o[3].x + o[3].y) * h, (o[3].x - o[3].y) * h);
     v[0] = cadd(e[0], o[0]);
     v[4] = csub(e[0], o[0]);
-    v[1] = cadd(e[1], o1);
-    v[5] = csub(e[1], o1);
+    v[1] = make_float2(fmaf(a.x, h, e[1].x), fmaf(a.y, h, e[1].y));
+    v[5] = make_float2(fmaf(-a.x, h, e[1].x), fmaf(-a.y, h, e[1].y));
     v[2] = cadd(e[2], o2);
     v[6] = csub(e[2], o2);
-    v[3] = cadd(e[3], o3);
-    v[7] = csub(e[3], o3);
+    v[3] = make_float2(fmaf(b.x, h, e[3].x), fmaf(b.y, h, e[3].y));
+    v[7] = make_float2(fmaf(-b.x, h, e[3].x), fmaf(-b.y, h, e[3].y));
 }
 
 template <int SH>
@@ -275,7 +276,8 @@ struct FftPad {
 // points per butterfly remain, then 2 or 4; a pass's stride Ns is the
 // product of the earlier radices.  Passes after the first read their
 // twiddles exp(2 pi i r kk / (Ns R)) (r = 1..R-1, kk < Ns) from a table
-// tw[off(Ns) + (r - 1) * Ns + kk] that each launch block fills once.
+// tw[off(Ns) + (r - 1) * Ns + kk] (fft_twiddle_table on the host, in double
+// rounded to float) that each launch block copies into LDS.
 template <int N>
 struct FftPlan {
     static constexpr int T = N / 8;
@@ -288,22 +290,6 @@ struct FftPlan {
     static constexpr int tw_size = tw_off(N);
     static constexpr int data_size = N + N / 8;  // the widest padding
 };
-
-template <int N, int NS>
-__device__ inline void fft_fill_twiddles(float2* tw, int t) {
-    using F = FftPlan<N>;
-    constexpr int R = F::radix(NS);
-    if constexpr (NS > 1) {
-        for (int e = t; e < (R - 1) * NS; e += F::T) {
-            const int r = e / NS + 1;
-            const int kk = e & (NS - 1);
-            float sn, cs;
-            sincospif(2.0f * (float)(r * kk) / (float)(NS * R), &sn, &cs);  // exact argument
-            tw[F::tw_off(NS) + e] = make_float2(cs, sn);
-        }
-    }
-    if constexpr (NS * R < N) fft_fill_twiddles<N, NS * R>(tw, t);
-}
 
 // Pass P of stride NS on the lane's elements z[m] = x[t + T*m].  With the
 // paddings used (pass 0: one complex in 32, its butterflies writing 8
@@ -359,7 +345,8 @@ __device__ inline void fft_passes(float2 (&z)[8], float2* lds, const float2* tw,
 
 template <int N>
 __global__ void __launch_bounds__(N / 8) k_coherent_fused(const float2* __restrict__ spec,
-                                                          const float2* __restrict__ fac, float* __restrict__ wf,
+                                                          const float2* __restrict__ fac,
+                                                          const float2* __restrict__ twg, float* __restrict__ wf,
                                                           int npairs, int nchan, int len_sft, int nfft, int bpw,
                                                           int nov_sub, int mbin_adj, int msamp, double dm,
                                                           double fmin, double foff, double tsamp_wf, float scale) {
@@ -368,7 +355,7 @@ __global__ void __launch_bounds__(N / 8) k_coherent_fused(const float2* __restri
     __shared__ float2 lds[F::data_size];
     __shared__ float2 tw[F::tw_size];
     const int t = (int)threadIdx.x;
-    fft_fill_twiddles<N, 1>(tw, t);
+    for (int e = t; e < F::tw_size; e += T) tw[e] = twg[e];
     const int seg = (int)blockIdx.x;
     const int c = seg / len_sft;
     const int k = seg - c * len_sft;
@@ -386,27 +373,46 @@ __global__ void __launch_bounds__(N / 8) k_coherent_fused(const float2* __restri
     const int b1 = min(nfft, b0 + bpw);
     float* out = wf + (int64_t)row * msamp;
     __syncthreads();  // the twiddle table
-    for (int b = b0; b < b1; ++b) {
-        float v[8];
+    // iterations (block b, polarisation p) in order, the next one's spectra
+    // loaded while this one's FFT runs
+    const int nit = (b1 - b0) * npairs;
+    const int64_t bstride = (int64_t)nchan * nbin;
+    const float2* src = spec + (int64_t)b0 * bstride + seg_off;
+    float2 zn[8];
 #pragma unroll
-        for (int m = 0; m < 8; ++m) v[m] = 0.0f;
-        const float2* src = spec + (int64_t)b * nchan * nbin + seg_off;
-        for (int p = 0; p < npairs; ++p, src += plane) {
-            float2 z[8];
+    for (int m = 0; m < 8; ++m) zn[m] = src[T * m];
+    float v[8];
 #pragma unroll
-            for (int m = 0; m < 8; ++m) z[m] = chirp_mul(src[T * m], d[m]);
-            fft_passes<N, 1, 0>(z, lds, tw, t);
-            // |x|^2 / nbin^2 added over the polarisations, in float
+    for (int m = 0; m < 8; ++m) v[m] = 0.0f;
+    int b = b0, p = 0;
+    for (int it = 0; it < nit; ++it) {
+        float2 z[8];
 #pragma unroll
-            for (int m = 0; m < 8; ++m) v[m] = fmaf(fmaf(z[m].x, z[m].x, z[m].y * z[m].y), scale, v[m]);
+        for (int m = 0; m < 8; ++m) z[m] = zn[m];
+        if (it + 1 < nit) {
+            const float2* nsrc = p + 1 < npairs ? src + (int64_t)(p + 1) * plane : src + bstride;
+#pragma unroll
+            for (int m = 0; m < 8; ++m) zn[m] = nsrc[T * m];
         }
 #pragma unroll
-        for (int m = 0; m < 8; ++m) {
-            const int n = t + T * m;
-            if (n >= nov_sub && n < N - nov_sub) {
-                const int s0 = b * mbin_adj + (n - nov_sub);  // + sh, modulo msamp
-                out[s0 >= msamp - sh ? s0 - (msamp - sh) : s0 + sh] = v[m];
+        for (int m = 0; m < 8; ++m) z[m] = cmul_fma(z[m], d[m]);
+        fft_passes<N, 1, 0>(z, lds, tw, t);
+        // |x|^2 / nbin^2 added over the polarisations, in float
+#pragma unroll
+        for (int m = 0; m < 8; ++m) v[m] = fmaf(fmaf(z[m].x, z[m].x, z[m].y * z[m].y), scale, v[m]);
+        if (++p == npairs) {
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                const int n = t + T * m;
+                if (n >= nov_sub && n < N - nov_sub) {
+                    const int s0 = b * mbin_adj + (n - nov_sub);  // + sh, modulo msamp
+                    out[s0 >= msamp - sh ? s0 - (msamp - sh) : s0 + sh] = v[m];
+                }
+                v[m] = 0.0f;
             }
+            p = 0;
+            ++b;
+            src += bstride;
         }
     }
 }
@@ -427,6 +433,7 @@ struct sps_coherent {
     DevBuf chirped;  // the chirped spectra, then their sub-band time series (unfused path)
     DevBuf factors;  // one DM's chirp factors [chan][nbin] (fused path)
     DevBuf taper;    // the sub-band taper per bin, mbin doubles
+    DevBuf twiddles; // the fused pass's FFT twiddle table (fft_twiddle_table)
     double bw_sub = 0, bw_chan = 0, inv_nbin = 0;
     ~sps_coherent() {
         if (have_fwd) (void)hipfftDestroy(fwd);
@@ -444,6 +451,24 @@ void coherent_chirp(const sps_coherent& h, const float* d_in, double dm, float* 
                        reinterpret_cast<float2*>(d_out), dm, s.nchan, s.len_sft, s.mbin(), s.nfft * (s.npol / 2),
                        s.fmin, h.bw_sub, h.bw_chan, h.taper.as<double>(), h.inv_nbin);
     SPS_HIP(hipGetLastError());
+}
+
+// The twiddle table of the fused pass's N-point inverse FFT, in the layout
+// FftPlan<N> reads: for each pass after the first (stride ns, radix R), the
+// (R - 1) * ns factors exp(2 pi i r kk / (ns R)), r-major.
+std::vector<float2> fft_twiddle_table(int n) {
+    std::vector<float2> tw;
+    const double pi = 3.14159265358979323846;
+    for (int ns = 8; ns < n;) {
+        const int r_ = (n / ns) >= 8 ? 8 : n / ns;
+        for (int e = 0; e < (r_ - 1) * ns; ++e) {
+            const int r = e / ns + 1, kk = e % ns;
+            const double a = 2.0 * pi * (double)(r * kk) / (double)(ns * r_);
+            tw.push_back(make_float2((float)std::cos(a), (float)std::sin(a)));
+        }
+        ns *= r_;
+    }
+    return tw;
 }
 
 // the fused pass applies: a power-of-two sub-band length the kernel is built
@@ -469,10 +494,12 @@ void coherent_detect(const sps_coherent& h, const float* d_buf, double dm, float
 }
 
 template <int N>
-void launch_fused(const CoherentShape& s, const float2* spec, const float2* fac, float* wf, double dm, int bpw,
-                  double foff, double tsamp_wf, float scale, hipStream_t st) {
+void launch_fused(const CoherentShape& s, const float2* spec, const float2* fac, const float2* twg, float* wf,
+                  double dm, int bpw, double foff, double tsamp_wf, float scale, hipStream_t st) {
+    static_assert(FftPlan<N>::tw_size > 0, "at least two passes");
     const dim3 grid((unsigned)s.rows(), (unsigned)ceil_div((int64_t)s.nfft, (int64_t)bpw));
-    hipLaunchKernelGGL(k_coherent_fused<N>, grid, dim3(N / 8), 0, st, spec, fac, wf, s.npol / 2, s.nchan, s.len_sft,
+    hipLaunchKernelGGL(k_coherent_fused<N>, grid, dim3(N / 8), 0, st, spec, fac, twg, wf, s.npol / 2, s.nchan,
+                       s.len_sft,
                        s.nfft, bpw, s.nov_sub(), s.mbin_adj(), (int)s.msamp(), dm, s.fmin, foff, tsamp_wf, scale);
 }
 
@@ -500,13 +527,14 @@ void coherent_fused(sps_coherent& h, const float* d_spectra, double dm, float* d
     const int64_t want = 8192;
     const int bpw = (int)std::max<int64_t>(1, std::min<int64_t>(s.nfft, ceil_div(s.rows() * s.nfft, want)));
     const float2* spec = reinterpret_cast<const float2*>(d_spectra);
+    const float2* twg = h.twiddles.as<float2>();
     switch (s.mbin()) {
-        case 128: launch_fused<128>(s, spec, fac, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
-        case 256: launch_fused<256>(s, spec, fac, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
-        case 512: launch_fused<512>(s, spec, fac, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
-        case 1024: launch_fused<1024>(s, spec, fac, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
-        case 2048: launch_fused<2048>(s, spec, fac, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
-        case 4096: launch_fused<4096>(s, spec, fac, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
+        case 128: launch_fused<128>(s, spec, fac, twg, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
+        case 256: launch_fused<256>(s, spec, fac, twg, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
+        case 512: launch_fused<512>(s, spec, fac, twg, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
+        case 1024: launch_fused<1024>(s, spec, fac, twg, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
+        case 2048: launch_fused<2048>(s, spec, fac, twg, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
+        case 4096: launch_fused<4096>(s, spec, fac, twg, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
         default: throw Error(SPS_ERR_RUNTIME, "fused coherent pass: unsupported sub-band length");
     }
     SPS_HIP(hipGetLastError());
@@ -552,6 +580,11 @@ SPS_API sps_status sps_coherent_create(float f_min, float f_max, int npol, int n
         for (int i = 0; i < mbin; ++i) taper[(size_t)i] = chirp_taper(i, mbin, h->bw_chan);
         h->taper.ensure(sizeof(double) * (size_t)mbin);
         SPS_HIP(hipMemcpy(h->taper.p, taper.data(), sizeof(double) * (size_t)mbin, hipMemcpyHostToDevice));
+        if (coherent_fusable(h->sh)) {
+            const std::vector<float2> tw = fft_twiddle_table(mbin);
+            h->twiddles.ensure(sizeof(float2) * tw.size());
+            SPS_HIP(hipMemcpy(h->twiddles.p, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice));
+        }
         *out = h.release();
     });
 }
