@@ -524,7 +524,7 @@ void coherent_fused(sps_coherent& h, const float* d_spectra, double dm, float* d
     const float scale = (float)(1.0 / ((double)s.nbin * (double)s.nbin));
     // blocks per launch block: enough launch blocks to fill the chip several
     // times over, each reusing its segment's factors for the rest
-    const int64_t want = 8192;
+    const int64_t want = 2048;
     const int bpw = (int)std::max<int64_t>(1, std::min<int64_t>(s.nfft, ceil_div(s.rows() * s.nfft, want)));
     const float2* spec = reinterpret_cast<const float2*>(d_spectra);
     const float2* twg = h.twiddles.as<float2>();

@@ -89,6 +89,10 @@ CHAIN_GEOM = GEOM + [
     (1300.0, 1500.0, 2, 2, 8, 8192, 2, 512, 2.0e-7),     # mbin 1024
     (1200.0, 1600.0, 4, 2, 4, 8192, 2, 1024, 1.6e-7),    # mbin 2048, the bench's sub-band length
     (1200.0, 1600.0, 2, 1, 4, 16384, 3, 2048, 1.0e-7),   # mbin 4096
+    # 1024 rows x 5 blocks: each launch block of the fused pass runs 3 blocks
+    # (the last one 2), both polarisations, the next spectra prefetched across
+    # block boundaries
+    (1200.0, 1600.0, 4, 512, 2, 256, 5, 32, 1.0e-6),     # mbin 128
     (1200.0, 1600.0, 4, 2, 8, 768, 2, 64, 1.0e-6),       # mbin 96: three-step path
     (1200.0, 1600.0, 2, 2, 3, 384, 2, 48, 1.0e-6),       # len_sft 3: three-step path
 ]
@@ -125,7 +129,7 @@ def test_chain_vs_double_fft(torch_cuda, geom):
         np.testing.assert_allclose(got, ref, rtol=2e-4, atol=2e-5 * float(ref.max()))
 
 
-@pytest.mark.parametrize("geom", [GEOM[0], CHAIN_GEOM[5]])
+@pytest.mark.parametrize("geom", [GEOM[0], CHAIN_GEOM[5], CHAIN_GEOM[7]])
 def test_fused_vs_steps(torch_cuda, geom):
     """The fused pass against the step entry points chained by hand (chirp,
     rocFFT inverse, intensity pass): the same chirp products and the same
