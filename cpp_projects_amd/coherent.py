@@ -3,6 +3,12 @@ steps 2-8 with CChunk_fly_gpu's on-the-fly chirp (CDMT/Chunk_gpu.cu:197-209,
 254-322; Chunk_fly_gpu.cu:94-164).  Spectra are complex64 device tensors laid
 out [pol][fft][chan][bin] (the unpack's output); the waterfall is a float32
 device tensor [nchan * len_sft][msamp] that the search takes as its input.
+
+CoherentChunk.process runs the whole of CChunk_gpu::process on one chunk of
+unpacked blocks: the forward FFT, then per coherent DM the waterfall, the
+statistics and normalisation, the CFdmt of the keep mask and of the waterfall
+and the S/N normalisation into that DM's rows of the output image, and at the
+end the clustering over the image with the candidates log.
 """
 import ctypes as C
 
@@ -17,6 +23,8 @@ class CoherentChunk:
     sampling time)."""
 
     def __init__(self, f_min, f_max, npol, nchan, len_sft, nbin, nfft, noverlap, tsamp):
+        self.f_min, self.f_max, self.len_sft, self.tsamp = float(f_min), float(f_max), int(len_sft), float(tsamp)
+        self._fdmt = None
         h = C.c_void_p()
         call("sps_coherent_create", float(f_min), float(f_max), int(npol), int(nchan), int(len_sft), int(nbin),
              int(nfft), int(noverlap), float(tsamp), C.byref(h))
@@ -70,3 +78,58 @@ class CoherentChunk:
         self._check(d_buf, 2 * self.spectra, "buffer")
         self._check(d_waterfall, self.rows * self.cols, "waterfall")
         call("sps_coherent_detect", self._h, _dptr(d_buf), float(dm), _dptr(d_waterfall), _stream())
+
+    def process(self, d_blocks, dms, sigma_bound, d_image=None, wnd=4, bins=(2, 1, 2), filename="candidates.log",
+                t_chunk_begin=0.0, bwrite=True):
+        """CChunk_gpu::process (CDMT/Chunk_gpu.cu:254-425) on unpacked blocks,
+        which the forward FFT overwrites with their spectra (step 2).  For
+        each coherent DM idm (steps 3-14): the waterfall; calc_fdmt_inputs
+        (in place, with the keep mask); the CFdmt (nchan * len_sft channels,
+        msamp columns, imaxDt = len_sft) of the mask and of the waterfall;
+        fdmt_normalization into rows [idm * len_sft, (idm + 1) * len_sft) of
+        the image.  Then clusterization_main over the image (steps 16-17:
+        widths 1..wnd, threshold sigma_bound, gap bins, the log written to
+        filename with the waterfall's sampling time len_sft * tsamp).  The
+        reference's power-of-two gather before the clustering keeps no
+        result and is not run.
+
+        Returns (image, candidates, delegates, group sizes): the float32
+        image [len(dms) * len_sft][msamp], the all-widths candidates and
+        the delegate indices into them with their group sizes (empty when
+        no value passes the threshold)."""
+        import torch
+
+        from .fdmt import CFdmtGpu
+        from .search import _write_delegates, clusterization, fdmt_normalization, statistical_preprocessing
+
+        dev = d_blocks.device
+        rows, msamp, L = self.rows, self.cols, self.len_sft
+        ndm = len(dms)
+        if d_image is None:
+            d_image = torch.empty((ndm * L, msamp), dtype=torch.float32, device=dev)
+        self._check(d_image, ndm * L * msamp, "image")
+        if self._fdmt is None:
+            self._fdmt = CFdmtGpu(self.f_min, self.f_max, rows, msamp, L)
+            self._wf = torch.empty((rows, msamp), dtype=torch.float32, device=dev)
+            self._norm_in = torch.empty((rows, msamp), dtype=torch.float32, device=dev)
+            self._norm = torch.empty((L, msamp), dtype=torch.float32, device=dev)
+            self._buff = torch.empty(2 * rows + 4, dtype=torch.float32, device=dev)
+        img = d_image.view(ndm * L, msamp)
+        self.forward(d_blocks)
+        for idm, dm in enumerate(dms):
+            self.waterfall(d_blocks, dm, self._wf)
+            statistical_preprocessing.calc_fdmt_inputs(self._wf, rows, msamp, self._buff, self._norm_in)
+            self._fdmt.process_image(self._norm_in, self._norm, False)
+            out = img[idm * L:(idm + 1) * L]
+            self._fdmt.process_image(self._wf, out, False)
+            fdmt_normalization(out, self._norm, L * msamp, out)
+        d_thr = torch.tensor([float(sigma_bound)], dtype=torch.float32, device=dev)
+        d_bins = torch.tensor(list(bins), dtype=torch.int32, device=dev)
+        ok, cands, n = clusterization.gather_candidates_in_heap_with_plan(img, ndm * L, msamp, d_thr, wnd)
+        if not ok:
+            empty = torch.empty(0, dtype=torch.int64)
+            return d_image, cands, empty, empty
+        dele, gsz = clusterization.cluster(cands, n, d_bins)
+        if bwrite:
+            _write_delegates(cands, n, dele.cpu().numpy(), gsz.cpu().numpy(), filename, L * self.tsamp, t_chunk_begin)
+        return d_image, cands, dele, gsz

@@ -9,19 +9,29 @@
 // (roll_rows_normalize_sum_kernel, transpose_unpadd_intensity, dedisperse)
 // for one coherent DM, leaving the [nchan * len_sft][msamp] float waterfall
 // that statistical_preprocessing::calc_fdmt_inputs and the FDMT take next.
-// Device pointers; asynchronous on the stream.
+// process() is the whole of CChunk_gpu::process on one chunk: those steps,
+// then per DM the statistics, the CFdmt of the keep mask and of the waterfall
+// and fdmt_normalization into the DM's rows of the output image, and the
+// clustering with its log.  Device pointers; asynchronous on the stream
+// except process(), which synchronises for the clustering.
 #pragma once
 
 #include <hip/hip_runtime_api.h>
 #include <hip/hip_vector_types.h>
 
 #include "../sps.h"
+#include <stdexcept>
+#include <string>
+#include <vector>
+
 #include "fdmt.hpp"
+#include "search.hpp"
 
 class CoherentChunk {
 public:
     CoherentChunk(float Fmin, float Fmax, int npol, int nchan, int len_sft, int nbin, int nfft, int noverlap,
-                  float tsamp) {
+                  float tsamp)
+        : m_Fmin(Fmin), m_Fmax(Fmax), m_len_sft(len_sft), m_tsamp(tsamp) {
         sps_detail::check(sps_coherent_create(Fmin, Fmax, npol, nchan, len_sft, nbin, nfft, noverlap, tsamp, &m_h));
         sps_detail::check(sps_coherent_get_shape(m_h, &m_rows, &m_cols, &m_spectra));
         int f = 0;
@@ -65,7 +75,64 @@ public:
             sps_coherent_detect(m_h, reinterpret_cast<const float*>(d_buf), dm, parr_wfall_disp, stream));
     }
 
+    // CChunk_gpu::process (Chunk_gpu.cu:254-425) on unpacked blocks, which
+    // the forward FFT overwrites with their spectra.  d_image receives
+    // coh_dms.size() * len_sft rows of msamp S/N values (m_pdoutImg); then
+    // clusterization_main over it (widths 1..wnd, d_bins: 3 device ints,
+    // the reference's {2, 1, 2}) writes filename with the waterfall's
+    // sampling time len_sft * tsamp.  The reference's power-of-two gather
+    // before the clustering keeps no result and is not run.  Returns
+    // clusterization_main's result (false: nothing above sigma_bound).
+    bool process(float2* pcmparrRawSignalCur, const std::vector<double>& coh_dms, float sigma_bound,
+                 float* d_image, int wnd, const int* d_bins, const std::string& filename = "candidates.log",
+                 float t_chunkBegin = 0.0f) {
+        const int rows = (int)m_rows, cols = (int)m_cols;
+        if (!m_have_fdmt) {
+            m_Fdmt = CFdmtGpu(m_Fmin, m_Fmax, rows, cols, m_len_sft);
+            m_have_fdmt = true;
+        }
+        float *wf = nullptr, *norm_in = nullptr, *norm = nullptr, *buff = nullptr, *thr = nullptr;
+        auto release = [&] {
+            for (float* p : {wf, norm_in, norm, buff, thr})
+                if (p) (void)hipFree(p);
+        };
+        try {
+            hip_ok(hipMalloc(&wf, sizeof(float) * (size_t)rows * cols));
+            hip_ok(hipMalloc(&norm_in, sizeof(float) * (size_t)rows * cols));
+            hip_ok(hipMalloc(&norm, sizeof(float) * (size_t)m_len_sft * cols));
+            hip_ok(hipMalloc(&buff, sizeof(float) * (size_t)(2 * rows + 4)));
+            hip_ok(hipMalloc(&thr, sizeof(float)));
+            hip_ok(hipMemcpy(thr, &sigma_bound, sizeof(float), hipMemcpyHostToDevice));
+            forward(pcmparrRawSignalCur);
+            for (size_t idm = 0; idm < coh_dms.size(); ++idm) {
+                float* out = d_image + idm * (size_t)m_len_sft * cols;
+                waterfall(pcmparrRawSignalCur, coh_dms[idm], wf);
+                statistical_preprocessing::calc_fdmt_inputs(wf, rows, cols, buff, norm_in);
+                m_Fdmt.process_image(norm_in, norm, false);
+                m_Fdmt.process_image(wf, out, false);
+                fdmt_normalization(out, norm, m_len_sft * cols, out);
+            }
+            const bool ok = clusterization::clusterization_main(d_image, m_len_sft * (int)coh_dms.size(), cols, *thr,
+                                                                wnd, d_bins, filename, (float)m_len_sft * m_tsamp,
+                                                                t_chunkBegin);
+            release();
+            return ok;
+        } catch (...) {
+            release();
+            throw;
+        }
+    }
+
 private:
+    static void hip_ok(hipError_t e) {
+        if (e != hipSuccess) throw std::runtime_error("HIP allocation or copy failed");
+    }
+
+    float m_Fmin, m_Fmax;
+    int m_len_sft;
+    float m_tsamp;
+    CFdmtGpu m_Fdmt;
+    bool m_have_fdmt = false;
     sps_coherent* m_h = nullptr;
     int64_t m_rows = 0, m_cols = 0, m_spectra = 0;
     bool m_fused = false;

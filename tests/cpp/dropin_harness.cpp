@@ -20,9 +20,11 @@
 #include <string>
 #include <vector>
 
+#include "sps/coherent.hpp"
 #include "sps/fdmt.hpp"
 #include "sps/search.hpp"
 #include "sps/thrust_vectors.hpp"
+#include "sps/unpack.hpp"
 
 static void dump(const std::string& fn, const void* h, size_t bytes) {
     FILE* f = fopen(fn.c_str(), "wb");
@@ -332,6 +334,48 @@ int main(int argc, char** argv) {
         HIP_OK(hipFree(d_img));
         HIP_OK(hipFree(d_thr));
         HIP_OK(hipFree(d_bins));
+
+        // CChunk_gpu::process through CoherentChunk: 8-bit GUPPI voltages
+        // (hashed noise with two broadband bursts; the test rebuilds the same
+        // bytes) unpacked, then the whole coherent chain and the clustering
+        {
+            const int npol = 4, nchan = 8, len_sft = 8, nbin = 1024, nfft = 4, nov = 128;
+            const int nsamp = (nbin - 2 * nov) * nfft;
+            std::vector<signed char> raw((size_t)nchan * nsamp * npol);
+            for (size_t i = 0; i < raw.size(); ++i) {
+                const int s = (int)((i / npol) % nsamp);
+                int v = (int)(((unsigned)i * 2654435761u >> 7) % 41) - 20;
+                if (s >= 1500 && s < 1564)
+                    v *= 4;
+                else if (s >= 400 && s < 432)
+                    v *= 2;
+                raw[i] = (signed char)v;
+            }
+            char* d_raw = nullptr;
+            float2* d_blocks = nullptr;
+            float* d_cimg = nullptr;
+            int* d_cbins = nullptr;
+            HIP_OK(hipMalloc(&d_raw, raw.size()));
+            HIP_OK(hipMemcpy(d_raw, raw.data(), raw.size(), hipMemcpyHostToDevice));
+            HIP_OK(hipMalloc(&d_blocks, sizeof(float2) * (size_t)(npol / 2) * nfft * nchan * nbin));
+            unpack_chunk_guppi_gpu(nsamp, nov, nbin, nfft, npol, nchan, d_raw, d_blocks);
+            CoherentChunk cc(1200.0f, 1600.0f, npol, nchan, len_sft, nbin, nfft, nov, 1e-6f);
+            const std::vector<double> dms{0.0, 30.0};
+            const size_t nimg = dms.size() * (size_t)len_sft * (size_t)cc.msamp();
+            HIP_OK(hipMalloc(&d_cimg, nimg * sizeof(float)));
+            const int h_cbins[3] = {2, 1, 2};
+            HIP_OK(hipMalloc(&d_cbins, sizeof(h_cbins)));
+            HIP_OK(hipMemcpy(d_cbins, h_cbins, sizeof(h_cbins), hipMemcpyHostToDevice));
+            const bool found = cc.process(d_blocks, dms, 6.0f, d_cimg, 4, d_cbins, out + "/coherent_candidates.log");
+            std::vector<float> cimg(nimg);
+            HIP_OK(hipMemcpy(cimg.data(), d_cimg, nimg * sizeof(float), hipMemcpyDeviceToHost));
+            dump(out + "/coherent_image.bin", cimg.data(), nimg * sizeof(float));
+            std::ofstream(out + "/coherent.txt") << (found ? 1 : 0) << " " << cc.msamp() << "\n";
+            HIP_OK(hipFree(d_raw));
+            HIP_OK(hipFree(d_blocks));
+            HIP_OK(hipFree(d_cimg));
+            HIP_OK(hipFree(d_cbins));
+        }
     } catch (const std::exception& e) {
         fprintf(stderr, "error: %s\n", e.what());
         return 1;

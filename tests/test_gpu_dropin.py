@@ -115,6 +115,8 @@ def test_cpp_dropin_harness(torch_cuda, tmp_path):
         got = det[key]
         assert got[0] == "1" and [int(t) for t in got[1:4]] == [w, r_, c], key
         assert np.float32(float(got[4])) == pytest.approx(v, rel=1e-6), key
+    # CChunk_gpu::process through CoherentChunk (sps/coherent.hpp)
+    check_coherent_process(str(tmp_path))
 
 
 def test_fdmtcpu_and_cfdmtcpu_host_buffers(torch_cuda):
@@ -251,3 +253,36 @@ def test_prefix_sums(torch_cuda, n):
     torch.cuda.synchronize()
     if n < (1 << 23):
         np.testing.assert_array_equal(ds.cpu().numpy(), np.cumsum(small.astype(np.int64)).astype(np.float32))
+
+
+def _hashed_guppi(npol, nchan, nsamp):
+    """The harness's raw chunk: hashed noise in [-20, 20], samples 1500..1563
+    times 4 and 400..431 times 2 in every channel and polarisation."""
+    i = np.arange(nchan * nsamp * npol, dtype=np.uint64)
+    v = ((((i * np.uint64(2654435761)) % np.uint64(1 << 32)) >> np.uint64(7)) % np.uint64(41)).astype(np.int64) - 20
+    s = (i // npol) % nsamp
+    v = np.where((s >= 1500) & (s < 1564), v * 4, np.where((s >= 400) & (s < 432), v * 2, v))
+    return v.astype(np.int8).reshape(nchan, nsamp, npol)
+
+
+def check_coherent_process(out_dir):
+    """The harness's CoherentChunk::process against the oracle's chain: the
+    S/N image within float-FFT tolerance of double FFTs, and candidates found
+    and logged."""
+    npol, nchan, len_sft, nbin, nfft, nov = 4, 8, 8, 1024, 4, 128
+    nsamp = (nbin - 2 * nov) * nfft
+    raw = _hashed_guppi(npol, nchan, nsamp)
+    blocks = orc.unpack("guppi", raw, nsamp, nov, nbin, nfft, npol, nchan)
+    rows, msamp = nchan * len_sft, nfft * (nbin // len_sft - 2 * (nov // len_sft))
+    cf = orc.CFdmt(1200.0, 1600.0, rows, msamp, len_sft)
+    ref = []
+    for dm in (0.0, 30.0):
+        wf = orc.coherent_chain(blocks, dm, 1200.0, 1600.0, nchan, len_sft, nbin, nfft, npol, nov, 1e-6)
+        wn, mask, _ = orc.normalise(wf, 1)
+        ref.append(orc.fdmt_normalization(cf.process(wn), cf.process(mask)))
+    ref = np.concatenate(ref)
+    found, ms = open(os.path.join(out_dir, "coherent.txt")).read().split()
+    assert int(ms) == msamp and int(found) == 1
+    got = np.fromfile(os.path.join(out_dir, "coherent_image.bin"), np.float32).reshape(ref.shape)
+    np.testing.assert_allclose(got, ref, rtol=1e-3, atol=1e-3 * float(np.abs(ref).max()))
+    assert os.path.getsize(os.path.join(out_dir, "coherent_candidates.log")) > 0
