@@ -369,6 +369,24 @@ def bench_coherent(args):
         t_fwd += ev[1].elapsed_time(ev[2]) / steps
         t_dm += ev[2].elapsed_time(ev[3]) / steps / len(dms)
     ok = bool(torch.isfinite(wf).all().item()) and float(wf.max().item()) > 0.0
+    # the whole of CChunk_gpu::process from the raw bytes: unpack, then the
+    # forward FFT and per DM the waterfall, statistics, two CFdmt transforms
+    # and the S/N normalisation, then the clustering (which synchronises)
+    img = torch.empty((len(dms) * C["len_sft"], cc.cols), dtype=torch.float32, device="cuda")
+
+    def chunk_process():
+        unpack.unpack_chunk_guppi(raw, nsamp, nov, nbin, nfft, npol, nchan, blocks)
+        return cc.process(blocks, dms, C["sigma_bound"], d_image=img, bwrite=False)
+
+    chunk_process()
+    torch.cuda.synchronize()
+    t_proc = 0.0
+    for _ in range(steps):
+        ev[0].record()
+        _, _, dele, _ = chunk_process()
+        ev[1].record()
+        torch.cuda.synchronize()
+        t_proc += ev[0].elapsed_time(ev[1]) / steps
     chunk = 8.0 * cc.spectra
     kept = chunk * (nbin // C["len_sft"] - 2 * (nov // C["len_sft"])) / (nbin // C["len_sft"])
     b_unpack = raw.numel() + chunk
@@ -384,6 +402,7 @@ def bench_coherent(args):
         "fused": cc.fused, "ms_per_chunk": t_unpack + t_fwd + t_dm * len(dms), "steps": steps,
         "unpack_ms": t_unpack, "forward_fft_ms": t_fwd, "ms_per_dm": t_dm,
         "waterfall_samples_per_s": cc.rows * cc.cols * len(dms) / ((t_unpack + t_fwd + t_dm * len(dms)) * 1e-3),
+        "process_ms_per_chunk": t_proc, "process_delegates": int(dele.numel()),
         "compulsory_GBs": {"unpack": gbs(b_unpack, t_unpack), "forward_fft": gbs(b_fwd, t_fwd),
                            "per_dm": gbs(b_dm, t_dm)},
         "frac_of_peak_per_dm": gbs(b_dm, t_dm) / 8000.0,

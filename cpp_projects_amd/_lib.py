@@ -96,6 +96,7 @@ SIGNATURES = {
     "sps_coherent_destroy": [_vp],
     "sps_coherent_get_shape": [_vp, _pi64, _pi64, _pi64],
     "sps_coherent_is_fused": [_vp, C.POINTER(C.c_int)],
+    "sps_fdmt_normalization_rows": [_vp, C.c_int64, C.c_int64, _vp, C.c_int64, _vp, _vp],
     "sps_coherent_forward": [_vp, _vp, _vp],
     "sps_coherent_waterfall": [_vp, _vp, C.c_double, _vp, _vp],
     "sps_coherent_chirp": [_vp, _vp, C.c_double, _vp, _vp],

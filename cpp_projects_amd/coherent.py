@@ -87,7 +87,11 @@ class CoherentChunk:
         (in place, with the keep mask); the CFdmt (nchan * len_sft channels,
         msamp columns, imaxDt = len_sft) of the mask and of the waterfall;
         fdmt_normalization into rows [idm * len_sft, (idm + 1) * len_sft) of
-        the image.  Then clusterization_main over the image (steps 16-17:
+        the image.  The mask is constant along each row, so its transform is
+        causal and constant from column len_sft - 1 on: only its first
+        4 * len_sft columns are transformed and the division extends the last
+        one (fdmt_normalization_rows; the same values as the full
+        transform).  Then clusterization_main over the image (steps 16-17:
         widths 1..wnd, threshold sigma_bound, gap bins, the log written to
         filename with the waterfall's sampling time len_sft * tsamp).  The
         reference's power-of-two gather before the clustering keeps no
@@ -100,7 +104,7 @@ class CoherentChunk:
         import torch
 
         from .fdmt import CFdmtGpu
-        from .search import _write_delegates, clusterization, fdmt_normalization, statistical_preprocessing
+        from .search import _write_delegates, clusterization, fdmt_normalization_rows, statistical_preprocessing
 
         dev = d_blocks.device
         rows, msamp, L = self.rows, self.cols, self.len_sft
@@ -108,21 +112,25 @@ class CoherentChunk:
         if d_image is None:
             d_image = torch.empty((ndm * L, msamp), dtype=torch.float32, device=dev)
         self._check(d_image, ndm * L * msamp, "image")
+        W = min(msamp, 4 * L)
         if self._fdmt is None:
             self._fdmt = CFdmtGpu(self.f_min, self.f_max, rows, msamp, L)
+            self._fdmt_mask = CFdmtGpu(self.f_min, self.f_max, rows, W, L)
             self._wf = torch.empty((rows, msamp), dtype=torch.float32, device=dev)
             self._norm_in = torch.empty((rows, msamp), dtype=torch.float32, device=dev)
-            self._norm = torch.empty((L, msamp), dtype=torch.float32, device=dev)
+            self._mask_w = torch.empty((rows, W), dtype=torch.float32, device=dev)
+            self._norm = torch.empty((L, W), dtype=torch.float32, device=dev)
             self._buff = torch.empty(2 * rows + 4, dtype=torch.float32, device=dev)
         img = d_image.view(ndm * L, msamp)
         self.forward(d_blocks)
         for idm, dm in enumerate(dms):
             self.waterfall(d_blocks, dm, self._wf)
             statistical_preprocessing.calc_fdmt_inputs(self._wf, rows, msamp, self._buff, self._norm_in)
-            self._fdmt.process_image(self._norm_in, self._norm, False)
+            self._mask_w.copy_(self._norm_in[:, :W])
+            self._fdmt_mask.process_image(self._mask_w, self._norm, False)
             out = img[idm * L:(idm + 1) * L]
             self._fdmt.process_image(self._wf, out, False)
-            fdmt_normalization(out, self._norm, L * msamp, out)
+            fdmt_normalization_rows(out, L, msamp, self._norm, W, out)
         d_thr = torch.tensor([float(sigma_bound)], dtype=torch.float32, device=dev)
         d_bins = torch.tensor(list(bins), dtype=torch.int32, device=dev)
         ok, cands, n = clusterization.gather_candidates_in_heap_with_plan(img, ndm * L, msamp, d_thr, wnd)

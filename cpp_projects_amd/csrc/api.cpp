@@ -964,6 +964,15 @@ SPS_API sps_status sps_fdmt_normalization(const float* d_arr, const float* d_nor
     });
 }
 
+SPS_API sps_status sps_fdmt_normalization_rows(const float* d_arr, int64_t rows, int64_t cols, const float* d_norm,
+                                               int64_t norm_cols, float* d_out, void* stream) {
+    return guarded([&] {
+        SPS_REQUIRE(d_arr && d_norm && d_out && rows >= 0 && cols >= 0, "invalid argument");
+        SPS_REQUIRE(norm_cols >= 1 && norm_cols <= cols, "norm_cols must lie in 1..cols");
+        launch_fdmt_normalization_rows(d_arr, rows, cols, d_norm, norm_cols, d_out, as_stream(stream));
+    });
+}
+
 // ---------------------------------------------------------------------------
 // candidates
 // ---------------------------------------------------------------------------

@@ -184,6 +184,8 @@ void launch_stats_finalize(const double* d_row_mean, const double* d_row_var, in
 void launch_apply_norm(float* x, float* mask, int64_t rows, int64_t cols, const float* d_mu, const float* d_sigma,
                        const uint8_t* d_keep, hipStream_t st);
 void launch_fdmt_normalization(const float* a, const float* n, int64_t len, float* out, hipStream_t st);
+void launch_fdmt_normalization_rows(const float* a, int64_t rows, int64_t cols, const float* n, int64_t norm_cols,
+                                    float* out, hipStream_t st);
 // S/N normalisation of the DM-time image by the mask transform: for t < norm_len
 // the divisor comes from d_norm (rows x norm_len), beyond it from *d_nkeep.
 void launch_snr_normalise(float* img, int64_t rows, int64_t cols, int64_t pitch, const float* d_norm,

@@ -231,6 +231,20 @@ __global__ void __launch_bounds__(kThreads) k_fdmt_normalization(const float* __
         out[i] = a[i] / sqrtf((float)((double)n[i] + 1.0E-8));
 }
 
+// fdmt_normalization with a causal norm image kept for its first norm_cols
+// columns: out[r][t] = a[r][t] / sqrtf(norm[r][min(t, norm_cols - 1)] + 1e-8)
+__global__ void __launch_bounds__(kThreads) k_fdmt_normalization_rows(const float* __restrict__ a, int64_t cols,
+                                                                      const float* __restrict__ n, int64_t norm_cols,
+                                                                      float* __restrict__ out) {
+    const int64_t r = blockIdx.y;
+    const float* nr = n + r * norm_cols;
+    const float tail = sqrtf((float)((double)nr[norm_cols - 1] + 1.0E-8));
+    for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < cols; t += (int64_t)gridDim.x * kThreads) {
+        const float d = t < norm_cols ? sqrtf((float)((double)nr[t] + 1.0E-8)) : tail;
+        out[r * cols + t] = a[r * cols + t] / d;
+    }
+}
+
 __global__ void __launch_bounds__(kThreads) k_snr_normalise(float* __restrict__ img, int64_t cols, int64_t pitch,
                                                             const float* __restrict__ norm, int64_t norm_len,
                                                             const double* __restrict__ stats) {
@@ -1540,6 +1554,15 @@ void launch_apply_norm(float* x, float* mask, int64_t rows, int64_t cols, const 
     if (rows > 65535) throw Error(SPS_ERR_INVALID_ARGUMENT, "more than 65535 rows");
     hipLaunchKernelGGL(k_apply_norm, row_grid(rows, cols), dim3(kThreads), 0, st, x, mask, cols, d_mu, d_sigma,
                        d_keep);
+    SPS_HIP(hipGetLastError());
+}
+
+void launch_fdmt_normalization_rows(const float* a, int64_t rows, int64_t cols, const float* n, int64_t norm_cols,
+                                    float* out, hipStream_t st) {
+    if (rows <= 0 || cols <= 0) return;
+    if (rows > 65535) throw Error(SPS_ERR_INVALID_ARGUMENT, "more than 65535 rows");
+    hipLaunchKernelGGL(k_fdmt_normalization_rows, row_grid(rows, cols), dim3(kThreads), 0, st, a, cols, n, norm_cols,
+                       out);
     SPS_HIP(hipGetLastError());
 }
 

@@ -76,6 +76,14 @@ def fdmt_normalization(d_arr, d_norm, length, d_out):
     call("sps_fdmt_normalization", _dptr(d_arr), _dptr(d_norm), int(length), _dptr(d_out), _stream())
 
 
+def fdmt_normalization_rows(d_arr, rows, cols, d_norm, norm_cols, d_out):
+    """fdmt_normalization over a rows x cols image with a norm image given
+    for its first norm_cols columns and constant along each row beyond:
+    d_out[r][t] = d_arr[r][t] / sqrtf(d_norm[r][min(t, norm_cols - 1)] + 1e-8)."""
+    call("sps_fdmt_normalization_rows", _dptr(d_arr), int(rows), int(cols), _dptr(d_norm), int(norm_cols),
+         _dptr(d_out), _stream())
+
+
 def _gather(d_fdmt, rows, cols, d_thr, wnd, all_widths, capacity=None):
     import torch
 

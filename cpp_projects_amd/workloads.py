@@ -31,7 +31,8 @@ BATCH = dict(nbeams=64, nch_choices=(1024, 2048, 4096), n_lo=1 << 14, n_hi=1 << 
 # (1200-1600 MHz), two polarisations, 32 sub-channels each, 2^16-bin FFT
 # blocks with 2^13 bins of overlap, 16 blocks per chunk, 8 coherent DMs
 COHERENT = dict(f_min=1200.0, f_max=1600.0, npol=4, nchan=64, len_sft=32, nbin=1 << 16, nfft=16, noverlap=1 << 13,
-                tsamp=1.6e-7, dms=(0.0, 25.0, 50.0, 75.0, 100.0, 125.0, 150.0, 175.0), seed=9)
+                tsamp=1.6e-7, dms=(0.0, 25.0, 50.0, 75.0, 100.0, 125.0, 150.0, 175.0), seed=9,
+                sigma_bound=6.0)
 LARGE = dict(nchans=8192, nsamps=1 << 20, dt_max=8191, box_q=9, threshold=6.0, npulses=50, seed=5, rfi_frac=0.1,
              rfi_sigma=5.0)
 

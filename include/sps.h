@@ -195,6 +195,15 @@ SPS_API sps_status sps_calc_fdmt_inputs(float* d_in, int64_t nrows, int64_t ncol
  * d_out[i] = d_arr[i] / sqrtf(d_norm[i] + 1e-8); d_out may alias d_arr. */
 SPS_API sps_status sps_fdmt_normalization(const float* d_arr, const float* d_norm, int64_t len, float* d_out,
                                           void* stream);
+/* The same division over a rows x cols image whose norm image is causal and
+ * constant along each row from column norm_cols - 1 on (the transform of a
+ * row-constant keep mask is, once t passes the largest shift), so only its
+ * first norm_cols columns are given: d_out[r][t] = d_arr[r][t] /
+ * sqrtf(d_norm[r][min(t, norm_cols - 1)] + 1e-8), d_norm rows x norm_cols
+ * (no reference counterpart: it lets CChunk_gpu::process transform a
+ * narrow slice of the mask; d_out may alias d_arr). */
+SPS_API sps_status sps_fdmt_normalization_rows(const float* d_arr, int64_t rows, int64_t cols, const float* d_norm,
+                                               int64_t norm_cols, float* d_out, void* stream);
 
 /* ---------------------------------------------------------------------- */
 /* boxcar candidate gathering                                              */

@@ -87,19 +87,25 @@ public:
                  float* d_image, int wnd, const int* d_bins, const std::string& filename = "candidates.log",
                  float t_chunkBegin = 0.0f) {
         const int rows = (int)m_rows, cols = (int)m_cols;
+        // the keep mask is constant along each row: its transform is causal and
+        // constant from column len_sft - 1 on, so only its first W columns are
+        // transformed and the division extends the last one
+        const int W = cols < 4 * m_len_sft ? cols : 4 * m_len_sft;
         if (!m_have_fdmt) {
             m_Fdmt = CFdmtGpu(m_Fmin, m_Fmax, rows, cols, m_len_sft);
+            m_FdmtMask = CFdmtGpu(m_Fmin, m_Fmax, rows, W, m_len_sft);
             m_have_fdmt = true;
         }
-        float *wf = nullptr, *norm_in = nullptr, *norm = nullptr, *buff = nullptr, *thr = nullptr;
+        float *wf = nullptr, *norm_in = nullptr, *mask_w = nullptr, *norm = nullptr, *buff = nullptr, *thr = nullptr;
         auto release = [&] {
-            for (float* p : {wf, norm_in, norm, buff, thr})
+            for (float* p : {wf, norm_in, mask_w, norm, buff, thr})
                 if (p) (void)hipFree(p);
         };
         try {
             hip_ok(hipMalloc(&wf, sizeof(float) * (size_t)rows * cols));
             hip_ok(hipMalloc(&norm_in, sizeof(float) * (size_t)rows * cols));
-            hip_ok(hipMalloc(&norm, sizeof(float) * (size_t)m_len_sft * cols));
+            hip_ok(hipMalloc(&mask_w, sizeof(float) * (size_t)rows * W));
+            hip_ok(hipMalloc(&norm, sizeof(float) * (size_t)m_len_sft * W));
             hip_ok(hipMalloc(&buff, sizeof(float) * (size_t)(2 * rows + 4)));
             hip_ok(hipMalloc(&thr, sizeof(float)));
             hip_ok(hipMemcpy(thr, &sigma_bound, sizeof(float), hipMemcpyHostToDevice));
@@ -108,9 +114,11 @@ public:
                 float* out = d_image + idm * (size_t)m_len_sft * cols;
                 waterfall(pcmparrRawSignalCur, coh_dms[idm], wf);
                 statistical_preprocessing::calc_fdmt_inputs(wf, rows, cols, buff, norm_in);
-                m_Fdmt.process_image(norm_in, norm, false);
+                hip_ok(hipMemcpy2DAsync(mask_w, sizeof(float) * W, norm_in, sizeof(float) * cols, sizeof(float) * W,
+                                        rows, hipMemcpyDeviceToDevice, nullptr));
+                m_FdmtMask.process_image(mask_w, norm, false);
                 m_Fdmt.process_image(wf, out, false);
-                fdmt_normalization(out, norm, m_len_sft * cols, out);
+                fdmt_normalization_rows(out, m_len_sft, cols, norm, W, out);
             }
             const bool ok = clusterization::clusterization_main(d_image, m_len_sft * (int)coh_dms.size(), cols, *thr,
                                                                 wnd, d_bins, filename, (float)m_len_sft * m_tsamp,
@@ -131,7 +139,7 @@ private:
     float m_Fmin, m_Fmax;
     int m_len_sft;
     float m_tsamp;
-    CFdmtGpu m_Fdmt;
+    CFdmtGpu m_Fdmt, m_FdmtMask;
     bool m_have_fdmt = false;
     sps_coherent* m_h = nullptr;
     int64_t m_rows = 0, m_cols = 0, m_spectra = 0;

@@ -61,6 +61,13 @@ inline int calcThreadsForMean_and_Disp(unsigned const int nCols) {
 }  // namespace statistical_preprocessing
 
 // Chunk_gpu.cuh:118-119 (a host call here rather than a kernel launch)
+// The division with a causal norm image given for its first norm_cols
+// columns and constant along each row beyond (sps_fdmt_normalization_rows).
+inline void fdmt_normalization_rows(const float* d_arr, int rows, int cols, const float* d_norm, int norm_cols,
+                                    float* d_out, hipStream_t stream = nullptr) {
+    sps_detail::check(sps_fdmt_normalization_rows(d_arr, rows, cols, d_norm, norm_cols, d_out, stream));
+}
+
 inline void fdmt_normalization(fdmt_type_* d_arr, fdmt_type_* d_norm, const int len, float* d_pOutArray) {
     sps_detail::check(sps_fdmt_normalization(d_arr, d_norm, len, d_pOutArray, nullptr));
 }

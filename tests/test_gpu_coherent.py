@@ -260,3 +260,33 @@ def test_process_pipeline_vs_oracle(torch_cuda, tmp_path):
     np.testing.assert_allclose(d["msnr"], d_ref["msnr"], rtol=1e-3)
     np.testing.assert_allclose(gsz.cpu().numpy(), gsz_ref, rtol=0.05)
     assert log.exists() and log.stat().st_size > 0
+
+
+def test_fdmt_normalization_rows(torch_cuda):
+    """sps_fdmt_normalization_rows: the oracle's fdmt_normalization
+    (Chunk_gpu.cu:841-852) of the norm image extended along each row from its
+    last given column, bit for bit, in place too; norm_cols outside 1..cols
+    is an error."""
+    from cpp_projects_amd import _lib
+    from cpp_projects_amd.search import fdmt_normalization_rows
+
+    torch = torch_cuda
+    rng = np.random.default_rng(17)
+    for rows, cols, w in ((7, 1000, 33), (3, 64, 64), (5, 300, 1)):
+        print("case normalization rows", rows, cols, w, flush=True)
+        a = rng.standard_normal((rows, cols)).astype(np.float32)
+        n = rng.uniform(0, 50, (rows, w)).astype(np.float32)
+        n[0, :] = 0.0
+        full = np.concatenate([n, np.repeat(n[:, -1:], cols - w, axis=1)], axis=1)
+        ref = orc.fdmt_normalization(a, full)
+        d_a, d_n = torch.from_numpy(a).cuda(), torch.from_numpy(n).cuda()
+        out = torch.empty_like(d_a)
+        fdmt_normalization_rows(d_a, rows, cols, d_n, w, out)
+        fdmt_normalization_rows(d_a, rows, cols, d_n, w, d_a)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(out.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+        np.testing.assert_array_equal(d_a.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+    d = torch.zeros((2, 8), device="cuda")
+    for w in (0, 9):
+        with pytest.raises(_lib.InvalidArgument):
+            fdmt_normalization_rows(d, 2, 8, d, w, d)

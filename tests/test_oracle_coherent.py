@@ -64,3 +64,20 @@ def test_tone_row_and_dispersion_shift():
         fr = (nrows - 1 - r) * foff + f_min
         sh = int(np.floor(4.148808e3 * dm * (1 / f_min ** 2 - 1 / fr ** 2) / tsw + 0.5))
         np.testing.assert_array_equal(b[r], np.roll(a[r], sh))
+
+
+def test_mask_transform_is_causal_and_flat():
+    """The premise of CoherentChunk.process's narrow mask transform
+    (fdmt_normalization_rows): the CFdmt (imaxDt = L) of a row-constant 0/1
+    keep mask equals, over its first W >= L columns, the CFdmt of those W
+    columns alone, and from column W - 1 on it is constant along each row --
+    bit for bit, at W = 4 L as the pipeline uses and at W = L."""
+    rng = np.random.default_rng(5)
+    for rows, L, cols in ((64, 8, 384), (256, 32, 1000), (128, 16, 300), (64, 8, 20)):
+        keep = (rng.random(rows) > 0.2).astype(np.float32)
+        mask = np.repeat(keep[:, None], cols, axis=1)
+        full = orc.CFdmt(1200.0, 1600.0, rows, cols, L).process(mask)
+        for W in (min(cols, 4 * L), L):
+            narrow = orc.CFdmt(1200.0, 1600.0, rows, W, L).process(np.ascontiguousarray(mask[:, :W]))
+            np.testing.assert_array_equal(full[:, :W], narrow)
+            np.testing.assert_array_equal(full[:, W:], np.repeat(narrow[:, W - 1:W], cols - W, axis=1))
