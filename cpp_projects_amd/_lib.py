@@ -114,6 +114,7 @@ SIGNATURES = {
     "sps_digitize": [_vp, _i64, _vp],
     "sps_boxcar_snr_adjoint": [_vp, _i64, _i64, _int, _vp, _vp],
     "sps_search_create": [C.POINTER(SearchParams), C.POINTER(_vp)],
+    "sps_search_create_cfdmt": [C.POINTER(SearchParams), _int, C.POINTER(_vp)],
     "sps_search_destroy": [_vp],
     "sps_search_run": [_vp, _vp, _vp, _vp, _vp, _vp],
     "sps_search_sort_candidates": [_vp, _vp, _vp, _pi64, _vp],

@@ -24,7 +24,7 @@ class CoherentChunk:
 
     def __init__(self, f_min, f_max, npol, nchan, len_sft, nbin, nfft, noverlap, tsamp):
         self.f_min, self.f_max, self.len_sft, self.tsamp = float(f_min), float(f_max), int(len_sft), float(tsamp)
-        self._fdmt = None
+        self._search = None
         h = C.c_void_p()
         call("sps_coherent_create", float(f_min), float(f_max), int(npol), int(nchan), int(len_sft), int(nbin),
              int(nfft), int(noverlap), float(tsamp), C.byref(h))
@@ -83,15 +83,14 @@ class CoherentChunk:
                 t_chunk_begin=0.0, bwrite=True):
         """CChunk_gpu::process (CDMT/Chunk_gpu.cu:254-425) on unpacked blocks,
         which the forward FFT overwrites with their spectra (step 2).  For
-        each coherent DM idm (steps 3-14): the waterfall; calc_fdmt_inputs
-        (in place, with the keep mask); the CFdmt (nchan * len_sft channels,
-        msamp columns, imaxDt = len_sft) of the mask and of the waterfall;
-        fdmt_normalization into rows [idm * len_sft, (idm + 1) * len_sft) of
-        the image.  The mask is constant along each row, so its transform is
-        causal and constant from column len_sft - 1 on: only its first
-        4 * len_sft columns are transformed and the division extends the last
-        one (fdmt_normalization_rows; the same values as the full
-        transform).  Then clusterization_main over the image (steps 16-17:
+        each coherent DM idm (steps 3-14): the waterfall, then one search call
+        over the CFdmt plan (nchan * len_sft channels, msamp columns, imaxDt =
+        len_sft; sps_search_create_cfdmt) that does calc_fdmt_inputs' statistics,
+        the transform of the keep mask (over its causal extent: the mask is
+        constant along each row), the transform of the waterfall with the
+        normalisation inside the init's loads, and fdmt_normalization inside
+        the last pass, into rows [idm * len_sft, (idm + 1) * len_sft) of the
+        image -- the same values as the reference's separate steps.  Then clusterization_main over the image (steps 16-17:
         widths 1..wnd, threshold sigma_bound, gap bins, the log written to
         filename with the waterfall's sampling time len_sft * tsamp).  The
         reference's power-of-two gather before the clustering keeps no
@@ -103,8 +102,7 @@ class CoherentChunk:
         no value passes the threshold)."""
         import torch
 
-        from .fdmt import CFdmtGpu
-        from .search import _write_delegates, clusterization, fdmt_normalization_rows, statistical_preprocessing
+        from .search import SingleBeamSearch, _write_delegates, clusterization
 
         dev = d_blocks.device
         rows, msamp, L = self.rows, self.cols, self.len_sft
@@ -112,25 +110,18 @@ class CoherentChunk:
         if d_image is None:
             d_image = torch.empty((ndm * L, msamp), dtype=torch.float32, device=dev)
         self._check(d_image, ndm * L * msamp, "image")
-        W = min(msamp, 4 * L)
-        if self._fdmt is None:
-            self._fdmt = CFdmtGpu(self.f_min, self.f_max, rows, msamp, L)
-            self._fdmt_mask = CFdmtGpu(self.f_min, self.f_max, rows, W, L)
+        if self._search is None:
+            # its own gather (widths 1, nothing stored) is not used
+            self._search = SingleBeamSearch(self.f_min, self.f_max, rows, msamp, L - 1, in_dtype="f32", box_q=0,
+                                            threshold=1e30, cand_capacity=0, cfdmt_imaxdt=L)
             self._wf = torch.empty((rows, msamp), dtype=torch.float32, device=dev)
-            self._norm_in = torch.empty((rows, msamp), dtype=torch.float32, device=dev)
-            self._mask_w = torch.empty((rows, W), dtype=torch.float32, device=dev)
-            self._norm = torch.empty((L, W), dtype=torch.float32, device=dev)
-            self._buff = torch.empty(2 * rows + 4, dtype=torch.float32, device=dev)
+            self._cands = torch.empty((1, 4), dtype=torch.int32, device=dev)
+            self._count = torch.zeros(1, dtype=torch.int64, device=dev)
         img = d_image.view(ndm * L, msamp)
         self.forward(d_blocks)
         for idm, dm in enumerate(dms):
             self.waterfall(d_blocks, dm, self._wf)
-            statistical_preprocessing.calc_fdmt_inputs(self._wf, rows, msamp, self._buff, self._norm_in)
-            self._mask_w.copy_(self._norm_in[:, :W])
-            self._fdmt_mask.process_image(self._mask_w, self._norm, False)
-            out = img[idm * L:(idm + 1) * L]
-            self._fdmt.process_image(self._wf, out, False)
-            fdmt_normalization_rows(out, L, msamp, self._norm, W, out)
+            self._search.run(self._wf, img[idm * L:(idm + 1) * L], self._cands, self._count)
         d_thr = torch.tensor([float(sigma_bound)], dtype=torch.float32, device=dev)
         d_bins = torch.tensor(list(bins), dtype=torch.int32, device=dev)
         ok, cands, n = clusterization.gather_candidates_in_heap_with_plan(img, ndm * L, msamp, d_thr, wnd)

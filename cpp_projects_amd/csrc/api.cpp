@@ -1289,7 +1289,10 @@ struct sps_search {
     }
 };
 
-static std::unique_ptr<sps_search> make_search(const sps_search_params* p) {
+// cfdmt_imaxdt > 0: the data transform is the legacy CFdmt plan
+// (CFdmtGpu(f_min, f_max, nchans, nsamps, imaxDt)) instead of the FDMT plan;
+// dt_max and tsamp are then unused
+static std::unique_ptr<sps_search> make_search(const sps_search_params* p, int cfdmt_imaxdt = 0) {
     SPS_REQUIRE(p->in_dtype == SPS_F32 || p->in_dtype == SPS_U8, "unknown dtype");
     SPS_REQUIRE(p->norm_mode >= SPS_NORM_NONE && p->norm_mode <= SPS_NORM_PER_CHANNEL, "unknown norm mode");
     SPS_REQUIRE(p->box_q >= 0 && p->box_q <= 13, "box_q out of range");
@@ -1297,8 +1300,14 @@ static std::unique_ptr<sps_search> make_search(const sps_search_params* p) {
     std::unique_ptr<sps_search> s(new sps_search());
     s->p = *p;
     s->f.reset(new sps_fdmt());
-    s->f->plan.reset(new FdmtPlan(p->f_min, p->f_max, p->nchans, p->nsamps, p->tsamp, p->dt_max, 1, 0));
-    s->f->eng = s->f->plan->lower(kInitMean);
+    if (cfdmt_imaxdt > 0) {
+        SPS_REQUIRE(p->nchans <= INT32_MAX && p->nsamps <= INT32_MAX, "CFdmt sizes must fit in int");
+        s->f->cplan.reset(new CFdmtPlan(p->f_min, p->f_max, (int)p->nchans, (int)p->nsamps, cfdmt_imaxdt));
+        s->f->eng = s->f->cplan->lower();
+    } else {
+        s->f->plan.reset(new FdmtPlan(p->f_min, p->f_max, p->nchans, p->nsamps, p->tsamp, p->dt_max, 1, 0));
+        s->f->eng = s->f->plan->lower(kInitMean);
+    }
     s->f->init_layout();
     s->ndt = s->f->eng.out_rows;
     // the mask transform is causal: samples t < L depend on inputs before
@@ -1331,6 +1340,15 @@ SPS_API sps_status sps_search_create(const sps_search_params* p, sps_search** ou
         SPS_REQUIRE(p && out, "null argument");
         *out = nullptr;
         *out = make_search(p).release();
+    });
+}
+
+SPS_API sps_status sps_search_create_cfdmt(const sps_search_params* p, int imaxdt, sps_search** out) {
+    return guarded([&] {
+        SPS_REQUIRE(p && out, "null argument");
+        SPS_REQUIRE(imaxdt >= 1, "imaxdt must be positive");
+        *out = nullptr;
+        *out = make_search(p, imaxdt).release();
     });
 }
 

@@ -262,7 +262,11 @@ class SingleBeamSearch:
     beam: normalise -> FDMT -> S/N normalisation -> boxcar gather."""
 
     def __init__(self, f_min, f_max, nchans, nsamps, dt_max, tsamp=1.0, in_dtype="u8", norm_mode=SPS_NORM_GLOBAL,
-                 snr_norm=True, box_q=8, threshold=6.0, cand_capacity=1 << 20):
+                 snr_norm=True, box_q=8, threshold=6.0, cand_capacity=1 << 20, cfdmt_imaxdt=None):
+        """cfdmt_imaxdt: run the legacy CFdmt plan (CFdmtGpu(f_min, f_max,
+        nchans, nsamps, cfdmt_imaxdt)) instead of the FDMT plan, as
+        CChunk_gpu::process does per coherent DM; dt_max and tsamp are then
+        unused."""
         p = SearchParams()
         p.f_min, p.f_max, p.tsamp = f_min, f_max, tsamp
         p.nchans, p.nsamps, p.dt_max = nchans, nsamps, dt_max
@@ -271,7 +275,10 @@ class SingleBeamSearch:
         p.threshold, p.cand_capacity = threshold, cand_capacity
         self.params = p
         self._h = C.c_void_p()
-        call("sps_search_create", C.byref(p), C.byref(self._h))
+        if cfdmt_imaxdt is None:
+            call("sps_search_create", C.byref(p), C.byref(self._h))
+        else:
+            call("sps_search_create_cfdmt", C.byref(p), int(cfdmt_imaxdt), C.byref(self._h))
         n = C.c_int64()
         call("sps_search_get_ndt", self._h, C.byref(n))
         self.ndt = n.value

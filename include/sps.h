@@ -405,6 +405,12 @@ typedef struct {
 } sps_search_params;
 
 SPS_API sps_status sps_search_create(const sps_search_params* p, sps_search** out);
+/* The same search over the legacy CFdmt plan, as CChunk_gpu::process runs
+ * it for each coherent DM (m_Fdmt = CFdmtGpu(Fmin, Fmax, nchan * len_sft,
+ * msamp, len_sft), Chunk_gpu.cu:175-181; statistics, the transforms of mask
+ * and waterfall and fdmt_normalization, :325-356): p->dt_max and p->tsamp
+ * are unused, the image has imaxdt rows. */
+SPS_API sps_status sps_search_create_cfdmt(const sps_search_params* p, int imaxdt, sps_search** out);
 SPS_API sps_status sps_search_destroy(sps_search* s);
 /* Runs one beam.  d_in: nchans x nsamps; d_dmt (may be NULL): the S/N
  * normalised DM-time image (ndt x nsamps); d_cands (cand_capacity) and

@@ -39,6 +39,7 @@ public:
         m_fused = f != 0;
     }
     ~CoherentChunk() {
+        if (m_search) (void)sps_search_destroy(m_search);
         if (m_h) (void)sps_coherent_destroy(m_h);
     }
     CoherentChunk(const CoherentChunk&) = delete;
@@ -87,38 +88,42 @@ public:
                  float* d_image, int wnd, const int* d_bins, const std::string& filename = "candidates.log",
                  float t_chunkBegin = 0.0f) {
         const int rows = (int)m_rows, cols = (int)m_cols;
-        // the keep mask is constant along each row: its transform is causal and
-        // constant from column len_sft - 1 on, so only its first W columns are
-        // transformed and the division extends the last one
-        const int W = cols < 4 * m_len_sft ? cols : 4 * m_len_sft;
-        if (!m_have_fdmt) {
-            m_Fdmt = CFdmtGpu(m_Fmin, m_Fmax, rows, cols, m_len_sft);
-            m_FdmtMask = CFdmtGpu(m_Fmin, m_Fmax, rows, W, m_len_sft);
-            m_have_fdmt = true;
+        if (!m_search) {
+            // calc_fdmt_inputs, the CFdmt (imaxDt = len_sft) of mask and
+            // waterfall and fdmt_normalization in one search call per DM; its
+            // own gather (widths 1, nothing stored) is not used
+            sps_search_params sp{};
+            sp.f_min = m_Fmin;
+            sp.f_max = m_Fmax;
+            sp.tsamp = m_tsamp;
+            sp.nchans = rows;
+            sp.nsamps = cols;
+            sp.dt_max = m_len_sft - 1;
+            sp.in_dtype = SPS_F32;
+            sp.norm_mode = SPS_NORM_GLOBAL;
+            sp.snr_norm = 1;
+            sp.box_q = 0;
+            sp.threshold = 1e30f;
+            sp.cand_capacity = 0;
+            sps_detail::check(sps_search_create_cfdmt(&sp, m_len_sft, &m_search));
         }
-        float *wf = nullptr, *norm_in = nullptr, *mask_w = nullptr, *norm = nullptr, *buff = nullptr, *thr = nullptr;
+        float *wf = nullptr, *thr = nullptr, *cand = nullptr, *cnt = nullptr;
         auto release = [&] {
-            for (float* p : {wf, norm_in, mask_w, norm, buff, thr})
+            for (float* p : {wf, thr, cand, cnt})
                 if (p) (void)hipFree(p);
         };
         try {
             hip_ok(hipMalloc(&wf, sizeof(float) * (size_t)rows * cols));
-            hip_ok(hipMalloc(&norm_in, sizeof(float) * (size_t)rows * cols));
-            hip_ok(hipMalloc(&mask_w, sizeof(float) * (size_t)rows * W));
-            hip_ok(hipMalloc(&norm, sizeof(float) * (size_t)m_len_sft * W));
-            hip_ok(hipMalloc(&buff, sizeof(float) * (size_t)(2 * rows + 4)));
             hip_ok(hipMalloc(&thr, sizeof(float)));
+            hip_ok(hipMalloc(&cand, sizeof(sps_cand)));
+            hip_ok(hipMalloc(&cnt, sizeof(uint64_t)));
             hip_ok(hipMemcpy(thr, &sigma_bound, sizeof(float), hipMemcpyHostToDevice));
             forward(pcmparrRawSignalCur);
             for (size_t idm = 0; idm < coh_dms.size(); ++idm) {
                 float* out = d_image + idm * (size_t)m_len_sft * cols;
                 waterfall(pcmparrRawSignalCur, coh_dms[idm], wf);
-                statistical_preprocessing::calc_fdmt_inputs(wf, rows, cols, buff, norm_in);
-                hip_ok(hipMemcpy2DAsync(mask_w, sizeof(float) * W, norm_in, sizeof(float) * cols, sizeof(float) * W,
-                                        rows, hipMemcpyDeviceToDevice, nullptr));
-                m_FdmtMask.process_image(mask_w, norm, false);
-                m_Fdmt.process_image(wf, out, false);
-                fdmt_normalization_rows(out, m_len_sft, cols, norm, W, out);
+                sps_detail::check(sps_search_run(m_search, wf, out, reinterpret_cast<sps_cand*>(cand),
+                                                 reinterpret_cast<uint64_t*>(cnt), nullptr));
             }
             const bool ok = clusterization::clusterization_main(d_image, m_len_sft * (int)coh_dms.size(), cols, *thr,
                                                                 wnd, d_bins, filename, (float)m_len_sft * m_tsamp,
@@ -139,8 +144,7 @@ private:
     float m_Fmin, m_Fmax;
     int m_len_sft;
     float m_tsamp;
-    CFdmtGpu m_Fdmt, m_FdmtMask;
-    bool m_have_fdmt = false;
+    sps_search* m_search = nullptr;
     sps_coherent* m_h = nullptr;
     int64_t m_rows = 0, m_cols = 0, m_spectra = 0;
     bool m_fused = false;

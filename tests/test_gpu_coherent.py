@@ -290,3 +290,35 @@ def test_fdmt_normalization_rows(torch_cuda):
     for w in (0, 9):
         with pytest.raises(_lib.InvalidArgument):
             fdmt_normalization_rows(d, 2, 8, d, w, d)
+
+
+def test_search_over_cfdmt_plan_vs_oracle(torch_cuda):
+    """sps_search_create_cfdmt: the per-DM search of CChunk_gpu::process
+    (Chunk_gpu.cu:325-356) in one call -- statistics, the CFdmt (imaxDt) of
+    the keep mask and of the waterfall with the normalisation inside the
+    init, the S/N division inside the last pass -- bit for bit against the
+    oracle's chain of the reference's steps (calc_fdmt_inputs, CFdmt of the
+    normalised waterfall and of the mask, fdmt_normalization), with one
+    outlier-variance row cleaned; up to the 2048 x 24576 waterfall of the
+    measured chunk."""
+    from cpp_projects_amd.search import SingleBeamSearch
+
+    torch = torch_cuda
+    rng = np.random.default_rng(8)
+    for rows, cols, L in ((64, 384, 8), (256, 1000, 32), (2048, 24576, 32)):
+        print("case search over CFdmt", rows, cols, L, flush=True)
+        x = (rng.standard_normal((rows, cols)) * 3 + 1).astype(np.float32)
+        x[5] *= 40
+        wn, mask, st = orc.normalise(x, 1)
+        assert st[2][5] == 0 and st[2].sum() == rows - 1
+        cf = orc.CFdmt(1200.0, 1600.0, rows, cols, L)
+        ref = orc.fdmt_normalization(cf.process(wn), cf.process(mask))
+        s = SingleBeamSearch(1200.0, 1600.0, rows, cols, L - 1, in_dtype="f32", box_q=2, threshold=1e30,
+                             cand_capacity=0, cfdmt_imaxdt=L)
+        assert s.ndt == L
+        dmt = torch.full((L, cols), float("nan"), dtype=torch.float32, device="cuda")
+        cands = torch.empty((1, 4), dtype=torch.int32, device="cuda")
+        count = torch.zeros(1, dtype=torch.int64, device="cuda")
+        s.run(torch.from_numpy(x).cuda(), dmt, cands, count)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(dmt.cpu().numpy().view(np.uint32), ref.view(np.uint32))
