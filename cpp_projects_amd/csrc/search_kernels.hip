@@ -122,10 +122,27 @@ __global__ void __launch_bounds__(kLanes) k_row_stats(const void* __restrict__ i
             }
         } else {
             const float* x = static_cast<const float*>(in) + r * cols;
-            for (int64_t t = t0; t < t0 + kChunk && t < cols; ++t) {
-                const double d = (double)x[t];
-                a += d;
-                b += d * d;
+            if (t0 + kChunk <= cols && ((reinterpret_cast<uintptr_t>(x + t0) & 15) == 0)) {
+                // a whole aligned chunk: four float4 loads, added in order
+                float4 v[kChunk / 4];
+#pragma unroll
+                for (int q = 0; q < kChunk / 4; ++q) v[q] = *reinterpret_cast<const float4*>(x + t0 + 4 * q);
+#pragma unroll
+                for (int q = 0; q < kChunk / 4; ++q) {
+                    const float e4[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const double d = (double)e4[e];
+                        a += d;
+                        b += d * d;
+                    }
+                }
+            } else {
+                for (int64_t t = t0; t < t0 + kChunk && t < cols; ++t) {
+                    const double d = (double)x[t];
+                    a += d;
+                    b += d * d;
+                }
             }
         }
     }
