@@ -107,6 +107,8 @@ class CoherentChunk:
         dev = d_blocks.device
         rows, msamp, L = self.rows, self.cols, self.len_sft
         ndm = len(dms)
+        if ndm == 0:
+            raise ValueError("process needs at least one coherent DM")
         if d_image is None:
             d_image = torch.empty((ndm * L, msamp), dtype=torch.float32, device=dev)
         self._check(d_image, ndm * L * msamp, "image")

@@ -88,6 +88,7 @@ public:
                  float* d_image, int wnd, const int* d_bins, const std::string& filename = "candidates.log",
                  float t_chunkBegin = 0.0f) {
         const int rows = (int)m_rows, cols = (int)m_cols;
+        if (coh_dms.empty()) throw std::invalid_argument("process needs at least one coherent DM");
         if (!m_search) {
             // calc_fdmt_inputs, the CFdmt (imaxDt = len_sft) of mask and
             // waterfall and fdmt_normalization in one search call per DM; its

@@ -260,6 +260,8 @@ def test_process_pipeline_vs_oracle(torch_cuda, tmp_path):
     np.testing.assert_allclose(d["msnr"], d_ref["msnr"], rtol=1e-3)
     np.testing.assert_allclose(gsz.cpu().numpy(), gsz_ref, rtol=0.05)
     assert log.exists() and log.stat().st_size > 0
+    with pytest.raises(ValueError):
+        cc.process(d_blocks, (), thr)
 
 
 def test_fdmt_normalization_rows(torch_cuda):
