@@ -12,6 +12,7 @@ end the clustering over the image with the candidates log.
 """
 import ctypes as C
 
+from . import _lib
 from ._lib import call
 from .fdmt import _dptr, _stream
 
@@ -41,8 +42,11 @@ class CoherentChunk:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
-            call("sps_coherent_destroy", h)
+        if h is not None and h.value:
+            try:
+                _lib.lib().sps_coherent_destroy(h)
+            except Exception:  # interpreter shutdown: the library may be gone
+                pass
             self._h = None
 
     def _check(self, t, n, what):
