@@ -1150,6 +1150,7 @@ struct sps_search {
     int64_t norm_len = 0;            // samples of the mask transform
     DevBuf rm, rv, stats, mu, sg, keep, zeros, ones_f;
     DevBuf norm_img, norm_a, norm_b; // mask transform output and its states
+    DevBuf mask_shift;               // [ndt][nchans] int16 channel shifts (empty: run the transform)
     DevBuf dmt;                      // internal DM-time image when the caller passes none
     bool ready = false;
     // clustering leg (sps_search_cluster): all-widths candidates, delegates
@@ -1269,9 +1270,15 @@ struct sps_search {
         MergeEpilogue epi{nullptr, 0, nullptr};
         const bool fuse = p.snr_norm && f->eng.niters > 0;
         if (p.snr_norm) {
-            ChanNorm mask{zeros.as<float>(), ones_f.as<float>(), keep.as<uint8_t>()};
             const int64_t L = norm_len;
-            f->run(nullptr, 0, 1, N, &mask, L, sc.nimg, L, sc.na, sc.nb, round_up(L, 4), st, nullptr, nullptr, true);
+            if (mask_shift.p) {
+                launch_mask_counts(mask_shift.as<int16_t>(), keep.as<uint8_t>(), ndt, p.nchans, (int)L, sc.nimg, L,
+                                   st);
+            } else {
+                ChanNorm mask{zeros.as<float>(), ones_f.as<float>(), keep.as<uint8_t>()};
+                f->run(nullptr, 0, 1, N, &mask, L, sc.nimg, L, sc.na, sc.nb, round_up(L, 4), st, nullptr, nullptr,
+                       true);
+            }
             epi = MergeEpilogue{sc.nimg, L, stats.as<double>()};
         }
         if (ev) SPS_HIP(hipEventRecord(ev[2], st));
@@ -1288,6 +1295,67 @@ struct sps_search {
         if (ev) SPS_HIP(hipEventRecord(ev[it + 5], st));
     }
 };
+
+// The mask transform's input is 1 on every sample of a kept channel and 0
+// elsewhere.  With the running-mean init its cells are exactly 1 from t = the
+// cell's dt on and 0 before (a mean of ones; cells t < dt are zero -- the
+// CFdmt init keeps partial means there instead, so it is not counted), and
+// each merge adds exact small
+// integers, a head counted from t = offset on.  So output row r at t counts
+// the kept channels whose accumulated shift along r's tree (init dt plus the
+// head offsets on the path) is <= t.  The shifts are tabulated here per
+// (output row, channel); a row that would hold a channel twice, or shifts
+// beyond int16, leave the table empty and the transform runs instead.
+static void build_mask_shifts(sps_search& s) {
+    const EnginePlan& e = s.f->eng;
+    const int64_t L = s.norm_len;
+    const int64_t R = e.out_rows, C = e.nchans;
+    // L + 256 LDS counters within 64 KiB
+    if (e.init_mode != kInitMean || L <= 0 || L > 16000 || R <= 0 || C <= 0 || R * C > ((int64_t)1 << 31)) return;
+    // state-0 row -> channel and init dt
+    std::vector<int32_t> chan0((size_t)e.rows[0], -1), dt0((size_t)e.rows[0], 0);
+    for (int64_t c = 0; c < C; ++c)
+        for (int64_t k = e.grid_off[(size_t)c]; k < e.grid_off[(size_t)c + 1]; ++k) {
+            const int64_t r0 = e.chan_row_off[(size_t)c] + (k - e.grid_off[(size_t)c]);
+            chan0[(size_t)r0] = (int32_t)c;
+            dt0[(size_t)r0] = e.grid[(size_t)k];
+        }
+    // per iteration: output row -> its record
+    std::vector<std::vector<int64_t>> rec((size_t)e.niters + 1);
+    for (int i = 1; i <= e.niters; ++i) {
+        rec[(size_t)i].assign((size_t)e.rows[(size_t)i], -1);
+        const auto& it = e.iter[(size_t)i];
+        for (size_t j = 0; j < it.size(); ++j) rec[(size_t)i][(size_t)it[j].out_row] = (int64_t)j;
+    }
+    std::vector<int16_t> tab((size_t)(R * C), (int16_t)32767);
+    struct Node {
+        int i;
+        int64_t row, base;
+    };
+    std::vector<Node> stack;
+    for (int64_t r = 0; r < R; ++r) {
+        int16_t* trow = tab.data() + (size_t)(r * C);
+        stack.assign(1, Node{e.niters, r, 0});
+        while (!stack.empty()) {
+            const Node n = stack.back();
+            stack.pop_back();
+            if (n.i == 0) {
+                const int32_t c = chan0[(size_t)n.row];
+                const int64_t sh = n.base + dt0[(size_t)n.row];
+                if (c < 0 || trow[c] != 32767 || sh < 0 || sh >= 32767) return;  // unexpected: keep the transform
+                trow[c] = (int16_t)sh;
+                continue;
+            }
+            const int64_t j = rec[(size_t)n.i][(size_t)n.row];
+            if (j < 0) return;
+            const RowDesc& d = e.iter[(size_t)n.i][(size_t)j];
+            stack.push_back(Node{n.i - 1, d.tail_row, n.base});
+            if (d.head_row >= 0) stack.push_back(Node{n.i - 1, d.head_row, n.base + d.offset});
+        }
+    }
+    s.mask_shift.ensure(sizeof(int16_t) * tab.size());
+    SPS_HIP(hipMemcpy(s.mask_shift.p, tab.data(), sizeof(int16_t) * tab.size(), hipMemcpyHostToDevice));
+}
 
 // cfdmt_imaxdt > 0: the data transform is the legacy CFdmt plan
 // (CFdmtGpu(f_min, f_max, nchans, nsamps, imaxDt)) instead of the FDMT plan;
@@ -1330,6 +1398,7 @@ static std::unique_ptr<sps_search> make_search(const sps_search_params* p, int c
     int64_t mx = 0;
     for (auto v : ext) mx = std::max(mx, v);
     s->norm_len = std::min<int64_t>(p->nsamps, mx + 1);
+    build_mask_shifts(*s);
     return s;
 }
 

@@ -188,6 +188,11 @@ void launch_fdmt_normalization_rows(const float* a, int64_t rows, int64_t cols, 
                                     float* out, hipStream_t st);
 // S/N normalisation of the DM-time image by the mask transform: for t < norm_len
 // the divisor comes from d_norm (rows x norm_len), beyond it from *d_nkeep.
+// The transform of a row-constant 0/1 keep mask over its first L columns,
+// counted: out[r][t] = #{c : keep[c], shift[r][c] <= t} (shift: the
+// accumulated delay of channel c in output row r, an int16 table; L <= 16000)
+void launch_mask_counts(const int16_t* shift, const uint8_t* keep, int64_t rows, int64_t nchans, int L, float* out,
+                        int64_t pitch, hipStream_t st);
 void launch_snr_normalise(float* img, int64_t rows, int64_t cols, int64_t pitch, const float* d_norm,
                           int64_t norm_len, const double* d_stats, hipStream_t st);
 

@@ -1591,6 +1591,55 @@ void launch_fdmt_normalization(const float* a, const float* n, int64_t len, floa
     SPS_HIP(hipGetLastError());
 }
 
+// One block per output row: the histogram of the kept channels' shifts in
+// LDS, its inclusive prefix sum (per-thread contiguous chunks, then a scan of
+// the chunk totals), written as floats.  Counts are integers below 2^24: exact.
+__global__ void __launch_bounds__(kThreads) k_mask_counts(const int16_t* __restrict__ shift,
+                                                          const uint8_t* __restrict__ keep, int64_t nchans, int L,
+                                                          float* __restrict__ out, int64_t pitch) {
+    extern __shared__ int hist[];  // L counters, then kThreads chunk totals
+    int* tot = hist + L;
+    const int64_t r = blockIdx.x;
+    for (int t = threadIdx.x; t < L; t += kThreads) hist[t] = 0;
+    __syncthreads();
+    const int16_t* sr = shift + r * nchans;
+    for (int64_t c = threadIdx.x; c < nchans; c += kThreads) {
+        const int s = sr[c];
+        if (keep[c] && s >= 0 && s < L) atomicAdd(&hist[s], 1);
+    }
+    __syncthreads();
+    const int chunk = (L + kThreads - 1) / kThreads;
+    const int t0 = threadIdx.x * chunk;
+    const int t1 = min(L, t0 + chunk);
+    int acc = 0;
+    for (int t = t0; t < t1; ++t) {
+        acc += hist[t];
+        hist[t] = acc;
+    }
+    // inclusive scan of the chunk totals (Hillis-Steele), then exclusive
+    int v = acc;
+    for (int off = 1; off < kThreads; off <<= 1) {
+        __syncthreads();
+        tot[threadIdx.x] = v;
+        __syncthreads();
+        if ((int)threadIdx.x >= off) v += tot[threadIdx.x - off];
+    }
+    const int base = v - acc;
+    for (int t = t0; t < t1; ++t) hist[t] += base;
+    __syncthreads();
+    float* o = out + r * pitch;
+    for (int t = threadIdx.x; t < L; t += kThreads) o[t] = (float)hist[t];
+}
+
+void launch_mask_counts(const int16_t* shift, const uint8_t* keep, int64_t rows, int64_t nchans, int L, float* out,
+                        int64_t pitch, hipStream_t st) {
+    if (rows <= 0 || L <= 0) return;
+    const size_t lds = sizeof(int) * ((size_t)L + kThreads);
+    hipLaunchKernelGGL(k_mask_counts, dim3((uint32_t)rows), dim3(kThreads), lds, st, shift, keep, nchans, L, out,
+                       pitch);
+    SPS_HIP(hipGetLastError());
+}
+
 void launch_snr_normalise(float* img, int64_t rows, int64_t cols, int64_t pitch, const float* d_norm,
                           int64_t norm_len, const double* d_stats, hipStream_t st) {
     if (rows <= 0 || cols <= 0) return;
