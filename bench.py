@@ -653,7 +653,7 @@ def main():
     stage_names = (["stats", "mask_transform", "init"] +
                    [f"pass{k}(L={schedule[k]})" + ("+init" if k == 0 and fused0 else "") for k in range(npass - 1)] +
                    [f"pass{npass - 1}(L={schedule[-1]})+snr"] + ["snr_norm_separate", "boxcar"])
-    stage_kernels = (["k_row_stats<true> + k_stats_finalize", "k_init_consec<2, true> + k_merge_fused<L, true, false>",
+    stage_kernels = (["k_row_stats<true> + k_stats_finalize", "k_mask_counts (counted mask transform)",
                       "k_init_consec<1, true>"] + [kname(k) for k in range(npass)] +
                      ["k_snr_normalise", "k_boxcar_pow2_pf<%d>" % min(max(c["box_q"], 8), 9)])
     kernels = {}
