@@ -2,6 +2,20 @@
 
 TEST INFRASTRUCTURE ONLY.  Imported by tests/, __graft_entry__.smoke() and the
 cpu_baseline leg of bench.py; the product package never imports this module.
+
+The coherent front-end (coherent_chirp, coherent_detect, coherent_chain) is
+restated here in numpy.  It is pinned by what the mathematics fixes
+(tests/test_oracle_coherent.py):
+  * at DM 0 the chirp factor is the real taper / nbin;
+  * the intensity pass conserves the kept intensity;
+  * a pure tone lands in the row the index maps predict (derived
+    independently in the test);
+  * each row's dispersion shift is its rounded delay;
+  * the CFdmt of a row-constant keep mask is causal and flat (the chunk
+    pipeline's premise).
+No reference test or fixture covers this path (cuFFT is not runnable here):
+its FFT rounding against cuFFT is parity unpinned, bounded by the tolerance
+against double-precision FFTs.
 """
 import ctypes as C
 import os
