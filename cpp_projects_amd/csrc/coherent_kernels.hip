@@ -343,9 +343,19 @@ __device__ inline void fft_passes(float2 (&z)[8], float2* lds, const float2* tw,
     }
 }
 
+// What chirp_factor needs besides the bin: the sub-band and channel widths,
+// the taper table and the exact 1/nbin (0 when nbin is not a power of two).
+struct ChirpArgs {
+    double bw_sub, bw_chan;
+    const double* taper;
+    double inv_nbin;
+};
+
+// fac == nullptr: each launch block computes its segment's factors itself
+// (the same chirp_factor values the table would hold)
 template <int N>
 __global__ void __launch_bounds__(N / 8) k_coherent_fused(const float2* __restrict__ spec,
-                                                          const float2* __restrict__ fac,
+                                                          const float2* __restrict__ fac, ChirpArgs ca,
                                                           const float2* __restrict__ twg, float* __restrict__ wf,
                                                           int npairs, int nchan, int len_sft, int nfft, int bpw,
                                                           int nov_sub, int mbin_adj, int msamp, double dm,
@@ -366,8 +376,18 @@ __global__ void __launch_bounds__(N / 8) k_coherent_fused(const float2* __restri
     const int nbin = N * len_sft;
     const int64_t seg_off = (int64_t)c * nbin + (int64_t)k * N + t;
     float2 d[8];
+    if (fac) {
 #pragma unroll
-    for (int m = 0; m < 8; ++m) d[m] = fac[seg_off + T * m];
+        for (int m = 0; m < 8; ++m) d[m] = fac[seg_off + T * m];
+    } else {
+        // segment k holds sub-band ich's bins (the half-row rotation)
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            int col;
+            d[m] = chirp_factor(c, ich, t + T * m, len_sft, N, dm, fmin, ca.bw_sub, ca.bw_chan, ca.taper, ca.inv_nbin,
+                                &col);
+        }
+    }
     const int64_t plane = (int64_t)nfft * nchan * nbin;
     const int b0 = (int)blockIdx.y * bpw;
     const int b1 = min(nfft, b0 + bpw);
@@ -431,7 +451,7 @@ struct sps_coherent {
     hipfftHandle fwd = 0, inv = 0;
     bool have_fwd = false, have_inv = false;
     DevBuf chirped;  // the chirped spectra, then their sub-band time series (unfused path)
-    DevBuf factors;  // one DM's chirp factors [chan][nbin] (fused path)
+    DevBuf factors;  // one DM's chirp factors [chan][nbin] (fused path reading a table)
     DevBuf taper;    // the sub-band taper per bin, mbin doubles
     DevBuf twiddles; // the fused pass's FFT twiddle table (fft_twiddle_table)
     double bw_sub = 0, bw_chan = 0, inv_nbin = 0;
@@ -493,12 +513,17 @@ void coherent_detect(const sps_coherent& h, const float* d_buf, double dm, float
     SPS_HIP(hipGetLastError());
 }
 
+// the fused pass computes each segment's chirp factors itself rather than
+// reading a per-DM factor table
+constexpr bool kFusedOwnFactors = true;
+
 template <int N>
-void launch_fused(const CoherentShape& s, const float2* spec, const float2* fac, const float2* twg, float* wf,
-                  double dm, int bpw, double foff, double tsamp_wf, float scale, hipStream_t st) {
+void launch_fused(const CoherentShape& s, const float2* spec, const float2* fac, const ChirpArgs& ca,
+                  const float2* twg, float* wf, double dm, int bpw, double foff, double tsamp_wf, float scale,
+                  hipStream_t st) {
     static_assert(FftPlan<N>::tw_size > 0, "at least two passes");
     const dim3 grid((unsigned)s.rows(), (unsigned)ceil_div((int64_t)s.nfft, (int64_t)bpw));
-    hipLaunchKernelGGL(k_coherent_fused<N>, grid, dim3(N / 8), 0, st, spec, fac, twg, wf, s.npol / 2, s.nchan,
+    hipLaunchKernelGGL(k_coherent_fused<N>, grid, dim3(N / 8), 0, st, spec, fac, ca, twg, wf, s.npol / 2, s.nchan,
                        s.len_sft,
                        s.nfft, bpw, s.nov_sub(), s.mbin_adj(), (int)s.msamp(), dm, s.fmin, foff, tsamp_wf, scale);
 }
@@ -513,12 +538,16 @@ void coherent_fft(sps_coherent& h, hipfftHandle plan, float* d, int dir, hipStre
 
 void coherent_fused(sps_coherent& h, const float* d_spectra, double dm, float* d_wf, hipStream_t st) {
     const CoherentShape& s = h.sh;
-    float2* fac = h.factors.as<float2>();
-    const dim3 fgrid((unsigned)ceil_div((int64_t)s.mbin(), (int64_t)kThreads), (unsigned)s.len_sft,
-                     (unsigned)s.nchan);
-    hipLaunchKernelGGL(k_coherent_factors, fgrid, dim3(kThreads), 0, st, fac, dm, s.len_sft, s.mbin(), s.fmin,
-                       h.bw_sub, h.bw_chan, h.taper.as<double>(), h.inv_nbin);
-    SPS_HIP(hipGetLastError());
+    const ChirpArgs ca{h.bw_sub, h.bw_chan, h.taper.as<double>(), h.inv_nbin};
+    float2* fac = nullptr;  // the fused pass computes its factors
+    if (!kFusedOwnFactors) {
+        fac = h.factors.as<float2>();
+        const dim3 fgrid((unsigned)ceil_div((int64_t)s.mbin(), (int64_t)kThreads), (unsigned)s.len_sft,
+                         (unsigned)s.nchan);
+        hipLaunchKernelGGL(k_coherent_factors, fgrid, dim3(kThreads), 0, st, fac, dm, s.len_sft, s.mbin(), s.fmin,
+                           h.bw_sub, h.bw_chan, h.taper.as<double>(), h.inv_nbin);
+        SPS_HIP(hipGetLastError());
+    }
     const double foff = (s.fmax - s.fmin) / ((double)s.nchan * s.len_sft);
     const double tsamp_wf = (double)((float)(unsigned)s.len_sft * s.tsamp);
     const float scale = (float)(1.0 / ((double)s.nbin * (double)s.nbin));
@@ -529,12 +558,12 @@ void coherent_fused(sps_coherent& h, const float* d_spectra, double dm, float* d
     const float2* spec = reinterpret_cast<const float2*>(d_spectra);
     const float2* twg = h.twiddles.as<float2>();
     switch (s.mbin()) {
-        case 128: launch_fused<128>(s, spec, fac, twg, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
-        case 256: launch_fused<256>(s, spec, fac, twg, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
-        case 512: launch_fused<512>(s, spec, fac, twg, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
-        case 1024: launch_fused<1024>(s, spec, fac, twg, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
-        case 2048: launch_fused<2048>(s, spec, fac, twg, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
-        case 4096: launch_fused<4096>(s, spec, fac, twg, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
+        case 128: launch_fused<128>(s, spec, fac, ca, twg, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
+        case 256: launch_fused<256>(s, spec, fac, ca, twg, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
+        case 512: launch_fused<512>(s, spec, fac, ca, twg, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
+        case 1024: launch_fused<1024>(s, spec, fac, ca, twg, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
+        case 2048: launch_fused<2048>(s, spec, fac, ca, twg, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
+        case 4096: launch_fused<4096>(s, spec, fac, ca, twg, d_wf, dm, bpw, foff, tsamp_wf, scale, st); break;
         default: throw Error(SPS_ERR_RUNTIME, "fused coherent pass: unsupported sub-band length");
     }
     SPS_HIP(hipGetLastError());
@@ -569,7 +598,8 @@ SPS_API sps_status sps_coherent_create(float f_min, float f_max, int npol, int n
                                     nfft * nchan * len_sft * (npol / 2)),
                      "inverse plan");
         h->have_inv = true;
-        if (coherent_fusable(h->sh)) h->factors.ensure(sizeof(float) * 2 * (size_t)nchan * nbin);
+        if (coherent_fusable(h->sh) && !kFusedOwnFactors)
+            h->factors.ensure(sizeof(float) * 2 * (size_t)nchan * nbin);
         // double bw = m_Fmax - m_Fmin: a float difference (Chunk_fly_gpu.cu:96)
         const double bw = (double)(f_max - f_min);
         h->bw_sub = bw / nchan;
