@@ -291,7 +291,7 @@ SPS_API sps_status sps_unpack_lofar(const int8_t* d_in, int64_t nsamp, int nover
  * :254-322 steps 2-8; Chunk_fly_gpu.cu:94-164 chirp).  A handle holds the
  * two rocFFT plans of CChunk_gpu::create_fft_plans (forward: nbin points,
  * nfft * nchan * npol/2 blocks; inverse: mbin = nbin / len_sft points, len_sft
- * times as many), a one-DM chirp factor table (nchan * nbin complex) and,
+ * times as many), the sub-band taper per bin and,
  * on the unfused path only, one scratch chunk.  Spectra are complex float
  * [pol][fft][chan][bin] (the unpack's layout, npol/2 polarisations); the
  * waterfall is float [nchan * len_sft][msamp], msamp = nfft * (mbin - 2 *
