@@ -324,8 +324,8 @@ def bench_coherent(args):
     the chirp, the sub-band inverse FFTs and the intensity waterfall that the
     search takes.  Compulsory bytes: unpack reads the raw chunk and writes the
     blocks; the forward FFT reads and writes the chunk once; per DM the fused
-    pass reads the spectra once and writes the waterfall (its 32 MB factor
-    table is not counted).  Where the chunk does not take the fused pass, per
+    pass reads the spectra once and writes the waterfall.  Where the chunk
+    does not take the fused pass, per
     DM the chirp reads and writes the chunk, the inverse FFT reads and writes
     it, the intensity pass reads the kept bins and writes the waterfall."""
     import torch
